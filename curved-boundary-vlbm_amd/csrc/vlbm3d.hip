@@ -1,0 +1,928 @@
+// libvlbm: D3Q6x12 vectorial LBM time step for MI355X (gfx950).
+//
+// Storage is the active-node list of the reference (cbw3d:3683-3722): every
+// field is a (components, n_active) plane, nodes in lexicographic (i, j, k)
+// order with k fastest, so the +-z neighbours of a node are adjacent in memory
+// and +-y / +-x neighbours sit one row / one plane away.
+//
+// One step (cbw3d:4549-4589) runs as
+//   k_collide   moments of f0 (U, u, u*), J flag                      per node
+//   k_sweep     local displacement sweep into u_next                  per node   [if due]
+//   k_commit    intermediate sweep commits                            per node   [sweeps > 1]
+//   k_record    F repair + deferred-collision record + final commit   per node
+//   k_gather    characteristic state of the traction links            per (link, comp)
+//   k_prepare   boundary values + normal-traction Newton              per link
+//   k_stream    pull stream: f1[q,p] = fpost(f0, record)[q, p - e_q]  per node
+//   k_links     BFL + characteristic populations of cut links         per link
+// Post-collision populations are never materialised: k_record stores 24
+// doubles per node from which fpost[q][a] = (1 - omega) f[q][a] + A[a] +
+// s_q B[axis(q)][a] follows exactly, and k_stream / k_links apply it while
+// pulling.  Populations therefore cross HBM once per step in each direction.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/vlbm3d.h"
+#include "vlbm3d_device.h"
+
+using namespace vlbm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const char *where, hipError_t e) {
+    g_err = std::string(where) + ": " + hipGetErrorString(e);
+    return -2;
+}
+int fail_msg(const std::string &m) {
+    g_err = m;
+    return -1;
+}
+
+constexpr int BLOCK = 256;
+constexpr int NXCD = 8;
+constexpr int RED_BLOCKS = 1024;
+
+// Record layout (24 planes): A_v[3], A_F[9], B_v[r*3+axis] (9), B_F[3].
+enum { R_AV = 0, R_AF = 3, R_BV = 12, R_BF = 21 };
+
+struct Params {
+    int n, L;
+    Material m;
+    double dt, c, omega, dx, dy, dz;
+    int src_mode;
+    double src[12];
+    int recon, solver;
+    double disp_relax, w2;
+    int disp_bid;
+    double blend;
+};
+
+// Grid sizes rounded up to a multiple of the XCD count; block b of the launch
+// is mapped to logical block (b % 8) * (G / 8) + b / 8 so that each XCD walks
+// one contiguous slab of nodes and neighbour planes stay in its own L2.
+__device__ __forceinline__ int xcd_block() {
+    const int G = gridDim.x;
+    const int per = G / NXCD;
+    const int b = blockIdx.x;
+    return (b % NXCD) * per + b / NXCD;
+}
+inline int xcd_grid(int n) {
+    int g = (n + BLOCK - 1) / BLOCK;
+    return ((g + NXCD - 1) / NXCD) * NXCD;
+}
+
+// Population access.  double planes hold f; float planes hold f - f_rest
+// where f_rest[q][a] = 1/6 on the diagonal of F (a = 3, 7, 11), 0 elsewhere.
+__device__ __forceinline__ double rest(int a) { return (a == 3 || a == 7 || a == 11) ? (1.0 / 6.0) : 0.0; }
+__device__ __forceinline__ double ld(const double *f, size_t i, int) { return f[i]; }
+__device__ __forceinline__ double ld(const float *f, size_t i, int a) { return (double)f[i] + rest(a); }
+__device__ __forceinline__ void st(double *f, size_t i, int, double v) { f[i] = v; }
+__device__ __forceinline__ void st(float *f, size_t i, int a, double v) { f[i] = (float)(v - rest(a)); }
+
+// fpost[q][a] at node m from pre-collision f and the node's record.
+template <typename T>
+__device__ __forceinline__ double fpost(const T *f, const double *rec, size_t n, int q, int a, int node,
+                                        double om1) {
+    const double fv = ld(f, (size_t)(q * 12 + a) * n + node, a);
+    const int axis = q % 3;
+    const double s = q < 3 ? 1.0 : -1.0;
+    double b = 0.0;
+    if (a < 3) {
+        b = rec[(size_t)(R_BV + 3 * a + axis) * n + node];
+    } else if ((a - 3) % 3 == axis) {
+        b = rec[(size_t)(R_BF + (a - 3) / 3) * n + node];
+    }
+    return om1 * fv + rec[(size_t)a * n + node] + s * b;
+}
+
+__global__ void k_reset_invalid(int32_t *invalid) { invalid[0] = 0; }
+
+// Moments with the half-step source; u = u* + dt/2 v and (collide only)
+// u* = u + dt/2 v.  cbw3d:2110-2189 and :2193-2280.
+template <typename T, bool COLLIDE>
+__global__ __launch_bounds__(BLOCK) void k_collide(const T *__restrict__ f, double *__restrict__ U,
+                                                   double *__restrict__ u, double *__restrict__ us,
+                                                   int32_t *__restrict__ invalid, Params P) {
+    const int p = xcd_block() * BLOCK + threadIdx.x;
+    if (p >= P.n) return;
+    const size_t n = P.n;
+    double Um[NC];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) Um[a] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int a = 0; a < NC; ++a) Um[a] += ld(f, (size_t)(q * 12 + a) * n + p, a);
+    const double hdt = 0.5 * P.dt;
+    if (P.src_mode == SRC_CONST) {
+#pragma unroll
+        for (int a = 0; a < NC; ++a) Um[a] += hdt * P.src[a];
+    } else if (P.src_mode == SRC_DAMP) {
+        const double denom = 1.0 + hdt * P.src[0];
+        Um[0] = Um[0] / denom;
+        Um[1] = Um[1] / denom;
+        Um[2] = Um[2] / denom;
+    }
+#pragma unroll
+    for (int a = 0; a < NC; ++a) U[a * n + p] = Um[a];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double uk = us[k * n + p] + hdt * Um[k];
+        u[k * n + p] = uk;
+        if (COLLIDE) us[k * n + p] = uk + hdt * Um[k];
+    }
+    if (det3(Um + 3) <= P.m.jf) atomicOr(invalid, 1);
+}
+
+// One relaxed nearest-neighbour compatibility sweep of u (cbw3d:2473-2527)
+// with the Dirichlet anchors of the node's own cut links gathered in link
+// order (cbw3d:2413-2469), so the sums are deterministic.
+__global__ __launch_bounds__(BLOCK) void k_sweep(
+    const double *__restrict__ U, const double *__restrict__ u, double *__restrict__ un,
+    const int32_t *__restrict__ nbr, const int32_t *__restrict__ loff, const int32_t *__restrict__ lids,
+    const int32_t *__restrict__ link_bid, const double *__restrict__ lxb, const double *__restrict__ lyb,
+    const double *__restrict__ lzb, const int32_t *__restrict__ ai, const int32_t *__restrict__ aj,
+    const int32_t *__restrict__ ak, const int32_t *__restrict__ comp_kind, const int32_t *__restrict__ mode_by_id,
+    const double *__restrict__ prm_by_id, double time, Params P) {
+    const int p = xcd_block() * BLOCK + threadIdx.x;
+    if (p >= P.n) return;
+    const size_t n = P.n;
+    double d[3] = {0.0, 0.0, 0.0}, r[3] = {0.0, 0.0, 0.0};
+    const int l0 = loff[p], l1 = loff[p + 1];
+    if (l1 > l0) {
+        double F[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) F[k] = U[(3 + k) * n + p];
+        const double xp = ((double)ai[p] + 0.5) * P.dx, yp = ((double)aj[p] + 0.5) * P.dy,
+                     zp = ((double)ak[p] + 0.5) * P.dz;
+        for (int t = l0; t < l1; ++t) {
+            const int ell = lids[t];
+            const int bid = link_bid[ell];
+            bool force_all = false;
+            if (P.disp_bid >= 0) {
+                if (bid != P.disp_bid) continue;
+                force_all = true;
+            }
+            const double x = lxb[ell], y = lyb[ell], z = lzb[ell];
+            const double db[3] = {x - xp, y - yp, z - zp};
+            const int mode = mode_by_id[bid];
+            const double *prm = prm_by_id + bid * MAX_IDS;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (!(force_all || comp_kind[c * MAX_IDS + bid] == BC_DIR)) continue;
+                const double ub = dirichlet_disp(mode, prm, c, x, y, z, time);
+                const double g = (F[3 * c] - (c == 0 ? 1.0 : 0.0)) * db[0] +
+                                 (F[3 * c + 1] - (c == 1 ? 1.0 : 0.0)) * db[1] +
+                                 (F[3 * c + 2] - (c == 2 ? 1.0 : 0.0)) * db[2];
+                d[c] += P.w2;
+                r[c] += P.w2 * (ub - g);
+            }
+        }
+    }
+    double Fp[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Fp[k] = U[(3 + k) * n + p];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int pn = nbr[q * n + p];
+        if (pn < 0) continue;
+        const double h[3] = {(double)dir_comp(q, 0) * P.dx, (double)dir_comp(q, 1) * P.dy,
+                             (double)dir_comp(q, 2) * P.dz};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double g = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                double s = Fp[3 * c + j] + U[(3 + 3 * c + j) * n + pn];
+                if (j == c) s -= 2.0;
+                g += s * h[j];
+            }
+            g = 0.5 * g;
+            r[c] += u[c * n + pn] - g;
+            d[c] += 1.0;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double uc = u[c * n + p];
+        un[c * n + p] = d[c] > 0.0 ? uc + P.disp_relax * (r[c] / d[c] - uc) : uc;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_commit(const double *__restrict__ un, double *__restrict__ u,
+                                                  double *__restrict__ us, int n) {
+    const int p = xcd_block() * BLOCK + threadIdx.x;
+    if (p >= n) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const size_t i = (size_t)c * n + p;
+        const double delta = un[i] - u[i];
+        u[i] = un[i];
+        us[i] = us[i] + delta;
+    }
+}
+
+// Centred / second-order one-sided / first-order derivative of u along one
+// axis (cbw3d:2544-2571).
+__device__ __forceinline__ double du_axis(const double *u, const int32_t *nbr, size_t n, int c, int p,
+                                          int axis, double h) {
+    const int pp = nbr[axis * n + p], pm = nbr[(axis + 3) * n + p];
+    const double *uc = u + c * n;
+    if (pp >= 0 && pm >= 0) return (uc[pp] - uc[pm]) / (2.0 * h);
+    if (pp >= 0) {
+        const int pp2 = nbr[axis * n + pp];
+        if (pp2 >= 0) return (-3.0 * uc[p] + 4.0 * uc[pp] - uc[pp2]) / (2.0 * h);
+        return (uc[pp] - uc[p]) / h;
+    }
+    if (pm >= 0) {
+        const int pm2 = nbr[(axis + 3) * n + pm];
+        if (pm2 >= 0) return (3.0 * uc[p] - 4.0 * uc[pm] + uc[pm2]) / (2.0 * h);
+        return (uc[p] - uc[pm]) / h;
+    }
+    return 0.0;
+}
+
+// Deformation-gradient update F <- F + theta (I + grad u - F) with the J
+// guard (cbw3d:2575-2672), folded with the BGK collision of cbw3d:2281-2316
+// into the per-node record, plus the last displacement commit.
+__global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const double *__restrict__ ugrad,
+                                                  double *__restrict__ u, double *__restrict__ us,
+                                                  double *__restrict__ rec, const int32_t *__restrict__ nbr,
+                                                  int repair, int commit, Params P) {
+    const int p = xcd_block() * BLOCK + threadIdx.x;
+    if (p >= P.n) return;
+    const size_t n = P.n;
+    double v[3], Fo[9];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[k] = U[k * n + p];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Fo[k] = U[(3 + k) * n + p];
+    double Po[9], dP[9], dF[9];
+    piola(Fo, P.m, Po);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        dP[k] = 0.0;
+        dF[k] = 0.0;
+    }
+    if (repair) {
+        double beta = P.blend;
+        beta = beta < 0.0 ? 0.0 : (beta > 1.0 ? 1.0 : beta);
+        const double hh[3] = {P.dx, P.dy, P.dz};
+        double Fn[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double t = (i == j ? 1.0 : 0.0) + du_axis(ugrad, nbr, n, i, p, j, hh[j]);
+                Fn[3 * i + j] = Fo[3 * i + j] + beta * (t - Fo[3 * i + j]);
+            }
+        if (det3(Fn) > P.m.jf) {
+            double Pn[9];
+            piola(Fn, P.m, Pn);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                dP[k] = Pn[k] - Po[k];
+                dF[k] = Fn[k] - Fo[k];
+                U[(3 + k) * n + p] = Fn[k];
+            }
+        }
+    }
+    if (commit) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const size_t i = (size_t)c * n + p;
+            const double delta = ugrad[i] - u[i];
+            u[i] = ugrad[i];
+            us[i] = us[i] + delta;
+        }
+    }
+    const double w = P.omega;
+    const double st = P.dt * (2.0 - w) * (1.0 / 12.0);
+    const double w6 = w / 6.0, w2c = w / (2.0 * P.c), h2c = 0.5 / P.c;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) rec[(R_AV + a) * n + p] = w6 * v[a] + st * source_comp(a, P.src_mode, P.src, v[0], v[1], v[2]);
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        rec[(R_AF + k) * n + p] = w6 * Fo[k] + st * source_comp(3 + k, P.src_mode, P.src, v[0], v[1], v[2]) + dF[k] / 6.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) rec[(R_BV + k) * n + p] = -w2c * Po[k] - h2c * dP[k];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) rec[(R_BF + i) * n + p] = -w2c * v[i];
+}
+
+// Pull stream of the post-collision populations; the entries whose upstream
+// node is missing belong to cut links and are written by k_links.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_stream(const T *__restrict__ f0, T *__restrict__ f1,
+                                                  const double *__restrict__ rec,
+                                                  const int32_t *__restrict__ nbr, Params P) {
+    const int p = xcd_block() * BLOCK + threadIdx.x;
+    if (p >= P.n) return;
+    const size_t n = P.n;
+    const double om1 = 1.0 - P.omega;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int src = nbr[opp(q) * n + p];
+        if (src < 0) continue;
+#pragma unroll
+        for (int a = 0; a < NC; ++a) st(f1, (size_t)(q * 12 + a) * n + p, a, fpost(f0, rec, n, q, a, src, om1));
+    }
+}
+
+// Characteristic state at each traction link: predictor value and gradient
+// of every state component from the precomputed weights (cbw3d:2676-2700).
+__global__ __launch_bounds__(BLOCK) void k_gather(const double *__restrict__ U, const int32_t *__restrict__ idx,
+                                                  const double *__restrict__ wts, double *__restrict__ cs,
+                                                  int n, int L) {
+    const int ell = blockIdx.x * BLOCK + threadIdx.x;
+    const int a = blockIdx.y;
+    if (ell >= L) return;
+    if (idx[ell] < 0) return;
+    double val = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+    const double *Ua = U + (size_t)a * n;
+    for (int m = 0; m < MAX_CHAR; ++m) {
+        const int p = idx[(size_t)m * L + ell];
+        if (p < 0) continue;
+        const double x = Ua[p];
+        val += wts[((size_t)0 * MAX_CHAR + m) * L + ell] * x;
+        gx += wts[((size_t)1 * MAX_CHAR + m) * L + ell] * x;
+        gy += wts[((size_t)2 * MAX_CHAR + m) * L + ell] * x;
+        gz += wts[((size_t)3 * MAX_CHAR + m) * L + ell] * x;
+    }
+    cs[((size_t)0 * NC + a) * L + ell] = val;
+    cs[((size_t)1 * NC + a) * L + ell] = gx;
+    cs[((size_t)2 * NC + a) * L + ell] = gy;
+    cs[((size_t)3 * NC + a) * L + ell] = gz;
+}
+
+struct LinkTables {
+    const int32_t *lp, *lup, *lq, *lbid;
+    const double *eta, *xb, *yb, *zb, *nx, *ny, *nz;
+    const int32_t *kind_by_id, *comp_kind, *mode_by_id;
+    const double *prm_by_id;
+    const int32_t *dyn_en;
+    const double *dyn_t;
+    int32_t *lkind, *lck;
+    double *lbv, *lFb, *lPb;
+    double *cs;
+};
+
+// Boundary values, F predictor and the normal-traction solve per link
+// (cbw3d:2888-3151).
+__global__ __launch_bounds__(128) void k_prepare(const double *__restrict__ U, LinkTables T, double t, Params P) {
+    const int ell = blockIdx.x * 128 + threadIdx.x;
+    if (ell >= P.L) return;
+    const size_t n = P.n, L = P.L;
+    const int bid = T.lbid[ell];
+    const int mode = T.mode_by_id[bid];
+    const int kk[3] = {T.comp_kind[0 * MAX_IDS + bid], T.comp_kind[1 * MAX_IDS + bid], T.comp_kind[2 * MAX_IDS + bid]};
+    const double x = T.xb[ell], y = T.yb[ell], z = T.zb[ell];
+    const double nv[3] = {T.nx[ell], T.ny[ell], T.nz[ell]};
+    const double *prm = T.prm_by_id + bid * MAX_IDS;
+    double tr[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) tr[c] = curved_value(mode, prm, c, x, y, z, nv, t, P.m.jf);
+    if (T.dyn_en[ell] != 0)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) tr[c] = T.dyn_t[c * L + ell];
+    T.lkind[ell] = T.kind_by_id[bid];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        T.lck[c * L + ell] = kk[c];
+        T.lbv[c * L + ell] = tr[c];
+    }
+    double Fb[9] = {1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0};
+    double Pb[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const bool any_neu = kk[0] == BC_NEU || kk[1] == BC_NEU || kk[2] == BC_NEU;
+    if (any_neu) {
+        const int p = T.lp[ell];
+        double F[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) F[k] = U[(3 + k) * n + p];
+        if (P.recon == R_LOCALF_BFL) {
+            const int up = T.lup[ell];
+            if (up >= 0) {
+                const double eta = T.eta[ell];
+#pragma unroll
+                for (int k = 0; k < 9; ++k) F[k] = F[k] + eta * (F[k] - U[(3 + k) * n + up]);
+            }
+        }
+        if (P.recon == R_LOCALF)
+#pragma unroll
+            for (int k = 0; k < 9; ++k) F[k] = T.cs[(size_t)(3 + k) * L + ell];
+        const bool all_neu = kk[0] == BC_NEU && kk[1] == BC_NEU && kk[2] == BC_NEU;
+        solve_Fb(F, nv, tr, kk, !all_neu, P.solver, P.m, Fb);
+        piola(Fb, P.m, Pb);
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        T.lFb[k * L + ell] = Fb[k];
+        T.lPb[k * L + ell] = Pb[k];
+    }
+}
+
+// Incoming populations of the cut links: BFL with the population-pair
+// identities (cbw3d:3155-3241), then the characteristic closure on the
+// traction rows (cbw3d:2813-2884).  `use_rec` selects post-collision sources
+// (time step) or the raw populations (standalone reconstruction).
+template <typename T>
+__global__ __launch_bounds__(128) void k_links(const T *__restrict__ fsrc, T *__restrict__ fdst,
+                                               const double *__restrict__ rec, const double *__restrict__ U,
+                                               LinkTables Tb, int32_t *__restrict__ invalid, int use_rec,
+                                               int has_char, Params P) {
+    const int ell = blockIdx.x * 128 + threadIdx.x;
+    if (ell >= P.L) return;
+    const size_t n = P.n, L = P.L;
+    const int p = Tb.lp[ell], qo = Tb.lq[ell], qi = opp(qo);
+    double eta = Tb.eta[ell];
+    if (P.recon == R_HALF) eta = 0.5;
+    const double din[3] = {(double)dir_comp(qi, 0), (double)dir_comp(qi, 1), (double)dir_comp(qi, 2)};
+    const int up = Tb.lup[ell];
+    const double v[3] = {Tb.lbv[ell], Tb.lbv[L + ell], Tb.lbv[2 * L + ell]};
+    const int kk[3] = {Tb.lck[ell], Tb.lck[L + ell], Tb.lck[2 * L + ell]};
+    const double om1 = 1.0 - P.omega;
+    double outv[NC];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) {
+        const int row = a < 3 ? a : (a - 3) / 3;
+        double D = 1.0, S = 0.0;
+        if (kk[row] == BC_DIR) {
+            if (a < 3) {
+                D = -1.0;
+                S = v[a] / 3.0;
+            } else {
+                const int col = (a - 3) - row * 3;
+                S = -din[col] * v[row] / P.c;
+            }
+        } else {
+            if (a < 3) {
+                S = -(din[0] * Tb.lPb[(3 * a) * L + ell] + din[1] * Tb.lPb[(3 * a + 1) * L + ell] +
+                      din[2] * Tb.lPb[(3 * a + 2) * L + ell]) / P.c;
+            } else {
+                D = -1.0;
+                S = Tb.lFb[(a - 3) * L + ell] / 3.0;
+            }
+        }
+        double fo, fi;
+        if (use_rec) {
+            fo = fpost(fsrc, rec, n, qo, a, p, om1);
+            fi = fpost(fsrc, rec, n, qi, a, p, om1);
+        } else {
+            fo = ld(fsrc, (size_t)(qo * 12 + a) * n + p, a);
+            fi = ld(fsrc, (size_t)(qi * 12 + a) * n + p, a);
+        }
+        double val;
+        if (eta >= 0.5) {
+            const double alpha = 1.0 / (2.0 * eta);
+            val = alpha * (D * fo + S) + (1.0 - alpha) * fi;
+        } else {
+            double upv = fo;
+            if (up >= 0) upv = use_rec ? fpost(fsrc, rec, n, qo, a, up, om1) : ld(fsrc, (size_t)(qo * 12 + a) * n + up, a);
+            val = D * (2.0 * eta * fo + (1.0 - 2.0 * eta) * upv) + S;
+        }
+        outv[a] = val;
+    }
+    const bool any_neu = kk[0] == BC_NEU || kk[1] == BC_NEU || kk[2] == BC_NEU;
+    if (has_char && any_neu) {
+        double Ub[NC], Gx[NC], Gy[NC], Gz[NC], inc[NC];
+#pragma unroll
+        for (int a = 0; a < NC; ++a) {
+            Gx[a] = Tb.cs[((size_t)1 * NC + a) * L + ell];
+            Gy[a] = Tb.cs[((size_t)2 * NC + a) * L + ell];
+            Gz[a] = Tb.cs[((size_t)3 * NC + a) * L + ell];
+            if (a < 3)
+                Ub[a] = kk[a] == BC_DIR ? v[a] : U[(size_t)a * n + p];
+            else
+                Ub[a] = Tb.lFb[(a - 3) * L + ell];
+        }
+        if (det_state(Ub) <= P.m.jf) atomicAdd(invalid, 1);
+        char_population(Ub, Gx, Gy, Gz, din, Tb.eta[ell], P.dt, P.c, P.omega, P.src_mode, P.src, P.m, inc);
+#pragma unroll
+        for (int a = 0; a < NC; ++a) {
+            const int row = a < 3 ? a : (a - 3) / 3;
+            if (kk[row] == BC_NEU) outv[a] = inc[a];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < NC; ++a) st(fdst, (size_t)(qi * 12 + a) * n + p, a, outv[a]);
+}
+
+// f = feq(U) - dt/(2Q) B(U) (vnew3d:535-560 as used by cbw3d:4251-4253).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_init_pop(const double *__restrict__ U, T *__restrict__ f, Params P) {
+    const int p = xcd_block() * BLOCK + threadIdx.x;
+    if (p >= P.n) return;
+    const size_t n = P.n;
+    double Uv[NC], Pf[9];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) Uv[a] = U[a * n + p];
+    piola(Uv + 3, P.m, Pf);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int axis = q % 3;
+        const double s = q < 3 ? 1.0 : -1.0;
+#pragma unroll
+        for (int a = 0; a < NC; ++a) {
+            double flux = 0.0;
+            if (a < 3)
+                flux = -Pf[3 * a + axis];
+            else if ((a - 3) % 3 == axis)
+                flux = -Uv[(a - 3) / 3];
+            const double B = source_comp(a, P.src_mode, P.src, Uv[0], Uv[1], Uv[2]);
+            double fv = (1.0 / 6.0) * (Uv[a] + 3.0 * (s * flux / P.c));
+            fv -= 0.5 * P.dt / 6.0 * B;
+            st(f, (size_t)(q * 12 + a) * n + p, a, fv);
+        }
+    }
+}
+
+__global__ void k_set_ustar(const double *__restrict__ disp, const double *__restrict__ U,
+                            double *__restrict__ us, double dt, int n) {
+    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= n) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) us[(size_t)c * n + p] = disp[(size_t)c * n + p] - 0.5 * dt * U[(size_t)c * n + p];
+}
+
+template <typename T>
+__global__ void k_copy_pop(const T *__restrict__ s, T *__restrict__ d, size_t count) {
+    const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < count) d[i] = s[i];
+}
+
+__global__ void k_pop_to_f64(const float *__restrict__ f, double *__restrict__ d, int n) {
+    const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= (size_t)FQ * n) return;
+    const int a = (int)((i / n) % 12);
+    d[i] = ld(f, i, a);
+}
+__global__ void k_pop_from_f64(const double *__restrict__ s, float *__restrict__ f, int n) {
+    const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= (size_t)FQ * n) return;
+    const int a = (int)((i / n) % 12);
+    st(f, i, a, s[i]);
+}
+
+// Deterministic two-pass reductions over U and u.
+__device__ __forceinline__ void red_combine(double *acc, const double *o) {
+    acc[0] += o[0];
+    acc[1] = fmax(acc[1], o[1]);
+    acc[2] = fmin(acc[2], o[2]);
+    acc[3] = fmax(acc[3], o[3]);
+    acc[4] = fmax(acc[4], o[4]);
+    acc[5] += o[5];
+}
+
+__device__ inline void block_reduce(double *acc, double (*sh)[6]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sh[t][k] = acc[k];
+    __syncthreads();
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+        if (t < s) red_combine(sh[t], sh[t + s]);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_reduce_partial(const double *__restrict__ U, const double *__restrict__ u,
+                                                          double *__restrict__ part, int n) {
+    __shared__ double sh[BLOCK][6];
+    double acc[6] = {0.0, 0.0, INFINITY, -INFINITY, 0.0, 0.0};
+    for (int p = blockIdx.x * BLOCK + threadIdx.x; p < n; p += gridDim.x * BLOCK) {
+        double Uv[NC];
+        double nonfinite = 0.0;
+#pragma unroll
+        for (int a = 0; a < NC; ++a) {
+            Uv[a] = U[(size_t)a * n + p];
+            if (!isfinite(Uv[a])) nonfinite += 1.0;
+        }
+        double uu = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double x = u[(size_t)c * n + p];
+            if (!isfinite(x)) nonfinite += 1.0;
+            uu += x * x;
+        }
+        const double v2 = Uv[0] * Uv[0] + Uv[1] * Uv[1] + Uv[2] * Uv[2];
+        const double J = det3(Uv + 3);
+        const double o[6] = {v2, sqrt(v2), J, J, sqrt(uu), nonfinite};
+        red_combine(acc, o);
+    }
+    block_reduce(acc, sh);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) part[blockIdx.x * 8 + k] = sh[0][k];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_reduce_final(double *__restrict__ part, int nparts,
+                                                        const int32_t *__restrict__ invalid,
+                                                        double *__restrict__ out) {
+    __shared__ double sh[BLOCK][6];
+    double acc[6] = {0.0, 0.0, INFINITY, -INFINITY, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nparts; b += BLOCK) red_combine(acc, part + b * 8);
+    block_reduce(acc, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) out[k] = sh[0][k];
+        out[6] = (double)invalid[0];
+        out[7] = 0.0;
+    }
+}
+
+}  // namespace
+
+struct vlbm3d_handle {
+    vlbm3d_desc d;
+    Params P;
+    LinkTables T;
+    hipStream_t stream;
+};
+
+static Params make_params(const vlbm3d_desc &d) {
+    Params P{};
+    P.n = d.n_active;
+    P.L = d.n_links;
+    P.m = Material{d.model, d.lam, d.mu, d.p0, d.p1, d.p2, d.j_floor};
+    P.dt = d.dt;
+    P.c = d.c;
+    P.omega = d.omega;
+    P.dx = d.dx;
+    P.dy = d.dy;
+    P.dz = d.dz;
+    P.src_mode = d.source_mode;
+    for (int k = 0; k < 12; ++k) P.src[k] = d.source_params[k];
+    P.recon = d.recon;
+    P.solver = d.normal_solver;
+    P.disp_relax = d.disp_relax;
+    P.w2 = d.disp_weight * d.disp_weight;
+    P.disp_bid = d.disp_boundary_id;
+    P.blend = d.compat_blend;
+    return P;
+}
+
+static LinkTables make_tables(const vlbm3d_desc &d) {
+    LinkTables T{};
+    T.lp = d.link_p;
+    T.lup = d.link_up;
+    T.lq = d.link_q;
+    T.lbid = d.link_bid;
+    T.eta = d.link_eta;
+    T.xb = d.link_xb;
+    T.yb = d.link_yb;
+    T.zb = d.link_zb;
+    T.nx = d.link_nx;
+    T.ny = d.link_ny;
+    T.nz = d.link_nz;
+    T.kind_by_id = d.kind_by_id;
+    T.comp_kind = d.comp_kind_by_id;
+    T.mode_by_id = d.mode_by_id;
+    T.prm_by_id = d.params_by_id;
+    T.dyn_en = d.dyn_enabled;
+    T.dyn_t = d.dyn_traction;
+    T.lkind = d.lkind;
+    T.lck = d.lcomp_kind;
+    T.lbv = d.lbv;
+    T.lFb = d.lFb;
+    T.lPb = d.lPb;
+    T.cs = d.char_state;
+    return T;
+}
+
+#define CHECK_LAUNCH(where)                               \
+    do {                                                  \
+        hipError_t e_ = hipGetLastError();                \
+        if (e_ != hipSuccess) return fail((where), e_);   \
+    } while (0)
+
+template <typename T>
+static int run_boundary(vlbm3d_handle *h, const T *src, T *dst, double t, int use_rec) {
+    const vlbm3d_desc &d = h->d;
+    if (d.n_links == 0) return 0;
+    if (d.has_char) {
+        dim3 g((d.n_links + BLOCK - 1) / BLOCK, NC);
+        hipLaunchKernelGGL(k_gather, g, dim3(BLOCK), 0, h->stream, d.U, d.char_indices, d.char_weights,
+                           d.char_state, d.n_active, d.n_links);
+        CHECK_LAUNCH("k_gather");
+    }
+    const int gl = (d.n_links + 127) / 128;
+    hipLaunchKernelGGL(k_prepare, dim3(gl), dim3(128), 0, h->stream, (const double *)d.U, h->T, t, h->P);
+    CHECK_LAUNCH("k_prepare");
+    hipLaunchKernelGGL((k_links<T>), dim3(gl), dim3(128), 0, h->stream, src, dst, (const double *)d.record,
+                       (const double *)d.U, h->T, d.invalid, use_rec, d.has_char, h->P);
+    CHECK_LAUNCH("k_links");
+    return 0;
+}
+
+template <typename T>
+static int run_steps(vlbm3d_handle *h, T *f0, T *f1, double time, int64_t steps, int nsteps) {
+    const vlbm3d_desc &d = h->d;
+    const int gn = xcd_grid(d.n_active);
+    const Params &P = h->P;
+    for (int k = 0; k < nsteps; ++k) {
+        hipLaunchKernelGGL(k_reset_invalid, dim3(1), dim3(1), 0, h->stream, d.invalid);
+        hipLaunchKernelGGL((k_collide<T, true>), dim3(gn), dim3(BLOCK), 0, h->stream, f0, d.U, d.u, d.u_star,
+                           d.invalid, P);
+        CHECK_LAUNCH("k_collide");
+        const double tb = time + 0.5 * d.dt;
+        const bool disp = d.disp_interval > 0 && steps % d.disp_interval == 0;
+        const bool compat = d.compat_interval > 0 && steps % d.compat_interval == 0;
+        if (disp) {
+            for (int s = 0; s < d.disp_sweeps; ++s) {
+                hipLaunchKernelGGL(k_sweep, dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)d.U,
+                                   (const double *)d.u, d.u_next, d.neighbor, d.node_link_off, d.node_link_ids,
+                                   d.link_bid, d.link_xb, d.link_yb, d.link_zb, d.active_i, d.active_j,
+                                   d.active_k, d.comp_kind_by_id, d.mode_by_id, d.params_by_id, tb, P);
+                CHECK_LAUNCH("k_sweep");
+                if (s + 1 < d.disp_sweeps) {
+                    hipLaunchKernelGGL(k_commit, dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)d.u_next,
+                                       d.u, d.u_star, d.n_active);
+                    CHECK_LAUNCH("k_commit");
+                }
+            }
+        }
+        hipLaunchKernelGGL(k_record, dim3(gn), dim3(BLOCK), 0, h->stream, d.U, disp ? (const double *)d.u_next : (const double *)d.u,
+                           d.u, d.u_star, (double *)d.record, d.neighbor, (int)compat, (int)disp, P);
+        CHECK_LAUNCH("k_record");
+        hipLaunchKernelGGL((k_stream<T>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, f1,
+                           (const double *)d.record, d.neighbor, P);
+        CHECK_LAUNCH("k_stream");
+        int rc = run_boundary<T>(h, f0, f1, tb, 1);
+        if (rc) return rc;
+        T *tmp = f0;
+        f0 = f1;
+        f1 = tmp;
+        time += d.dt;
+        steps += 1;
+    }
+    return 0;
+}
+
+extern "C" {
+
+const char *vlbm_last_error(void) { return g_err.c_str(); }
+int vlbm_abi_version(void) { return VLBM3D_ABI_VERSION; }
+
+int vlbm3d_create(const vlbm3d_desc *desc, vlbm3d_handle **out) {
+    if (!desc || !out) return fail_msg("vlbm3d_create: null argument");
+    if (desc->struct_size != sizeof(vlbm3d_desc))
+        return fail_msg("vlbm3d_create: descriptor size mismatch (" + std::to_string(desc->struct_size) +
+                        " != " + std::to_string(sizeof(vlbm3d_desc)) + ")");
+    if (desc->abi_version != VLBM3D_ABI_VERSION) return fail_msg("vlbm3d_create: ABI version mismatch");
+    if (desc->n_active <= 0) return fail_msg("vlbm3d_create: n_active must be positive");
+    if (desc->n_links < 0) return fail_msg("vlbm3d_create: n_links must be non-negative");
+    if (desc->precision != VLBM_PREC_F64 && desc->precision != VLBM_PREC_F32)
+        return fail_msg("vlbm3d_create: unknown precision");
+    if (desc->recon == R_COMPAT) return fail_msg("vlbm3d_create: compat_bfl reconstruction is not provided");
+    if (desc->recon < 0 || desc->recon > R_LOCALF_BFL) return fail_msg("vlbm3d_create: unknown reconstruction");
+    if (desc->disp_sweeps < 1 || desc->disp_sweeps > 8) return fail_msg("vlbm3d_create: disp_sweeps must be in [1,8]");
+    if ((size_t)desc->n_active * FQ >= (size_t)1 << 40) return fail_msg("vlbm3d_create: lattice too large");
+    const void *req[] = {desc->U, desc->u, desc->u_star, desc->u_next, desc->record, desc->invalid,
+                         desc->reduce_scratch, desc->neighbor, desc->active_i, desc->active_j, desc->active_k,
+                         desc->node_link_off, desc->kind_by_id, desc->comp_kind_by_id, desc->mode_by_id,
+                         desc->params_by_id};
+    for (const void *ptr : req)
+        if (!ptr) return fail_msg("vlbm3d_create: a required device pointer is null");
+    if (desc->n_links > 0) {
+        const void *lreq[] = {desc->link_p, desc->link_up, desc->link_q, desc->link_bid, desc->link_eta,
+                              desc->link_xb, desc->link_yb, desc->link_zb, desc->link_nx, desc->link_ny,
+                              desc->link_nz, desc->dyn_enabled, desc->dyn_traction, desc->lkind,
+                              desc->lcomp_kind, desc->lbv, desc->lFb, desc->lPb};
+        for (const void *ptr : lreq)
+            if (!ptr) return fail_msg("vlbm3d_create: a required link pointer is null");
+        if (desc->n_anchor_links > 0 && !desc->node_link_ids)
+            return fail_msg("vlbm3d_create: node_link_ids is null");
+        if (desc->has_char && (!desc->char_indices || !desc->char_weights || !desc->char_state))
+            return fail_msg("vlbm3d_create: characteristic tables are null");
+    }
+    vlbm3d_handle *h = new vlbm3d_handle;
+    h->d = *desc;
+    h->P = make_params(*desc);
+    h->T = make_tables(*desc);
+    h->stream = (hipStream_t)desc->stream;
+    *out = h;
+    return 0;
+}
+
+int vlbm3d_destroy(vlbm3d_handle *h) {
+    delete h;
+    return 0;
+}
+
+int vlbm3d_step(vlbm3d_handle *h, void *f0, void *f1, double time, int64_t steps, int32_t nsteps) {
+    if (!h || !f0 || !f1) return fail_msg("vlbm3d_step: null argument");
+    if (nsteps < 0) return fail_msg("vlbm3d_step: nsteps must be non-negative");
+    if (h->d.precision == VLBM_PREC_F64)
+        return run_steps<double>(h, (double *)f0, (double *)f1, time, steps, nsteps);
+    return run_steps<float>(h, (float *)f0, (float *)f1, time, steps, nsteps);
+}
+
+int vlbm3d_refresh(vlbm3d_handle *h, const void *f) {
+    if (!h || !f) return fail_msg("vlbm3d_refresh: null argument");
+    const vlbm3d_desc &d = h->d;
+    hipLaunchKernelGGL(k_reset_invalid, dim3(1), dim3(1), 0, h->stream, d.invalid);
+    const int gn = xcd_grid(d.n_active);
+    if (d.precision == VLBM_PREC_F64)
+        hipLaunchKernelGGL((k_collide<double, false>), dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)f, d.U,
+                           d.u, d.u_star, d.invalid, h->P);
+    else
+        hipLaunchKernelGGL((k_collide<float, false>), dim3(gn), dim3(BLOCK), 0, h->stream, (const float *)f, d.U,
+                           d.u, d.u_star, d.invalid, h->P);
+    CHECK_LAUNCH("k_refresh");
+    return 0;
+}
+
+int vlbm3d_apply_boundary(vlbm3d_handle *h, void *f_src, void *f_dst, double t, int32_t in_place) {
+    if (!h || !f_src || !f_dst) return fail_msg("vlbm3d_apply_boundary: null argument");
+    const vlbm3d_desc &d = h->d;
+    if (d.n_links == 0) return 0;
+    const size_t count = (size_t)FQ * d.n_active;
+    const int gc = (int)((count + BLOCK - 1) / BLOCK);
+    if (d.precision == VLBM_PREC_F64) {
+        double *s = (double *)f_src, *t2 = (double *)f_dst;
+        if (in_place) {
+            hipLaunchKernelGGL(k_copy_pop<double>, dim3(gc), dim3(BLOCK), 0, h->stream, s, t2, count);
+            CHECK_LAUNCH("k_copy_pop");
+            return run_boundary<double>(h, t2, s, t, 0);
+        }
+        return run_boundary<double>(h, s, t2, t, 0);
+    }
+    float *s = (float *)f_src, *t2 = (float *)f_dst;
+    if (in_place) {
+        hipLaunchKernelGGL(k_copy_pop<float>, dim3(gc), dim3(BLOCK), 0, h->stream, s, t2, count);
+        CHECK_LAUNCH("k_copy_pop");
+        return run_boundary<float>(h, t2, s, t, 0);
+    }
+    return run_boundary<float>(h, s, t2, t, 0);
+}
+
+int vlbm3d_init_populations(vlbm3d_handle *h, void *f) {
+    if (!h || !f) return fail_msg("vlbm3d_init_populations: null argument");
+    const int gn = xcd_grid(h->d.n_active);
+    if (h->d.precision == VLBM_PREC_F64)
+        hipLaunchKernelGGL(k_init_pop<double>, dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)h->d.U,
+                           (double *)f, h->P);
+    else
+        hipLaunchKernelGGL(k_init_pop<float>, dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)h->d.U,
+                           (float *)f, h->P);
+    CHECK_LAUNCH("k_init_pop");
+    return 0;
+}
+
+int vlbm3d_set_ustar(vlbm3d_handle *h, const double *disp) {
+    if (!h || !disp) return fail_msg("vlbm3d_set_ustar: null argument");
+    const int g = (h->d.n_active + BLOCK - 1) / BLOCK;
+    hipLaunchKernelGGL(k_set_ustar, dim3(g), dim3(BLOCK), 0, h->stream, disp, (const double *)h->d.U, h->d.u_star,
+                       h->d.dt, h->d.n_active);
+    CHECK_LAUNCH("k_set_ustar");
+    return 0;
+}
+
+int vlbm3d_reduce(vlbm3d_handle *h, double *out) {
+    if (!h || !out) return fail_msg("vlbm3d_reduce: null argument");
+    const vlbm3d_desc &d = h->d;
+    int nb = (d.n_active + BLOCK - 1) / BLOCK;
+    if (nb > RED_BLOCKS) nb = RED_BLOCKS;
+    double *part = d.reduce_scratch;
+    double *res = d.reduce_scratch + (size_t)RED_BLOCKS * 8;
+    hipLaunchKernelGGL(k_reduce_partial, dim3(nb), dim3(BLOCK), 0, h->stream, (const double *)d.U,
+                       (const double *)d.u, part, d.n_active);
+    CHECK_LAUNCH("k_reduce_partial");
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(BLOCK), 0, h->stream, part, nb, (const int32_t *)d.invalid, res);
+    CHECK_LAUNCH("k_reduce_final");
+    hipError_t e = hipMemcpyAsync(out, res, sizeof(double) * VLBM_RED_COUNT, hipMemcpyDeviceToHost, h->stream);
+    if (e != hipSuccess) return fail("vlbm3d_reduce: copy", e);
+    e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail("vlbm3d_reduce: sync", e);
+    return 0;
+}
+
+int vlbm3d_populations_to_f64(vlbm3d_handle *h, const void *f, double *dst) {
+    if (!h || !f || !dst) return fail_msg("vlbm3d_populations_to_f64: null argument");
+    const size_t count = (size_t)FQ * h->d.n_active;
+    const int g = (int)((count + BLOCK - 1) / BLOCK);
+    if (h->d.precision == VLBM_PREC_F64)
+        hipLaunchKernelGGL(k_copy_pop<double>, dim3(g), dim3(BLOCK), 0, h->stream, (const double *)f, dst, count);
+    else
+        hipLaunchKernelGGL(k_pop_to_f64, dim3(g), dim3(BLOCK), 0, h->stream, (const float *)f, dst, h->d.n_active);
+    CHECK_LAUNCH("populations_to_f64");
+    return 0;
+}
+
+int vlbm3d_populations_from_f64(vlbm3d_handle *h, const double *src, void *f) {
+    if (!h || !f || !src) return fail_msg("vlbm3d_populations_from_f64: null argument");
+    const size_t count = (size_t)FQ * h->d.n_active;
+    const int g = (int)((count + BLOCK - 1) / BLOCK);
+    if (h->d.precision == VLBM_PREC_F64)
+        hipLaunchKernelGGL(k_copy_pop<double>, dim3(g), dim3(BLOCK), 0, h->stream, src, (double *)f, count);
+    else
+        hipLaunchKernelGGL(k_pop_from_f64, dim3(g), dim3(BLOCK), 0, h->stream, src, (float *)f, h->d.n_active);
+    CHECK_LAUNCH("populations_from_f64");
+    return 0;
+}
+
+}  // extern "C"

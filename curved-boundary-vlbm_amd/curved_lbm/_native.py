@@ -1,0 +1,166 @@
+"""ctypes binding of libvlbm.so, the HIP (gfx950) implementation of the time step.
+
+The library is loaded from ``curved_lbm/_lib/libvlbm.so`` (built in-tree by
+``build_native.py``).  There is no fallback: if the library or a GPU is
+missing the solver refuses to construct.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libvlbm.so"
+
+ABI_VERSION = 1
+PREC_F64 = 0
+PREC_F32 = 1
+
+RED_SUM_V2, RED_MAX_V, RED_MIN_J, RED_MAX_J, RED_MAX_U, RED_NONFINITE, RED_INVALID = range(7)
+RED_COUNT = 8
+RED_SCRATCH = 1024 * 8 + 8
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class Vlbm3dDesc(C.Structure):
+    """Mirror of ``vlbm3d_desc`` in include/vlbm3d.h (field order matters)."""
+
+    _fields_ = [
+        ("struct_size", C.c_uint32),
+        ("abi_version", C.c_int32),
+        ("n_active", C.c_int32),
+        ("n_links", C.c_int32),
+        ("precision", C.c_int32),
+        ("model", C.c_int32),
+        ("lam", C.c_double),
+        ("mu", C.c_double),
+        ("p0", C.c_double),
+        ("p1", C.c_double),
+        ("p2", C.c_double),
+        ("j_floor", C.c_double),
+        ("dt", C.c_double),
+        ("c", C.c_double),
+        ("omega", C.c_double),
+        ("dx", C.c_double),
+        ("dy", C.c_double),
+        ("dz", C.c_double),
+        ("source_mode", C.c_int32),
+        ("source_params", C.c_double * 12),
+        ("recon", C.c_int32),
+        ("normal_solver", C.c_int32),
+        ("disp_interval", C.c_int32),
+        ("disp_sweeps", C.c_int32),
+        ("disp_relax", C.c_double),
+        ("disp_weight", C.c_double),
+        ("disp_boundary_id", C.c_int32),
+        ("compat_interval", C.c_int32),
+        ("compat_blend", C.c_double),
+        ("has_char", C.c_int32),
+        ("n_anchor_links", C.c_int32),
+        ("U", C.c_void_p),
+        ("u", C.c_void_p),
+        ("u_star", C.c_void_p),
+        ("u_next", C.c_void_p),
+        ("record", C.c_void_p),
+        ("invalid", C.c_void_p),
+        ("reduce_scratch", C.c_void_p),
+        ("neighbor", C.c_void_p),
+        ("active_i", C.c_void_p),
+        ("active_j", C.c_void_p),
+        ("active_k", C.c_void_p),
+        ("node_link_off", C.c_void_p),
+        ("node_link_ids", C.c_void_p),
+        ("link_p", C.c_void_p),
+        ("link_up", C.c_void_p),
+        ("link_q", C.c_void_p),
+        ("link_bid", C.c_void_p),
+        ("link_eta", C.c_void_p),
+        ("link_xb", C.c_void_p),
+        ("link_yb", C.c_void_p),
+        ("link_zb", C.c_void_p),
+        ("link_nx", C.c_void_p),
+        ("link_ny", C.c_void_p),
+        ("link_nz", C.c_void_p),
+        ("char_indices", C.c_void_p),
+        ("char_weights", C.c_void_p),
+        ("char_state", C.c_void_p),
+        ("kind_by_id", C.c_void_p),
+        ("comp_kind_by_id", C.c_void_p),
+        ("mode_by_id", C.c_void_p),
+        ("params_by_id", C.c_void_p),
+        ("dyn_enabled", C.c_void_p),
+        ("dyn_traction", C.c_void_p),
+        ("lkind", C.c_void_p),
+        ("lcomp_kind", C.c_void_p),
+        ("lbv", C.c_void_p),
+        ("lFb", C.c_void_p),
+        ("lPb", C.c_void_p),
+        ("stream", C.c_void_p),
+    ]
+
+
+EXPORTED = (
+    "vlbm3d_create",
+    "vlbm3d_destroy",
+    "vlbm3d_step",
+    "vlbm3d_refresh",
+    "vlbm3d_apply_boundary",
+    "vlbm3d_init_populations",
+    "vlbm3d_set_ustar",
+    "vlbm3d_reduce",
+    "vlbm3d_populations_to_f64",
+    "vlbm3d_populations_from_f64",
+    "vlbm_last_error",
+    "vlbm_abi_version",
+)
+
+_lib = None
+
+
+def load_library(path: Path | None = None) -> C.CDLL:
+    """Load libvlbm.so once and declare its signatures.  Raises if missing."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"HIP library {p} is missing; build it with `python curved-boundary-vlbm_amd/build_native.py`"
+        )
+    lib = C.CDLL(str(p))
+    vp = C.c_void_p
+    lib.vlbm_last_error.restype = C.c_char_p
+    lib.vlbm_last_error.argtypes = []
+    lib.vlbm_abi_version.restype = C.c_int
+    lib.vlbm_abi_version.argtypes = []
+    sigs = {
+        "vlbm3d_create": [C.POINTER(Vlbm3dDesc), C.POINTER(vp)],
+        "vlbm3d_destroy": [vp],
+        "vlbm3d_step": [vp, vp, vp, C.c_double, C.c_int64, C.c_int32],
+        "vlbm3d_refresh": [vp, vp],
+        "vlbm3d_apply_boundary": [vp, vp, vp, C.c_double, C.c_int32],
+        "vlbm3d_init_populations": [vp, vp],
+        "vlbm3d_set_ustar": [vp, vp],
+        "vlbm3d_reduce": [vp, _dp],
+        "vlbm3d_populations_to_f64": [vp, vp, vp],
+        "vlbm3d_populations_from_f64": [vp, vp, vp],
+    }
+    for name, args in sigs.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    if lib.vlbm_abi_version() != ABI_VERSION:
+        raise RuntimeError("libvlbm ABI version mismatch; rebuild the library")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = load_library().vlbm_last_error().decode(errors="replace")
+        if rc == -1:
+            raise ValueError(f"{what}: {msg}")
+        raise RuntimeError(f"{what}: {msg}")

@@ -1,0 +1,1368 @@
+"""Three-dimensional curved-boundary D3Q6x12 hyperelastic solver on MI355X.
+
+Drop-in for the reference module src/curved_lbm/three_d/curved_boundary_warp3d.py
+(abbreviated cbw3d below): same class names, constructor arguments, methods,
+attribute names and array layouts.  The reference material occupies
+phi(X) <= 0; active nodes collide and stream on a Cartesian lattice; velocity
+Dirichlet cut links use BFL interpolation; ``local_f`` traction links use a
+local tangential prediction, a nonlinear nominal-traction solve and a
+characteristic reconstruction; a local displacement sweep and a
+deformation-gradient update couple the kinematic fields.
+
+Host work here is geometry preprocessing (level set -> compacted cut-link
+lists, neighbour tables, characteristic weights) and orchestration.  Every
+time step runs in libvlbm.so (hand-written HIP for gfx950) on device arrays
+held as torch tensors.  The ``compat_bfl`` reconstruction (tangent least
+squares) is not provided.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+import json
+import math
+import os
+from pathlib import Path
+from typing import Callable, Literal
+
+import numpy as np
+
+from . import vector_nonlinear_elastic_warp3d as _base
+from .characteristic_stencil3d import MAX_CHARACTERISTIC_SAMPLES, build_characteristic_weights
+
+Q = int(_base.Q)
+NCOMP = int(_base.NCOMP)
+FQ = int(_base.FQ)
+BC_DIRICHLET = int(_base.BC_DIRICHLET)
+BC_NEUMANN = int(_base.BC_NEUMANN)
+VALUE_ZERO = int(_base.VALUE_ZERO)
+VALUE_CONSTANT = int(_base.VALUE_CONSTANT)
+VALUE_AFFINE_VELOCITY = int(_base.VALUE_AFFINE_VELOCITY)
+SOURCE_CONSTANT = int(_base.SOURCE_CONSTANT)
+SOURCE_DAMPING = int(_base.SOURCE_DAMPING)
+WarpHyperelasticMaterial3D = _base.WarpHyperelasticMaterial3D
+MaterialName = _base.MaterialName
+SourceName = _base.SourceName
+default_device = _base.default_device
+source_id = _base.source_id
+source_name = _base.source_name
+_equilibrium_np = _base._equilibrium_np
+_source_np = _base._source_np
+_first_piola_np = _base._first_piola_np
+_state_to_F_np = _base._state_to_F_np
+_det3_np = _base._det3_np
+_cofactor_np = _base._cofactor_np
+
+DIRS3 = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [-1, 0, 0], [0, -1, 0], [0, 0, -1]], dtype=np.int32)
+
+CURVED3D_VALUE_ZERO = VALUE_ZERO
+CURVED3D_VALUE_CONSTANT_VECTOR = VALUE_CONSTANT
+CURVED3D_VALUE_AFFINE_VELOCITY = VALUE_AFFINE_VELOCITY
+CURVED3D_VALUE_RAMP_VECTOR = 200
+CURVED3D_VALUE_SINE2_HOLD_VECTOR = 204
+CURVED3D_VALUE_PIOLA_NORMAL = 201
+CURVED3D_VALUE_NORMAL_TRACTION = 202
+CURVED3D_VALUE_SINE2_NORMAL_TRACTION = 203
+CURVED3D_VALUE_AFFINE_NEO_HOOKE_PIOLA_NORMAL = 205
+CURVED3D_VALUE_Z_PULL_TORSION_TRACTION = 206
+CURVED3D_VALUE_Z_STRETCH_EXACT_TWIST_VELOCITY = 207
+# Extension: nominal traction amp * min(t / t_ramp, 1) * n, params (amp, t_ramp).
+CURVED3D_VALUE_LINEAR_RAMP_NORMAL_TRACTION = 208
+CURVED_VALUE_ZERO = CURVED3D_VALUE_ZERO
+CURVED_VALUE_CONSTANT_VECTOR = CURVED3D_VALUE_CONSTANT_VECTOR
+CURVED_VALUE_AFFINE_VELOCITY = CURVED3D_VALUE_AFFINE_VELOCITY
+CURVED_VALUE_RAMP_VECTOR = CURVED3D_VALUE_RAMP_VECTOR
+CURVED_VALUE_SINE2_HOLD_VECTOR = CURVED3D_VALUE_SINE2_HOLD_VECTOR
+CURVED_VALUE_PIOLA_NORMAL = CURVED3D_VALUE_PIOLA_NORMAL
+CURVED_VALUE_NORMAL_TRACTION = CURVED3D_VALUE_NORMAL_TRACTION
+CURVED_VALUE_SINE2_NORMAL_TRACTION = CURVED3D_VALUE_SINE2_NORMAL_TRACTION
+CURVED_VALUE_AFFINE_NEO_HOOKE_PIOLA_NORMAL = CURVED3D_VALUE_AFFINE_NEO_HOOKE_PIOLA_NORMAL
+CURVED_VALUE_Z_PULL_TORSION_TRACTION = CURVED3D_VALUE_Z_PULL_TORSION_TRACTION
+CURVED_VALUE_Z_STRETCH_EXACT_TWIST_VELOCITY = CURVED3D_VALUE_Z_STRETCH_EXACT_TWIST_VELOCITY
+CURVED_VALUE_LINEAR_RAMP_NORMAL_TRACTION = CURVED3D_VALUE_LINEAR_RAMP_NORMAL_TRACTION
+MATERIAL_NEO_HOOKE = int(_base.MATERIAL_NEO_HOOKE)
+BOUNDARY_RECON_BFL = 0
+BOUNDARY_RECON_HALF_WAY = 1
+BOUNDARY_RECON_COMPAT_BFL = 2
+BOUNDARY_RECON_LOCAL_F = 3
+BOUNDARY_RECON_LOCAL_F_BFL = 4
+NORMAL_JACOBIAN_FINITE_DIFFERENCE = 0
+NORMAL_JACOBIAN_ACOUSTIC = 1
+NORMAL_SOLVER_ENERGY = 2
+MAX_LSQ_STENCIL = 32
+MAX_BOUNDARY_IDS = 16
+BOUNDARY_ID_OUTER = 1
+BOUNDARY_ID_INNER = 2
+BoundaryKind = Literal["dirichlet", "neumann"]
+ComponentBoundaryKinds = tuple[BoundaryKind, BoundaryKind, BoundaryKind]
+BoundaryReconstruction = Literal["bfl", "halfway", "compat_bfl", "local_f", "local_f_bfl"]
+Precision = Literal["fp64", "fp32"]
+
+
+def neumann_solver_name(name: str | None, material: WarpHyperelasticMaterial3D) -> str:
+    """'newton' (default, or $XLB_NEUMANN_SOLVER) or 'energy' (neo-Hooke only); cbw3d:112-127."""
+    value = os.environ.get("XLB_NEUMANN_SOLVER", "newton") if name is None else str(name)
+    key = value.strip().lower()
+    if key not in ("newton", "energy"):
+        raise ValueError(f"unsupported Neumann solver: {value!r}")
+    if key == "energy" and (
+        material.id != MATERIAL_NEO_HOOKE
+        or not np.isfinite(material.mu)
+        or not np.isfinite(material.lam)
+        or material.mu <= 0.0
+        or material.lam < 0.0
+    ):
+        raise ValueError("energy Neumann solver requires quadratic-volumetric neo_hooke, mu > 0, lambda >= 0")
+    return key
+
+
+def normal_jacobian_id(name: str | None = None) -> int:
+    value = os.environ.get("XLB_NORMAL_JACOBIAN", "acoustic") if name is None else str(name)
+    key = value.strip().lower().replace("-", "_")
+    if key in ("finite_difference", "finite", "fd"):
+        return NORMAL_JACOBIAN_FINITE_DIFFERENCE
+    if key in ("acoustic", "acoustic_tensor", "analytic", "analytical"):
+        return NORMAL_JACOBIAN_ACOUSTIC
+    raise ValueError(f"unsupported normal Jacobian mode: {value!r}")
+
+
+def normal_jacobian_name(mode: int) -> str:
+    return {NORMAL_JACOBIAN_FINITE_DIFFERENCE: "finite_difference", NORMAL_JACOBIAN_ACOUSTIC: "acoustic"}.get(
+        int(mode), f"unknown({mode})"
+    )
+
+
+def _normal_jacobian_for_material(material: WarpHyperelasticMaterial3D, mode: str | None) -> int:
+    # The acoustic closed form is specific to quadratic-volumetric neo-Hooke;
+    # the SPH energy defaults to the finite-difference Jacobian.
+    if mode is None and material.model == "sph_neo_hooke":
+        return NORMAL_JACOBIAN_FINITE_DIFFERENCE
+    return normal_jacobian_id(mode)
+
+
+def _kind_id(k: str) -> int:
+    if k == "dirichlet":
+        return BC_DIRICHLET
+    if k == "neumann":
+        return BC_NEUMANN
+    raise ValueError(k)
+
+
+_RECON_ALIASES = {
+    "bfl": BOUNDARY_RECON_BFL,
+    "bouzidi": BOUNDARY_RECON_BFL,
+    "bouzidi_firdaouss_lallemand": BOUNDARY_RECON_BFL,
+    "halfway": BOUNDARY_RECON_HALF_WAY,
+    "half_way": BOUNDARY_RECON_HALF_WAY,
+    "half-way": BOUNDARY_RECON_HALF_WAY,
+    "compat_bfl": BOUNDARY_RECON_COMPAT_BFL,
+    "local_f": BOUNDARY_RECON_LOCAL_F,
+    "fully_local": BOUNDARY_RECON_LOCAL_F,
+    "local_f_bfl": BOUNDARY_RECON_LOCAL_F_BFL,
+    "single_history": BOUNDARY_RECON_LOCAL_F_BFL,
+}
+_RECON_NAMES = {0: "bfl", 1: "halfway", 2: "compat_bfl", 3: "local_f", 4: "local_f_bfl"}
+
+
+def _recon_id(k: str) -> int:
+    try:
+        return _RECON_ALIASES[k]
+    except KeyError:
+        raise ValueError(f"unsupported boundary_reconstruction: {k}") from None
+
+
+def _recon_name(rid: int) -> str:
+    return _RECON_NAMES.get(int(rid), f"unknown({rid})")
+
+
+@dataclass(frozen=True)
+class CurvedBoundarySpec3D:
+    """Per-boundary-id kinds, value modes and parameters (cbw3d:194-257)."""
+
+    kind: BoundaryKind = "dirichlet"
+    value_mode: int = CURVED3D_VALUE_ZERO
+    params: tuple[float, ...] = ()
+    id_kinds: dict[int, BoundaryKind] | None = None
+    id_value_modes: dict[int, int] | None = None
+    id_params: dict[int, tuple[float, ...]] | None = None
+    label: str = ""
+    component_kinds: ComponentBoundaryKinds | None = None
+    id_component_kinds: dict[int, ComponentBoundaryKinds] | None = None
+
+    def tables(self, max_ids: int = MAX_BOUNDARY_IDS) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        kind = np.full(max_ids, _kind_id(self.kind), dtype=np.int32)
+        mode = np.full(max_ids, int(self.value_mode), dtype=np.int32)
+        base = np.zeros(16, dtype=np.float64)
+        vals = tuple(self.params)[:16]
+        base[: len(vals)] = [float(v) for v in vals]
+        params = np.tile(base[None, :], (max_ids, 1))
+        for i, v in (self.id_kinds or {}).items():
+            if 0 <= int(i) < max_ids:
+                kind[int(i)] = _kind_id(v)
+        for i, v in (self.id_value_modes or {}).items():
+            if 0 <= int(i) < max_ids:
+                mode[int(i)] = int(v)
+        for i, vals in (self.id_params or {}).items():
+            if 0 <= int(i) < max_ids:
+                row = np.zeros(16)
+                vv = tuple(vals)[:16]
+                row[: len(vv)] = [float(v) for v in vv]
+                params[int(i)] = row
+        return kind, mode, np.ascontiguousarray(params)
+
+    def component_kind_table(self, max_ids: int = MAX_BOUNDARY_IDS) -> np.ndarray:
+        """Per displacement row (x, y, z) boundary kinds; ``component_kinds``
+        and ``id_component_kinds`` override the broadcast of ``kind``."""
+        kind, _, _ = self.tables(max_ids)
+        out = np.repeat(kind[None, :], 3, axis=0)
+        if self.component_kinds is not None:
+            values = tuple(self.component_kinds)
+            if len(values) != 3:
+                raise ValueError("component_kinds must contain exactly three kinds")
+            for c, v in enumerate(values):
+                out[c, :] = _kind_id(v)
+            for i, v in (self.id_kinds or {}).items():
+                if 0 <= int(i) < max_ids:
+                    out[:, int(i)] = _kind_id(v)
+        for i, values in (self.id_component_kinds or {}).items():
+            if not 0 <= int(i) < max_ids:
+                continue
+            values = tuple(values)
+            if len(values) != 3:
+                raise ValueError("id_component_kinds values must contain exactly three kinds")
+            for c, v in enumerate(values):
+                out[c, int(i)] = _kind_id(v)
+        return np.ascontiguousarray(out)
+
+
+@dataclass(frozen=True)
+class SphereDomain:
+    center: tuple[float, float, float] = (0.5, 0.5, 0.5)
+    radius: float = 0.35
+
+    def phi(self, x: float, y: float, z: float) -> float:
+        cx, cy, cz = self.center
+        return math.sqrt((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2) - self.radius
+
+    def normal(self, x: float, y: float, z: float) -> tuple[float, float, float]:
+        cx, cy, cz = self.center
+        rx, ry, rz = x - cx, y - cy, z - cz
+        r = max(math.sqrt(rx * rx + ry * ry + rz * rz), 1e-30)
+        return rx / r, ry / r, rz / r
+
+
+def _elementwise(fn, *arrays, nout=1):
+    """Apply a scalar callable element by element (exact scalar semantics)."""
+    shape = np.broadcast(*arrays).shape
+    flat = [np.broadcast_to(a, shape).ravel() for a in arrays]
+    outs = [np.empty(flat[0].size) for _ in range(nout)]
+    for m in range(flat[0].size):
+        res = fn(*(float(a[m]) for a in flat))
+        if nout == 1:
+            outs[0][m] = float(res)
+        else:
+            for o in range(nout):
+                outs[o][m] = float(res[o])
+    outs = [o.reshape(shape) for o in outs]
+    return outs[0] if nout == 1 else outs
+
+
+@dataclass
+class CurvedBoundaryGeometry3D:
+    """Active-node mask and compacted cut-link list of a level-set body."""
+
+    nx: int
+    ny: int
+    nz: int
+    length_x: float
+    length_y: float
+    length_z: float
+    active: np.ndarray
+    link_i: np.ndarray
+    link_j: np.ndarray
+    link_k: np.ndarray
+    link_q: np.ndarray
+    link_boundary_id: np.ndarray
+    link_eta: np.ndarray
+    link_xb: np.ndarray
+    link_yb: np.ndarray
+    link_zb: np.ndarray
+    link_nx: np.ndarray
+    link_ny: np.ndarray
+    link_nz: np.ndarray
+    label: str = "implicit"
+
+    @property
+    def dx(self):
+        return self.length_x / float(self.nx)
+
+    @property
+    def dy(self):
+        return self.length_y / float(self.ny)
+
+    @property
+    def dz(self):
+        return self.length_z / float(self.nz)
+
+    @property
+    def n_active(self):
+        return int(np.count_nonzero(self.active))
+
+    @property
+    def n_links(self):
+        return int(self.link_i.size)
+
+    def as_metadata(self):
+        return {
+            "label": self.label, "nx": self.nx, "ny": self.ny, "nz": self.nz,
+            "dx": self.dx, "dy": self.dy, "dz": self.dz,
+            "n_active": self.n_active, "n_cut_links": self.n_links,
+        }
+
+    @classmethod
+    def from_level_set(
+        cls,
+        *,
+        nx: int,
+        ny: int,
+        nz: int,
+        length_x: float,
+        length_y: float,
+        length_z: float,
+        phi: Callable[[float, float, float], float],
+        normal: Callable[[float, float, float], tuple[float, float, float]] | None = None,
+        boundary_id: Callable[[float, float, float, float, float, float], int] | None = None,
+        label: str = "level_set",
+        root_iterations: int = 42,
+        phi_vectorized: Callable | None = None,
+        normal_vectorized: Callable | None = None,
+        phi_chunk_size: int = 64,
+    ):
+        """Cut links of the body phi <= 0 (cbw3d:333-721).
+
+        A link is every (active node, direction) whose neighbour is inactive
+        or off the lattice.  Its fraction eta locates phi = 0 by a 17-sample
+        sign search then ``root_iterations`` bisections, clipped to
+        [1e-6, 1 - 1e-6].  Links are ordered node-major (then direction) for
+        scalar callables and direction-major (then node) when
+        ``phi_vectorized`` is given, as in the reference.
+        """
+        dx, dy, dz = length_x / nx, length_y / ny, length_z / nz
+        if nx < 4 or ny < 4 or nz < 4:
+            raise ValueError("nx,ny,nz>=4 required")
+        if max(abs(dx - dy), abs(dx - dz)) > 1e-14:
+            raise ValueError("D3Q6 requires dx=dy=dz")
+        xs = (np.arange(nx) + 0.5) * dx
+        ys = (np.arange(ny) + 0.5) * dy
+        zs = (np.arange(nz) + 0.5) * dz
+        ph = np.empty((nx, ny, nz), float)
+        if phi_vectorized is None:
+            for i, x in enumerate(xs):
+                for j, y in enumerate(ys):
+                    for k, z in enumerate(zs):
+                        ph[i, j, k] = float(phi(float(x), float(y), float(z)))
+        else:
+            X2, Y2 = np.meshgrid(xs, ys, indexing="ij")
+            chunk = max(1, int(phi_chunk_size))
+            for k0 in range(0, nz, chunk):
+                k1 = min(nz, k0 + chunk)
+                vals = np.asarray(phi_vectorized(X2[:, :, None], Y2[:, :, None], zs[None, None, k0:k1]), float)
+                ph[:, :, k0:k1] = np.broadcast_to(vals, ph[:, :, k0:k1].shape)
+        active = ph <= 0.0
+        if not np.any(active):
+            raise ValueError("no active material nodes")
+
+        def phiv(x, y, z):
+            if phi_vectorized is None:
+                return _elementwise(phi, x, y, z)
+            out = np.asarray(phi_vectorized(np.asarray(x), np.asarray(y), np.asarray(z)), float)
+            return np.broadcast_to(out, np.broadcast(x, y, z).shape)
+
+        def normalv(x, y, z):
+            shape = np.broadcast(x, y, z).shape
+            if normal_vectorized is not None:
+                vals = normal_vectorized(np.asarray(x), np.asarray(y), np.asarray(z))
+                if isinstance(vals, (tuple, list)):
+                    a, b, c = (np.asarray(v, float) for v in vals[:3])
+                else:
+                    arr = np.asarray(vals, float)
+                    if arr.shape[:1] == (3,):
+                        a, b, c = arr[0], arr[1], arr[2]
+                    elif arr.shape[-1:] == (3,):
+                        a, b, c = arr[..., 0], arr[..., 1], arr[..., 2]
+                    else:
+                        raise ValueError("normal_vectorized must return a 3-tuple or an array with a 3-vector axis")
+                a, b, c = (np.broadcast_to(v, shape).astype(float, copy=False) for v in (a, b, c))
+                r = np.maximum(np.sqrt(a * a + b * b + c * c), 1.0e-30)
+                return a / r, b / r, c / r
+
+            def one(xx, yy, zz):
+                if normal is not None:
+                    a, b, c = normal(xx, yy, zz)
+                    a, b, c = float(a), float(b), float(c)
+                else:
+                    h = 0.5 * dx
+                    a = (float(phi(xx + h, yy, zz)) - float(phi(xx - h, yy, zz))) / (2 * h)
+                    b = (float(phi(xx, yy + h, zz)) - float(phi(xx, yy - h, zz))) / (2 * h)
+                    c = (float(phi(xx, yy, zz + h)) - float(phi(xx, yy, zz - h))) / (2 * h)
+                r = max(math.sqrt(a * a + b * b + c * c), 1.0e-30)
+                return a / r, b / r, c / r
+
+            return tuple(_elementwise(one, x, y, z, nout=3))
+
+        # Enumerate cut (node, direction) pairs.
+        cut_by_q = []
+        for q, (di, dj, dk) in enumerate(DIRS3):
+            nb = np.zeros_like(active)
+            src = [slice(None)] * 3
+            dst = [slice(None)] * 3
+            for ax, d in enumerate((di, dj, dk)):
+                if d == 1:
+                    dst[ax], src[ax] = slice(None, -1), slice(1, None)
+                elif d == -1:
+                    dst[ax], src[ax] = slice(1, None), slice(None, -1)
+            nb[tuple(dst)] = active[tuple(src)]
+            cut_by_q.append(active & ~nb)
+        if phi_vectorized is not None:
+            parts = [np.nonzero(c) + (np.full(np.count_nonzero(c), q),) for q, c in enumerate(cut_by_q)]
+            ii = np.concatenate([p[0] for p in parts]).astype(np.int64)
+            jj = np.concatenate([p[1] for p in parts]).astype(np.int64)
+            kk = np.concatenate([p[2] for p in parts]).astype(np.int64)
+            qq = np.concatenate([p[3] for p in parts]).astype(np.int64)
+        else:
+            stack = np.stack(cut_by_q, axis=-1)  # (nx, ny, nz, Q), node-major then q
+            ii, jj, kk, qq = (a.astype(np.int64) for a in np.nonzero(stack))
+        dirs = DIRS3[qq].astype(np.float64)
+        x0, y0, z0 = xs[ii], ys[jj], zs[kk]
+        x1 = x0 + dirs[:, 0] * dx
+        y1 = y0 + dirs[:, 1] * dy
+        z1 = z0 + dirs[:, 2] * dz
+        L = ii.size
+        lo = np.zeros(L)
+        hi = np.ones(L)
+        if L:
+            p0 = ph[ii, jj, kk]
+            p1 = phiv(x1, y1, z1)
+            needs = np.flatnonzero(~((p0 <= 0.0) & (p1 >= 0.0)))
+            if needs.size:
+                found = np.zeros(needs.size, bool)
+                lo_n = np.full(needs.size, 0.5)
+                hi_n = np.full(needs.size, 0.5)
+                sample = np.linspace(0.0, 1.0, 17)
+                prev_s, prev_v = sample[0], p0[needs]
+                for ss in sample[1:]:
+                    pm = phiv(
+                        x0[needs] + ss * (x1[needs] - x0[needs]),
+                        y0[needs] + ss * (y1[needs] - y0[needs]),
+                        z0[needs] + ss * (z1[needs] - z0[needs]),
+                    )
+                    hit = (~found) & (prev_v <= 0.0) & (pm >= 0.0)
+                    lo_n[hit] = prev_s
+                    hi_n[hit] = ss
+                    found |= hit
+                    prev_s, prev_v = ss, pm
+                lo[needs] = lo_n
+                hi[needs] = hi_n
+            for _ in range(root_iterations):
+                mid = 0.5 * (lo + hi)
+                inside = phiv(x0 + mid * (x1 - x0), y0 + mid * (y1 - y0), z0 + mid * (z1 - z0)) <= 0.0
+                lo = np.where(inside, mid, lo)
+                hi = np.where(inside, hi, mid)
+        eta = np.clip(0.5 * (lo + hi), 1.0e-6, 1.0 - 1.0e-6)
+        xb = x0 + eta * (x1 - x0)
+        yb = y0 + eta * (y1 - y0)
+        zb = z0 + eta * (z1 - z0)
+        if L:
+            nxv, nyv, nzv = normalv(xb, yb, zb)
+        else:
+            nxv = nyv = nzv = np.empty(0)
+        if boundary_id is None:
+            bid = np.ones(L, dtype=np.int64)
+        else:
+            bid = np.array(
+                [int(boundary_id(float(xb[m]), float(yb[m]), float(zb[m]), float(nxv[m]), float(nyv[m]), float(nzv[m])))
+                 for m in range(L)], dtype=np.int64,
+            )
+        bid = np.clip(bid, 0, MAX_BOUNDARY_IDS - 1)
+
+        def c32(a):
+            return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+        def c64(a):
+            return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+        return cls(
+            nx, ny, nz, float(length_x), float(length_y), float(length_z),
+            np.ascontiguousarray(active.astype(np.int32)),
+            c32(ii), c32(jj), c32(kk), c32(qq), c32(bid), c64(eta), c64(xb), c64(yb), c64(zb),
+            c64(nxv), c64(nyv), c64(nzv), label,
+        )
+
+    @classmethod
+    def sphere(cls, *, nx: int, ny: int, nz: int, length_x: float = 1.0, length_y: float = 1.0,
+               length_z: float = 1.0, center=(0.5, 0.5, 0.5), radius: float = 0.35):
+        sph = SphereDomain(tuple(center), radius)
+        return cls.from_level_set(
+            nx=nx, ny=ny, nz=nz, length_x=length_x, length_y=length_y, length_z=length_z,
+            phi=sph.phi, normal=sph.normal, label=f"sphere(center={tuple(center)},r={radius})",
+        )
+
+    @classmethod
+    def spherical_shell(cls, *, nx: int, ny: int, nz: int, length_x: float = 1.0, length_y: float = 1.0,
+                        length_z: float = 1.0, center=(0.5, 0.5, 0.5), inner_radius: float = 0.18,
+                        outer_radius: float = 0.42):
+        if not 0.0 < inner_radius < outer_radius:
+            raise ValueError("spherical_shell requires 0 < inner_radius < outer_radius")
+        cx, cy, cz = center
+
+        def rr(x, y, z):
+            return math.sqrt((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2)
+
+        def phi(x, y, z):
+            r = rr(x, y, z)
+            return max(r - outer_radius, inner_radius - r)
+
+        def normal(x, y, z):
+            r = max(rr(x, y, z), 1.0e-30)
+            ex, ey, ez = (x - cx) / r, (y - cy) / r, (z - cz) / r
+            if abs(r - inner_radius) <= abs(r - outer_radius):
+                return -ex, -ey, -ez
+            return ex, ey, ez
+
+        def bid(x, y, z, nxv, nyv, nzv):
+            r = rr(x, y, z)
+            return BOUNDARY_ID_INNER if abs(r - inner_radius) <= abs(r - outer_radius) else BOUNDARY_ID_OUTER
+
+        return cls.from_level_set(
+            nx=nx, ny=ny, nz=nz, length_x=length_x, length_y=length_y, length_z=length_z,
+            phi=phi, normal=normal, boundary_id=bid,
+            label=f"spherical_shell(center={tuple(center)},ri={inner_radius},ro={outer_radius})",
+        )
+
+
+def _sparse_tables_from_geometry(geometry: CurvedBoundaryGeometry3D) -> dict[str, np.ndarray]:
+    """Active-node ids (lexicographic, k fastest), neighbour table (6, n) with
+    -1 for missing neighbours, link node and upstream node (cbw3d:3683-3722)."""
+    act = np.asarray(geometry.active, dtype=bool)
+    ai, aj, ak = (np.ascontiguousarray(a.astype(np.int32)) for a in np.nonzero(act))
+    n = int(ai.size)
+    active_id = np.full(act.shape, -1, dtype=np.int32)
+    active_id[ai, aj, ak] = np.arange(n, dtype=np.int32)
+    shape = np.array(act.shape)
+    neighbor = np.full((Q, n), -1, dtype=np.int32)
+    ijk = np.stack((ai, aj, ak), axis=1).astype(np.int64)
+    for q in range(Q):
+        t = ijk + DIRS3[q]
+        ok = np.all((t >= 0) & (t < shape), axis=1)
+        neighbor[q, ok] = active_id[t[ok, 0], t[ok, 1], t[ok, 2]]
+    li, lj, lk = (np.asarray(a, np.int64) for a in (geometry.link_i, geometry.link_j, geometry.link_k))
+    link_p = active_id[li, lj, lk].astype(np.int32) if li.size else np.zeros(0, np.int32)
+    up = np.stack((li, lj, lk), axis=1) - DIRS3[np.asarray(geometry.link_q, np.int64)] if li.size else np.zeros((0, 3), np.int64)
+    link_up = np.full(li.size, -1, dtype=np.int32)
+    if li.size:
+        ok = np.all((up >= 0) & (up < shape), axis=1)
+        link_up[ok] = active_id[up[ok, 0], up[ok, 1], up[ok, 2]]
+    return {
+        "active_i": ai, "active_j": aj, "active_k": ak,
+        "active_id": np.ascontiguousarray(active_id),
+        "neighbor": np.ascontiguousarray(neighbor),
+        "link_p": np.ascontiguousarray(link_p),
+        "link_upstream": np.ascontiguousarray(link_up),
+    }
+
+
+# ---- host boundary-value helpers (cbw3d:802-960) -----------------------------
+def _affine_neo_hooke_F_P_np(p, t):
+    pp = np.zeros(16)
+    pp[: min(16, len(p))] = p[: min(16, len(p))]
+    tt = float(t) if pp[11] <= 0.0 else min(float(t), float(pp[11]))
+    F = np.eye(3) + tt * pp[:9].reshape(3, 3)
+    J = float(np.linalg.det(F))
+    H = _cofactor_np(F) / max(J, np.finfo(float).eps)
+    lam, mu = float(pp[9]), float(pp[10])
+    return F, mu * (F - H) + 0.5 * lam * (J * J - 1.0) * H
+
+
+def _sine2(t, active, denom):
+    if denom <= 0:
+        denom = 2.0 * active
+    return 1.0 if active <= 0 or t >= active else math.sin(math.pi * t / denom) ** 2
+
+
+def _value_np(mode, p, comp, x, y, z, n, t):
+    """Host evaluation of a boundary value (same modes as the device)."""
+    pp = np.zeros(16)
+    pp[: min(16, len(p))] = p[: min(16, len(p))]
+    n = np.asarray(n, float)
+    if mode == CURVED3D_VALUE_ZERO:
+        return 0.0
+    if mode == CURVED3D_VALUE_CONSTANT_VECTOR:
+        return float(pp[comp])
+    if mode == CURVED3D_VALUE_RAMP_VECTOR:
+        scale = 1.0 if pp[3] <= 0 or t >= pp[3] else math.sin(math.pi * t / pp[3]) ** 2
+        return float(pp[comp] * scale)
+    if mode == CURVED3D_VALUE_AFFINE_VELOCITY:
+        if pp[12] > 0 and t > pp[12]:
+            return 0.0
+        return float((pp[:9].reshape(3, 3) @ np.array([x, y, z]) + pp[9:12])[comp])
+    if mode == CURVED3D_VALUE_SINE2_HOLD_VECTOR:
+        return float(pp[comp] * _sine2(t, pp[3], pp[4]))
+    if mode == CURVED3D_VALUE_PIOLA_NORMAL:
+        return float(pp[:9].reshape(3, 3)[comp, :] @ n)
+    if mode == CURVED3D_VALUE_AFFINE_NEO_HOOKE_PIOLA_NORMAL:
+        _, P = _affine_neo_hooke_F_P_np(pp, t)
+        return float(P[comp, :] @ n)
+    if mode == CURVED3D_VALUE_NORMAL_TRACTION:
+        return float(pp[0] * n[comp])
+    if mode == CURVED3D_VALUE_SINE2_NORMAL_TRACTION:
+        return float(pp[0] * _sine2(t, pp[1], pp[2]) * n[comp])
+    if mode == CURVED3D_VALUE_LINEAR_RAMP_NORMAL_TRACTION:
+        scale = 1.0 if pp[1] <= 0 or t >= pp[1] else t / pp[1]
+        return float(pp[0] * scale * n[comp])
+    if mode == CURVED3D_VALUE_Z_PULL_TORSION_TRACTION:
+        scale = _sine2(t, pp[4], pp[5])
+        if comp == 0:
+            return float(-scale * pp[3] * (y - pp[1]))
+        if comp == 1:
+            return float(scale * pp[3] * (x - pp[0]))
+        return float(scale * pp[2])
+    if mode == CURVED3D_VALUE_Z_STRETCH_EXACT_TWIST_VELOCITY:
+        active = float(pp[4])
+        if active <= 0.0 or t >= active:
+            return 0.0
+        phase = math.pi * t / active
+        s = 0.5 * (1.0 - math.cos(phase))
+        sdot = 0.5 * math.pi / active * math.sin(phase)
+        theta, thetadot = pp[3] * s, pp[3] * sdot
+        ddx, ddy = x - pp[0], y - pp[1]
+        if comp == 0:
+            return float(thetadot * (-math.sin(theta) * ddx - math.cos(theta) * ddy))
+        if comp == 1:
+            return float(thetadot * (math.cos(theta) * ddx - math.sin(theta) * ddy))
+        return float(pp[2] * sdot)
+    return 0.0
+
+
+def _basis_np(n):
+    n = np.asarray(n, float)
+    n = n / max(np.linalg.norm(n), np.finfo(float).eps)
+    ref = np.array([1.0, 0, 0]) if abs(n[0]) < 0.75 else np.array([0, 1.0, 0])
+    t1 = np.cross(n, ref)
+    t1 = t1 / max(np.linalg.norm(t1), np.finfo(float).eps)
+    return t1, np.cross(n, t1)
+
+
+def _solve_Fb_np(Fg, n, T, mat, j_floor=1e-12):
+    """Host Newton solve of P(F_b) n = T for the normal image (cbw3d:875-912)."""
+    n = np.asarray(n, float)
+    n = n / max(np.linalg.norm(n), np.finfo(float).eps)
+    t1, t2 = _basis_np(n)
+    Fg = np.asarray(Fg, float).reshape(3, 3)
+    h, g1, g2 = Fg @ n, Fg @ t1, Fg @ t2
+
+    def make(hv):
+        return np.outer(hv, n) + np.outer(g1, t1) + np.outer(g2, t2)
+
+    def res(hv):
+        return _first_piola_np(make(hv), mat) @ n - T
+
+    for _ in range(24):
+        r = res(h)
+        if np.linalg.norm(r) < 1e-11:
+            break
+        Jm = np.zeros((3, 3))
+        for a in range(3):
+            hp = h.copy()
+            hp[a] += 1e-6
+            Jm[:, a] = (res(hp) - r) / 1e-6
+        try:
+            d = np.linalg.solve(Jm, -r)
+        except np.linalg.LinAlgError:
+            break
+        step = 1.0
+        for _ls in range(16):
+            hc = h + step * d
+            if np.linalg.det(make(hc)) > j_floor:
+                h = hc
+                break
+            step *= 0.5
+    return make(h)
+
+
+def constant_piola_params3d(material: WarpHyperelasticMaterial3D, F: np.ndarray) -> tuple[float, ...]:
+    return tuple(float(x) for x in _first_piola_np(np.asarray(F, float).reshape(3, 3), material).reshape(-1))
+
+
+def relative_l2_active(error, reference, mask):
+    e, r, m = np.asarray(error), np.asarray(reference), np.asarray(mask, bool)
+    return float(np.sqrt(np.sum(e[..., m] ** 2)) / max(np.sqrt(np.sum(r[..., m] ** 2)), np.finfo(float).eps))
+
+
+class WarpSparseCurvedBoundaryLBM3D:
+    """Active-node sparse curved 3-D solver (cbw3d:3852-4802) on one HIP device.
+
+    Storage is O(n_active): populations (72, n_active), fields (c, n_active).
+    ``precision="fp32"`` stores populations as float deviations from the rest
+    state while moments, kinematics, traction solves and reductions stay FP64.
+    """
+
+    def __init__(
+        self,
+        geometry: CurvedBoundaryGeometry3D,
+        *,
+        material: WarpHyperelasticMaterial3D,
+        boundary: CurvedBoundarySpec3D,
+        lattice_speed: float = 5.0,
+        collision_omega: float = 1.8,
+        boundary_reconstruction: BoundaryReconstruction = "bfl",
+        device: str | None = None,
+        check_linearized_cfl: bool = True,
+        jacobian_floor: float = 1e-12,
+        source_mode: SourceName = "none",
+        source_vector: tuple[float, ...] = (0.0,) * NCOMP,
+        damping_gamma: float = 0.0,
+        compatibility_projection_interval: int = 0,
+        compatibility_projection_iterations: int = 160,
+        compatibility_projection_relax: float = 0.85,
+        compatibility_projection_weight: float = 20.0,
+        compatibility_projection_repair_F: bool = False,
+        compatibility_projection_repair_F_band_width: int = -1,
+        compatibility_projection_repair_F_blend: float = 1.0,
+        compatibility_projection_boundary_id: int = -1,
+        local_displacement_interval: int = 0,
+        local_displacement_sweeps: int = 1,
+        local_displacement_relax: float = 0.85,
+        local_displacement_boundary_weight: float = 20.0,
+        local_displacement_boundary_id: int = -1,
+        local_compatibility_interval: int = 0,
+        local_compatibility_blend: float = 0.1,
+        normal_jacobian: Literal["finite_difference", "acoustic"] | None = None,
+        neumann_solver: Literal["newton", "energy"] | None = None,
+        precision: Precision = "fp64",
+    ):
+        self.geometry = geometry
+        self.nx, self.ny, self.nz = geometry.nx, geometry.ny, geometry.nz
+        self.length_x, self.length_y, self.length_z = geometry.length_x, geometry.length_y, geometry.length_z
+        self.dx, self.dy, self.dz = geometry.dx, geometry.dy, geometry.dz
+        self.material = material
+        self.boundary = boundary
+        self.lattice_speed = float(lattice_speed)
+        self.collision_omega = float(collision_omega)
+        if not 0.0 < self.collision_omega <= 2.0:
+            raise ValueError("collision_omega must be in (0,2]")
+        self.boundary_reconstruction = _recon_id(boundary_reconstruction)
+        self.boundary_reconstruction_name = _recon_name(self.boundary_reconstruction)
+        if self.boundary_reconstruction == BOUNDARY_RECON_COMPAT_BFL:
+            raise ValueError("boundary_reconstruction='compat_bfl' is not provided by this solver")
+        self.normal_jacobian = _normal_jacobian_for_material(material, normal_jacobian)
+        self.normal_jacobian_name = normal_jacobian_name(self.normal_jacobian)
+        self.neumann_solver_name = neumann_solver_name(neumann_solver, material)
+        self._normal_solver_mode = (
+            NORMAL_SOLVER_ENERGY if self.neumann_solver_name == "energy" else self.normal_jacobian
+        )
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        self.precision = precision
+        self.dt = self.dx / self.lattice_speed
+        self.time = 0.0
+        self.steps = 0
+        self.shape = (self.nx, self.ny, self.nz)
+        self.jacobian_floor = float(jacobian_floor)
+        if check_linearized_cfl and material.linearized_stability_ratio(self.lattice_speed) >= 1.0:
+            raise ValueError("linearized D3Q6 stability ratio must be below one")
+        self.source_mode = source_id(source_mode)
+        self.source_params = np.zeros(NCOMP)
+        if self.source_mode == SOURCE_CONSTANT:
+            vals = tuple(source_vector)[:NCOMP]
+            self.source_params[: len(vals)] = [float(v) for v in vals]
+        elif self.source_mode == SOURCE_DAMPING:
+            self.source_params[0] = float(damping_gamma)
+        if int(compatibility_projection_interval) != 0:
+            raise ValueError("the fully-local solver forbids compatibility_projection_interval != 0")
+        self.compatibility_projection_interval = 0
+        self.compatibility_projection_iterations = int(max(1, compatibility_projection_iterations))
+        self.compatibility_projection_relax = float(compatibility_projection_relax)
+        self.compatibility_projection_weight = float(compatibility_projection_weight)
+        self.compatibility_projection_repair_F = False
+        self.compatibility_projection_repair_F_band_width = int(compatibility_projection_repair_F_band_width)
+        self.compatibility_projection_repair_F_blend = float(compatibility_projection_repair_F_blend)
+        self.compatibility_projection_boundary_id = int(compatibility_projection_boundary_id)
+        self.local_displacement_interval = int(max(0, local_displacement_interval))
+        self.local_displacement_sweeps = int(local_displacement_sweeps)
+        self.local_displacement_relax = float(local_displacement_relax)
+        self.local_displacement_boundary_weight = float(local_displacement_boundary_weight)
+        self.local_displacement_boundary_id = int(local_displacement_boundary_id)
+        self.local_compatibility_interval = int(max(0, local_compatibility_interval))
+        self.local_compatibility_blend = float(local_compatibility_blend)
+        if not 1 <= self.local_displacement_sweeps <= 8:
+            raise ValueError("local_displacement_sweeps must be in [1,8]")
+        if not 0.0 <= self.local_displacement_relax <= 1.0:
+            raise ValueError("local_displacement_relax must be in [0,1]")
+        if self.local_displacement_boundary_weight < 0.0:
+            raise ValueError("local_displacement_boundary_weight must be non-negative")
+        if not 0.0 <= self.local_compatibility_blend <= 1.0:
+            raise ValueError("local_compatibility_blend must be in [0,1]")
+        self._x_dense = self._y_dense = self._z_dense = None
+        self.active_np = np.ascontiguousarray(geometry.active.astype(np.int32))
+        tables = _sparse_tables_from_geometry(geometry)
+        self.tables = tables
+        self.active_i, self.active_j, self.active_k = tables["active_i"], tables["active_j"], tables["active_k"]
+        self.active_id_np = tables["active_id"]
+        self.n_active = int(self.active_i.size)
+        self.n_nodes = self.n_active
+        self.tangent_lsq_active = False
+        self.kind_by_id, self.mode_by_id, self.params_by_id = boundary.tables(MAX_BOUNDARY_IDS)
+        self.component_kind_by_id = boundary.component_kind_table(MAX_BOUNDARY_IDS)
+        link_ids = np.clip(np.asarray(geometry.link_boundary_id, dtype=np.int32), 0, MAX_BOUNDARY_IDS - 1)
+        neumann_rows = self.component_kind_by_id[:, link_ids] == BC_NEUMANN
+        self._has_neumann_cut_links = bool(link_ids.size and np.any(neumann_rows))
+        self.characteristic_stencil = None
+        if self.boundary_reconstruction == BOUNDARY_RECON_LOCAL_F and self._has_neumann_cut_links:
+            self.characteristic_stencil = build_characteristic_weights(
+                geometry, tables, link_mask=np.any(neumann_rows, axis=0)
+            )
+        self._setup_device(device, tables)
+        self.initialize_identity()
+
+    # ---- device setup ------------------------------------------------------
+    def _setup_device(self, device, tables):
+        import torch
+
+        from .. import _native
+        from ..device import DeviceArray, PopulationArray, hip_device
+
+        self._torch = torch
+        self._lib = _native.load_library()
+        self.device = str(hip_device(device))
+        dev = torch.device(self.device)
+        n, L = self.n_active, self.geometry.n_links
+        g = self.geometry
+        i32, f64 = np.int32, np.float64
+
+        def dev_i(a):
+            return DeviceArray.from_numpy(a, i32, dev)
+
+        def dev_f(a):
+            return DeviceArray.from_numpy(a, f64, dev)
+
+        self.active_i_wp, self.active_j_wp, self.active_k_wp = (dev_i(tables[k]) for k in ("active_i", "active_j", "active_k"))
+        self.neighbor = dev_i(tables["neighbor"])
+        self.link_p = dev_i(tables["link_p"])
+        self.link_upstream = dev_i(tables["link_upstream"])
+        self.link_q = dev_i(g.link_q)
+        self.link_boundary_id = dev_i(g.link_boundary_id)
+        self.link_eta, self.link_xb, self.link_yb, self.link_zb = (dev_f(a) for a in (g.link_eta, g.link_xb, g.link_yb, g.link_zb))
+        self.link_nx, self.link_ny, self.link_nz = (dev_f(a) for a in (g.link_nx, g.link_ny, g.link_nz))
+        # Links of each node in ascending link order, for the Dirichlet anchors.
+        lp = np.asarray(tables["link_p"], np.int64)
+        order = np.argsort(lp, kind="stable").astype(np.int32)
+        counts = np.bincount(lp, minlength=n) if L else np.zeros(n, np.int64)
+        off = np.zeros(n + 1, np.int32)
+        off[1:] = np.cumsum(counts)
+        self.node_link_off = dev_i(off)
+        self.node_link_ids = dev_i(order if L else np.zeros(1, i32))
+        if self.characteristic_stencil is not None:
+            self.char_indices = dev_i(self.characteristic_stencil["indices"])
+            self.char_weights = dev_f(self.characteristic_stencil["weights"])
+            self.char_state = DeviceArray.zeros((4, NCOMP, L), f64, dev)
+        else:
+            self.char_indices = DeviceArray.zeros((1,), i32, dev)
+            self.char_weights = DeviceArray.zeros((1,), f64, dev)
+            self.char_state = DeviceArray.zeros((1, 1, 1), f64, dev)
+        self.kind_by_id_wp = dev_i(self.kind_by_id)
+        self.component_kind_by_id_wp = dev_i(self.component_kind_by_id)
+        self.mode_by_id_wp = dev_i(self.mode_by_id)
+        self.params_by_id_wp = dev_f(self.params_by_id)
+        self.link_kind = DeviceArray.zeros((L,), i32, dev)
+        self.link_component_kind = DeviceArray.zeros((3, L), i32, dev)
+        self.link_bv = DeviceArray.zeros((3, L), f64, dev)
+        self.link_Fb = DeviceArray.zeros((9, L), f64, dev)
+        self.link_Pb = DeviceArray.zeros((9, L), f64, dev)
+        self.dynamic_traction_override_enabled = DeviceArray.zeros((L,), i32, dev)
+        self.dynamic_traction_override = DeviceArray.zeros((3, L), f64, dev)
+        pop_dtype = np.float64 if self.precision == "fp64" else np.float32
+        self.f0 = PopulationArray(DeviceArray.zeros((FQ, n), pop_dtype, dev).tensor, self)
+        self.f1 = PopulationArray(DeviceArray.zeros((FQ, n), pop_dtype, dev).tensor, self)
+        self.U = DeviceArray.zeros((NCOMP, n), f64, dev)
+        self.u = DeviceArray.zeros((3, n), f64, dev)
+        self.u_star = DeviceArray.zeros((3, n), f64, dev)
+        self.u_local_next = DeviceArray.zeros((3, n), f64, dev)
+        self.record = DeviceArray.zeros((24, n), f64, dev)
+        self.invalid = DeviceArray.zeros((1,), i32, dev)
+        self._reduce_scratch = DeviceArray.zeros((_native.RED_SCRATCH,), f64, dev)
+        self.source_params_wp = dev_f(self.source_params)
+        self._stream = torch.cuda.current_stream(dev)
+
+        d = _native.Vlbm3dDesc()
+        d.struct_size = C.sizeof(_native.Vlbm3dDesc)
+        d.abi_version = _native.ABI_VERSION
+        d.n_active, d.n_links = n, L
+        d.precision = _native.PREC_F64 if self.precision == "fp64" else _native.PREC_F32
+        m = self.material
+        d.model, d.lam, d.mu, d.p0, d.p1, d.p2 = m.id, m.lam, m.mu, m.param0, m.param1, m.param2
+        d.j_floor = self.jacobian_floor
+        d.dt, d.c, d.omega = self.dt, self.lattice_speed, self.collision_omega
+        d.dx, d.dy, d.dz = self.dx, self.dy, self.dz
+        d.source_mode = self.source_mode
+        for k in range(12):
+            d.source_params[k] = float(self.source_params[k])
+        d.recon = self.boundary_reconstruction
+        d.normal_solver = self._normal_solver_mode
+        d.disp_interval = self.local_displacement_interval
+        d.disp_sweeps = self.local_displacement_sweeps
+        d.disp_relax = self.local_displacement_relax
+        d.disp_weight = self.local_displacement_boundary_weight
+        d.disp_boundary_id = self.local_displacement_boundary_id
+        d.compat_interval = self.local_compatibility_interval
+        d.compat_blend = self.local_compatibility_blend
+        d.has_char = 1 if self.characteristic_stencil is not None else 0
+        d.n_anchor_links = L
+        ptrs = {
+            "U": self.U, "u": self.u, "u_star": self.u_star, "u_next": self.u_local_next,
+            "record": self.record, "invalid": self.invalid, "reduce_scratch": self._reduce_scratch,
+            "neighbor": self.neighbor, "active_i": self.active_i_wp, "active_j": self.active_j_wp,
+            "active_k": self.active_k_wp, "node_link_off": self.node_link_off, "node_link_ids": self.node_link_ids,
+            "link_p": self.link_p, "link_up": self.link_upstream, "link_q": self.link_q,
+            "link_bid": self.link_boundary_id, "link_eta": self.link_eta, "link_xb": self.link_xb,
+            "link_yb": self.link_yb, "link_zb": self.link_zb, "link_nx": self.link_nx, "link_ny": self.link_ny,
+            "link_nz": self.link_nz, "char_indices": self.char_indices, "char_weights": self.char_weights,
+            "char_state": self.char_state, "kind_by_id": self.kind_by_id_wp,
+            "comp_kind_by_id": self.component_kind_by_id_wp, "mode_by_id": self.mode_by_id_wp,
+            "params_by_id": self.params_by_id_wp, "dyn_enabled": self.dynamic_traction_override_enabled,
+            "dyn_traction": self.dynamic_traction_override, "lkind": self.link_kind,
+            "lcomp_kind": self.link_component_kind, "lbv": self.link_bv, "lFb": self.link_Fb, "lPb": self.link_Pb,
+        }
+        for name, arr in ptrs.items():
+            setattr(d, name, arr.ptr or None)
+        d.stream = self._stream.cuda_stream or None
+        self._desc = d
+        h = C.c_void_p()
+        _native.check(self._lib.vlbm3d_create(C.byref(d), C.byref(h)), "vlbm3d_create")
+        self._handle = h
+        self._native = _native
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            try:
+                self._lib.vlbm3d_destroy(h)
+            except Exception:
+                pass
+            self._handle = None
+
+    def _check(self, rc, what):
+        self._native.check(rc, what)
+
+    def _rebind_dynamic(self):
+        self._desc.dyn_enabled = self.dynamic_traction_override_enabled.ptr or None
+        self._desc.dyn_traction = self.dynamic_traction_override.ptr or None
+        self._lib.vlbm3d_destroy(self._handle)
+        h = C.c_void_p()
+        self._check(self._lib.vlbm3d_create(C.byref(self._desc), C.byref(h)), "vlbm3d_create")
+        self._handle = h
+
+    def _populations_to_host(self, arr) -> np.ndarray:
+        tmp = self._torch.empty((FQ, self.n_active), dtype=self._torch.float64, device=self.device)
+        self._check(self._lib.vlbm3d_populations_to_f64(self._handle, arr.ptr, tmp.data_ptr()), "populations_to_f64")
+        self._torch.cuda.synchronize(self.device)
+        return tmp.cpu().numpy()
+
+    def _populations_from_host(self, arr, host: np.ndarray) -> None:
+        if host.shape != (FQ, self.n_active):
+            raise ValueError(f"populations must have shape {(FQ, self.n_active)}")
+        tmp = self._torch.from_numpy(np.ascontiguousarray(host)).to(self.device)
+        self._check(self._lib.vlbm3d_populations_from_f64(self._handle, tmp.data_ptr(), arr.ptr), "populations_from_f64")
+        self._torch.cuda.synchronize(self.device)
+
+    # ---- coordinates and bookkeeping ---------------------------------------
+    def _build_dense_coordinates(self) -> None:
+        if self._x_dense is None:
+            xs = (np.arange(self.nx) + 0.5) * self.dx
+            ys = (np.arange(self.ny) + 0.5) * self.dy
+            zs = (np.arange(self.nz) + 0.5) * self.dz
+            self._x_dense, self._y_dense, self._z_dense = np.meshgrid(xs, ys, zs, indexing="ij")
+
+    @property
+    def x(self):
+        self._build_dense_coordinates()
+        return self._x_dense
+
+    @property
+    def y(self):
+        self._build_dense_coordinates()
+        return self._y_dense
+
+    @property
+    def z(self):
+        self._build_dense_coordinates()
+        return self._z_dense
+
+    def active_indices(self):
+        return self.active_i.copy(), self.active_j.copy(), self.active_k.copy()
+
+    def active_coordinates(self):
+        return (
+            (self.active_i.astype(np.float64) + 0.5) * self.dx,
+            (self.active_j.astype(np.float64) + 0.5) * self.dy,
+            (self.active_k.astype(np.float64) + 0.5) * self.dz,
+        )
+
+    def memory_estimate_bytes(self) -> dict[str, int]:
+        pop_bytes = 8 if self.precision == "fp64" else 4
+        L = self.geometry.n_links
+        return {
+            "populations": int(2 * FQ * self.n_active * pop_bytes),
+            "fields": int((NCOMP + 3 + 3 + 3 + 24) * self.n_active * 8),
+            "connectivity": int((Q * self.n_active + self.n_active + 1 + 3 * L) * 4),
+            "link_data": int((1 + 3 + 9 + 9 + 3) * L * 8 + L * 4),
+        }
+
+    # ---- dynamic traction overrides (cbw3d:4115-4174) -----------------------
+    def _validate_dynamic_traction_override(self, enabled, traction) -> None:
+        nl = self.geometry.n_links
+        if tuple(enabled.shape) != (nl,):
+            raise ValueError(f"dynamic traction enable mask must have shape {(nl,)}")
+        if tuple(traction.shape) != (3, nl):
+            raise ValueError(f"dynamic traction values must have shape {(3, nl)}")
+        if enabled.dtype != self._torch.int32:
+            raise TypeError("dynamic traction enable mask must use int32")
+        if traction.dtype != self._torch.float64:
+            raise TypeError("dynamic traction values must use float64")
+        if enabled.tensor.device != self._torch.device(self.device) or traction.tensor.device != self._torch.device(self.device):
+            raise ValueError(f"dynamic traction buffers must reside on {self.device}")
+
+    def bind_dynamic_traction_override(self, enabled, traction) -> None:
+        """Bind live per-link device buffers (DeviceArray) without copying."""
+        self._validate_dynamic_traction_override(enabled, traction)
+        self.dynamic_traction_override_enabled = enabled
+        self.dynamic_traction_override = traction
+        self._rebind_dynamic()
+
+    def update_dynamic_traction_override(self, *, enabled=None, traction=None) -> None:
+        nl = self.geometry.n_links
+        if enabled is not None:
+            arr = enabled.numpy() if hasattr(enabled, "numpy") and not isinstance(enabled, np.ndarray) else enabled
+            arr = np.ascontiguousarray(arr, dtype=np.int32)
+            if arr.shape != (nl,):
+                raise ValueError(f"dynamic traction enable mask must have shape {(nl,)}")
+            self.dynamic_traction_override_enabled.assign(arr)
+        if traction is not None:
+            arr = traction.numpy() if hasattr(traction, "numpy") and not isinstance(traction, np.ndarray) else traction
+            arr = np.ascontiguousarray(arr, dtype=np.float64)
+            if arr.shape != (3, nl):
+                raise ValueError(f"dynamic traction values must have shape {(3, nl)}")
+            self.dynamic_traction_override.assign(arr)
+
+    def clear_dynamic_traction_override(self) -> None:
+        self.dynamic_traction_override_enabled.zero_()
+
+    # ---- initialisation (cbw3d:4176-4279) ----------------------------------
+    def _active_from_dense(self, arr, ncomp, name):
+        arr = np.asarray(arr, dtype=np.float64)
+        if arr.shape == (ncomp, self.n_active):
+            return np.ascontiguousarray(arr)
+        if arr.shape != (ncomp, self.nx, self.ny, self.nz):
+            raise ValueError(
+                f"{name} must have shape {(ncomp, self.nx, self.ny, self.nz)} or {(ncomp, self.n_active)}"
+            )
+        return np.ascontiguousarray(arr[:, self.active_np.astype(bool)])
+
+    def _scatter_to_dense(self, active_values, ncomp, *, identity_U=False):
+        out = np.zeros((ncomp, self.nx, self.ny, self.nz), dtype=np.float64)
+        if identity_U and ncomp == NCOMP:
+            out[3] = out[7] = out[11] = 1.0
+        out[:, self.active_i, self.active_j, self.active_k] = active_values
+        return out
+
+    def initialize_identity(self, *, init_order: int = 1):
+        U = np.zeros((NCOMP, self.n_active), dtype=np.float64)
+        U[3] = U[7] = U[11] = 1.0
+        self._initialize_from_active(U, np.zeros((3, self.n_active)), init_order=init_order)
+
+    def initialize_from_numpy(self, *, U, displacement=None, init_order: int = 1,
+                              apply_initial_boundaries: bool = True, initial_boundary_time: float = 0.0):
+        U_active = self._active_from_dense(U, NCOMP, "U")
+        disp = np.zeros((3, self.n_active)) if displacement is None else self._active_from_dense(displacement, 3, "displacement")
+        self._initialize_from_active(
+            U_active, disp, init_order=init_order,
+            apply_initial_boundaries=apply_initial_boundaries, initial_boundary_time=initial_boundary_time,
+        )
+
+    def _initialize_from_active(self, U_active, disp_active, *, init_order=1, apply_initial_boundaries=True,
+                                initial_boundary_time=0.0):
+        if init_order != 1:
+            raise ValueError(
+                "sparse 3-D curved solver currently supports init_order=1; use dense class for higher-order initialization"
+            )
+        self.U.assign(U_active)
+        self._check(self._lib.vlbm3d_init_populations(self._handle, self.f0.ptr), "init_populations")
+        self.f1.zero_()
+        self.u_star.assign(np.ascontiguousarray(disp_active - 0.5 * self.dt * U_active[:3]))
+        self.time = 0.0
+        self.steps = 0
+        self.refresh_current()
+        if apply_initial_boundaries:
+            self.apply_boundary_reconstruction(initial_boundary_time, in_place=True)
+            self.refresh_current()
+            disp = self._torch.from_numpy(np.ascontiguousarray(disp_active, dtype=np.float64)).to(self.device)
+            self._check(self._lib.vlbm3d_set_ustar(self._handle, disp.data_ptr()), "set_ustar")
+            self._torch.cuda.synchronize(self.device)
+            self.refresh_current()
+
+    # ---- time stepping -------------------------------------------------------
+    def apply_boundary_reconstruction(self, t: float, *, in_place: bool = False):
+        if self.geometry.n_links == 0:
+            return
+        self._check(
+            self._lib.vlbm3d_apply_boundary(self._handle, self.f0.ptr, self.f1.ptr, float(t), 1 if in_place else 0),
+            "apply_boundary",
+        )
+
+    def refresh_current(self):
+        self._check(self._lib.vlbm3d_refresh(self._handle, self.f0.ptr), "refresh")
+
+    def run(self, nsteps: int) -> None:
+        """Advance ``nsteps`` steps in one library call (same result as
+        calling ``step()`` nsteps times)."""
+        nsteps = int(nsteps)
+        if nsteps <= 0:
+            return
+        self._check(
+            self._lib.vlbm3d_step(self._handle, self.f0.ptr, self.f1.ptr, float(self.time), int(self.steps), nsteps),
+            "step",
+        )
+        for _ in range(nsteps):
+            self.time += self.dt
+        self.steps += nsteps
+        if nsteps % 2 == 1:
+            self.f0, self.f1 = self.f1, self.f0
+
+    def step(self):
+        self.run(1)
+
+    def steps_until(self, t_end: float) -> int:
+        """Number of steps run_until(t_end) takes from the current time."""
+        t, k = self.time, 0
+        while t < float(t_end) - 0.5 * self.dt:
+            t += self.dt
+            k += 1
+        return k
+
+    def run_until(self, t_end: float, *, chunk: int = 256):
+        remaining = self.steps_until(t_end)
+        while remaining > 0:
+            k = min(chunk, remaining)
+            self.run(k)
+            remaining -= k
+        self.refresh_current()
+        self._torch.cuda.synchronize(self.device)
+
+    def reductions(self) -> dict[str, float]:
+        """Device reductions of the current U and u (no refresh)."""
+        out = (C.c_double * 8)()
+        self._check(self._lib.vlbm3d_reduce(self._handle, out), "reduce")
+        n = max(self.n_active, 1)
+        return {
+            "rms_v": math.sqrt(out[0] / n), "max_abs_v": out[1], "min_J": out[2], "max_J": out[3],
+            "max_abs_u": out[4], "nonfinite": int(out[5]), "invalid": int(out[6]),
+        }
+
+    def residual(self) -> float:
+        """Equilibrium residual rms_v = sqrt(mean_active |v|^2) of the current
+        populations (refreshes U first), as cases/spherical_shell.py:351-361."""
+        self.refresh_current()
+        return self.reductions()["rms_v"]
+
+    def run_to_equilibrium(self, tol: float, *, check_every: int = 100, hold_checks: int = 1,
+                           min_time: float = 0.0, max_steps: int = 10**7) -> dict:
+        """Step until rms_v <= tol on ``hold_checks`` consecutive checks."""
+        held = 0
+        while self.steps < max_steps:
+            self.run(min(check_every, max_steps - self.steps))
+            red = self._refreshed_reductions()
+            if red["nonfinite"] or red["min_J"] <= 0.0:
+                return {"converged": False, "steps": self.steps, **red}
+            held = held + 1 if (self.time >= min_time and red["rms_v"] <= tol) else 0
+            if held >= hold_checks:
+                return {"converged": True, "steps": self.steps, **red}
+        return {"converged": False, "steps": self.steps, **self._refreshed_reductions()}
+
+    def _refreshed_reductions(self):
+        self.refresh_current()
+        return self.reductions()
+
+    # ---- accessors (cbw3d:4596-4802) ----------------------------------------
+    def active_mask(self):
+        return self.active_np.astype(bool)
+
+    def active_system_state(self):
+        return np.asarray(self.U.numpy())
+
+    def active_displacement(self):
+        return np.asarray(self.u.numpy())
+
+    def active_ustar(self):
+        return np.asarray(self.u_star.numpy())
+
+    def active_populations(self):
+        return np.asarray(self.f0.numpy())
+
+    def active_deformation_gradient(self):
+        return _state_to_F_np(self.active_system_state())
+
+    def active_deformation_jacobian(self):
+        return _det3_np(self.active_deformation_gradient())
+
+    def active_piola(self):
+        return _first_piola_np(self.active_deformation_gradient(), self.material)
+
+    def active_sigma(self):
+        F = self.active_deformation_gradient()
+        P = _first_piola_np(F, self.material)
+        J = np.maximum(_det3_np(F), self.jacobian_floor)
+        return np.einsum("ia...,ja...->ij...", P, F) / J
+
+    def numpy_field(self, name):
+        if name == "U":
+            return self._scatter_to_dense(self.U.numpy(), NCOMP, identity_U=True)
+        if name == "u":
+            return self._scatter_to_dense(self.u.numpy(), 3)
+        if name == "u_star":
+            return self._scatter_to_dense(self.u_star.numpy(), 3)
+        if name == "f0":
+            return self._scatter_to_dense(self.f0.numpy(), FQ)
+        if name == "f1":
+            return self._scatter_to_dense(self.f1.numpy(), FQ)
+        if name in ("P", "sigma"):
+            U = self.U.numpy()
+            F = U[3:12].reshape(3, 3, self.n_active)
+            P = _first_piola_np(F, self.material)
+            if name == "P":
+                return self._scatter_to_dense(P.reshape(9, self.n_active), 9)
+            J = np.maximum(_det3_np(F), self.jacobian_floor)
+            sigma = np.einsum("ia...,ja...->ij...", P, F) / J
+            return self._scatter_to_dense(sigma.reshape(9, self.n_active), 9)
+        raise AttributeError(name)
+
+    def system_state(self):
+        return self.numpy_field("U")
+
+    def displacement(self):
+        return self.numpy_field("u")
+
+    def deformation_gradient(self):
+        return _state_to_F_np(self.system_state())
+
+    def deformation_jacobian(self):
+        return _det3_np(self.deformation_gradient())
+
+    def invalid_state(self):
+        U = self.U.numpy()
+        J = _det3_np(U[3:12].reshape(3, 3, self.n_active))
+        return int(self.invalid.numpy()[0]) != 0 or not np.isfinite(U).all() or not np.all(J > 0)
+
+    def configure_compatibility_projection(self, *, interval=1, iterations=160, relax=0.85, weight=20.0,
+                                           repair_F=False, repair_F_band_width=-1, repair_F_blend=1.0,
+                                           boundary_id=-1) -> None:
+        if int(interval) != 0 or bool(repair_F):
+            raise ValueError("global compatibility projection is not available in the fully-local solver")
+
+    def project_compatible_displacement(self, *, iterations=None, repair_F=None) -> None:
+        raise RuntimeError("global compatibility projection is disabled by design")
+
+    def _local_displacement_due(self) -> bool:
+        return self.local_displacement_interval > 0 and self.steps % self.local_displacement_interval == 0
+
+    def _local_compatibility_due(self) -> bool:
+        return self.local_compatibility_interval > 0 and self.steps % self.local_compatibility_interval == 0
+
+    def summary(self):
+        self.refresh_current()
+        red = self.reductions()
+        mem = self.memory_estimate_bytes()
+        disp_radius = int(self.local_displacement_sweeps) if self.local_displacement_interval > 0 else 0
+        composed = disp_radius + 2 if self.local_compatibility_interval > 0 else disp_radius
+        link_ids = np.clip(self.geometry.link_boundary_id, 0, MAX_BOUNDARY_IDS - 1)
+        has_char = self.characteristic_stencil is not None
+        n_ok = self.n_active > 0
+        return {
+            "device": self.device,
+            "storage": "sparse_active",
+            "precision": self.precision,
+            "geometry": self.geometry.label,
+            "boundary_label": self.boundary.label,
+            "model": self.material.model,
+            "nx": self.nx, "ny": self.ny, "nz": self.nz,
+            "n_active": int(self.n_active),
+            "n_cut_links": int(self.geometry.n_links),
+            "steps": int(self.steps),
+            "time": float(self.time),
+            "dx": float(self.dx), "dt": float(self.dt),
+            "lattice_speed": float(self.lattice_speed),
+            "collision_omega": float(self.collision_omega),
+            "boundary_reconstruction": self.boundary_reconstruction_name,
+            "population_reconstruction": "dirichlet_bfl_neumann_characteristic" if has_char else "bfl",
+            "boundary_stencil_radius": 3 if has_char else 1,
+            "characteristic_cut_links": int(np.count_nonzero(np.any(self.component_kind_by_id[:, link_ids] == BC_NEUMANN, axis=0))) if has_char else 0,
+            "bfl_cut_links": int(np.count_nonzero(np.any(self.component_kind_by_id[:, link_ids] == BC_DIRICHLET, axis=0))) if has_char else int(self.geometry.n_links),
+            "normal_jacobian": self.normal_jacobian_name,
+            "neumann_solver": self.neumann_solver_name,
+            "has_neumann_cut_links": bool(self._has_neumann_cut_links),
+            "surface_tangent_lsq_active": False,
+            "compatibility_projection": {"interval": 0, "active_in_step": False},
+            "local_method": {
+                "displacement_interval": int(self.local_displacement_interval),
+                "displacement_sweeps": int(self.local_displacement_sweeps),
+                "displacement_relax": float(self.local_displacement_relax),
+                "boundary_weight": float(self.local_displacement_boundary_weight),
+                "boundary_id": int(self.local_displacement_boundary_id),
+                "F_repair_interval": int(self.local_compatibility_interval),
+                "F_repair_blend": float(self.local_compatibility_blend),
+                "kernel_stencil_radius": 2 if self.local_compatibility_interval > 0 else (1 if disp_radius > 0 else 0),
+                "displacement_dependency_radius": disp_radius,
+                "composed_compatibility_radius": composed,
+            },
+            "memory_estimate_MiB": {k: float(v / 1024**2) for k, v in mem.items()},
+            "max_abs_u": float(red["max_abs_u"]) if n_ok else 0.0,
+            "max_abs_v": float(red["max_abs_v"]) if n_ok else 0.0,
+            "min_J": float(red["min_J"]) if n_ok else 0.0,
+            "max_J": float(red["max_J"]) if n_ok else 0.0,
+            "finite": bool(red["nonfinite"] == 0 and red["min_J"] > 0) if n_ok else False,
+        }
+
+    def save_npz(self, path):
+        self.refresh_current()
+        path = Path(path)
+        path.parent.mkdir(parents=True, exist_ok=True)
+        xa, ya, za = self.active_coordinates()
+        U = self.active_system_state()
+        u = self.active_displacement()
+        F = _state_to_F_np(U)
+        P = _first_piola_np(F, self.material)
+        J = np.maximum(_det3_np(F), self.jacobian_floor)
+        sigma = np.einsum("ia...,ja...->ij...", P, F) / J
+        np.savez_compressed(
+            path, x_active=xa, y_active=ya, z_active=za,
+            active_i=self.active_i, active_j=self.active_j, active_k=self.active_k,
+            shape=np.array([self.nx, self.ny, self.nz], dtype=np.int32),
+            dx=np.array(self.dx), dy=np.array(self.dy), dz=np.array(self.dz),
+            u=u, U=U, F=F, P=P, sigma=sigma, time=np.array(self.time),
+            storage=np.array("sparse_active"), metadata=np.array(json.dumps(self.summary())),
+        )
+
+
+WarpCurvedBoundaryLBM3D = WarpSparseCurvedBoundaryLBM3D
+
+__all__ = [
+    "WarpHyperelasticMaterial3D", "CurvedBoundarySpec3D", "CurvedBoundaryGeometry3D",
+    "WarpCurvedBoundaryLBM3D", "WarpSparseCurvedBoundaryLBM3D", "constant_piola_params3d",
+    "relative_l2_active", "SphereDomain",
+    "CURVED3D_VALUE_ZERO", "CURVED3D_VALUE_CONSTANT_VECTOR", "CURVED3D_VALUE_AFFINE_VELOCITY",
+    "CURVED3D_VALUE_RAMP_VECTOR", "CURVED3D_VALUE_SINE2_HOLD_VECTOR", "CURVED3D_VALUE_PIOLA_NORMAL",
+    "CURVED3D_VALUE_NORMAL_TRACTION", "CURVED3D_VALUE_SINE2_NORMAL_TRACTION",
+    "CURVED3D_VALUE_AFFINE_NEO_HOOKE_PIOLA_NORMAL", "CURVED3D_VALUE_Z_PULL_TORSION_TRACTION",
+    "CURVED3D_VALUE_Z_STRETCH_EXACT_TWIST_VELOCITY", "CURVED3D_VALUE_LINEAR_RAMP_NORMAL_TRACTION",
+    "BOUNDARY_RECON_BFL", "BOUNDARY_RECON_HALF_WAY", "BOUNDARY_RECON_COMPAT_BFL", "BOUNDARY_RECON_LOCAL_F",
+    "BOUNDARY_ID_OUTER", "BOUNDARY_ID_INNER",
+]

@@ -1,0 +1,278 @@
+"""Python harness of the CPU FP64 oracle (oracle/vlbm3d_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg, never by the product package.
+
+``OracleSolver3D`` takes the same constructor arguments as
+``curved_lbm.three_d.curved_boundary_warp3d.WarpCurvedBoundaryLBM3D`` and
+builds the same host tables (active ids, neighbours, cut links,
+characteristic weights) with the product's geometry code, then advances the
+state with the C restatement of the reference step.  State arrays are numpy
+arrays in the reference layout.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "build" / "libvlbm3d_oracle.so"
+
+_dp = C.c_void_p
+
+
+class _State(C.Structure):
+    _fields_ = [
+        ("n_active", C.c_int32), ("n_links", C.c_int32),
+        ("model", C.c_int32),
+        ("lam", C.c_double), ("mu", C.c_double), ("p0", C.c_double), ("p1", C.c_double),
+        ("p2", C.c_double), ("j_floor", C.c_double),
+        ("dt", C.c_double), ("c", C.c_double), ("omega", C.c_double),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("source_mode", C.c_int32), ("source_params", C.c_double * 12),
+        ("recon", C.c_int32), ("normal_solver", C.c_int32),
+        ("disp_interval", C.c_int32), ("disp_sweeps", C.c_int32),
+        ("disp_relax", C.c_double), ("disp_weight", C.c_double),
+        ("disp_boundary_id", C.c_int32), ("compat_interval", C.c_int32),
+        ("compat_blend", C.c_double), ("has_char", C.c_int32),
+        ("f0", _dp), ("f1", _dp), ("U", _dp), ("u", _dp), ("u_star", _dp), ("u_next", _dp),
+        ("bc_diag", _dp), ("bc_rhs", _dp), ("invalid", _dp),
+        ("neighbor", _dp), ("active_i", _dp), ("active_j", _dp), ("active_k", _dp),
+        ("link_p", _dp), ("link_up", _dp), ("link_q", _dp), ("link_bid", _dp),
+        ("link_eta", _dp), ("link_xb", _dp), ("link_yb", _dp), ("link_zb", _dp),
+        ("link_nx", _dp), ("link_ny", _dp), ("link_nz", _dp),
+        ("char_indices", _dp), ("char_weights", _dp), ("char_state", _dp),
+        ("kind_by_id", _dp), ("comp_kind_by_id", _dp), ("mode_by_id", _dp), ("params_by_id", _dp),
+        ("dyn_enabled", _dp), ("dyn_traction", _dp),
+        ("lkind", _dp), ("lcomp_kind", _dp), ("lbv", _dp), ("lFb", _dp), ("lPb", _dp),
+    ]
+
+
+_lib = None
+
+
+def build(quiet: bool = True) -> Path:
+    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True,
+                   stdout=subprocess.DEVNULL if quiet else None)
+    return LIB
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        lib = C.CDLL(str(LIB))
+        for name in ("or3d_refresh", "or3d_step", "or3d_apply_boundary", "or3d_set_ustar",
+                     "or3d_init_populations", "or3d_threads"):
+            getattr(lib, name).restype = C.c_int
+        lib.or3d_step.argtypes = [C.POINTER(_State), C.c_double, C.c_int64, C.c_int]
+        lib.or3d_apply_boundary.argtypes = [C.POINTER(_State), C.c_double, C.c_int]
+        lib.or3d_refresh.argtypes = [C.POINTER(_State)]
+        lib.or3d_set_ustar.argtypes = [C.POINTER(_State), _dp]
+        lib.or3d_init_populations.argtypes = [C.POINTER(_State), _dp]
+        lib.or3d_piola.argtypes = [_dp, C.c_int] + [C.c_double] * 6 + [_dp]
+        lib.or3d_piola.restype = None
+        lib.or3d_du_axis_all.argtypes = [C.POINTER(_State), _dp, _dp]
+        lib.or3d_du_axis_all.restype = None
+        _lib = lib
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def piola(F: np.ndarray, material, j_floor: float = 1e-12) -> np.ndarray:
+    """Oracle constitutive map for a single 3x3 F."""
+    lib = load()
+    Fi = np.ascontiguousarray(np.asarray(F, np.float64).reshape(9))
+    P = np.zeros(9)
+    lib.or3d_piola(_p(Fi), material.id, material.lam, material.mu, material.param0, material.param1,
+                   material.param2, j_floor, _p(P))
+    return P.reshape(3, 3)
+
+
+class OracleSolver3D:
+    """The reference 3D sparse step on the CPU with the reference's API subset
+    used by the tests: initialize_*, step/run, apply_boundary_reconstruction,
+    refresh_current, and numpy state attributes f0, f1, U, u, u_star, link_*."""
+
+    def __init__(self, geometry, *, material, boundary, lattice_speed=5.0, collision_omega=1.8,
+                 boundary_reconstruction="bfl", jacobian_floor=1e-12, source_mode="none",
+                 source_vector=(0.0,) * 12, damping_gamma=0.0, local_displacement_interval=0,
+                 local_displacement_sweeps=1, local_displacement_relax=0.85,
+                 local_displacement_boundary_weight=20.0, local_displacement_boundary_id=-1,
+                 local_compatibility_interval=0, local_compatibility_blend=0.1, normal_jacobian=None,
+                 neumann_solver=None, check_linearized_cfl=True, threads: int | None = None):
+        from curved_lbm.three_d import curved_boundary_warp3d as cb
+
+        self.cb = cb
+        self.geometry = geometry
+        self.material = material
+        self.lattice_speed = float(lattice_speed)
+        self.collision_omega = float(collision_omega)
+        self.dx, self.dy, self.dz = geometry.dx, geometry.dy, geometry.dz
+        self.dt = self.dx / self.lattice_speed
+        self.jacobian_floor = float(jacobian_floor)
+        self.recon = cb._recon_id(boundary_reconstruction)
+        if self.recon == cb.BOUNDARY_RECON_COMPAT_BFL:
+            raise ValueError("compat_bfl is not restated by the oracle")
+        if check_linearized_cfl and material.linearized_stability_ratio(self.lattice_speed) >= 1.0:
+            raise ValueError("linearized D3Q6 stability ratio must be below one")
+        nj = cb._normal_jacobian_for_material(material, normal_jacobian)
+        solver = cb.NORMAL_SOLVER_ENERGY if cb.neumann_solver_name(neumann_solver, material) == "energy" else nj
+        self.source_mode = cb.source_id(source_mode)
+        sp = np.zeros(12)
+        if self.source_mode == cb.SOURCE_CONSTANT:
+            vals = tuple(source_vector)[:12]
+            sp[: len(vals)] = vals
+        elif self.source_mode == cb.SOURCE_DAMPING:
+            sp[0] = float(damping_gamma)
+        self.source_params = sp
+        tables = cb._sparse_tables_from_geometry(geometry)
+        self.tables = tables
+        n = int(tables["active_i"].size)
+        L = geometry.n_links
+        self.n_active = n
+        kind, mode, params = boundary.tables(cb.MAX_BOUNDARY_IDS)
+        ck = boundary.component_kind_table(cb.MAX_BOUNDARY_IDS)
+        lids = np.clip(np.asarray(geometry.link_boundary_id, np.int32), 0, cb.MAX_BOUNDARY_IDS - 1)
+        neu_rows = ck[:, lids] == cb.BC_NEUMANN
+        self.characteristic_stencil = None
+        if self.recon == cb.BOUNDARY_RECON_LOCAL_F and L and np.any(neu_rows):
+            self.characteristic_stencil = cb.build_characteristic_weights(
+                geometry, tables, link_mask=np.any(neu_rows, axis=0))
+        i32, f64 = np.int32, np.float64
+        a = self._arrays = {}
+
+        def keep(name, arr, dtype):
+            a[name] = np.ascontiguousarray(np.asarray(arr, dtype=dtype))
+            return a[name]
+
+        self.f0 = keep("f0", np.zeros((72, n)), f64)
+        self.f1 = keep("f1", np.zeros((72, n)), f64)
+        self.U = keep("U", np.zeros((12, n)), f64)
+        self.u = keep("u", np.zeros((3, n)), f64)
+        self.u_star = keep("u_star", np.zeros((3, n)), f64)
+        keep("u_next", np.zeros((3, n)), f64)
+        keep("bc_diag", np.zeros((3, n)), f64)
+        keep("bc_rhs", np.zeros((3, n)), f64)
+        self.invalid = keep("invalid", np.zeros(1), i32)
+        self.neighbor = keep("neighbor", tables["neighbor"], i32)
+        for k in ("active_i", "active_j", "active_k"):
+            keep(k, tables[k], i32)
+        self.link_p = keep("link_p", tables["link_p"], i32)
+        keep("link_up", tables["link_upstream"], i32)
+        keep("link_q", geometry.link_q, i32)
+        keep("link_bid", geometry.link_boundary_id, i32)
+        for k in ("eta", "xb", "yb", "zb", "nx", "ny", "nz"):
+            keep("link_" + k, getattr(geometry, "link_" + k), f64)
+        if self.characteristic_stencil is not None:
+            keep("char_indices", self.characteristic_stencil["indices"], i32)
+            keep("char_weights", self.characteristic_stencil["weights"], f64)
+            self.char_state = keep("char_state", np.zeros((4, 12, L)), f64)
+        else:
+            keep("char_indices", np.zeros(1), i32)
+            keep("char_weights", np.zeros(1), f64)
+            self.char_state = keep("char_state", np.zeros(1), f64)
+        keep("kind_by_id", kind, i32)
+        keep("comp_kind_by_id", ck, i32)
+        keep("mode_by_id", mode, i32)
+        keep("params_by_id", params, f64)
+        self.dyn_enabled = keep("dyn_enabled", np.zeros(max(L, 1)), i32)
+        self.dyn_traction = keep("dyn_traction", np.zeros((3, max(L, 1))), f64)
+        self.link_kind = keep("lkind", np.zeros(max(L, 1)), i32)
+        self.link_component_kind = keep("lcomp_kind", np.zeros((3, max(L, 1))), i32)
+        self.link_bv = keep("lbv", np.zeros((3, max(L, 1))), f64)
+        self.link_Fb = keep("lFb", np.zeros((9, max(L, 1))), f64)
+        self.link_Pb = keep("lPb", np.zeros((9, max(L, 1))), f64)
+
+        s = self._s = _State()
+        s.n_active, s.n_links = n, L
+        s.model, s.lam, s.mu = material.id, material.lam, material.mu
+        s.p0, s.p1, s.p2, s.j_floor = material.param0, material.param1, material.param2, self.jacobian_floor
+        s.dt, s.c, s.omega = self.dt, self.lattice_speed, self.collision_omega
+        s.dx, s.dy, s.dz = self.dx, self.dy, self.dz
+        s.source_mode = self.source_mode
+        for k in range(12):
+            s.source_params[k] = sp[k]
+        s.recon, s.normal_solver = self.recon, solver
+        s.disp_interval = int(max(0, local_displacement_interval))
+        s.disp_sweeps = int(local_displacement_sweeps)
+        s.disp_relax = float(local_displacement_relax)
+        s.disp_weight = float(local_displacement_boundary_weight)
+        s.disp_boundary_id = int(local_displacement_boundary_id)
+        s.compat_interval = int(max(0, local_compatibility_interval))
+        s.compat_blend = float(local_compatibility_blend)
+        s.has_char = 1 if self.characteristic_stencil is not None else 0
+        for name, arr in a.items():
+            setattr(s, name, _p(arr))
+        self._lib = load()
+        if threads is not None:
+            os.environ["OMP_NUM_THREADS"] = str(int(threads))
+        self.time = 0.0
+        self.steps = 0
+        self.initialize_identity()
+
+    # population buffers alternate like the reference's f0/f1 swap
+    def _sync_pointers(self):
+        if self._s.f0 != self.f0.ctypes.data:
+            self.f0, self.f1 = self.f1, self.f0
+
+    def active_coordinates(self):
+        t = self.tables
+        return ((t["active_i"] + 0.5) * self.dx, (t["active_j"] + 0.5) * self.dy, (t["active_k"] + 0.5) * self.dz)
+
+    def refresh_current(self):
+        self._lib.or3d_refresh(C.byref(self._s))
+
+    def apply_boundary_reconstruction(self, t, *, in_place=False):
+        self._lib.or3d_apply_boundary(C.byref(self._s), float(t), 1 if in_place else 0)
+
+    def initialize_identity(self):
+        U = np.zeros((12, self.n_active))
+        U[3] = U[7] = U[11] = 1.0
+        self.initialize_from_numpy(U=U, displacement=np.zeros((3, self.n_active)))
+
+    def initialize_from_numpy(self, *, U, displacement=None, apply_initial_boundaries=True,
+                              initial_boundary_time=0.0):
+        U = np.ascontiguousarray(np.asarray(U, np.float64))
+        disp = np.zeros((3, self.n_active)) if displacement is None else np.ascontiguousarray(displacement, np.float64)
+        self._lib.or3d_init_populations(C.byref(self._s), _p(U))
+        self.f1[...] = 0.0
+        self.u_star[...] = disp - 0.5 * self.dt * U[:3]
+        self.time = 0.0
+        self.steps = 0
+        self.refresh_current()
+        if apply_initial_boundaries:
+            self.apply_boundary_reconstruction(initial_boundary_time, in_place=True)
+            self.refresh_current()
+            self._lib.or3d_set_ustar(C.byref(self._s), _p(disp))
+            self.refresh_current()
+
+    def run(self, nsteps):
+        self._lib.or3d_step(C.byref(self._s), float(self.time), int(self.steps), int(nsteps))
+        for _ in range(int(nsteps)):
+            self.time += self.dt
+        self.steps += int(nsteps)
+        self._sync_pointers()
+
+    def step(self):
+        self.run(1)
+
+    def du_axis_all(self, u):
+        u = np.ascontiguousarray(np.asarray(u, np.float64))
+        out = np.zeros((3, 3, self.n_active))
+        self._lib.or3d_du_axis_all(C.byref(self._s), _p(u), _p(out))
+        return out
+
+    def rms_v(self):
+        self.refresh_current()
+        return float(np.sqrt(np.mean(np.sum(self.U[:3] ** 2, axis=0))))

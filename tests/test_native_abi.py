@@ -1,0 +1,71 @@
+"""The C ABI library loads and exports every entry point of
+include/vlbm3d.h; descriptor layout agrees with the library.  No GPU work."""
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+from curved_lbm import _native
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "vlbm3d.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(vlbm\w*)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _native.load_library()
+    names = declared_functions()
+    assert len(names) >= 10
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(_native.EXPORTED)
+
+
+def test_descriptor_layout_matches_library():
+    """vlbm3d_create checks struct_size before anything else; with all
+    pointers null it must fail on the pointer check, not on the size."""
+    lib = _native.load_library()
+    d = _native.Vlbm3dDesc()
+    d.struct_size = C.sizeof(_native.Vlbm3dDesc)
+    d.abi_version = _native.ABI_VERSION
+    d.n_active = 10
+    d.disp_sweeps = 1
+    h = C.c_void_p()
+    rc = lib.vlbm3d_create(C.byref(d), C.byref(h))
+    assert rc == -1
+    assert b"null" in lib.vlbm_last_error()
+    d.struct_size -= 8
+    assert lib.vlbm3d_create(C.byref(d), C.byref(h)) == -1
+    assert b"size mismatch" in lib.vlbm_last_error()
+
+
+def test_header_struct_fields_match_binding():
+    text = HEADER.read_text()
+    body = text[text.index("typedef struct {"): text.index("} vlbm3d_desc;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl or decl.startswith("typedef"):
+            decl = decl.replace("typedef struct {", "").strip()
+            if not decl:
+                continue
+        parts = decl.replace("*", " ").split()
+        tail = " ".join(parts[1:]) if parts[0] not in ("const",) else " ".join(parts[2:])
+        if parts[0] == "const":
+            tail = " ".join(parts[2:])
+        for item in tail.split(","):
+            item = item.strip()
+            if item:
+                names.append(re.sub(r"\[.*\]", "", item).strip())
+    assert names == [f[0] for f in _native.Vlbm3dDesc._fields_]
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    with pytest.raises(RuntimeError):
+        _native.load_library(tmp_path / "libvlbm.so")
