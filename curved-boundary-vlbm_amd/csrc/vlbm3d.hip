@@ -24,6 +24,8 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/vlbm3d.h"
 #include "vlbm3d_device.h"
@@ -636,11 +638,48 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_final(double *__restrict__ par
 
 }  // namespace
 
+// Optional per-kernel timing with HIP events recorded on the launch stream.
+struct Timing {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
+    hipEvent_t get() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+};
+
 struct vlbm3d_handle {
     vlbm3d_desc d;
     Params P;
     LinkTables T;
     hipStream_t stream;
+    Timing timing;
+};
+
+// RAII bracket around one launch: records a begin/end event pair on the
+// stream when timing is enabled.
+struct TimedSlot {
+    vlbm3d_handle *h;
+    int slot;
+    hipEvent_t b = nullptr;
+    TimedSlot(vlbm3d_handle *hh, int s) : h(hh), slot(s) {
+        if (h->timing.on) {
+            b = h->timing.get();
+            if (b && hipEventRecord(b, h->stream) != hipSuccess) b = nullptr;
+        }
+    }
+    ~TimedSlot() {
+        if (b) {
+            hipEvent_t e = h->timing.get();
+            if (e && hipEventRecord(e, h->stream) == hipSuccess) h->timing.marks.push_back({slot, {b, e}});
+        }
+    }
 };
 
 static Params make_params(const vlbm3d_desc &d) {
@@ -704,14 +743,19 @@ static int run_boundary(vlbm3d_handle *h, const T *src, T *dst, double t, int us
     const vlbm3d_desc &d = h->d;
     if (d.n_links == 0) return 0;
     if (d.has_char) {
+        TimedSlot ts(h, VLBM_SLOT_GATHER);
         dim3 g((d.n_links + BLOCK - 1) / BLOCK, NC);
         hipLaunchKernelGGL(k_gather, g, dim3(BLOCK), 0, h->stream, d.U, d.char_indices, d.char_weights,
                            d.char_state, d.n_active, d.n_links);
         CHECK_LAUNCH("k_gather");
     }
     const int gl = (d.n_links + 127) / 128;
-    hipLaunchKernelGGL(k_prepare, dim3(gl), dim3(128), 0, h->stream, (const double *)d.U, h->T, t, h->P);
-    CHECK_LAUNCH("k_prepare");
+    {
+        TimedSlot ts(h, VLBM_SLOT_PREPARE);
+        hipLaunchKernelGGL(k_prepare, dim3(gl), dim3(128), 0, h->stream, (const double *)d.U, h->T, t, h->P);
+        CHECK_LAUNCH("k_prepare");
+    }
+    TimedSlot ts(h, VLBM_SLOT_LINKS);
     hipLaunchKernelGGL((k_links<T>), dim3(gl), dim3(128), 0, h->stream, src, dst, (const double *)d.record,
                        (const double *)d.U, h->T, d.invalid, use_rec, d.has_char, h->P);
     CHECK_LAUNCH("k_links");
@@ -724,15 +768,20 @@ static int run_steps(vlbm3d_handle *h, T *f0, T *f1, double time, int64_t steps,
     const int gn = xcd_grid(d.n_active);
     const Params &P = h->P;
     for (int k = 0; k < nsteps; ++k) {
+        TimedSlot whole(h, VLBM_SLOT_STEP);
         hipLaunchKernelGGL(k_reset_invalid, dim3(1), dim3(1), 0, h->stream, d.invalid);
-        hipLaunchKernelGGL((k_collide<T, true>), dim3(gn), dim3(BLOCK), 0, h->stream, f0, d.U, d.u, d.u_star,
-                           d.invalid, P);
-        CHECK_LAUNCH("k_collide");
+        {
+            TimedSlot ts(h, VLBM_SLOT_COLLIDE);
+            hipLaunchKernelGGL((k_collide<T, true>), dim3(gn), dim3(BLOCK), 0, h->stream, f0, d.U, d.u, d.u_star,
+                               d.invalid, P);
+            CHECK_LAUNCH("k_collide");
+        }
         const double tb = time + 0.5 * d.dt;
         const bool disp = d.disp_interval > 0 && steps % d.disp_interval == 0;
         const bool compat = d.compat_interval > 0 && steps % d.compat_interval == 0;
         if (disp) {
             for (int s = 0; s < d.disp_sweeps; ++s) {
+                TimedSlot ts(h, VLBM_SLOT_SWEEP);
                 hipLaunchKernelGGL(k_sweep, dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)d.U,
                                    (const double *)d.u, d.u_next, d.neighbor, d.node_link_off, d.node_link_ids,
                                    d.link_bid, d.link_xb, d.link_yb, d.link_zb, d.active_i, d.active_j,
@@ -745,12 +794,19 @@ static int run_steps(vlbm3d_handle *h, T *f0, T *f1, double time, int64_t steps,
                 }
             }
         }
-        hipLaunchKernelGGL(k_record, dim3(gn), dim3(BLOCK), 0, h->stream, d.U, disp ? (const double *)d.u_next : (const double *)d.u,
-                           d.u, d.u_star, (double *)d.record, d.neighbor, (int)compat, (int)disp, P);
-        CHECK_LAUNCH("k_record");
-        hipLaunchKernelGGL((k_stream<T>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, f1,
-                           (const double *)d.record, d.neighbor, P);
-        CHECK_LAUNCH("k_stream");
+        {
+            TimedSlot ts(h, VLBM_SLOT_RECORD);
+            hipLaunchKernelGGL(k_record, dim3(gn), dim3(BLOCK), 0, h->stream, d.U,
+                               disp ? (const double *)d.u_next : (const double *)d.u, d.u, d.u_star,
+                               (double *)d.record, d.neighbor, (int)compat, (int)disp, P);
+            CHECK_LAUNCH("k_record");
+        }
+        {
+            TimedSlot ts(h, VLBM_SLOT_STREAM);
+            hipLaunchKernelGGL((k_stream<T>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, f1,
+                               (const double *)d.record, d.neighbor, P);
+            CHECK_LAUNCH("k_stream");
+        }
         int rc = run_boundary<T>(h, f0, f1, tb, 1);
         if (rc) return rc;
         T *tmp = f0;
@@ -809,7 +865,41 @@ int vlbm3d_create(const vlbm3d_desc *desc, vlbm3d_handle **out) {
 }
 
 int vlbm3d_destroy(vlbm3d_handle *h) {
+    if (!h) return 0;
+    for (hipEvent_t e : h->timing.pool) (void)hipEventDestroy(e);
     delete h;
+    return 0;
+}
+
+int vlbm3d_timing_enable(vlbm3d_handle *h, int32_t enable) {
+    if (!h) return fail_msg("vlbm3d_timing_enable: null handle");
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail("vlbm3d_timing_enable: sync", e);
+    h->timing.marks.clear();
+    h->timing.used = 0;
+    h->timing.on = enable != 0;
+    return 0;
+}
+
+int vlbm3d_timing_read(vlbm3d_handle *h, double *ms, int32_t *counts, int32_t nslots) {
+    if (!h || !ms || !counts) return fail_msg("vlbm3d_timing_read: null argument");
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail("vlbm3d_timing_read: sync", e);
+    for (int k = 0; k < nslots; ++k) {
+        ms[k] = 0.0;
+        counts[k] = 0;
+    }
+    for (auto &m : h->timing.marks) {
+        float t = 0.0f;
+        e = hipEventElapsedTime(&t, m.second.first, m.second.second);
+        if (e != hipSuccess) return fail("vlbm3d_timing_read: elapsed", e);
+        if (m.first >= 0 && m.first < nslots) {
+            ms[m.first] += t;
+            counts[m.first] += 1;
+        }
+    }
+    h->timing.marks.clear();
+    h->timing.used = 0;
     return 0;
 }
 

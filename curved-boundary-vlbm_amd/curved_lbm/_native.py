@@ -112,9 +112,13 @@ EXPORTED = (
     "vlbm3d_reduce",
     "vlbm3d_populations_to_f64",
     "vlbm3d_populations_from_f64",
+    "vlbm3d_timing_enable",
+    "vlbm3d_timing_read",
     "vlbm_last_error",
     "vlbm_abi_version",
 )
+
+SLOTS = ("collide", "sweep", "commit", "record", "stream", "gather", "prepare", "links", "step")
 
 _lib = None
 
@@ -146,6 +150,8 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vlbm3d_reduce": [vp, _dp],
         "vlbm3d_populations_to_f64": [vp, vp, vp],
         "vlbm3d_populations_from_f64": [vp, vp, vp],
+        "vlbm3d_timing_enable": [vp, C.c_int32],
+        "vlbm3d_timing_read": [vp, _dp, _ip, C.c_int32],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)

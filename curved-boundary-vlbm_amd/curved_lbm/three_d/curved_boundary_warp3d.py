@@ -1158,6 +1158,18 @@ class WarpSparseCurvedBoundaryLBM3D:
         self.refresh_current()
         self._torch.cuda.synchronize(self.device)
 
+    def timing(self, enable: bool = True) -> None:
+        """Start (or stop) per-kernel HIP-event timing on the solver stream."""
+        self._check(self._lib.vlbm3d_timing_enable(self._handle, 1 if enable else 0), "timing_enable")
+
+    def kernel_times(self) -> dict[str, dict[str, float]]:
+        """Summed milliseconds and launch counts per kernel since timing()."""
+        ns = len(self._native.SLOTS)
+        ms = (C.c_double * ns)()
+        cnt = (C.c_int32 * ns)()
+        self._check(self._lib.vlbm3d_timing_read(self._handle, ms, cnt, ns), "timing_read")
+        return {name: {"ms": float(ms[k]), "count": int(cnt[k])} for k, name in enumerate(self._native.SLOTS)}
+
     def reductions(self) -> dict[str, float]:
         """Device reductions of the current U and u (no refresh)."""
         out = (C.c_double * 8)()

@@ -125,6 +125,24 @@ int vlbm3d_reduce(vlbm3d_handle *h, double *out);
 /* Converts populations between the stored precision and double planes. */
 int vlbm3d_populations_to_f64(vlbm3d_handle *h, const void *f, double *dst);
 int vlbm3d_populations_from_f64(vlbm3d_handle *h, const double *src, void *f);
+/* Per-kernel timing with HIP events on the solver stream (for measurement).
+ * vlbm3d_timing_enable(h, 1) starts recording (and clears earlier marks);
+ * vlbm3d_timing_read synchronises the stream and returns, per slot
+ * VLBM_SLOT_*, the summed kernel milliseconds and launch counts since the
+ * last enable/read.  No reference counterpart (the reference reports only
+ * wall-clock elapsed seconds per case, cases/reproduce.py). */
+#define VLBM_SLOT_COLLIDE 0
+#define VLBM_SLOT_SWEEP 1
+#define VLBM_SLOT_COMMIT 2
+#define VLBM_SLOT_RECORD 3
+#define VLBM_SLOT_STREAM 4
+#define VLBM_SLOT_GATHER 5
+#define VLBM_SLOT_PREPARE 6
+#define VLBM_SLOT_LINKS 7
+#define VLBM_SLOT_STEP 8
+#define VLBM_NSLOTS 9
+int vlbm3d_timing_enable(vlbm3d_handle *h, int32_t enable);
+int vlbm3d_timing_read(vlbm3d_handle *h, double *ms, int32_t *counts, int32_t nslots);
 const char *vlbm_last_error(void);
 int vlbm_abi_version(void);
 

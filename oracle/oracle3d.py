@@ -110,7 +110,8 @@ class OracleSolver3D:
                  local_displacement_sweeps=1, local_displacement_relax=0.85,
                  local_displacement_boundary_weight=20.0, local_displacement_boundary_id=-1,
                  local_compatibility_interval=0, local_compatibility_blend=0.1, normal_jacobian=None,
-                 neumann_solver=None, check_linearized_cfl=True, threads: int | None = None):
+                 neumann_solver=None, check_linearized_cfl=True, threads: int | None = None,
+                 tables=None, characteristic_stencil=None):
         from curved_lbm.three_d import curved_boundary_warp3d as cb
 
         self.cb = cb
@@ -136,7 +137,9 @@ class OracleSolver3D:
         elif self.source_mode == cb.SOURCE_DAMPING:
             sp[0] = float(damping_gamma)
         self.source_params = sp
-        tables = cb._sparse_tables_from_geometry(geometry)
+        if threads is not None:
+            os.environ["OMP_NUM_THREADS"] = str(int(threads))
+        tables = cb._sparse_tables_from_geometry(geometry) if tables is None else tables
         self.tables = tables
         n = int(tables["active_i"].size)
         L = geometry.n_links
@@ -145,8 +148,8 @@ class OracleSolver3D:
         ck = boundary.component_kind_table(cb.MAX_BOUNDARY_IDS)
         lids = np.clip(np.asarray(geometry.link_boundary_id, np.int32), 0, cb.MAX_BOUNDARY_IDS - 1)
         neu_rows = ck[:, lids] == cb.BC_NEUMANN
-        self.characteristic_stencil = None
-        if self.recon == cb.BOUNDARY_RECON_LOCAL_F and L and np.any(neu_rows):
+        self.characteristic_stencil = characteristic_stencil
+        if characteristic_stencil is None and self.recon == cb.BOUNDARY_RECON_LOCAL_F and L and np.any(neu_rows):
             self.characteristic_stencil = cb.build_characteristic_weights(
                 geometry, tables, link_mask=np.any(neu_rows, axis=0))
         i32, f64 = np.int32, np.float64
@@ -215,8 +218,6 @@ class OracleSolver3D:
         for name, arr in a.items():
             setattr(s, name, _p(arr))
         self._lib = load()
-        if threads is not None:
-            os.environ["OMP_NUM_THREADS"] = str(int(threads))
         self.time = 0.0
         self.steps = 0
         self.initialize_identity()
