@@ -46,6 +46,8 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=40, help="oracle steps timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--chunk", type=int, default=500, help="steps per library call")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the run, copy a MIB-sized device buffer twice (reference kernel for counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -124,6 +126,14 @@ def main() -> int:
             roofline["traffic"] = json.loads(prof.read_text()).get("hbm_bytes_per_launch")
         except (ValueError, OSError):
             pass
+
+    if args.copy_calibration > 0:
+        src = torch.ones(args.copy_calibration * 2**17, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     cpu = None
     if not args.no_cpu:

@@ -86,6 +86,37 @@ __device__ __forceinline__ double ld(const float *f, size_t i, int a) { return (
 __device__ __forceinline__ void st(double *f, size_t i, int, double v) { f[i] = v; }
 __device__ __forceinline__ void st(float *f, size_t i, int a, double v) { f[i] = (float)(v - rest(a)); }
 
+// Streaming variants: populations are touched once per step, so they bypass
+// the caches' retention and leave L2 to the records and neighbour tables.
+__device__ __forceinline__ double ld_nt(const double *f, size_t i, int) { return __builtin_nontemporal_load(f + i); }
+__device__ __forceinline__ double ld_nt(const float *f, size_t i, int a) {
+    return (double)__builtin_nontemporal_load(f + i) + rest(a);
+}
+// Store v and return the value as stored (rounded to the plane precision).
+__device__ __forceinline__ double st_nt(double *f, size_t i, int, double v) {
+    __builtin_nontemporal_store(v, f + i);
+    return v;
+}
+__device__ __forceinline__ double st_nt(float *f, size_t i, int a, double v) {
+    const float x = (float)(v - rest(a));
+    __builtin_nontemporal_store(x, f + i);
+    return (double)x + rest(a);
+}
+
+// fpost[q][a] at `node` from its pre-collision value fv and the node's record.
+__device__ __forceinline__ double fpost_v(double fv, const double *rec, size_t n, int q, int a, int node,
+                                          double om1) {
+    const int axis = q % 3;
+    const double s = q < 3 ? 1.0 : -1.0;
+    double b = 0.0;
+    if (a < 3) {
+        b = rec[(size_t)(R_BV + 3 * a + axis) * n + node];
+    } else if ((a - 3) % 3 == axis) {
+        b = rec[(size_t)(R_BF + (a - 3) / 3) * n + node];
+    }
+    return om1 * fv + rec[(size_t)a * n + node] + s * b;
+}
+
 // fpost[q][a] at node m from pre-collision f and the node's record.
 template <typename T>
 __device__ __forceinline__ double fpost(const T *f, const double *rec, size_t n, int q, int a, int node,
@@ -318,48 +349,108 @@ __global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const 
 }
 
 // Pull stream of the post-collision populations; the entries whose upstream
-// node is missing belong to cut links and are written by k_links.
-template <typename T>
+// node is missing belong to cut links and were written by k_links.  With
+// NEXT the kernel also performs the moment part of the next step's collide
+// (cbw3d:2213-2280): U = moments of the new populations with the half-step
+// source, u = u* + dt/2 v, u* = u + dt/2 v, and the J check.
+template <typename T, bool NEXT>
 __global__ __launch_bounds__(BLOCK) void k_stream(const T *__restrict__ f0, T *__restrict__ f1,
                                                   const double *__restrict__ rec,
-                                                  const int32_t *__restrict__ nbr, Params P) {
+                                                  const int32_t *__restrict__ nbr, double *__restrict__ U,
+                                                  double *__restrict__ u, double *__restrict__ us,
+                                                  int32_t *__restrict__ invalid, Params P) {
     const int p = xcd_block() * BLOCK + threadIdx.x;
     if (p >= P.n) return;
     const size_t n = P.n;
     const double om1 = 1.0 - P.omega;
+    double Um[NC];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) Um[a] = 0.0;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const int src = nbr[opp(q) * n + p];
-        if (src < 0) continue;
+        if (src >= 0) {
+            double fv[NC];
 #pragma unroll
-        for (int a = 0; a < NC; ++a) st(f1, (size_t)(q * 12 + a) * n + p, a, fpost(f0, rec, n, q, a, src, om1));
+            for (int a = 0; a < NC; ++a) fv[a] = ld_nt(f0, (size_t)(q * 12 + a) * n + src, a);
+#pragma unroll
+            for (int a = 0; a < NC; ++a) {
+                const double x = st_nt(f1, (size_t)(q * 12 + a) * n + p, a, fpost_v(fv[a], rec, n, q, a, src, om1));
+                if (NEXT) Um[a] += x;
+            }
+        } else if (NEXT) {
+#pragma unroll
+            for (int a = 0; a < NC; ++a) Um[a] += ld(f1, (size_t)(q * 12 + a) * n + p, a);
+        }
+    }
+    if (NEXT) {
+        const double hdt = 0.5 * P.dt;
+        if (P.src_mode == SRC_CONST) {
+#pragma unroll
+            for (int a = 0; a < NC; ++a) Um[a] += hdt * P.src[a];
+        } else if (P.src_mode == SRC_DAMP) {
+            const double denom = 1.0 + hdt * P.src[0];
+            Um[0] = Um[0] / denom;
+            Um[1] = Um[1] / denom;
+            Um[2] = Um[2] / denom;
+        }
+#pragma unroll
+        for (int a = 0; a < NC; ++a) U[a * n + p] = Um[a];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double uk = us[k * n + p] + hdt * Um[k];
+            u[k * n + p] = uk;
+            us[k * n + p] = uk + hdt * Um[k];
+        }
+        if (det3(Um + 3) <= P.m.jf) atomicOr(invalid, 1);
     }
 }
 
 // Characteristic state at each traction link: predictor value and gradient
-// of every state component from the precomputed weights (cbw3d:2676-2700).
-__global__ __launch_bounds__(BLOCK) void k_gather(const double *__restrict__ U, const int32_t *__restrict__ idx,
-                                                  const double *__restrict__ wts, double *__restrict__ cs,
-                                                  int n, int L) {
-    const int ell = blockIdx.x * BLOCK + threadIdx.x;
-    const int a = blockIdx.y;
-    if (ell >= L) return;
-    if (idx[ell] < 0) return;
-    double val = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
-    const double *Ua = U + (size_t)a * n;
-    for (int m = 0; m < MAX_CHAR; ++m) {
-        const int p = idx[(size_t)m * L + ell];
-        if (p < 0) continue;
-        const double x = Ua[p];
-        val += wts[((size_t)0 * MAX_CHAR + m) * L + ell] * x;
-        gx += wts[((size_t)1 * MAX_CHAR + m) * L + ell] * x;
-        gy += wts[((size_t)2 * MAX_CHAR + m) * L + ell] * x;
-        gz += wts[((size_t)3 * MAX_CHAR + m) * L + ell] * x;
+// of every state component (cbw3d:2676-2700).  Four lanes share a link and
+// each accumulates three components, so the stencil tables are read once per
+// link (the four lanes' table loads coalesce into one request).
+__global__ __launch_bounds__(BLOCK) void k_gather(const double *__restrict__ U, const int32_t *__restrict__ clink,
+                                                  const int32_t *__restrict__ gidx, const double *__restrict__ gw,
+                                                  const int32_t *__restrict__ pidx, const double *__restrict__ pw,
+                                                  double *__restrict__ cs, int n, int L, int Lc) {
+    constexpr int NG = MAX_CHAR - 16;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    const int c = t >> 2;
+    const int a0 = 3 * (t & 3);
+    if (c >= Lc) return;
+    const double *Ua = U + (size_t)a0 * n;
+    double val[3] = {0.0, 0.0, 0.0}, gx[3] = {0.0, 0.0, 0.0}, gy[3] = {0.0, 0.0, 0.0}, gz[3] = {0.0, 0.0, 0.0};
+    for (int m = 0; m < NG; ++m) {
+        const int p = gidx[(size_t)m * Lc + c];
+        if (p < 0) break;
+        const double w1 = gw[((size_t)0 * NG + m) * Lc + c];
+        const double w2 = gw[((size_t)1 * NG + m) * Lc + c];
+        const double w3 = gw[((size_t)2 * NG + m) * Lc + c];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double x = Ua[(size_t)k * n + p];
+            gx[k] += w1 * x;
+            gy[k] += w2 * x;
+            gz[k] += w3 * x;
+        }
     }
-    cs[((size_t)0 * NC + a) * L + ell] = val;
-    cs[((size_t)1 * NC + a) * L + ell] = gx;
-    cs[((size_t)2 * NC + a) * L + ell] = gy;
-    cs[((size_t)3 * NC + a) * L + ell] = gz;
+    for (int m = 0; m < 16; ++m) {
+        const int p = pidx[(size_t)m * Lc + c];
+        if (p < 0) continue;
+        const double w0 = pw[(size_t)m * Lc + c];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) val[k] += w0 * Ua[(size_t)k * n + p];
+    }
+    const int ell = clink[c];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int a = a0 + k;
+        cs[((size_t)0 * NC + a) * L + ell] = val[k];
+        cs[((size_t)1 * NC + a) * L + ell] = gx[k];
+        cs[((size_t)2 * NC + a) * L + ell] = gy[k];
+        cs[((size_t)3 * NC + a) * L + ell] = gz[k];
+    }
 }
 
 struct LinkTables {
@@ -742,11 +833,12 @@ template <typename T>
 static int run_boundary(vlbm3d_handle *h, const T *src, T *dst, double t, int use_rec) {
     const vlbm3d_desc &d = h->d;
     if (d.n_links == 0) return 0;
-    if (d.has_char) {
+    if (d.has_char && d.n_char_links > 0) {
         TimedSlot ts(h, VLBM_SLOT_GATHER);
-        dim3 g((d.n_links + BLOCK - 1) / BLOCK, NC);
-        hipLaunchKernelGGL(k_gather, g, dim3(BLOCK), 0, h->stream, d.U, d.char_indices, d.char_weights,
-                           d.char_state, d.n_active, d.n_links);
+        const int g = (int)(((size_t)d.n_char_links * 4 + BLOCK - 1) / BLOCK);
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BLOCK), 0, h->stream, (const double *)d.U, d.char_link,
+                           d.char_gidx, d.char_gw, d.char_pidx, d.char_pw, d.char_state, d.n_active, d.n_links,
+                           d.n_char_links);
         CHECK_LAUNCH("k_gather");
     }
     const int gl = (d.n_links + 127) / 128;
@@ -762,20 +854,29 @@ static int run_boundary(vlbm3d_handle *h, const T *src, T *dst, double t, int us
     return 0;
 }
 
+// One library call runs nsteps reference steps.  The collide of the first
+// step is a kernel of its own; the collide of every later step is fused into
+// the previous step's stream kernel (moments of the freshly written
+// populations), so the populations are read once per step.  The cut-link
+// kernel therefore runs before the stream kernel, which reads the link rows
+// back to complete each node's moments.  The last step of the call streams
+// without the fused collide, so U, u and u* end as the reference leaves them
+// (post-repair state, committed displacement).
 template <typename T>
 static int run_steps(vlbm3d_handle *h, T *f0, T *f1, double time, int64_t steps, int nsteps) {
     const vlbm3d_desc &d = h->d;
     const int gn = xcd_grid(d.n_active);
     const Params &P = h->P;
+    if (nsteps <= 0) return 0;
+    hipLaunchKernelGGL(k_reset_invalid, dim3(1), dim3(1), 0, h->stream, d.invalid);
+    {
+        TimedSlot ts(h, VLBM_SLOT_COLLIDE);
+        hipLaunchKernelGGL((k_collide<T, true>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, d.U, d.u,
+                           d.u_star, d.invalid, P);
+        CHECK_LAUNCH("k_collide");
+    }
     for (int k = 0; k < nsteps; ++k) {
         TimedSlot whole(h, VLBM_SLOT_STEP);
-        hipLaunchKernelGGL(k_reset_invalid, dim3(1), dim3(1), 0, h->stream, d.invalid);
-        {
-            TimedSlot ts(h, VLBM_SLOT_COLLIDE);
-            hipLaunchKernelGGL((k_collide<T, true>), dim3(gn), dim3(BLOCK), 0, h->stream, f0, d.U, d.u, d.u_star,
-                               d.invalid, P);
-            CHECK_LAUNCH("k_collide");
-        }
         const double tb = time + 0.5 * d.dt;
         const bool disp = d.disp_interval > 0 && steps % d.disp_interval == 0;
         const bool compat = d.compat_interval > 0 && steps % d.compat_interval == 0;
@@ -788,6 +889,7 @@ static int run_steps(vlbm3d_handle *h, T *f0, T *f1, double time, int64_t steps,
                                    d.active_k, d.comp_kind_by_id, d.mode_by_id, d.params_by_id, tb, P);
                 CHECK_LAUNCH("k_sweep");
                 if (s + 1 < d.disp_sweeps) {
+                    TimedSlot tc(h, VLBM_SLOT_COMMIT);
                     hipLaunchKernelGGL(k_commit, dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)d.u_next,
                                        d.u, d.u_star, d.n_active);
                     CHECK_LAUNCH("k_commit");
@@ -801,14 +903,20 @@ static int run_steps(vlbm3d_handle *h, T *f0, T *f1, double time, int64_t steps,
                                (double *)d.record, d.neighbor, (int)compat, (int)disp, P);
             CHECK_LAUNCH("k_record");
         }
-        {
-            TimedSlot ts(h, VLBM_SLOT_STREAM);
-            hipLaunchKernelGGL((k_stream<T>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, f1,
-                               (const double *)d.record, d.neighbor, P);
-            CHECK_LAUNCH("k_stream");
-        }
         int rc = run_boundary<T>(h, f0, f1, tb, 1);
         if (rc) return rc;
+        const bool next = k + 1 < nsteps;
+        if (next) hipLaunchKernelGGL(k_reset_invalid, dim3(1), dim3(1), 0, h->stream, d.invalid);
+        {
+            TimedSlot ts(h, VLBM_SLOT_STREAM);
+            if (next)
+                hipLaunchKernelGGL((k_stream<T, true>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, f1,
+                                   (const double *)d.record, d.neighbor, d.U, d.u, d.u_star, d.invalid, P);
+            else
+                hipLaunchKernelGGL((k_stream<T, false>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, f1,
+                                   (const double *)d.record, d.neighbor, d.U, d.u, d.u_star, d.invalid, P);
+            CHECK_LAUNCH("k_stream");
+        }
         T *tmp = f0;
         f0 = f1;
         f1 = tmp;
@@ -852,7 +960,9 @@ int vlbm3d_create(const vlbm3d_desc *desc, vlbm3d_handle **out) {
             if (!ptr) return fail_msg("vlbm3d_create: a required link pointer is null");
         if (desc->n_anchor_links > 0 && !desc->node_link_ids)
             return fail_msg("vlbm3d_create: node_link_ids is null");
-        if (desc->has_char && (!desc->char_indices || !desc->char_weights || !desc->char_state))
+        if (desc->has_char && (!desc->char_state ||
+                               (desc->n_char_links > 0 && (!desc->char_link || !desc->char_gidx || !desc->char_gw ||
+                                                           !desc->char_pidx || !desc->char_pw))))
             return fail_msg("vlbm3d_create: characteristic tables are null");
     }
     vlbm3d_handle *h = new vlbm3d_handle;

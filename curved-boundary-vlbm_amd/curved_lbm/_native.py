@@ -12,7 +12,7 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libvlbm.so"
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 PREC_F64 = 0
 PREC_F32 = 1
 
@@ -59,6 +59,7 @@ class Vlbm3dDesc(C.Structure):
         ("compat_blend", C.c_double),
         ("has_char", C.c_int32),
         ("n_anchor_links", C.c_int32),
+        ("n_char_links", C.c_int32),
         ("U", C.c_void_p),
         ("u", C.c_void_p),
         ("u_star", C.c_void_p),
@@ -83,8 +84,11 @@ class Vlbm3dDesc(C.Structure):
         ("link_nx", C.c_void_p),
         ("link_ny", C.c_void_p),
         ("link_nz", C.c_void_p),
-        ("char_indices", C.c_void_p),
-        ("char_weights", C.c_void_p),
+        ("char_link", C.c_void_p),
+        ("char_gidx", C.c_void_p),
+        ("char_gw", C.c_void_p),
+        ("char_pidx", C.c_void_p),
+        ("char_pw", C.c_void_p),
         ("char_state", C.c_void_p),
         ("kind_by_id", C.c_void_p),
         ("comp_kind_by_id", C.c_void_p),

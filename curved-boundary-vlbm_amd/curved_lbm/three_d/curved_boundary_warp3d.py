@@ -694,6 +694,26 @@ def constant_piola_params3d(material: WarpHyperelasticMaterial3D, F: np.ndarray)
     return tuple(float(x) for x in _first_piola_np(np.asarray(F, float).reshape(3, 3), material).reshape(-1))
 
 
+def compact_characteristic_tables(stencil) -> dict[str, np.ndarray]:
+    """Device layout of the characteristic stencils: only links that have a
+    stencil, gradient weights over the 48 fit samples and predictor weights
+    over the 16 probe samples (the other rows of the (4, 64, L) weights are
+    structurally zero)."""
+    idx = np.asarray(stencil["indices"])
+    w = np.asarray(stencil["weights"])
+    g = MAX_CHARACTERISTIC_SAMPLES - 16
+    links = np.flatnonzero(idx[0] >= 0).astype(np.int32)
+    if np.any(w[0, :g, links] != 0.0) or np.any(w[1:, g:, links] != 0.0):
+        raise ValueError("characteristic weights are not in gradient/predictor block form")
+    return {
+        "link": links,
+        "gidx": np.ascontiguousarray(idx[:g, links]),
+        "gw": np.ascontiguousarray(w[1:, :g][:, :, links]),
+        "pidx": np.ascontiguousarray(idx[g:, links]),
+        "pw": np.ascontiguousarray(w[0, g:][:, links]),
+    }
+
+
 def relative_l2_active(error, reference, mask):
     e, r, m = np.asarray(error), np.asarray(reference), np.asarray(mask, bool)
     return float(np.sqrt(np.sum(e[..., m] ** 2)) / max(np.sqrt(np.sum(r[..., m] ** 2)), np.finfo(float).eps))
@@ -863,12 +883,18 @@ class WarpSparseCurvedBoundaryLBM3D:
         self.node_link_off = dev_i(off)
         self.node_link_ids = dev_i(order if L else np.zeros(1, i32))
         if self.characteristic_stencil is not None:
-            self.char_indices = dev_i(self.characteristic_stencil["indices"])
-            self.char_weights = dev_f(self.characteristic_stencil["weights"])
+            comp = compact_characteristic_tables(self.characteristic_stencil)
+            self.n_char_links = int(comp["link"].size)
+            self.char_link = dev_i(comp["link"])
+            self.char_gidx = dev_i(comp["gidx"])
+            self.char_gw = dev_f(comp["gw"])
+            self.char_pidx = dev_i(comp["pidx"])
+            self.char_pw = dev_f(comp["pw"])
             self.char_state = DeviceArray.zeros((4, NCOMP, L), f64, dev)
         else:
-            self.char_indices = DeviceArray.zeros((1,), i32, dev)
-            self.char_weights = DeviceArray.zeros((1,), f64, dev)
+            self.n_char_links = 0
+            self.char_link = self.char_gidx = self.char_pidx = DeviceArray.zeros((1,), i32, dev)
+            self.char_gw = self.char_pw = DeviceArray.zeros((1,), f64, dev)
             self.char_state = DeviceArray.zeros((1, 1, 1), f64, dev)
         self.kind_by_id_wp = dev_i(self.kind_by_id)
         self.component_kind_by_id_wp = dev_i(self.component_kind_by_id)
@@ -918,6 +944,7 @@ class WarpSparseCurvedBoundaryLBM3D:
         d.compat_blend = self.local_compatibility_blend
         d.has_char = 1 if self.characteristic_stencil is not None else 0
         d.n_anchor_links = L
+        d.n_char_links = self.n_char_links
         ptrs = {
             "U": self.U, "u": self.u, "u_star": self.u_star, "u_next": self.u_local_next,
             "record": self.record, "invalid": self.invalid, "reduce_scratch": self._reduce_scratch,
@@ -926,7 +953,8 @@ class WarpSparseCurvedBoundaryLBM3D:
             "link_p": self.link_p, "link_up": self.link_upstream, "link_q": self.link_q,
             "link_bid": self.link_boundary_id, "link_eta": self.link_eta, "link_xb": self.link_xb,
             "link_yb": self.link_yb, "link_zb": self.link_zb, "link_nx": self.link_nx, "link_ny": self.link_ny,
-            "link_nz": self.link_nz, "char_indices": self.char_indices, "char_weights": self.char_weights,
+            "link_nz": self.link_nz, "char_link": self.char_link, "char_gidx": self.char_gidx,
+            "char_gw": self.char_gw, "char_pidx": self.char_pidx, "char_pw": self.char_pw,
             "char_state": self.char_state, "kind_by_id": self.kind_by_id_wp,
             "comp_kind_by_id": self.component_kind_by_id_wp, "mode_by_id": self.mode_by_id_wp,
             "params_by_id": self.params_by_id_wp, "dyn_enabled": self.dynamic_traction_override_enabled,
