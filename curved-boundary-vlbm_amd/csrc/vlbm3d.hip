@@ -406,45 +406,69 @@ __global__ __launch_bounds__(BLOCK) void k_stream(const T *__restrict__ f0, T *_
     }
 }
 
+// Node-major copy of U at the characteristic sample nodes: one 96-byte
+// record per sample instead of twelve plane-strided loads in the gather.
+__global__ __launch_bounds__(BLOCK) void k_pack(const double *__restrict__ U, const int32_t *__restrict__ nodes,
+                                                double *__restrict__ pack, int n, int ns) {
+    const int s = blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= ns) return;
+    const int p = nodes[s];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) pack[(size_t)s * NC + a] = U[(size_t)a * n + p];
+}
+
 // Characteristic state at each traction link: predictor value and gradient
-// of every state component (cbw3d:2676-2700).  Four lanes share a link and
-// each accumulates three components, so the stencil tables are read once per
-// link (the four lanes' table loads coalesce into one request).
-__global__ __launch_bounds__(BLOCK) void k_gather(const double *__restrict__ U, const int32_t *__restrict__ clink,
+// of every state component (cbw3d:2676-2700).  Two lanes share a link, each
+// accumulating six components from the packed sample records; the pair's
+// table loads coalesce and each sample costs one 96-byte line.
+__global__ __launch_bounds__(BLOCK) void k_gather(const double *__restrict__ pack, const int32_t *__restrict__ clink,
                                                   const int32_t *__restrict__ gidx, const double *__restrict__ gw,
                                                   const int32_t *__restrict__ pidx, const double *__restrict__ pw,
-                                                  double *__restrict__ cs, int n, int L, int Lc) {
+                                                  double *__restrict__ cs, int L, int Lc) {
     constexpr int NG = MAX_CHAR - 16;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
-    const int c = t >> 2;
-    const int a0 = 3 * (t & 3);
+    const int c = t >> 1;
+    const int a0 = 6 * (t & 1);
     if (c >= Lc) return;
-    const double *Ua = U + (size_t)a0 * n;
-    double val[3] = {0.0, 0.0, 0.0}, gx[3] = {0.0, 0.0, 0.0}, gy[3] = {0.0, 0.0, 0.0}, gz[3] = {0.0, 0.0, 0.0};
+    double val[6], gx[6], gy[6], gz[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) val[k] = gx[k] = gy[k] = gz[k] = 0.0;
     for (int m = 0; m < NG; ++m) {
-        const int p = gidx[(size_t)m * Lc + c];
-        if (p < 0) break;
+        const int s = gidx[(size_t)m * Lc + c];
+        if (s < 0) break;
         const double w1 = gw[((size_t)0 * NG + m) * Lc + c];
         const double w2 = gw[((size_t)1 * NG + m) * Lc + c];
         const double w3 = gw[((size_t)2 * NG + m) * Lc + c];
+        const double2 *rp = reinterpret_cast<const double2 *>(pack + (size_t)s * NC + a0);
+        double x[6];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const double x = Ua[(size_t)k * n + p];
-            gx[k] += w1 * x;
-            gy[k] += w2 * x;
-            gz[k] += w3 * x;
+            const double2 v = rp[k];
+            x[2 * k] = v.x;
+            x[2 * k + 1] = v.y;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            gx[k] += w1 * x[k];
+            gy[k] += w2 * x[k];
+            gz[k] += w3 * x[k];
         }
     }
     for (int m = 0; m < 16; ++m) {
-        const int p = pidx[(size_t)m * Lc + c];
-        if (p < 0) continue;
+        const int s = pidx[(size_t)m * Lc + c];
+        if (s < 0) continue;
         const double w0 = pw[(size_t)m * Lc + c];
+        const double2 *rp = reinterpret_cast<const double2 *>(pack + (size_t)s * NC + a0);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) val[k] += w0 * Ua[(size_t)k * n + p];
+        for (int k = 0; k < 3; ++k) {
+            const double2 v = rp[k];
+            val[2 * k] += w0 * v.x;
+            val[2 * k + 1] += w0 * v.y;
+        }
     }
     const int ell = clink[c];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 6; ++k) {
         const int a = a0 + k;
         cs[((size_t)0 * NC + a) * L + ell] = val[k];
         cs[((size_t)1 * NC + a) * L + ell] = gx[k];
@@ -539,10 +563,14 @@ __global__ __launch_bounds__(128) void k_links(const T *__restrict__ fsrc, T *__
     const double v[3] = {Tb.lbv[ell], Tb.lbv[L + ell], Tb.lbv[2 * L + ell]};
     const int kk[3] = {Tb.lck[ell], Tb.lck[L + ell], Tb.lck[2 * L + ell]};
     const double om1 = 1.0 - P.omega;
+    const bool char_rows = has_char && (kk[0] == BC_NEU || kk[1] == BC_NEU || kk[2] == BC_NEU);
     double outv[NC];
 #pragma unroll
     for (int a = 0; a < NC; ++a) {
         const int row = a < 3 ? a : (a - 3) / 3;
+        outv[a] = 0.0;
+        // Traction rows are overwritten by the characteristic closure below.
+        if (char_rows && kk[row] == BC_NEU) continue;
         double D = 1.0, S = 0.0;
         if (kk[row] == BC_DIR) {
             if (a < 3) {
@@ -580,8 +608,7 @@ __global__ __launch_bounds__(128) void k_links(const T *__restrict__ fsrc, T *__
         }
         outv[a] = val;
     }
-    const bool any_neu = kk[0] == BC_NEU || kk[1] == BC_NEU || kk[2] == BC_NEU;
-    if (has_char && any_neu) {
+    if (char_rows) {
         double Ub[NC], Gx[NC], Gy[NC], Gz[NC], inc[NC];
 #pragma unroll
         for (int a = 0; a < NC; ++a) {
@@ -835,9 +862,13 @@ static int run_boundary(vlbm3d_handle *h, const T *src, T *dst, double t, int us
     if (d.n_links == 0) return 0;
     if (d.has_char && d.n_char_links > 0) {
         TimedSlot ts(h, VLBM_SLOT_GATHER);
-        const int g = (int)(((size_t)d.n_char_links * 4 + BLOCK - 1) / BLOCK);
-        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BLOCK), 0, h->stream, (const double *)d.U, d.char_link,
-                           d.char_gidx, d.char_gw, d.char_pidx, d.char_pw, d.char_state, d.n_active, d.n_links,
+        const int gp = (d.n_char_nodes + BLOCK - 1) / BLOCK;
+        hipLaunchKernelGGL(k_pack, dim3(gp), dim3(BLOCK), 0, h->stream, (const double *)d.U, d.char_nodes,
+                           d.char_pack, d.n_active, d.n_char_nodes);
+        CHECK_LAUNCH("k_pack");
+        const int g = (int)(((size_t)d.n_char_links * 2 + BLOCK - 1) / BLOCK);
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BLOCK), 0, h->stream, (const double *)d.char_pack, d.char_link,
+                           d.char_gidx, d.char_gw, d.char_pidx, d.char_pw, d.char_state, d.n_links,
                            d.n_char_links);
         CHECK_LAUNCH("k_gather");
     }
@@ -962,7 +993,8 @@ int vlbm3d_create(const vlbm3d_desc *desc, vlbm3d_handle **out) {
             return fail_msg("vlbm3d_create: node_link_ids is null");
         if (desc->has_char && (!desc->char_state ||
                                (desc->n_char_links > 0 && (!desc->char_link || !desc->char_gidx || !desc->char_gw ||
-                                                           !desc->char_pidx || !desc->char_pw))))
+                                                           !desc->char_pidx || !desc->char_pw || !desc->char_nodes ||
+                                                           !desc->char_pack || desc->n_char_nodes <= 0))))
             return fail_msg("vlbm3d_create: characteristic tables are null");
     }
     vlbm3d_handle *h = new vlbm3d_handle;

@@ -12,7 +12,7 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libvlbm.so"
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 PREC_F64 = 0
 PREC_F32 = 1
 
@@ -60,6 +60,7 @@ class Vlbm3dDesc(C.Structure):
         ("has_char", C.c_int32),
         ("n_anchor_links", C.c_int32),
         ("n_char_links", C.c_int32),
+        ("n_char_nodes", C.c_int32),
         ("U", C.c_void_p),
         ("u", C.c_void_p),
         ("u_star", C.c_void_p),
@@ -85,6 +86,8 @@ class Vlbm3dDesc(C.Structure):
         ("link_ny", C.c_void_p),
         ("link_nz", C.c_void_p),
         ("char_link", C.c_void_p),
+        ("char_nodes", C.c_void_p),
+        ("char_pack", C.c_void_p),
         ("char_gidx", C.c_void_p),
         ("char_gw", C.c_void_p),
         ("char_pidx", C.c_void_p),

@@ -705,11 +705,16 @@ def compact_characteristic_tables(stencil) -> dict[str, np.ndarray]:
     links = np.flatnonzero(idx[0] >= 0).astype(np.int32)
     if np.any(w[0, :g, links] != 0.0) or np.any(w[1:, g:, links] != 0.0):
         raise ValueError("characteristic weights are not in gradient/predictor block form")
+    sub = idx[:, links]
+    nodes = np.unique(sub[sub >= 0]).astype(np.int32)
+    slot = np.searchsorted(nodes, np.maximum(sub, 0)).astype(np.int32)
+    slot = np.where(sub >= 0, slot, -1).astype(np.int32)
     return {
         "link": links,
-        "gidx": np.ascontiguousarray(idx[:g, links]),
+        "nodes": nodes,
+        "gidx": np.ascontiguousarray(slot[:g]),
         "gw": np.ascontiguousarray(w[1:, :g][:, :, links]),
-        "pidx": np.ascontiguousarray(idx[g:, links]),
+        "pidx": np.ascontiguousarray(slot[g:]),
         "pw": np.ascontiguousarray(w[0, g:][:, links]),
     }
 
@@ -885,15 +890,19 @@ class WarpSparseCurvedBoundaryLBM3D:
         if self.characteristic_stencil is not None:
             comp = compact_characteristic_tables(self.characteristic_stencil)
             self.n_char_links = int(comp["link"].size)
+            self.n_char_nodes = int(comp["nodes"].size)
             self.char_link = dev_i(comp["link"])
+            self.char_nodes = dev_i(comp["nodes"])
+            self.char_pack = DeviceArray.zeros((max(self.n_char_nodes, 1), NCOMP), f64, dev)
             self.char_gidx = dev_i(comp["gidx"])
             self.char_gw = dev_f(comp["gw"])
             self.char_pidx = dev_i(comp["pidx"])
             self.char_pw = dev_f(comp["pw"])
             self.char_state = DeviceArray.zeros((4, NCOMP, L), f64, dev)
         else:
-            self.n_char_links = 0
-            self.char_link = self.char_gidx = self.char_pidx = DeviceArray.zeros((1,), i32, dev)
+            self.n_char_links = self.n_char_nodes = 0
+            self.char_link = self.char_gidx = self.char_pidx = self.char_nodes = DeviceArray.zeros((1,), i32, dev)
+            self.char_pack = DeviceArray.zeros((1,), f64, dev)
             self.char_gw = self.char_pw = DeviceArray.zeros((1,), f64, dev)
             self.char_state = DeviceArray.zeros((1, 1, 1), f64, dev)
         self.kind_by_id_wp = dev_i(self.kind_by_id)
@@ -945,6 +954,7 @@ class WarpSparseCurvedBoundaryLBM3D:
         d.has_char = 1 if self.characteristic_stencil is not None else 0
         d.n_anchor_links = L
         d.n_char_links = self.n_char_links
+        d.n_char_nodes = self.n_char_nodes
         ptrs = {
             "U": self.U, "u": self.u, "u_star": self.u_star, "u_next": self.u_local_next,
             "record": self.record, "invalid": self.invalid, "reduce_scratch": self._reduce_scratch,
@@ -953,7 +963,8 @@ class WarpSparseCurvedBoundaryLBM3D:
             "link_p": self.link_p, "link_up": self.link_upstream, "link_q": self.link_q,
             "link_bid": self.link_boundary_id, "link_eta": self.link_eta, "link_xb": self.link_xb,
             "link_yb": self.link_yb, "link_zb": self.link_zb, "link_nx": self.link_nx, "link_ny": self.link_ny,
-            "link_nz": self.link_nz, "char_link": self.char_link, "char_gidx": self.char_gidx,
+            "link_nz": self.link_nz, "char_link": self.char_link, "char_nodes": self.char_nodes,
+            "char_pack": self.char_pack, "char_gidx": self.char_gidx,
             "char_gw": self.char_gw, "char_pidx": self.char_pidx, "char_pw": self.char_pw,
             "char_state": self.char_state, "kind_by_id": self.kind_by_id_wp,
             "comp_kind_by_id": self.component_kind_by_id_wp, "mode_by_id": self.mode_by_id_wp,

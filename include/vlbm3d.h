@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define VLBM3D_ABI_VERSION 2
+#define VLBM3D_ABI_VERSION 3
 #define VLBM3D_MAX_IDS 16
 #define VLBM3D_MAX_CHAR 64
 #define VLBM3D_RECORD 24
@@ -67,6 +67,7 @@ typedef struct {
     int32_t has_char;
     int32_t n_anchor_links; /* length of node_link_ids */
     int32_t n_char_links;   /* number of traction links with a characteristic stencil */
+    int32_t n_char_nodes;   /* number of distinct stencil sample nodes */
     /* state (device) */
     double *U, *u, *u_star, *u_next; /* (12,n), (3,n), (3,n), (3,n) */
     void *record;                    /* (24,n) deferred-collision record, double */
@@ -80,11 +81,13 @@ typedef struct {
     const int32_t *link_p, *link_up, *link_q, *link_bid;
     const double *link_eta, *link_xb, *link_yb, *link_zb, *link_nx, *link_ny, *link_nz;
     /* characteristic stencils of the traction links, compacted: sample
-     * lists are -1 padded after the last valid sample */
+     * lists hold indices into char_nodes, -1 padded after the last sample */
     const int32_t *char_link;        /* (Lc) link index of each stencil */
-    const int32_t *char_gidx;        /* (48,Lc) gradient sample nodes */
+    const int32_t *char_nodes;       /* (Ns) active node of each sample slot */
+    double *char_pack;               /* (Ns,12) node-major copy of U at the sample nodes */
+    const int32_t *char_gidx;        /* (48,Lc) gradient samples (into char_nodes) */
     const double *char_gw;           /* (3,48,Lc) x/y/z gradient weights */
-    const int32_t *char_pidx;        /* (16,Lc) predictor sample nodes */
+    const int32_t *char_pidx;        /* (16,Lc) predictor samples (into char_nodes) */
     const double *char_pw;           /* (16,Lc) predictor weights */
     double *char_state;              /* (4,12,L) predictor and gradient of U */
     const int32_t *kind_by_id;       /* (16) */
