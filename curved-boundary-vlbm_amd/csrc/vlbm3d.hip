@@ -283,10 +283,13 @@ __device__ __forceinline__ double du_axis(const double *u, const int32_t *nbr, s
 // Deformation-gradient update F <- F + theta (I + grad u - F) with the J
 // guard (cbw3d:2575-2672), folded with the BGK collision of cbw3d:2281-2316
 // into the per-node record, plus the last displacement commit.
+// The post-repair state goes to the U planes when write_U is set and, for
+// the characteristic sample nodes, straight into the gather's packed records.
 __global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const double *__restrict__ ugrad,
                                                   double *__restrict__ u, double *__restrict__ us,
                                                   double *__restrict__ rec, const int32_t *__restrict__ nbr,
-                                                  int repair, int commit, Params P) {
+                                                  const int32_t *__restrict__ pack_slot, double *__restrict__ pack,
+                                                  int repair, int commit, int write_U, Params P) {
     const int p = xcd_block() * BLOCK + threadIdx.x;
     if (p >= P.n) return;
     const size_t n = P.n;
@@ -295,8 +298,10 @@ __global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const 
     for (int k = 0; k < 3; ++k) v[k] = U[k * n + p];
 #pragma unroll
     for (int k = 0; k < 9; ++k) Fo[k] = U[(3 + k) * n + p];
-    double Po[9], dP[9], dF[9];
+    double Po[9], dP[9], dF[9], Fnew[9];
     piola(Fo, P.m, Po);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Fnew[k] = Fo[k];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         dP[k] = 0.0;
@@ -321,8 +326,21 @@ __global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const 
             for (int k = 0; k < 9; ++k) {
                 dP[k] = Pn[k] - Po[k];
                 dF[k] = Fn[k] - Fo[k];
-                U[(3 + k) * n + p] = Fn[k];
+                Fnew[k] = Fn[k];
             }
+            if (write_U)
+#pragma unroll
+                for (int k = 0; k < 9; ++k) U[(3 + k) * n + p] = Fn[k];
+        }
+    }
+    if (pack_slot) {
+        const int sl = pack_slot[p];
+        if (sl >= 0) {
+            double *pr = pack + (size_t)sl * NC;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) pr[k] = v[k];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) pr[3 + k] = Fnew[k];
         }
     }
     if (commit) {
@@ -426,7 +444,7 @@ __global__ __launch_bounds__(BLOCK) void k_gather(const double *__restrict__ pac
                                                   const int32_t *__restrict__ pidx, const double *__restrict__ pw,
                                                   double *__restrict__ cs, int L, int Lc) {
     constexpr int NG = MAX_CHAR - 16;
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    const int t = xcd_block() * BLOCK + threadIdx.x;
     const int c = t >> 1;
     const int a0 = 6 * (t & 1);
     if (c >= Lc) return;
@@ -856,17 +874,21 @@ static LinkTables make_tables(const vlbm3d_desc &d) {
         if (e_ != hipSuccess) return fail((where), e_);   \
     } while (0)
 
+// use_rec: the sources are pre-collision populations plus the per-node
+// records of this step, and k_record has already filled the gather pack.
 template <typename T>
 static int run_boundary(vlbm3d_handle *h, const T *src, T *dst, double t, int use_rec) {
     const vlbm3d_desc &d = h->d;
     if (d.n_links == 0) return 0;
     if (d.has_char && d.n_char_links > 0) {
         TimedSlot ts(h, VLBM_SLOT_GATHER);
-        const int gp = (d.n_char_nodes + BLOCK - 1) / BLOCK;
-        hipLaunchKernelGGL(k_pack, dim3(gp), dim3(BLOCK), 0, h->stream, (const double *)d.U, d.char_nodes,
-                           d.char_pack, d.n_active, d.n_char_nodes);
-        CHECK_LAUNCH("k_pack");
-        const int g = (int)(((size_t)d.n_char_links * 2 + BLOCK - 1) / BLOCK);
+        if (!use_rec) {
+            const int gp = (d.n_char_nodes + BLOCK - 1) / BLOCK;
+            hipLaunchKernelGGL(k_pack, dim3(gp), dim3(BLOCK), 0, h->stream, (const double *)d.U, d.char_nodes,
+                               d.char_pack, d.n_active, d.n_char_nodes);
+            CHECK_LAUNCH("k_pack");
+        }
+        const int g = xcd_grid((int)((size_t)d.n_char_links * 2));
         hipLaunchKernelGGL(k_gather, dim3(g), dim3(BLOCK), 0, h->stream, (const double *)d.char_pack, d.char_link,
                            d.char_gidx, d.char_gw, d.char_pidx, d.char_pw, d.char_state, d.n_links,
                            d.n_char_links);
@@ -929,9 +951,15 @@ static int run_steps(vlbm3d_handle *h, T *f0, T *f1, double time, int64_t steps,
         }
         {
             TimedSlot ts(h, VLBM_SLOT_RECORD);
+            // Within a call only the gather pack consumes the post-repair F of
+            // the local_f closure; the U planes get it on the call's last step.
+            const bool last = k + 1 == nsteps;
+            const bool pack = d.has_char && d.n_char_links > 0;
+            const int write_U = (last || d.recon != R_LOCALF || !pack) ? 1 : 0;
             hipLaunchKernelGGL(k_record, dim3(gn), dim3(BLOCK), 0, h->stream, d.U,
                                disp ? (const double *)d.u_next : (const double *)d.u, d.u, d.u_star,
-                               (double *)d.record, d.neighbor, (int)compat, (int)disp, P);
+                               (double *)d.record, d.neighbor, pack ? d.char_pack_slot : (const int32_t *)nullptr,
+                               d.char_pack, (int)compat, (int)disp, write_U, P);
             CHECK_LAUNCH("k_record");
         }
         int rc = run_boundary<T>(h, f0, f1, tb, 1);

@@ -88,6 +88,7 @@ class Vlbm3dDesc(C.Structure):
         ("char_link", C.c_void_p),
         ("char_nodes", C.c_void_p),
         ("char_pack", C.c_void_p),
+        ("char_pack_slot", C.c_void_p),
         ("char_gidx", C.c_void_p),
         ("char_gw", C.c_void_p),
         ("char_pidx", C.c_void_p),

@@ -694,7 +694,7 @@ def constant_piola_params3d(material: WarpHyperelasticMaterial3D, F: np.ndarray)
     return tuple(float(x) for x in _first_piola_np(np.asarray(F, float).reshape(3, 3), material).reshape(-1))
 
 
-def compact_characteristic_tables(stencil) -> dict[str, np.ndarray]:
+def compact_characteristic_tables(stencil, link_p=None) -> dict[str, np.ndarray]:
     """Device layout of the characteristic stencils: only links that have a
     stencil, gradient weights over the 48 fit samples and predictor weights
     over the 16 probe samples (the other rows of the (4, 64, L) weights are
@@ -705,6 +705,8 @@ def compact_characteristic_tables(stencil) -> dict[str, np.ndarray]:
     links = np.flatnonzero(idx[0] >= 0).astype(np.int32)
     if np.any(w[0, :g, links] != 0.0) or np.any(w[1:, g:, links] != 0.0):
         raise ValueError("characteristic weights are not in gradient/predictor block form")
+    # Spatial order (by link node) so that a block's stencils share samples.
+    links = links[np.argsort(np.asarray(link_p)[links], kind="stable")] if link_p is not None else links
     sub = idx[:, links]
     nodes = np.unique(sub[sub >= 0]).astype(np.int32)
     slot = np.searchsorted(nodes, np.maximum(sub, 0)).astype(np.int32)
@@ -888,12 +890,15 @@ class WarpSparseCurvedBoundaryLBM3D:
         self.node_link_off = dev_i(off)
         self.node_link_ids = dev_i(order if L else np.zeros(1, i32))
         if self.characteristic_stencil is not None:
-            comp = compact_characteristic_tables(self.characteristic_stencil)
+            comp = compact_characteristic_tables(self.characteristic_stencil, tables["link_p"])
             self.n_char_links = int(comp["link"].size)
             self.n_char_nodes = int(comp["nodes"].size)
             self.char_link = dev_i(comp["link"])
             self.char_nodes = dev_i(comp["nodes"])
             self.char_pack = DeviceArray.zeros((max(self.n_char_nodes, 1), NCOMP), f64, dev)
+            slot = np.full(n, -1, np.int32)
+            slot[comp["nodes"]] = np.arange(self.n_char_nodes, dtype=np.int32)
+            self.char_pack_slot = dev_i(slot)
             self.char_gidx = dev_i(comp["gidx"])
             self.char_gw = dev_f(comp["gw"])
             self.char_pidx = dev_i(comp["pidx"])
@@ -903,6 +908,7 @@ class WarpSparseCurvedBoundaryLBM3D:
             self.n_char_links = self.n_char_nodes = 0
             self.char_link = self.char_gidx = self.char_pidx = self.char_nodes = DeviceArray.zeros((1,), i32, dev)
             self.char_pack = DeviceArray.zeros((1,), f64, dev)
+            self.char_pack_slot = DeviceArray.zeros((1,), i32, dev)
             self.char_gw = self.char_pw = DeviceArray.zeros((1,), f64, dev)
             self.char_state = DeviceArray.zeros((1, 1, 1), f64, dev)
         self.kind_by_id_wp = dev_i(self.kind_by_id)
@@ -964,7 +970,7 @@ class WarpSparseCurvedBoundaryLBM3D:
             "link_bid": self.link_boundary_id, "link_eta": self.link_eta, "link_xb": self.link_xb,
             "link_yb": self.link_yb, "link_zb": self.link_zb, "link_nx": self.link_nx, "link_ny": self.link_ny,
             "link_nz": self.link_nz, "char_link": self.char_link, "char_nodes": self.char_nodes,
-            "char_pack": self.char_pack, "char_gidx": self.char_gidx,
+            "char_pack": self.char_pack, "char_pack_slot": self.char_pack_slot, "char_gidx": self.char_gidx,
             "char_gw": self.char_gw, "char_pidx": self.char_pidx, "char_pw": self.char_pw,
             "char_state": self.char_state, "kind_by_id": self.kind_by_id_wp,
             "comp_kind_by_id": self.component_kind_by_id_wp, "mode_by_id": self.mode_by_id_wp,

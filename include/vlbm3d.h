@@ -85,6 +85,7 @@ typedef struct {
     const int32_t *char_link;        /* (Lc) link index of each stencil */
     const int32_t *char_nodes;       /* (Ns) active node of each sample slot */
     double *char_pack;               /* (Ns,12) node-major copy of U at the sample nodes */
+    const int32_t *char_pack_slot;   /* (n) sample slot of each active node, -1 if none */
     const int32_t *char_gidx;        /* (48,Lc) gradient samples (into char_nodes) */
     const double *char_gw;           /* (3,48,Lc) x/y/z gradient weights */
     const int32_t *char_pidx;        /* (16,Lc) predictor samples (into char_nodes) */
