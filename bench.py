@@ -46,6 +46,11 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=40, help="oracle steps timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--chunk", type=int, default=500, help="steps per library call")
+    ap.add_argument("--equilibrium", action="store_true",
+                    help="run FP64 and FP32 from rest to the rms_v tolerance and report wall-s, steps and FP32 error")
+    ap.add_argument("--tol", type=float, default=1e-9, help="equilibrium residual rms_v tolerance")
+    ap.add_argument("--check-every", type=int, default=200)
+    ap.add_argument("--max-steps", type=int, default=60000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy a MIB-sized device buffer twice (reference kernel for counter scaling)")
     args = ap.parse_args()
@@ -59,6 +64,9 @@ def main() -> int:
 
     from curved_lbm.three_d import curved_boundary_warp3d as cb
     import gpu_cases
+
+    if args.equilibrium:
+        return equilibrium(args)
 
     t_setup = time.time()
     g, kw = gpu_cases.traction_shell(args.n, traction=0.03, ramp_steps=4000)
@@ -182,6 +190,59 @@ def main() -> int:
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+def equilibrium(args) -> int:
+    """Wall-seconds from rest to rms_v <= tol (the equilibrium residual),
+    FP64 populations vs FP32 populations, on the same shell; FP32 error in
+    u, F and J relative to the FP64 solution at equilibrium."""
+    import numpy as np
+    import torch
+
+    from curved_lbm.three_d import curved_boundary_warp3d as cb
+    import gpu_cases
+
+    g, kw = gpu_cases.traction_shell(args.n, traction=0.03, ramp_steps=4000)
+    ramp_time = 4000 * g.dx / kw["lattice_speed"]
+    res = {}
+    states = {}
+    for prec in ("fp64", "fp32"):
+        s = cb.WarpCurvedBoundaryLBM3D(g, **kw, precision=prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = s.run_to_equilibrium(args.tol, check_every=args.check_every, min_time=ramp_time,
+                                   max_steps=args.max_steps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        s.refresh_current()
+        U, u = s.U.numpy(), s.u.numpy()
+        states[prec] = (U, u)
+        res[prec] = {"converged": out["converged"], "steps": out["steps"], "wall_s": round(wall, 3),
+                     "displacement_rate": out["displacement_rate"], "rms_v": out["rms_v"],
+                     "min_J": out["min_J"], "max_abs_u": out["max_abs_u"],
+                     "history_every_10th": [(int(a), float(b), float(c)) for a, b, c in out["history"][::10]]}
+        print(f"[equilibrium] {prec}: {res[prec]}", file=sys.stderr, flush=True)
+        del s
+    (U64, u64), (U32, u32) = states["fp64"], states["fp32"]
+    J64 = cb._det3_np(U64[3:].reshape(3, 3, -1))
+    J32 = cb._det3_np(U32[3:].reshape(3, 3, -1))
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    eye = np.eye(3).reshape(9, 1)
+    err = {"u_rel_l2": rel(u32, u64), "F_rel_l2": rel(U32[3:], U64[3:]),
+           "F_minus_I_rel_l2": rel(U32[3:] - eye, U64[3:] - eye), "J_max_abs": float(np.max(np.abs(J32 - J64)))}
+    out = {
+        "metric": "wall-s to equilibrium (rms displacement rate <= %.0e), D3Q6x12 shell %d^3" % (args.tol, args.n),
+        "fp64": res["fp64"], "fp32": res["fp32"],
+        "speedup_fp32_vs_fp64": round(res["fp64"]["wall_s"] / res["fp32"]["wall_s"], 3),
+        "fp32_error_vs_fp64": err,
+        "config": {"lattice": [args.n] * 3, "n_active": int(g.n_active), "check_every": args.check_every,
+                   "min_time": ramp_time},
     }
     print(json.dumps(out))
     return 0

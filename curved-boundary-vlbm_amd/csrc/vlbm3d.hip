@@ -707,6 +707,7 @@ __global__ void k_pop_from_f64(const double *__restrict__ s, float *__restrict__
 }
 
 // Deterministic two-pass reductions over U and u.
+constexpr int NRED = 7;
 __device__ __forceinline__ void red_combine(double *acc, const double *o) {
     acc[0] += o[0];
     acc[1] = fmax(acc[1], o[1]);
@@ -714,12 +715,13 @@ __device__ __forceinline__ void red_combine(double *acc, const double *o) {
     acc[3] = fmax(acc[3], o[3]);
     acc[4] = fmax(acc[4], o[4]);
     acc[5] += o[5];
+    acc[6] += o[6];
 }
 
-__device__ inline void block_reduce(double *acc, double (*sh)[6]) {
+__device__ inline void block_reduce(double *acc, double (*sh)[NRED]) {
     const int t = threadIdx.x;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) sh[t][k] = acc[k];
+    for (int k = 0; k < NRED; ++k) sh[t][k] = acc[k];
     __syncthreads();
     for (int s = BLOCK / 2; s > 0; s >>= 1) {
         if (t < s) red_combine(sh[t], sh[t + s]);
@@ -728,9 +730,10 @@ __device__ inline void block_reduce(double *acc, double (*sh)[6]) {
 }
 
 __global__ __launch_bounds__(BLOCK) void k_reduce_partial(const double *__restrict__ U, const double *__restrict__ u,
+                                                          const double *__restrict__ uprev,
                                                           double *__restrict__ part, int n) {
-    __shared__ double sh[BLOCK][6];
-    double acc[6] = {0.0, 0.0, INFINITY, -INFINITY, 0.0, 0.0};
+    __shared__ double sh[BLOCK][NRED];
+    double acc[NRED] = {0.0, 0.0, INFINITY, -INFINITY, 0.0, 0.0, 0.0};
     for (int p = blockIdx.x * BLOCK + threadIdx.x; p < n; p += gridDim.x * BLOCK) {
         double Uv[NC];
         double nonfinite = 0.0;
@@ -739,36 +742,40 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_partial(const double *__restri
             Uv[a] = U[(size_t)a * n + p];
             if (!isfinite(Uv[a])) nonfinite += 1.0;
         }
-        double uu = 0.0;
+        double uu = 0.0, du = 0.0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const double x = u[(size_t)c * n + p];
             if (!isfinite(x)) nonfinite += 1.0;
             uu += x * x;
+            if (uprev) {
+                const double dd = x - uprev[(size_t)c * n + p];
+                du += dd * dd;
+            }
         }
         const double v2 = Uv[0] * Uv[0] + Uv[1] * Uv[1] + Uv[2] * Uv[2];
         const double J = det3(Uv + 3);
-        const double o[6] = {v2, sqrt(v2), J, J, sqrt(uu), nonfinite};
+        const double o[NRED] = {v2, sqrt(v2), J, J, sqrt(uu), nonfinite, du};
         red_combine(acc, o);
     }
     block_reduce(acc, sh);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < 6; ++k) part[blockIdx.x * 8 + k] = sh[0][k];
+        for (int k = 0; k < NRED; ++k) part[blockIdx.x * 8 + k] = sh[0][k];
 }
 
 __global__ __launch_bounds__(BLOCK) void k_reduce_final(double *__restrict__ part, int nparts,
                                                         const int32_t *__restrict__ invalid,
                                                         double *__restrict__ out) {
-    __shared__ double sh[BLOCK][6];
-    double acc[6] = {0.0, 0.0, INFINITY, -INFINITY, 0.0, 0.0};
+    __shared__ double sh[BLOCK][NRED];
+    double acc[NRED] = {0.0, 0.0, INFINITY, -INFINITY, 0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < nparts; b += BLOCK) red_combine(acc, part + b * 8);
     block_reduce(acc, sh);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) out[k] = sh[0][k];
         out[6] = (double)invalid[0];
-        out[7] = 0.0;
+        out[7] = sh[0][6];
     }
 }
 
@@ -1142,7 +1149,7 @@ int vlbm3d_set_ustar(vlbm3d_handle *h, const double *disp) {
     return 0;
 }
 
-int vlbm3d_reduce(vlbm3d_handle *h, double *out) {
+int vlbm3d_reduce(vlbm3d_handle *h, const double *u_prev, double *out) {
     if (!h || !out) return fail_msg("vlbm3d_reduce: null argument");
     const vlbm3d_desc &d = h->d;
     int nb = (d.n_active + BLOCK - 1) / BLOCK;
@@ -1150,7 +1157,7 @@ int vlbm3d_reduce(vlbm3d_handle *h, double *out) {
     double *part = d.reduce_scratch;
     double *res = d.reduce_scratch + (size_t)RED_BLOCKS * 8;
     hipLaunchKernelGGL(k_reduce_partial, dim3(nb), dim3(BLOCK), 0, h->stream, (const double *)d.U,
-                       (const double *)d.u, part, d.n_active);
+                       (const double *)d.u, u_prev, part, d.n_active);
     CHECK_LAUNCH("k_reduce_partial");
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(BLOCK), 0, h->stream, part, nb, (const int32_t *)d.invalid, res);
     CHECK_LAUNCH("k_reduce_final");

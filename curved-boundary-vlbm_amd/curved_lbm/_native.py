@@ -12,11 +12,11 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libvlbm.so"
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 PREC_F64 = 0
 PREC_F32 = 1
 
-RED_SUM_V2, RED_MAX_V, RED_MIN_J, RED_MAX_J, RED_MAX_U, RED_NONFINITE, RED_INVALID = range(7)
+RED_SUM_V2, RED_MAX_V, RED_MIN_J, RED_MAX_J, RED_MAX_U, RED_NONFINITE, RED_INVALID, RED_SUM_DU2 = range(8)
 RED_COUNT = 8
 RED_SCRATCH = 1024 * 8 + 8
 
@@ -155,7 +155,7 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vlbm3d_apply_boundary": [vp, vp, vp, C.c_double, C.c_int32],
         "vlbm3d_init_populations": [vp, vp],
         "vlbm3d_set_ustar": [vp, vp],
-        "vlbm3d_reduce": [vp, _dp],
+        "vlbm3d_reduce": [vp, vp, _dp],
         "vlbm3d_populations_to_f64": [vp, vp, vp],
         "vlbm3d_populations_from_f64": [vp, vp, vp],
         "vlbm3d_timing_enable": [vp, C.c_int32],

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define VLBM3D_ABI_VERSION 3
+#define VLBM3D_ABI_VERSION 4
 #define VLBM3D_MAX_IDS 16
 #define VLBM3D_MAX_CHAR 64
 #define VLBM3D_RECORD 24
@@ -111,6 +111,7 @@ typedef struct {
 #define VLBM_RED_MAX_U 4    /* max |u| */
 #define VLBM_RED_NONFINITE 5 /* count of non-finite entries of U and u */
 #define VLBM_RED_INVALID 6  /* device invalid flag */
+#define VLBM_RED_SUM_DU2 7  /* sum over active nodes of |u - u_prev|^2 (0 when u_prev is NULL) */
 #define VLBM_RED_COUNT 8
 
 int vlbm3d_create(const vlbm3d_desc *desc, vlbm3d_handle **out);
@@ -130,8 +131,10 @@ int vlbm3d_apply_boundary(vlbm3d_handle *h, void *f_src, void *f_dst, double t, 
 int vlbm3d_init_populations(vlbm3d_handle *h, void *f);
 /* u_star = disp - dt/2 v from a (3,n) device array. */
 int vlbm3d_set_ustar(vlbm3d_handle *h, const double *disp);
-/* Synchronous reductions over U and u (see VLBM_RED_*); out has 8 slots. */
-int vlbm3d_reduce(vlbm3d_handle *h, double *out);
+/* Synchronous reductions over U and u (see VLBM_RED_*); out has 8 slots.
+ * u_prev (nullable, (3,n) device) is an earlier copy of u for the
+ * displacement-rate equilibrium residual. */
+int vlbm3d_reduce(vlbm3d_handle *h, const double *u_prev, double *out);
 /* Converts populations between the stored precision and double planes. */
 int vlbm3d_populations_to_f64(vlbm3d_handle *h, const void *f, double *dst);
 int vlbm3d_populations_from_f64(vlbm3d_handle *h, const double *src, void *f);
