@@ -1,0 +1,1065 @@
+// libvlbm: D2Q4x6 vectorial LBM time step for MI355X (gfx950).
+//
+// Storage is the reference's dense (components, nx, ny) layout, node index
+// p = i*ny + j, so the +-y neighbours of a node are adjacent in memory and
+// the +-x neighbours one row away.  Every per-node kernel runs over the
+// compacted list of active nodes, so the inactive part of the box is never
+// read or written.
+//
+// One step (cbw2d:4180-4256) runs as
+//   k2_moments  moments of f0 (U, u, u*), J flag                     first step of a call only
+//   k2_sweep    local displacement sweep into u_next                 per node   [if due]
+//   k2_commit   intermediate sweep commits                           per node   [sweeps > 1]
+//   k2_record   F repair + deferred-collision record + final commit  per node
+//   k2_links    boundary value, predictor, normal-traction Newton,
+//               BFL / characteristic populations of every cut link   per link
+//   k2_stream   pull stream fused with the next step's moments       per node
+// Post-collision populations are never materialised: k2_record stores 12
+// doubles per node from which fpost[q][a] = (1 - omega) f[q][a] + A[a] +
+// s_q B[axis(q)][a] follows exactly (including the equilibrium change of the
+// F repair), and k2_stream / k2_links apply it while pulling.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "../../include/vlbm2d.h"
+#include "vlbm2d_device.h"
+#include "vlbm_common.h"
+
+using namespace vlbm2;
+using vlbm_common::BLOCK;
+using vlbm_common::fail;
+using vlbm_common::fail_msg;
+using vlbm_common::TimedSlot;
+using vlbm_common::Timing;
+using vlbm_common::xcd_block;
+using vlbm_common::xcd_grid;
+
+namespace {
+
+constexpr int LBLOCK = 128;
+constexpr int RED_BLOCKS = 1024;
+
+// Record layout (12 planes): A[6], B_v[r*2+axis] (4), B_F[r] (2).
+enum { R_A = 0, R_BV = 6, R_BF = 10 };
+
+struct Params2 {
+    int nx, ny, na, L;
+    size_t N;
+    Material2 m;
+    double dt, c, omega, dx, dy;
+    int src_mode;
+    double src[6];
+    int recon, solver;
+    double disp_relax, w2;
+    double blend;
+    int interior_only;
+    int use_dyn;
+};
+
+struct Links2 {
+    const int32_t *lp, *lq, *lbid;
+    const double *eta, *xb, *yb, *nx, *ny;
+    const int32_t *lsq_p;
+    const double *wb, *wx, *wy;
+    const int32_t *valid;
+    const int32_t *kind_by_id, *mode_by_id;
+    const double *prm_by_id;
+    const double *dyn_tx, *dyn_ty;
+    const int32_t *act;
+};
+
+// Population access.  double planes hold f; float planes hold f - f_rest
+// where f_rest[q][a] = 1/4 on the diagonal of F (a = 2, 5), 0 elsewhere.
+__device__ __forceinline__ double rest(int a) { return (a == 2 || a == 5) ? 0.25 : 0.0; }
+__device__ __forceinline__ double ld(const double *f, size_t i, int) { return f[i]; }
+__device__ __forceinline__ double ld(const float *f, size_t i, int a) { return (double)f[i] + rest(a); }
+__device__ __forceinline__ void st(double *f, size_t i, int, double v) { f[i] = v; }
+__device__ __forceinline__ void st(float *f, size_t i, int a, double v) { f[i] = (float)(v - rest(a)); }
+__device__ __forceinline__ double ld_nt(const double *f, size_t i, int) { return __builtin_nontemporal_load(f + i); }
+__device__ __forceinline__ double ld_nt(const float *f, size_t i, int a) {
+    return (double)__builtin_nontemporal_load(f + i) + rest(a);
+}
+// Store v and return the value as stored (rounded to the plane precision).
+__device__ __forceinline__ double st_nt(double *f, size_t i, int, double v) {
+    __builtin_nontemporal_store(v, f + i);
+    return v;
+}
+__device__ __forceinline__ double st_nt(float *f, size_t i, int a, double v) {
+    const float x = (float)(v - rest(a));
+    __builtin_nontemporal_store(x, f + i);
+    return (double)x + rest(a);
+}
+
+// fpost[q][a] at node p from its pre-collision value fv and the node's record.
+__device__ __forceinline__ double fpost_v(double fv, const double *rec, size_t N, int q, int a, int p, double om1) {
+    const int axis = q & 1;
+    const double s = q < 2 ? 1.0 : -1.0;
+    double b = 0.0;
+    if (a < 2) {
+        b = rec[(size_t)(R_BV + 2 * a + axis) * N + p];
+    } else if (((a - 2) & 1) == axis) {
+        b = rec[(size_t)(R_BF + (a - 2) / 2) * N + p];
+    }
+    return om1 * fv + rec[(size_t)(R_A + a) * N + p] + s * b;
+}
+
+__global__ void k2_reset_invalid(int32_t *invalid) { invalid[0] = 0; }
+
+// Moments with the half-step source; u = u* + dt/2 v and (collide only)
+// u* = u + dt/2 v; J flag.  cbw2d:1692-1790 and :3128-3195.
+template <typename T, bool COLLIDE>
+__global__ __launch_bounds__(BLOCK) void k2_moments(const T *__restrict__ f, double *__restrict__ U,
+                                                    double *__restrict__ u, double *__restrict__ us,
+                                                    int32_t *__restrict__ invalid,
+                                                    const int32_t *__restrict__ list, Params2 P) {
+    const int t = xcd_block() * BLOCK + threadIdx.x;
+    if (t >= P.na) return;
+    const int p = list[t];
+    const size_t N = P.N;
+    double Um[NC];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) Um[a] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int a = 0; a < NC; ++a) Um[a] += ld(f, (size_t)(q * 6 + a) * N + p, a);
+    const double hdt = 0.5 * P.dt;
+    if (P.src_mode == SRC_CONST) {
+#pragma unroll
+        for (int a = 0; a < NC; ++a) Um[a] += hdt * P.src[a];
+    } else if (P.src_mode == SRC_DAMP) {
+        const double den = 1.0 + hdt * P.src[0];
+        Um[0] = Um[0] / den;
+        Um[1] = Um[1] / den;
+    }
+#pragma unroll
+    for (int a = 0; a < NC; ++a) U[a * N + p] = Um[a];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const double uc = us[c * N + p] + hdt * Um[c];
+        u[c * N + p] = uc;
+        if (COLLIDE) us[c * N + p] = uc + hdt * Um[c];
+    }
+    if (Um[2] * Um[5] - Um[3] * Um[4] <= P.m.jf) atomicOr(invalid, 1);
+}
+
+// One relaxed nearest-neighbour compatibility sweep of u (cbw2d:3421-3490)
+// with the Dirichlet cut-point anchors of the node's own links
+// (cbw2d:3288-3417) summed in link order, so the sums are deterministic.
+__global__ __launch_bounds__(BLOCK) void k2_sweep(const double *__restrict__ U, const double *__restrict__ u,
+                                                  double *__restrict__ un, const int32_t *__restrict__ list,
+                                                  const int32_t *__restrict__ act, const int32_t *__restrict__ aoff,
+                                                  const int32_t *__restrict__ aids, Links2 T, double time,
+                                                  Params2 P) {
+    const int t = xcd_block() * BLOCK + threadIdx.x;
+    if (t >= P.na) return;
+    const int p = list[t];
+    const int ny = P.ny;
+    const int i = p / ny, j = p - i * ny;
+    const size_t N = P.N;
+    const double *U2 = U + 2 * N, *U3 = U + 3 * N, *U4 = U + 4 * N, *U5 = U + 5 * N;
+    const double *u0 = u, *u1 = u + N;
+    double diag = 0.0, rx = 0.0, ry = 0.0;
+    const int l0 = aoff[t], l1 = aoff[t + 1];
+    for (int l = l0; l < l1; ++l) {
+        const int ell = aids[l];
+        const int bid = T.lbid[ell];
+        const int mode = T.mode_by_id[bid];
+        const double *prm = T.prm_by_id + bid * 8;
+        const double xb = T.xb[ell], yb = T.yb[ell], nxb = T.nx[ell], nyb = T.ny[ell];
+        double ub0 = 0.0, ub1 = 0.0;
+        if (mode == V_CONST) {
+            ub0 = time * value2(0, mode, prm, xb, yb, nxb, nyb, 0.0);
+            ub1 = time * value2(1, mode, prm, xb, yb, nxb, nyb, 0.0);
+        } else if (mode == V_AFFINE_VEL) {
+            double at = time;
+            if (prm[6] > 0.0 && at > prm[6]) at = prm[6];
+            ub0 = at * value2(0, mode, prm, xb, yb, nxb, nyb, 0.0);
+            ub1 = at * value2(1, mode, prm, xb, yb, nxb, nyb, 0.0);
+        }
+        const double dbx = xb - ((double)i + 0.5) * P.dx;
+        const double dby = yb - ((double)j + 0.5) * P.dy;
+        const double tx = ub0 - ((U2[p] - 1.0) * dbx + U3[p] * dby);
+        const double ty = ub1 - (U4[p] * dbx + (U5[p] - 1.0) * dby);
+        diag += P.w2;
+        rx += P.w2 * tx;
+        ry += P.w2 * ty;
+    }
+    const double dx = P.dx, dy = P.dy;
+    if (i + 1 < P.nx && act[p + ny] != 0) {
+        const int q = p + ny;
+        const double gx = 0.5 * dx * ((U2[p] - 1.0) + (U2[q] - 1.0));
+        const double gy = 0.5 * dx * (U4[p] + U4[q]);
+        rx += u0[q] - gx;
+        ry += u1[q] - gy;
+        diag += 1.0;
+    }
+    if (i > 0 && act[p - ny] != 0) {
+        const int q = p - ny;
+        const double gx = 0.5 * dx * ((U2[q] - 1.0) + (U2[p] - 1.0));
+        const double gy = 0.5 * dx * (U4[q] + U4[p]);
+        rx += u0[q] + gx;
+        ry += u1[q] + gy;
+        diag += 1.0;
+    }
+    if (j + 1 < ny && act[p + 1] != 0) {
+        const int q = p + 1;
+        const double gx = 0.5 * dy * (U3[p] + U3[q]);
+        const double gy = 0.5 * dy * ((U5[p] - 1.0) + (U5[q] - 1.0));
+        rx += u0[q] - gx;
+        ry += u1[q] - gy;
+        diag += 1.0;
+    }
+    if (j > 0 && act[p - 1] != 0) {
+        const int q = p - 1;
+        const double gx = 0.5 * dy * (U3[q] + U3[p]);
+        const double gy = 0.5 * dy * ((U5[q] - 1.0) + (U5[p] - 1.0));
+        rx += u0[q] + gx;
+        ry += u1[q] + gy;
+        diag += 1.0;
+    }
+    const double a0 = u0[p], a1 = u1[p];
+    if (diag > 0.0) {
+        un[p] = a0 + P.disp_relax * (rx / diag - a0);
+        un[N + p] = a1 + P.disp_relax * (ry / diag - a1);
+    } else {
+        un[p] = a0;
+        un[N + p] = a1;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k2_commit(const double *__restrict__ un, double *__restrict__ u,
+                                                   double *__restrict__ us, const int32_t *__restrict__ list,
+                                                   int na, size_t N) {
+    const int t = xcd_block() * BLOCK + threadIdx.x;
+    if (t >= na) return;
+    const int p = list[t];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const size_t k = (size_t)c * N + p;
+        const double delta = un[k] - u[k];
+        u[k] = un[k];
+        us[k] = us[k] + delta;
+    }
+}
+
+// Centred / second-order one-sided / first-order one-sided derivative of one
+// displacement component plane along x (axis 0) or y (axis 1) over active
+// nodes (cbw2d:3213-3272).
+__device__ __forceinline__ double du2(const double *uc, const int32_t *act, int p, int i, int j, int axis,
+                                      const Params2 &P) {
+    const int n = axis == 0 ? P.nx : P.ny;
+    const int k = axis == 0 ? i : j;
+    const int s = axis == 0 ? P.ny : 1;
+    const double h = axis == 0 ? P.dx : P.dy;
+    if (k > 0 && k + 1 < n && act[p - s] != 0 && act[p + s] != 0) return (uc[p + s] - uc[p - s]) / (2.0 * h);
+    if (k + 1 < n && act[p + s] != 0) {
+        if (k + 2 < n && act[p + 2 * s] != 0) return (-3.0 * uc[p] + 4.0 * uc[p + s] - uc[p + 2 * s]) / (2.0 * h);
+        return (uc[p + s] - uc[p]) / h;
+    }
+    if (k > 0 && act[p - s] != 0) {
+        if (k > 1 && act[p - 2 * s] != 0) return (3.0 * uc[p] - 4.0 * uc[p - s] + uc[p - 2 * s]) / (2.0 * h);
+        return (uc[p] - uc[p - s]) / h;
+    }
+    return 0.0;
+}
+
+// Deformation-gradient update F <- F + alpha (I + grad u - F) with the
+// halving J guard (cbw2d:3522-3615), folded with the BGK collision of
+// cbw2d:1820-1843 into the per-node record, plus the last displacement
+// commit.  ug is the displacement the repair differentiates (the sweep's
+// u_next when a sweep ran, else u).
+__global__ __launch_bounds__(BLOCK) void k2_record(double *__restrict__ U, const double *__restrict__ ug,
+                                                   double *__restrict__ u, double *__restrict__ us,
+                                                   double *__restrict__ rec, const int32_t *__restrict__ list,
+                                                   const int32_t *__restrict__ act, int repair, int commit,
+                                                   Params2 P) {
+    const int t = xcd_block() * BLOCK + threadIdx.x;
+    if (t >= P.na) return;
+    const int p = list[t];
+    const int ny = P.ny, nx = P.nx;
+    const int i = p / ny, j = p - i * ny;
+    const size_t N = P.N;
+    const double v0 = U[p], v1 = U[N + p];
+    double Fo[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) Fo[k] = U[(2 + k) * N + p];
+    double Po[4], dP[4] = {0.0, 0.0, 0.0, 0.0}, dF[4] = {0.0, 0.0, 0.0, 0.0};
+    piola2(Fo, P.m, Po);
+    bool do_rep = repair != 0;
+    if (do_rep && P.interior_only) {
+        if (i == 0 || i + 1 >= nx || j == 0 || j + 1 >= ny)
+            do_rep = false;
+        else if (act[p - ny] == 0 || act[p + ny] == 0 || act[p - 1] == 0 || act[p + 1] == 0)
+            do_rep = false;
+    }
+    if (do_rep) {
+        const double tt[4] = {1.0 + du2(ug, act, p, i, j, 0, P), du2(ug, act, p, i, j, 1, P),
+                              du2(ug + N, act, p, i, j, 0, P), 1.0 + du2(ug + N, act, p, i, j, 1, P)};
+        double alpha = P.blend;
+        alpha = alpha < 0.0 ? 0.0 : (alpha > 1.0 ? 1.0 : alpha);
+        double Fn[4];
+        for (int g = 0; g < 10; ++g) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) Fn[k] = Fo[k] + alpha * (tt[k] - Fo[k]);
+            if (Fn[0] * Fn[3] - Fn[1] * Fn[2] > P.m.jf) break;
+            alpha = 0.5 * alpha;
+        }
+        double Pn[4];
+        piola2(Fn, P.m, Pn);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            dP[k] = Pn[k] - Po[k];
+            dF[k] = Fn[k] - Fo[k];
+            U[(2 + k) * N + p] = Fn[k];
+        }
+    }
+    if (commit) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const size_t k = (size_t)c * N + p;
+            const double delta = ug[k] - u[k];
+            u[k] = ug[k];
+            us[k] = us[k] + delta;
+        }
+    }
+    const double w = P.omega;
+    const double st = P.dt * (2.0 - w) * 0.125;
+    const double w4 = 0.25 * w, w2c = w / (2.0 * P.c), h2c = 0.5 / P.c;
+    rec[(R_A + 0) * N + p] = w4 * v0 + st * source2(0, P.src_mode, P.src, v0, v1);
+    rec[(R_A + 1) * N + p] = w4 * v1 + st * source2(1, P.src_mode, P.src, v0, v1);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        rec[(R_A + 2 + k) * N + p] = w4 * Fo[k] + st * source2(2 + k, P.src_mode, P.src, v0, v1) + 0.25 * dF[k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) rec[(R_BV + k) * N + p] = -w2c * Po[k] - h2c * dP[k];
+    rec[(R_BF + 0) * N + p] = -w2c * v0;
+    rec[(R_BF + 1) * N + p] = -w2c * v1;
+}
+
+// Quadratic one-sided reconstruction of U[comp] along the inward link
+// coordinate (cbw2d:1443-1480).
+__device__ inline double interp_link(const double *U, const int32_t *act, size_t N, int comp, int p, int i, int j,
+                                     int qo, double sv, const Params2 &P) {
+    const double *Uc = U + comp * N;
+    const double v0 = Uc[p];
+    const int i1 = i - cx(qo), j1 = j - cy(qo);
+    if (i1 >= 0 && i1 < P.nx && j1 >= 0 && j1 < P.ny && act[i1 * P.ny + j1] != 0) {
+        const double v1 = Uc[i1 * P.ny + j1];
+        const int i2 = i - 2 * cx(qo), j2 = j - 2 * cy(qo);
+        if (i2 >= 0 && i2 < P.nx && j2 >= 0 && j2 < P.ny && act[i2 * P.ny + j2] != 0) {
+            const double v2 = Uc[i2 * P.ny + j2];
+            const double l0 = 0.5 * (sv + 1.0) * (sv + 2.0);
+            const double l1 = -sv * (sv + 2.0);
+            const double l2 = 0.5 * sv * (sv + 1.0);
+            return l0 * v0 + l1 * v1 + l2 * v2;
+        }
+        return (sv + 1.0) * v0 - sv * v1;
+    }
+    return v0;
+}
+
+// Normal-probe / LSQ weighted sum of U[comp] over a link's stencil (cbw2d:1202-1221).
+__device__ __forceinline__ double lsq_sum(const double *U, size_t N, const int32_t *lsq_p, const double *w, int L,
+                                          int ell, int comp) {
+    double out = 0.0;
+    for (int m = 0; m < MAX_LSQ; ++m) {
+        const int pp = lsq_p[(size_t)m * L + ell];
+        if (pp >= 0) out += w[(size_t)m * L + ell] * U[comp * N + pp];
+    }
+    return out;
+}
+
+// Incoming populations of every cut link (cbw2d:2064-3098 for the bfl,
+// lsq_bfl, local_f and local_f_bfl reconstructions).  use_rec selects
+// post-collision sources (time step: pre-collision f plus this step's
+// records) or populations read as they are (standalone reconstruction).
+template <typename T>
+__global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T *__restrict__ fdst,
+                                                   const double *__restrict__ rec, const double *__restrict__ U,
+                                                   Links2 Tb, int use_rec, double time, Params2 P) {
+    const int ell = blockIdx.x * LBLOCK + threadIdx.x;
+    if (ell >= P.L) return;
+    const size_t N = P.N;
+    const int L = P.L;
+    const int p = Tb.lp[ell], qo = Tb.lq[ell], qi = (qo + 2) & 3;
+    const int ny = P.ny;
+    const int i = p / ny, j = p - i * ny;
+    int bid = Tb.lbid[ell];
+    if (bid < 0 || bid >= MAX_IDS) bid = 0;
+    const int kind = Tb.kind_by_id[bid], mode = Tb.mode_by_id[bid];
+    const double *prm = Tb.prm_by_id + bid * 8;
+    const double eta = Tb.eta[ell], xb = Tb.xb[ell], yb = Tb.yb[ell], nxb = Tb.nx[ell], nyb = Tb.ny[ell];
+    const double dxi = (double)cx(qi), dyi = (double)cy(qi);
+    double bv0 = value2(0, mode, prm, xb, yb, nxb, nyb, time);
+    double bv1 = value2(1, mode, prm, xb, yb, nxb, nyb, time);
+    if (kind == BC_NEU && P.use_dyn) {
+        bv0 = Tb.dyn_tx[ell];
+        bv1 = Tb.dyn_ty[ell];
+    }
+    const int iu = i - cx(qo), ju = j - cy(qo);
+    const bool up_ok = iu >= 0 && iu < P.nx && ju >= 0 && ju < ny && Tb.act[iu * ny + ju] != 0;
+    const int pu = up_ok ? iu * ny + ju : p;
+    const bool valid = Tb.valid[ell] != 0;
+    double Fg[4];
+    if (P.recon == R_LOCALF) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            Fg[k] = valid ? lsq_sum(U, N, Tb.lsq_p, Tb.wb, L, ell, 2 + k)
+                          : interp_link(U, Tb.act, N, 2 + k, p, i, j, qo, eta, P);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double uc = U[(2 + k) * N + p];
+            Fg[k] = up_ok ? uc + eta * (uc - U[(2 + k) * N + pu]) : uc;
+        }
+        if (P.recon == R_LSQ_BFL && valid)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) Fg[k] = lsq_sum(U, N, Tb.lsq_p, Tb.wb, L, ell, 2 + k);
+    }
+    double Fb[4] = {Fg[0], Fg[1], Fg[2], Fg[3]};
+    if (kind == BC_NEU) solve_normal2(Fg, nxb, nyb, bv0, bv1, P.solver, P.m, Fb);
+    double Pb[4];
+    piola2(Fb, P.m, Pb);
+
+    // Characteristic closure of the traction rows (cbw2d:2456-2886, 3034-3097).
+    const bool use_char = P.recon == R_LOCALF && kind == BC_NEU && valid;
+    bool char_ok = false;
+    double cv[6], Acw[6], Pc[4];
+    if (use_char) {
+        cv[0] = U[p];
+        cv[1] = U[N + p];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) cv[2 + k] = Fb[k];
+        double gx[6], gy[6];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
+            gx[a] = lsq_sum(U, N, Tb.lsq_p, Tb.wx, L, ell, a);
+            gy[a] = lsq_sum(U, N, Tb.lsq_p, Tb.wy, L, ell, a);
+        }
+        const double shx = -eta * P.dt * P.c * (double)cx(qo);
+        const double shy = -eta * P.dt * P.c * (double)cy(qo);
+        double d[6];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) d[a] = shx * gx[a] + shy * gy[a];
+        double sc = 1.0;
+        for (int g = 0; g < 10; ++g) {
+            const double t11 = cv[2] + sc * d[2], t12 = cv[3] + sc * d[3];
+            const double t21 = cv[4] + sc * d[4], t22 = cv[5] + sc * d[5];
+            if (t11 * t22 - t12 * t21 > 10.0 * P.m.jf) break;
+            sc = 0.5 * sc;
+        }
+#pragma unroll
+        for (int a = 0; a < 6; ++a) cv[a] = cv[a] + sc * d[a];
+        double jx[6], jy[6], cw[6];
+        flux_jvp2(0, cv, gx, P.m, jx);
+        flux_jvp2(1, cv, gy, P.m, jy);
+#pragma unroll
+        for (int a = 0; a < 6; ++a)
+            cw[a] = source2(a, P.src_mode, P.src, cv[0], cv[1]) - jx[a] - jy[a] + P.c * (dxi * gx[a] + dyi * gy[a]);
+        if (cv[2] * cv[5] - cv[3] * cv[4] > P.m.jf) {
+            char_ok = true;
+            piola2(cv + 2, P.m, Pc);
+            double ax[6], ay[6];
+            flux_jvp2(0, cv, cw, P.m, ax);
+            flux_jvp2(1, cv, cw, P.m, ay);
+#pragma unroll
+            for (int a = 0; a < 6; ++a) Acw[a] = 0.25 * (cw[a] + (2.0 / P.c) * (dxi * ax[a] + dyi * ay[a]));
+        }
+    }
+    const double om1 = 1.0 - P.omega;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+        double val;
+        if (char_ok) {
+            const double geq = feq2(qi, a, cv, Pc, P.c);
+            const double src = source2(a, P.src_mode, P.src, cv[0], cv[1]);
+            val = geq - P.dt / P.omega * Acw[a] + P.dt * (1.0 / P.omega - 0.5) * 0.25 * src;
+        } else {
+            double D, S;
+            if (kind == BC_DIR) {
+                if (a < 2) {
+                    D = -1.0;
+                    S = 0.5 * (a == 0 ? bv0 : bv1);
+                } else {
+                    const int k = a - 2;
+                    D = 1.0;
+                    S = -((k & 1) == 0 ? dxi : dyi) * (k < 2 ? bv0 : bv1) / P.c;
+                }
+            } else {
+                if (a < 2) {
+                    D = 1.0;
+                    S = -(dxi * Pb[2 * a] + dyi * Pb[2 * a + 1]) / P.c;
+                } else {
+                    D = -1.0;
+                    S = 0.5 * Fb[a - 2];
+                }
+            }
+            double fo, fi;
+            if (use_rec) {
+                fo = fpost_v(ld(fsrc, (size_t)(qo * 6 + a) * N + p, a), rec, N, qo, a, p, om1);
+                fi = fpost_v(ld(fsrc, (size_t)(qi * 6 + a) * N + p, a), rec, N, qi, a, p, om1);
+            } else {
+                fo = ld(fsrc, (size_t)(qo * 6 + a) * N + p, a);
+                fi = ld(fsrc, (size_t)(qi * 6 + a) * N + p, a);
+            }
+            if (eta >= 0.5) {
+                const double alpha = 1.0 / (2.0 * eta);
+                val = alpha * (D * fo + S) + (1.0 - alpha) * fi;
+            } else {
+                double upv = fo;
+                if (up_ok)
+                    upv = use_rec ? fpost_v(ld(fsrc, (size_t)(qo * 6 + a) * N + pu, a), rec, N, qo, a, pu, om1)
+                                  : ld(fsrc, (size_t)(qo * 6 + a) * N + pu, a);
+                val = D * (2.0 * eta * fo + (1.0 - 2.0 * eta) * upv) + S;
+            }
+        }
+        st(fdst, (size_t)(qi * 6 + a) * N + p, a, val);
+    }
+}
+
+// Pull stream of the post-collision populations; entries whose upstream
+// node is inactive or off the lattice belong to cut links and were written
+// by k2_links.  With NEXT the kernel also performs the moment part of the
+// next step's collide (cbw2d:1731-1790).
+template <typename T, bool NEXT>
+__global__ __launch_bounds__(BLOCK) void k2_stream(const T *__restrict__ f0, T *__restrict__ f1,
+                                                   const double *__restrict__ rec, const int32_t *__restrict__ list,
+                                                   const int32_t *__restrict__ act, double *__restrict__ U,
+                                                   double *__restrict__ u, double *__restrict__ us,
+                                                   int32_t *__restrict__ invalid, Params2 P) {
+    const int t = xcd_block() * BLOCK + threadIdx.x;
+    if (t >= P.na) return;
+    const int p = list[t];
+    const int ny = P.ny;
+    const int i = p / ny, j = p - i * ny;
+    const size_t N = P.N;
+    const double om1 = 1.0 - P.omega;
+    double Um[NC];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) Um[a] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int iu = i - cx(q), ju = j - cy(q);
+        const bool ok = iu >= 0 && iu < P.nx && ju >= 0 && ju < ny && act[iu * ny + ju] != 0;
+        if (ok) {
+            const int src = iu * ny + ju;
+            double fv[NC];
+#pragma unroll
+            for (int a = 0; a < NC; ++a) fv[a] = ld_nt(f0, (size_t)(q * 6 + a) * N + src, a);
+#pragma unroll
+            for (int a = 0; a < NC; ++a) {
+                const double x = st_nt(f1, (size_t)(q * 6 + a) * N + p, a, fpost_v(fv[a], rec, N, q, a, src, om1));
+                if (NEXT) Um[a] += x;
+            }
+        } else if (NEXT) {
+#pragma unroll
+            for (int a = 0; a < NC; ++a) Um[a] += ld(f1, (size_t)(q * 6 + a) * N + p, a);
+        }
+    }
+    if (NEXT) {
+        const double hdt = 0.5 * P.dt;
+        if (P.src_mode == SRC_CONST) {
+#pragma unroll
+            for (int a = 0; a < NC; ++a) Um[a] += hdt * P.src[a];
+        } else if (P.src_mode == SRC_DAMP) {
+            const double den = 1.0 + hdt * P.src[0];
+            Um[0] = Um[0] / den;
+            Um[1] = Um[1] / den;
+        }
+#pragma unroll
+        for (int a = 0; a < NC; ++a) U[a * N + p] = Um[a];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const double uc = us[c * N + p] + hdt * Um[c];
+            u[c * N + p] = uc;
+            us[c * N + p] = uc + hdt * Um[c];
+        }
+        if (Um[2] * Um[5] - Um[3] * Um[4] <= P.m.jf) atomicOr(invalid, 1);
+    }
+}
+
+// f = feq(U) - dt/(2Q) B(U) at active nodes (cbw2d:3873-3876).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k2_init_pop(const double *__restrict__ U, T *__restrict__ f,
+                                                     const int32_t *__restrict__ list, Params2 P) {
+    const int t = xcd_block() * BLOCK + threadIdx.x;
+    if (t >= P.na) return;
+    const int p = list[t];
+    const size_t N = P.N;
+    double Uv[NC], Pf[4];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) Uv[a] = U[a * N + p];
+    piola2(Uv + 2, P.m, Pf);
+    const double cs = 0.5 * P.dt / 4.0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int a = 0; a < NC; ++a)
+            st(f, (size_t)(q * 6 + a) * N + p, a, feq2(q, a, Uv, Pf, P.c) - cs * source2(a, P.src_mode, P.src, Uv[0], Uv[1]));
+}
+
+__global__ __launch_bounds__(BLOCK) void k2_set_ustar(const double *__restrict__ disp, const double *__restrict__ U,
+                                                      double *__restrict__ us, const int32_t *__restrict__ list,
+                                                      double dt, int na, size_t N) {
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= na) return;
+    const int p = list[t];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) us[c * N + p] = disp[c * N + p] - 0.5 * dt * U[c * N + p];
+}
+
+__global__ __launch_bounds__(BLOCK) void k2_stress(const double *__restrict__ U, double *__restrict__ Pout,
+                                                   double *__restrict__ sig, const int32_t *__restrict__ list,
+                                                   Params2 P) {
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= P.na) return;
+    const int p = list[t];
+    const size_t N = P.N;
+    double F[4], Pf[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) F[k] = U[(2 + k) * N + p];
+    piola2(F, P.m, Pf);
+    const double Js = clamp_j(F[0] * F[3] - F[1] * F[2], P.m.jf);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) Pout[k * N + p] = Pf[k];
+    sig[0 * N + p] = (Pf[0] * F[0] + Pf[1] * F[1]) / Js;
+    sig[1 * N + p] = (Pf[0] * F[2] + Pf[1] * F[3]) / Js;
+    sig[2 * N + p] = (Pf[2] * F[0] + Pf[3] * F[1]) / Js;
+    sig[3 * N + p] = (Pf[2] * F[2] + Pf[3] * F[3]) / Js;
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k2_post(const T *__restrict__ f, T *__restrict__ fp,
+                                                 const double *__restrict__ rec, const int32_t *__restrict__ list,
+                                                 Params2 P) {
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= P.na) return;
+    const int p = list[t];
+    const double om1 = 1.0 - P.omega;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int a = 0; a < NC; ++a) {
+            const size_t k = (size_t)(q * 6 + a) * P.N + p;
+            st(fp, k, a, fpost_v(ld(f, k, a), rec, P.N, q, a, p, om1));
+        }
+}
+
+template <typename T>
+__global__ void k2_copy(const T *__restrict__ s, T *__restrict__ d, size_t count) {
+    const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < count) d[i] = s[i];
+}
+__global__ void k2_to_f64(const float *__restrict__ f, double *__restrict__ d, size_t N) {
+    const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= (size_t)FQ * N) return;
+    const int a = (int)((i / N) % 6);
+    d[i] = ld(f, i, a);
+}
+__global__ void k2_from_f64(const double *__restrict__ s, float *__restrict__ f, size_t N) {
+    const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= (size_t)FQ * N) return;
+    const int a = (int)((i / N) % 6);
+    st(f, i, a, s[i]);
+}
+
+// Deterministic two-pass reductions over the active nodes.
+constexpr int NRED = 7;
+__device__ __forceinline__ void red_combine(double *acc, const double *o) {
+    acc[0] += o[0];
+    acc[1] = fmax(acc[1], o[1]);
+    acc[2] = fmin(acc[2], o[2]);
+    acc[3] = fmax(acc[3], o[3]);
+    acc[4] = fmax(acc[4], o[4]);
+    acc[5] += o[5];
+    acc[6] += o[6];
+}
+__device__ inline void block_reduce(double *acc, double (*sh)[NRED]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < NRED; ++k) sh[t][k] = acc[k];
+    __syncthreads();
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+        if (t < s) red_combine(sh[t], sh[t + s]);
+        __syncthreads();
+    }
+}
+__global__ __launch_bounds__(BLOCK) void k2_reduce_partial(const double *__restrict__ U, const double *__restrict__ u,
+                                                           const double *__restrict__ uprev,
+                                                           const int32_t *__restrict__ list,
+                                                           double *__restrict__ part, int na, size_t N) {
+    __shared__ double sh[BLOCK][NRED];
+    double acc[NRED] = {0.0, 0.0, INFINITY, -INFINITY, 0.0, 0.0, 0.0};
+    for (int t = blockIdx.x * BLOCK + threadIdx.x; t < na; t += gridDim.x * BLOCK) {
+        const int p = list[t];
+        double Uv[NC];
+        double nonfinite = 0.0;
+#pragma unroll
+        for (int a = 0; a < NC; ++a) {
+            Uv[a] = U[(size_t)a * N + p];
+            if (!isfinite(Uv[a])) nonfinite += 1.0;
+        }
+        double uu = 0.0, du = 0.0;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const double x = u[(size_t)c * N + p];
+            if (!isfinite(x)) nonfinite += 1.0;
+            uu += x * x;
+            if (uprev) {
+                const double dd = x - uprev[(size_t)c * N + p];
+                du += dd * dd;
+            }
+        }
+        const double v2 = Uv[0] * Uv[0] + Uv[1] * Uv[1];
+        const double J = Uv[2] * Uv[5] - Uv[3] * Uv[4];
+        const double o[NRED] = {v2, sqrt(v2), J, J, sqrt(uu), nonfinite, du};
+        red_combine(acc, o);
+    }
+    block_reduce(acc, sh);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NRED; ++k) part[blockIdx.x * 8 + k] = sh[0][k];
+}
+__global__ __launch_bounds__(BLOCK) void k2_reduce_final(double *__restrict__ part, int nparts,
+                                                         const int32_t *__restrict__ invalid,
+                                                         double *__restrict__ out) {
+    __shared__ double sh[BLOCK][NRED];
+    double acc[NRED] = {0.0, 0.0, INFINITY, -INFINITY, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nparts; b += BLOCK) red_combine(acc, part + b * 8);
+    block_reduce(acc, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) out[k] = sh[0][k];
+        out[6] = (double)invalid[0];
+        out[7] = sh[0][6];
+    }
+}
+
+}  // namespace
+
+struct vlbm2d_handle {
+    vlbm2d_desc d;
+    Params2 P;
+    Links2 T;
+    hipStream_t stream;
+    Timing timing;
+};
+
+static Params2 make_params2(const vlbm2d_desc &d) {
+    Params2 P{};
+    P.nx = d.nx;
+    P.ny = d.ny;
+    P.na = d.n_active;
+    P.L = d.n_links;
+    P.N = (size_t)d.nx * (size_t)d.ny;
+    P.m = Material2{d.model, d.lam, d.mu, d.p0, d.p1, d.p2, d.j_floor};
+    P.dt = d.dt;
+    P.c = d.c;
+    P.omega = d.omega;
+    P.dx = d.dx;
+    P.dy = d.dy;
+    P.src_mode = d.source_mode;
+    for (int k = 0; k < 6; ++k) P.src[k] = d.source_params[k];
+    P.recon = d.recon;
+    P.solver = d.normal_solver;
+    P.disp_relax = d.disp_relax;
+    P.w2 = d.disp_weight * d.disp_weight;
+    P.blend = d.compat_blend;
+    P.interior_only = d.compat_interior_only;
+    P.use_dyn = d.use_dyn;
+    return P;
+}
+
+static Links2 make_links2(const vlbm2d_desc &d) {
+    Links2 T{};
+    T.lp = d.link_p;
+    T.lq = d.link_q;
+    T.lbid = d.link_bid;
+    T.eta = d.link_eta;
+    T.xb = d.link_xb;
+    T.yb = d.link_yb;
+    T.nx = d.link_nx;
+    T.ny = d.link_ny;
+    T.lsq_p = d.lsq_p;
+    T.wb = d.lsq_wb;
+    T.wx = d.lsq_wx;
+    T.wy = d.lsq_wy;
+    T.valid = d.lsq_valid;
+    T.kind_by_id = d.kind_by_id;
+    T.mode_by_id = d.mode_by_id;
+    T.prm_by_id = d.params_by_id;
+    T.dyn_tx = d.dyn_tx;
+    T.dyn_ty = d.dyn_ty;
+    T.act = d.active;
+    return T;
+}
+
+#define CHECK_LAUNCH(where) VLBM_CHECK_LAUNCH(where)
+
+template <typename T>
+static int run_links2(vlbm2d_handle *h, const T *src, T *dst, double t, int use_rec) {
+    const vlbm2d_desc &d = h->d;
+    if (d.n_links == 0) return 0;
+    TimedSlot ts(h->timing, h->stream, VLBM_SLOT_LINKS);
+    const int gl = (d.n_links + LBLOCK - 1) / LBLOCK;
+    hipLaunchKernelGGL((k2_links<T>), dim3(gl), dim3(LBLOCK), 0, h->stream, src, dst, (const double *)d.record,
+                       (const double *)d.U, h->T, use_rec, t, h->P);
+    CHECK_LAUNCH("k2_links");
+    return 0;
+}
+
+// One library call runs nsteps reference steps; the collide of every step
+// after the first is fused into the previous step's stream kernel, the cut
+// links run before the stream kernel, and the last step streams without the
+// fused collide so that U, u and u* end as the reference leaves them.
+template <typename T>
+static int run_steps2(vlbm2d_handle *h, T *f0, T *f1, double time, int64_t steps, int nsteps) {
+    const vlbm2d_desc &d = h->d;
+    const Params2 &P = h->P;
+    const int gn = xcd_grid(d.n_active);
+    if (nsteps <= 0) return 0;
+    hipLaunchKernelGGL(k2_reset_invalid, dim3(1), dim3(1), 0, h->stream, d.invalid);
+    {
+        TimedSlot ts(h->timing, h->stream, VLBM_SLOT_COLLIDE);
+        hipLaunchKernelGGL((k2_moments<T, true>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, d.U, d.u,
+                           d.u_star, d.invalid, d.active_list, P);
+        CHECK_LAUNCH("k2_moments");
+    }
+    for (int k = 0; k < nsteps; ++k) {
+        TimedSlot whole(h->timing, h->stream, VLBM_SLOT_STEP);
+        const double tb = time + 0.5 * d.dt;
+        const bool disp = d.disp_interval > 0 && steps % d.disp_interval == 0;
+        const bool compat = d.compat_interval > 0 && steps % d.compat_interval == 0;
+        if (disp) {
+            for (int s = 0; s < d.disp_sweeps; ++s) {
+                TimedSlot ts(h->timing, h->stream, VLBM_SLOT_SWEEP);
+                hipLaunchKernelGGL(k2_sweep, dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)d.U,
+                                   (const double *)d.u, d.u_next, d.active_list, d.active, d.anchor_off,
+                                   d.anchor_ids, h->T, tb, P);
+                CHECK_LAUNCH("k2_sweep");
+                if (s + 1 < d.disp_sweeps) {
+                    TimedSlot tc(h->timing, h->stream, VLBM_SLOT_COMMIT);
+                    hipLaunchKernelGGL(k2_commit, dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)d.u_next, d.u,
+                                       d.u_star, d.active_list, d.n_active, P.N);
+                    CHECK_LAUNCH("k2_commit");
+                }
+            }
+        }
+        {
+            TimedSlot ts(h->timing, h->stream, VLBM_SLOT_RECORD);
+            hipLaunchKernelGGL(k2_record, dim3(gn), dim3(BLOCK), 0, h->stream, d.U,
+                               disp ? (const double *)d.u_next : (const double *)d.u, d.u, d.u_star, d.record,
+                               d.active_list, d.active, (int)compat, (int)disp, P);
+            CHECK_LAUNCH("k2_record");
+        }
+        int rc = run_links2<T>(h, f0, f1, tb, 1);
+        if (rc) return rc;
+        const bool next = k + 1 < nsteps;
+        if (next) hipLaunchKernelGGL(k2_reset_invalid, dim3(1), dim3(1), 0, h->stream, d.invalid);
+        {
+            TimedSlot ts(h->timing, h->stream, VLBM_SLOT_STREAM);
+            if (next)
+                hipLaunchKernelGGL((k2_stream<T, true>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, f1,
+                                   (const double *)d.record, d.active_list, d.active, d.U, d.u, d.u_star, d.invalid,
+                                   P);
+            else
+                hipLaunchKernelGGL((k2_stream<T, false>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, f1,
+                                   (const double *)d.record, d.active_list, d.active, d.U, d.u, d.u_star, d.invalid,
+                                   P);
+            CHECK_LAUNCH("k2_stream");
+        }
+        T *tmp = f0;
+        f0 = f1;
+        f1 = tmp;
+        time += d.dt;
+        steps += 1;
+    }
+    return 0;
+}
+
+extern "C" {
+
+int vlbm2d_create(const vlbm2d_desc *desc, vlbm2d_handle **out) {
+    if (!desc || !out) return fail_msg("vlbm2d_create: null argument");
+    if (desc->struct_size != sizeof(vlbm2d_desc))
+        return fail_msg("vlbm2d_create: descriptor size mismatch (" + std::to_string(desc->struct_size) +
+                        " != " + std::to_string(sizeof(vlbm2d_desc)) + ")");
+    if (desc->abi_version != VLBM2D_ABI_VERSION) return fail_msg("vlbm2d_create: ABI version mismatch");
+    if (desc->nx < 4 || desc->ny < 4) return fail_msg("vlbm2d_create: nx and ny must be at least four");
+    if (desc->n_active <= 0 || (size_t)desc->n_active > (size_t)desc->nx * desc->ny)
+        return fail_msg("vlbm2d_create: n_active out of range");
+    if ((size_t)desc->nx * desc->ny * FQ >= (size_t)1 << 40) return fail_msg("vlbm2d_create: lattice too large");
+    if ((size_t)desc->nx * desc->ny >= (size_t)1 << 31) return fail_msg("vlbm2d_create: lattice exceeds int32 indexing");
+    if (desc->n_links < 0) return fail_msg("vlbm2d_create: n_links must be non-negative");
+    if (desc->precision != VLBM_PREC_F64 && desc->precision != VLBM_PREC_F32)
+        return fail_msg("vlbm2d_create: unknown precision");
+    if (desc->recon != R_BFL && desc->recon != R_LSQ_BFL && desc->recon != R_LOCALF && desc->recon != R_LOCALF_BFL)
+        return fail_msg("vlbm2d_create: reconstruction not provided (bfl, lsq_bfl, local_f, local_f_bfl)");
+    if (desc->disp_sweeps < 1 || desc->disp_sweeps > 8) return fail_msg("vlbm2d_create: disp_sweeps must be in [1,8]");
+    const void *req[] = {desc->U, desc->u, desc->u_star, desc->u_next, desc->record, desc->invalid,
+                         desc->reduce_scratch, desc->active, desc->active_list, desc->anchor_off,
+                         desc->kind_by_id, desc->mode_by_id, desc->params_by_id};
+    for (const void *ptr : req)
+        if (!ptr) return fail_msg("vlbm2d_create: a required device pointer is null");
+    if (desc->n_links > 0) {
+        const void *lreq[] = {desc->link_p, desc->link_q, desc->link_bid, desc->link_eta, desc->link_xb,
+                              desc->link_yb, desc->link_nx, desc->link_ny, desc->lsq_p, desc->lsq_wb,
+                              desc->lsq_wx, desc->lsq_wy, desc->lsq_valid, desc->dyn_tx, desc->dyn_ty};
+        for (const void *ptr : lreq)
+            if (!ptr) return fail_msg("vlbm2d_create: a required link pointer is null");
+    }
+    if (desc->n_anchor_links > 0 && !desc->anchor_ids) return fail_msg("vlbm2d_create: anchor_ids is null");
+    vlbm2d_handle *h = new vlbm2d_handle;
+    h->d = *desc;
+    h->P = make_params2(*desc);
+    h->T = make_links2(*desc);
+    h->stream = (hipStream_t)desc->stream;
+    *out = h;
+    return 0;
+}
+
+int vlbm2d_destroy(vlbm2d_handle *h) {
+    if (!h) return 0;
+    h->timing.destroy();
+    delete h;
+    return 0;
+}
+
+int vlbm2d_timing_enable(vlbm2d_handle *h, int32_t enable) {
+    if (!h) return fail_msg("vlbm2d_timing_enable: null handle");
+    return h->timing.enable(h->stream, enable);
+}
+
+int vlbm2d_timing_read(vlbm2d_handle *h, double *ms, int32_t *counts, int32_t nslots) {
+    if (!h || !ms || !counts) return fail_msg("vlbm2d_timing_read: null argument");
+    return h->timing.read(h->stream, ms, counts, nslots);
+}
+
+int vlbm2d_step(vlbm2d_handle *h, void *f0, void *f1, double time, int64_t steps, int32_t nsteps) {
+    if (!h || !f0 || !f1) return fail_msg("vlbm2d_step: null argument");
+    if (nsteps < 0) return fail_msg("vlbm2d_step: nsteps must be non-negative");
+    if (h->d.precision == VLBM_PREC_F64)
+        return run_steps2<double>(h, (double *)f0, (double *)f1, time, steps, nsteps);
+    return run_steps2<float>(h, (float *)f0, (float *)f1, time, steps, nsteps);
+}
+
+int vlbm2d_refresh(vlbm2d_handle *h, const void *f) {
+    if (!h || !f) return fail_msg("vlbm2d_refresh: null argument");
+    const vlbm2d_desc &d = h->d;
+    hipLaunchKernelGGL(k2_reset_invalid, dim3(1), dim3(1), 0, h->stream, d.invalid);
+    const int gn = xcd_grid(d.n_active);
+    if (d.precision == VLBM_PREC_F64)
+        hipLaunchKernelGGL((k2_moments<double, false>), dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)f, d.U,
+                           d.u, d.u_star, d.invalid, d.active_list, h->P);
+    else
+        hipLaunchKernelGGL((k2_moments<float, false>), dim3(gn), dim3(BLOCK), 0, h->stream, (const float *)f, d.U,
+                           d.u, d.u_star, d.invalid, d.active_list, h->P);
+    CHECK_LAUNCH("k2_refresh");
+    return 0;
+}
+
+int vlbm2d_apply_boundary(vlbm2d_handle *h, const void *f_src, void *f_dst, double t) {
+    if (!h || !f_src || !f_dst) return fail_msg("vlbm2d_apply_boundary: null argument");
+    if (f_src == f_dst) return fail_msg("vlbm2d_apply_boundary: source and target must differ");
+    if (h->d.precision == VLBM_PREC_F64) return run_links2<double>(h, (const double *)f_src, (double *)f_dst, t, 0);
+    return run_links2<float>(h, (const float *)f_src, (float *)f_dst, t, 0);
+}
+
+int vlbm2d_init_populations(vlbm2d_handle *h, void *f) {
+    if (!h || !f) return fail_msg("vlbm2d_init_populations: null argument");
+    const int gn = xcd_grid(h->d.n_active);
+    if (h->d.precision == VLBM_PREC_F64)
+        hipLaunchKernelGGL(k2_init_pop<double>, dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)h->d.U,
+                           (double *)f, h->d.active_list, h->P);
+    else
+        hipLaunchKernelGGL(k2_init_pop<float>, dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)h->d.U,
+                           (float *)f, h->d.active_list, h->P);
+    CHECK_LAUNCH("k2_init_pop");
+    return 0;
+}
+
+int vlbm2d_set_ustar(vlbm2d_handle *h, const double *disp) {
+    if (!h || !disp) return fail_msg("vlbm2d_set_ustar: null argument");
+    const int g = (h->d.n_active + BLOCK - 1) / BLOCK;
+    hipLaunchKernelGGL(k2_set_ustar, dim3(g), dim3(BLOCK), 0, h->stream, disp, (const double *)h->d.U, h->d.u_star,
+                       h->d.active_list, h->d.dt, h->d.n_active, h->P.N);
+    CHECK_LAUNCH("k2_set_ustar");
+    return 0;
+}
+
+int vlbm2d_stress(vlbm2d_handle *h, double *P, double *sigma) {
+    if (!h || !P || !sigma) return fail_msg("vlbm2d_stress: null argument");
+    const int g = (h->d.n_active + BLOCK - 1) / BLOCK;
+    hipLaunchKernelGGL(k2_stress, dim3(g), dim3(BLOCK), 0, h->stream, (const double *)h->d.U, P, sigma,
+                       h->d.active_list, h->P);
+    CHECK_LAUNCH("k2_stress");
+    return 0;
+}
+
+int vlbm2d_post_collision(vlbm2d_handle *h, const void *f_prev, void *fpost) {
+    if (!h || !f_prev || !fpost) return fail_msg("vlbm2d_post_collision: null argument");
+    const int g = (h->d.n_active + BLOCK - 1) / BLOCK;
+    if (h->d.precision == VLBM_PREC_F64)
+        hipLaunchKernelGGL(k2_post<double>, dim3(g), dim3(BLOCK), 0, h->stream, (const double *)f_prev,
+                           (double *)fpost, (const double *)h->d.record, h->d.active_list, h->P);
+    else
+        hipLaunchKernelGGL(k2_post<float>, dim3(g), dim3(BLOCK), 0, h->stream, (const float *)f_prev,
+                           (float *)fpost, (const double *)h->d.record, h->d.active_list, h->P);
+    CHECK_LAUNCH("k2_post");
+    return 0;
+}
+
+int vlbm2d_set_dynamic_traction(vlbm2d_handle *h, int32_t enabled) {
+    if (!h) return fail_msg("vlbm2d_set_dynamic_traction: null handle");
+    h->P.use_dyn = enabled != 0;
+    h->d.use_dyn = enabled != 0;
+    return 0;
+}
+
+int vlbm2d_reduce(vlbm2d_handle *h, const double *u_prev, double *out) {
+    if (!h || !out) return fail_msg("vlbm2d_reduce: null argument");
+    const vlbm2d_desc &d = h->d;
+    int nb = (d.n_active + BLOCK - 1) / BLOCK;
+    if (nb > RED_BLOCKS) nb = RED_BLOCKS;
+    double *part = d.reduce_scratch;
+    double *res = d.reduce_scratch + (size_t)RED_BLOCKS * 8;
+    hipLaunchKernelGGL(k2_reduce_partial, dim3(nb), dim3(BLOCK), 0, h->stream, (const double *)d.U,
+                       (const double *)d.u, u_prev, d.active_list, part, d.n_active, h->P.N);
+    CHECK_LAUNCH("k2_reduce_partial");
+    hipLaunchKernelGGL(k2_reduce_final, dim3(1), dim3(BLOCK), 0, h->stream, part, nb, (const int32_t *)d.invalid, res);
+    CHECK_LAUNCH("k2_reduce_final");
+    hipError_t e = hipMemcpyAsync(out, res, sizeof(double) * VLBM_RED_COUNT, hipMemcpyDeviceToHost, h->stream);
+    if (e != hipSuccess) return fail("vlbm2d_reduce: copy", e);
+    e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail("vlbm2d_reduce: sync", e);
+    return 0;
+}
+
+int vlbm2d_populations_to_f64(vlbm2d_handle *h, const void *f, double *dst) {
+    if (!h || !f || !dst) return fail_msg("vlbm2d_populations_to_f64: null argument");
+    const size_t count = (size_t)FQ * h->P.N;
+    const int g = (int)((count + BLOCK - 1) / BLOCK);
+    if (h->d.precision == VLBM_PREC_F64)
+        hipLaunchKernelGGL(k2_copy<double>, dim3(g), dim3(BLOCK), 0, h->stream, (const double *)f, dst, count);
+    else
+        hipLaunchKernelGGL(k2_to_f64, dim3(g), dim3(BLOCK), 0, h->stream, (const float *)f, dst, h->P.N);
+    CHECK_LAUNCH("vlbm2d_populations_to_f64");
+    return 0;
+}
+
+int vlbm2d_populations_from_f64(vlbm2d_handle *h, const double *src, void *f) {
+    if (!h || !f || !src) return fail_msg("vlbm2d_populations_from_f64: null argument");
+    const size_t count = (size_t)FQ * h->P.N;
+    const int g = (int)((count + BLOCK - 1) / BLOCK);
+    if (h->d.precision == VLBM_PREC_F64)
+        hipLaunchKernelGGL(k2_copy<double>, dim3(g), dim3(BLOCK), 0, h->stream, src, (double *)f, count);
+    else
+        hipLaunchKernelGGL(k2_from_f64, dim3(g), dim3(BLOCK), 0, h->stream, src, (float *)f, h->P.N);
+    CHECK_LAUNCH("vlbm2d_populations_from_f64");
+    return 0;
+}
+
+}  // extern "C"

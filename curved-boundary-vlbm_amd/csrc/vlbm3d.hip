@@ -29,24 +29,24 @@
 
 #include "../../include/vlbm3d.h"
 #include "vlbm3d_device.h"
+#include "vlbm_common.h"
 
 using namespace vlbm;
+using vlbm_common::BLOCK;
+using vlbm_common::fail;
+using vlbm_common::fail_msg;
+using vlbm_common::Timing;
+using vlbm_common::TimedSlot;
+using vlbm_common::xcd_block;
+using vlbm_common::xcd_grid;
+
+std::string &vlbm_common::last_error() {
+    static thread_local std::string err;
+    return err;
+}
 
 namespace {
 
-thread_local std::string g_err;
-
-int fail(const char *where, hipError_t e) {
-    g_err = std::string(where) + ": " + hipGetErrorString(e);
-    return -2;
-}
-int fail_msg(const std::string &m) {
-    g_err = m;
-    return -1;
-}
-
-constexpr int BLOCK = 256;
-constexpr int NXCD = 8;
 constexpr int RED_BLOCKS = 1024;
 
 // Record layout (24 planes): A_v[3], A_F[9], B_v[r*3+axis] (9), B_F[3].
@@ -63,20 +63,6 @@ struct Params {
     int disp_bid;
     double blend;
 };
-
-// Grid sizes rounded up to a multiple of the XCD count; block b of the launch
-// is mapped to logical block (b % 8) * (G / 8) + b / 8 so that each XCD walks
-// one contiguous slab of nodes and neighbour planes stay in its own L2.
-__device__ __forceinline__ int xcd_block() {
-    const int G = gridDim.x;
-    const int per = G / NXCD;
-    const int b = blockIdx.x;
-    return (b % NXCD) * per + b / NXCD;
-}
-inline int xcd_grid(int n) {
-    int g = (n + BLOCK - 1) / BLOCK;
-    return ((g + NXCD - 1) / NXCD) * NXCD;
-}
 
 // Population access.  double planes hold f; float planes hold f - f_rest
 // where f_rest[q][a] = 1/6 on the diagonal of F (a = 3, 7, 11), 0 elsewhere.
@@ -781,48 +767,12 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_final(double *__restrict__ par
 
 }  // namespace
 
-// Optional per-kernel timing with HIP events recorded on the launch stream.
-struct Timing {
-    bool on = false;
-    std::vector<hipEvent_t> pool;
-    size_t used = 0;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
-    hipEvent_t get() {
-        if (used == pool.size()) {
-            hipEvent_t e;
-            if (hipEventCreate(&e) != hipSuccess) return nullptr;
-            pool.push_back(e);
-        }
-        return pool[used++];
-    }
-};
-
 struct vlbm3d_handle {
     vlbm3d_desc d;
     Params P;
     LinkTables T;
     hipStream_t stream;
     Timing timing;
-};
-
-// RAII bracket around one launch: records a begin/end event pair on the
-// stream when timing is enabled.
-struct TimedSlot {
-    vlbm3d_handle *h;
-    int slot;
-    hipEvent_t b = nullptr;
-    TimedSlot(vlbm3d_handle *hh, int s) : h(hh), slot(s) {
-        if (h->timing.on) {
-            b = h->timing.get();
-            if (b && hipEventRecord(b, h->stream) != hipSuccess) b = nullptr;
-        }
-    }
-    ~TimedSlot() {
-        if (b) {
-            hipEvent_t e = h->timing.get();
-            if (e && hipEventRecord(e, h->stream) == hipSuccess) h->timing.marks.push_back({slot, {b, e}});
-        }
-    }
 };
 
 static Params make_params(const vlbm3d_desc &d) {
@@ -875,11 +825,7 @@ static LinkTables make_tables(const vlbm3d_desc &d) {
     return T;
 }
 
-#define CHECK_LAUNCH(where)                               \
-    do {                                                  \
-        hipError_t e_ = hipGetLastError();                \
-        if (e_ != hipSuccess) return fail((where), e_);   \
-    } while (0)
+#define CHECK_LAUNCH(where) VLBM_CHECK_LAUNCH(where)
 
 // use_rec: the sources are pre-collision populations plus the per-node
 // records of this step, and k_record has already filled the gather pack.
@@ -888,7 +834,7 @@ static int run_boundary(vlbm3d_handle *h, const T *src, T *dst, double t, int us
     const vlbm3d_desc &d = h->d;
     if (d.n_links == 0) return 0;
     if (d.has_char && d.n_char_links > 0) {
-        TimedSlot ts(h, VLBM_SLOT_GATHER);
+        TimedSlot ts(h->timing, h->stream, VLBM_SLOT_GATHER);
         if (!use_rec) {
             const int gp = (d.n_char_nodes + BLOCK - 1) / BLOCK;
             hipLaunchKernelGGL(k_pack, dim3(gp), dim3(BLOCK), 0, h->stream, (const double *)d.U, d.char_nodes,
@@ -903,11 +849,11 @@ static int run_boundary(vlbm3d_handle *h, const T *src, T *dst, double t, int us
     }
     const int gl = (d.n_links + 127) / 128;
     {
-        TimedSlot ts(h, VLBM_SLOT_PREPARE);
+        TimedSlot ts(h->timing, h->stream, VLBM_SLOT_PREPARE);
         hipLaunchKernelGGL(k_prepare, dim3(gl), dim3(128), 0, h->stream, (const double *)d.U, h->T, t, h->P);
         CHECK_LAUNCH("k_prepare");
     }
-    TimedSlot ts(h, VLBM_SLOT_LINKS);
+    TimedSlot ts(h->timing, h->stream, VLBM_SLOT_LINKS);
     hipLaunchKernelGGL((k_links<T>), dim3(gl), dim3(128), 0, h->stream, src, dst, (const double *)d.record,
                        (const double *)d.U, h->T, d.invalid, use_rec, d.has_char, h->P);
     CHECK_LAUNCH("k_links");
@@ -930,26 +876,26 @@ static int run_steps(vlbm3d_handle *h, T *f0, T *f1, double time, int64_t steps,
     if (nsteps <= 0) return 0;
     hipLaunchKernelGGL(k_reset_invalid, dim3(1), dim3(1), 0, h->stream, d.invalid);
     {
-        TimedSlot ts(h, VLBM_SLOT_COLLIDE);
+        TimedSlot ts(h->timing, h->stream, VLBM_SLOT_COLLIDE);
         hipLaunchKernelGGL((k_collide<T, true>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, d.U, d.u,
                            d.u_star, d.invalid, P);
         CHECK_LAUNCH("k_collide");
     }
     for (int k = 0; k < nsteps; ++k) {
-        TimedSlot whole(h, VLBM_SLOT_STEP);
+        TimedSlot whole(h->timing, h->stream, VLBM_SLOT_STEP);
         const double tb = time + 0.5 * d.dt;
         const bool disp = d.disp_interval > 0 && steps % d.disp_interval == 0;
         const bool compat = d.compat_interval > 0 && steps % d.compat_interval == 0;
         if (disp) {
             for (int s = 0; s < d.disp_sweeps; ++s) {
-                TimedSlot ts(h, VLBM_SLOT_SWEEP);
+                TimedSlot ts(h->timing, h->stream, VLBM_SLOT_SWEEP);
                 hipLaunchKernelGGL(k_sweep, dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)d.U,
                                    (const double *)d.u, d.u_next, d.neighbor, d.node_link_off, d.node_link_ids,
                                    d.link_bid, d.link_xb, d.link_yb, d.link_zb, d.active_i, d.active_j,
                                    d.active_k, d.comp_kind_by_id, d.mode_by_id, d.params_by_id, tb, P);
                 CHECK_LAUNCH("k_sweep");
                 if (s + 1 < d.disp_sweeps) {
-                    TimedSlot tc(h, VLBM_SLOT_COMMIT);
+                    TimedSlot tc(h->timing, h->stream, VLBM_SLOT_COMMIT);
                     hipLaunchKernelGGL(k_commit, dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)d.u_next,
                                        d.u, d.u_star, d.n_active);
                     CHECK_LAUNCH("k_commit");
@@ -957,7 +903,7 @@ static int run_steps(vlbm3d_handle *h, T *f0, T *f1, double time, int64_t steps,
             }
         }
         {
-            TimedSlot ts(h, VLBM_SLOT_RECORD);
+            TimedSlot ts(h->timing, h->stream, VLBM_SLOT_RECORD);
             // Within a call only the gather pack consumes the post-repair F of
             // the local_f closure; the U planes get it on the call's last step.
             const bool last = k + 1 == nsteps;
@@ -974,7 +920,7 @@ static int run_steps(vlbm3d_handle *h, T *f0, T *f1, double time, int64_t steps,
         const bool next = k + 1 < nsteps;
         if (next) hipLaunchKernelGGL(k_reset_invalid, dim3(1), dim3(1), 0, h->stream, d.invalid);
         {
-            TimedSlot ts(h, VLBM_SLOT_STREAM);
+            TimedSlot ts(h->timing, h->stream, VLBM_SLOT_STREAM);
             if (next)
                 hipLaunchKernelGGL((k_stream<T, true>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, f1,
                                    (const double *)d.record, d.neighbor, d.U, d.u, d.u_star, d.invalid, P);
@@ -994,7 +940,7 @@ static int run_steps(vlbm3d_handle *h, T *f0, T *f1, double time, int64_t steps,
 
 extern "C" {
 
-const char *vlbm_last_error(void) { return g_err.c_str(); }
+const char *vlbm_last_error(void) { return vlbm_common::last_error().c_str(); }
 int vlbm_abi_version(void) { return VLBM3D_ABI_VERSION; }
 
 int vlbm3d_create(const vlbm3d_desc *desc, vlbm3d_handle **out) {
@@ -1043,41 +989,19 @@ int vlbm3d_create(const vlbm3d_desc *desc, vlbm3d_handle **out) {
 
 int vlbm3d_destroy(vlbm3d_handle *h) {
     if (!h) return 0;
-    for (hipEvent_t e : h->timing.pool) (void)hipEventDestroy(e);
+    h->timing.destroy();
     delete h;
     return 0;
 }
 
 int vlbm3d_timing_enable(vlbm3d_handle *h, int32_t enable) {
     if (!h) return fail_msg("vlbm3d_timing_enable: null handle");
-    hipError_t e = hipStreamSynchronize(h->stream);
-    if (e != hipSuccess) return fail("vlbm3d_timing_enable: sync", e);
-    h->timing.marks.clear();
-    h->timing.used = 0;
-    h->timing.on = enable != 0;
-    return 0;
+    return h->timing.enable(h->stream, enable);
 }
 
 int vlbm3d_timing_read(vlbm3d_handle *h, double *ms, int32_t *counts, int32_t nslots) {
     if (!h || !ms || !counts) return fail_msg("vlbm3d_timing_read: null argument");
-    hipError_t e = hipStreamSynchronize(h->stream);
-    if (e != hipSuccess) return fail("vlbm3d_timing_read: sync", e);
-    for (int k = 0; k < nslots; ++k) {
-        ms[k] = 0.0;
-        counts[k] = 0;
-    }
-    for (auto &m : h->timing.marks) {
-        float t = 0.0f;
-        e = hipEventElapsedTime(&t, m.second.first, m.second.second);
-        if (e != hipSuccess) return fail("vlbm3d_timing_read: elapsed", e);
-        if (m.first >= 0 && m.first < nslots) {
-            ms[m.first] += t;
-            counts[m.first] += 1;
-        }
-    }
-    h->timing.marks.clear();
-    h->timing.used = 0;
-    return 0;
+    return h->timing.read(h->stream, ms, counts, nslots);
 }
 
 int vlbm3d_step(vlbm3d_handle *h, void *f0, void *f1, double time, int64_t steps, int32_t nsteps) {

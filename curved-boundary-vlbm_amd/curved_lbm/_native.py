@@ -13,6 +13,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libvlbm.so"
 
 ABI_VERSION = 4
+ABI2_VERSION = 1
 PREC_F64 = 0
 PREC_F32 = 1
 
@@ -109,6 +110,75 @@ class Vlbm3dDesc(C.Structure):
     ]
 
 
+class Vlbm2dDesc(C.Structure):
+    """Mirror of ``vlbm2d_desc`` in include/vlbm2d.h (field order matters)."""
+
+    _fields_ = [
+        ("struct_size", C.c_uint32),
+        ("abi_version", C.c_int32),
+        ("nx", C.c_int32),
+        ("ny", C.c_int32),
+        ("n_active", C.c_int32),
+        ("n_links", C.c_int32),
+        ("precision", C.c_int32),
+        ("model", C.c_int32),
+        ("lam", C.c_double),
+        ("mu", C.c_double),
+        ("p0", C.c_double),
+        ("p1", C.c_double),
+        ("p2", C.c_double),
+        ("j_floor", C.c_double),
+        ("dt", C.c_double),
+        ("c", C.c_double),
+        ("omega", C.c_double),
+        ("dx", C.c_double),
+        ("dy", C.c_double),
+        ("source_mode", C.c_int32),
+        ("source_params", C.c_double * 6),
+        ("recon", C.c_int32),
+        ("normal_solver", C.c_int32),
+        ("disp_interval", C.c_int32),
+        ("disp_sweeps", C.c_int32),
+        ("disp_relax", C.c_double),
+        ("disp_weight", C.c_double),
+        ("compat_interval", C.c_int32),
+        ("compat_blend", C.c_double),
+        ("compat_interior_only", C.c_int32),
+        ("n_anchor_links", C.c_int32),
+        ("U", C.c_void_p),
+        ("u", C.c_void_p),
+        ("u_star", C.c_void_p),
+        ("u_next", C.c_void_p),
+        ("record", C.c_void_p),
+        ("invalid", C.c_void_p),
+        ("reduce_scratch", C.c_void_p),
+        ("active", C.c_void_p),
+        ("active_list", C.c_void_p),
+        ("anchor_off", C.c_void_p),
+        ("anchor_ids", C.c_void_p),
+        ("link_p", C.c_void_p),
+        ("link_q", C.c_void_p),
+        ("link_bid", C.c_void_p),
+        ("link_eta", C.c_void_p),
+        ("link_xb", C.c_void_p),
+        ("link_yb", C.c_void_p),
+        ("link_nx", C.c_void_p),
+        ("link_ny", C.c_void_p),
+        ("lsq_p", C.c_void_p),
+        ("lsq_wb", C.c_void_p),
+        ("lsq_wx", C.c_void_p),
+        ("lsq_wy", C.c_void_p),
+        ("lsq_valid", C.c_void_p),
+        ("kind_by_id", C.c_void_p),
+        ("mode_by_id", C.c_void_p),
+        ("params_by_id", C.c_void_p),
+        ("dyn_tx", C.c_void_p),
+        ("dyn_ty", C.c_void_p),
+        ("use_dyn", C.c_int32),
+        ("stream", C.c_void_p),
+    ]
+
+
 EXPORTED = (
     "vlbm3d_create",
     "vlbm3d_destroy",
@@ -122,6 +192,21 @@ EXPORTED = (
     "vlbm3d_populations_from_f64",
     "vlbm3d_timing_enable",
     "vlbm3d_timing_read",
+    "vlbm2d_create",
+    "vlbm2d_destroy",
+    "vlbm2d_step",
+    "vlbm2d_refresh",
+    "vlbm2d_apply_boundary",
+    "vlbm2d_init_populations",
+    "vlbm2d_set_ustar",
+    "vlbm2d_stress",
+    "vlbm2d_post_collision",
+    "vlbm2d_set_dynamic_traction",
+    "vlbm2d_reduce",
+    "vlbm2d_populations_to_f64",
+    "vlbm2d_populations_from_f64",
+    "vlbm2d_timing_enable",
+    "vlbm2d_timing_read",
     "vlbm_last_error",
     "vlbm_abi_version",
 )
@@ -160,6 +245,21 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vlbm3d_populations_from_f64": [vp, vp, vp],
         "vlbm3d_timing_enable": [vp, C.c_int32],
         "vlbm3d_timing_read": [vp, _dp, _ip, C.c_int32],
+        "vlbm2d_create": [C.POINTER(Vlbm2dDesc), C.POINTER(vp)],
+        "vlbm2d_destroy": [vp],
+        "vlbm2d_step": [vp, vp, vp, C.c_double, C.c_int64, C.c_int32],
+        "vlbm2d_refresh": [vp, vp],
+        "vlbm2d_apply_boundary": [vp, vp, vp, C.c_double],
+        "vlbm2d_init_populations": [vp, vp],
+        "vlbm2d_set_ustar": [vp, vp],
+        "vlbm2d_stress": [vp, vp, vp],
+        "vlbm2d_post_collision": [vp, vp, vp],
+        "vlbm2d_set_dynamic_traction": [vp, C.c_int32],
+        "vlbm2d_reduce": [vp, vp, _dp],
+        "vlbm2d_populations_to_f64": [vp, vp, vp],
+        "vlbm2d_populations_from_f64": [vp, vp, vp],
+        "vlbm2d_timing_enable": [vp, C.c_int32],
+        "vlbm2d_timing_read": [vp, _dp, _ip, C.c_int32],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)

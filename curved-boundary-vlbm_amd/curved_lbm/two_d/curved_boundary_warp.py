@@ -1,0 +1,1249 @@
+"""Two-dimensional curved-boundary D2Q4x6 hyperelastic solver on MI355X.
+
+Drop-in for the reference module src/curved_lbm/two_d/curved_boundary_warp.py
+(abbreviated cbw2d below): same class names, constructor arguments, methods,
+attributes and dense array layouts ``(components, nx, ny, 1)``.  The material
+occupies phi(X) <= 0.  Velocity Dirichlet cut links use the population-pair
+identities with BFL interpolation; ``local_f`` traction links use a
+normal-probe predictor, the nonlinear normal-image traction solve and the
+characteristic closure.  A local displacement sweep and a J-guarded
+deformation-gradient update couple the kinematic fields.
+
+Host work is geometry preprocessing and orchestration; each time step runs
+in libvlbm.so (HIP, gfx950).  Provided reconstructions: ``bfl``, ``lsq_bfl``
+(``normal_probe_bfl``), ``local_f``, ``local_f_bfl``; the BGK collision.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+import json
+import math
+import os
+from pathlib import Path
+from typing import Callable, Literal
+
+import numpy as np
+
+from .vector_nonlinear_elastic_warp import (
+    BC_DIRICHLET,
+    BC_NEUMANN,
+    FQ,
+    MATERIAL_NEO_HOOKE,
+    NCOMP,
+    Q,
+    SOURCE_CONSTANT,
+    SOURCE_DAMPING,
+    SOURCE_NONE,
+    VALUE_AFFINE_VELOCITY,
+    VALUE_CONSTANT,
+    VALUE_ZERO,
+    WarpHyperelasticMaterial,
+    _CX,
+    _CY,
+    _equilibrium_np,
+    _first_piola_np,
+    _second_order_populations_masked_np,
+    _second_order_populations_np,
+    _source_np,
+    default_device,
+    source_id,
+    source_name,
+)
+
+CURVED_VALUE_ZERO = VALUE_ZERO
+CURVED_VALUE_CONSTANT_VECTOR = VALUE_CONSTANT
+CURVED_VALUE_AFFINE_VELOCITY = VALUE_AFFINE_VELOCITY
+CURVED_VALUE_PIOLA_NORMAL = 101
+CURVED_VALUE_RADIAL_TRACTION = 102
+CURVED_VALUE_RADIAL_TRACTION_SINE2_HOLD = 103
+CURVED_VALUE_RADIAL_TRACTION_SINE2_PULSE = 104
+# Extension: nominal traction amp * min(t / t_ramp, 1) * n, params (amp, t_ramp).
+CURVED_VALUE_RADIAL_TRACTION_LINEAR_RAMP = 105
+
+BOUNDARY_RECON_BFL = 0
+BOUNDARY_RECON_QBFL = 1
+BOUNDARY_RECON_LSQ_BFL = 2
+BOUNDARY_RECON_REG_BFL = 3
+BOUNDARY_RECON_RNEE = 4
+BOUNDARY_RECON_PAIR_NEE = 5
+BOUNDARY_RECON_SAME_NEE = 6
+BOUNDARY_RECON_POLY = 7
+BOUNDARY_RECON_LOWQ_REG_BFL = 8
+BOUNDARY_RECON_COMPAT_BFL = 9
+BOUNDARY_RECON_COMPAT_NEE = 10
+BOUNDARY_RECON_LOCAL_CHAR2 = 11
+BOUNDARY_RECON_LOCAL_F = 12
+BOUNDARY_RECON_LOCAL_F_BFL = 13
+PROVIDED_RECONSTRUCTIONS = (BOUNDARY_RECON_BFL, BOUNDARY_RECON_LSQ_BFL, BOUNDARY_RECON_LOCAL_F,
+                            BOUNDARY_RECON_LOCAL_F_BFL)
+
+NORMAL_JACOBIAN_FINITE_DIFFERENCE = 0
+NORMAL_JACOBIAN_ACOUSTIC = 1
+NORMAL_SOLVER_ENERGY = 2
+MAX_LSQ_STENCIL = 24
+MAX_BOUNDARY_IDS = 8
+BOUNDARY_ID_OUTER = 1
+BOUNDARY_ID_INNER = 2
+BoundaryKind = Literal["dirichlet", "neumann"]
+
+
+def neumann_solver_name(name: str | None, material: WarpHyperelasticMaterial) -> str:
+    value = os.environ.get("XLB_NEUMANN_SOLVER", "newton") if name is None else str(name)
+    key = value.strip().lower()
+    if key not in ("newton", "energy"):
+        raise ValueError(f"unsupported Neumann solver: {value!r}")
+    if key == "energy" and (
+        material.id != MATERIAL_NEO_HOOKE or not np.isfinite(material.mu) or not np.isfinite(material.lam)
+        or material.mu <= 0.0 or material.lam < 0.0
+    ):
+        raise ValueError("energy Neumann solver requires quadratic-volumetric neo_hooke, mu > 0, lambda >= 0")
+    return key
+
+
+def normal_jacobian_id(name: str | None = None) -> int:
+    value = os.environ.get("XLB_NORMAL_JACOBIAN", "acoustic") if name is None else str(name)
+    key = value.strip().lower().replace("-", "_")
+    if key in ("finite_difference", "finite", "fd"):
+        return NORMAL_JACOBIAN_FINITE_DIFFERENCE
+    if key in ("acoustic", "acoustic_tensor", "analytic", "analytical"):
+        return NORMAL_JACOBIAN_ACOUSTIC
+    raise ValueError(f"unsupported normal Jacobian mode: {value!r}")
+
+
+def normal_jacobian_name(mode: int) -> str:
+    return {0: "finite_difference", 1: "acoustic"}.get(int(mode), f"unknown({mode})")
+
+
+_RECON_BY_NAME = {
+    "bfl": 0, "qbfl": 1, "lsq_bfl": 2, "normal_probe_bfl": 2, "regularized_bfl": 3, "regularized_nee": 4,
+    "pair_nee": 5, "same_nee": 6, "poly_ibb": 7, "lowq_reg_bfl": 8, "compat_bfl": 9, "compat_nee": 10,
+    "local_char2": 11, "char2": 11, "characteristic2": 11, "second_order": 11, "strict2": 11,
+    "local_f": 12, "fully_local": 12, "local_f_bfl": 13, "single_history": 13,
+}
+_RECON_NAME = {0: "bfl", 1: "qbfl", 2: "lsq_bfl", 3: "regularized_bfl", 4: "regularized_nee", 5: "pair_nee",
+               6: "same_nee", 7: "poly_ibb", 8: "lowq_reg_bfl", 9: "compat_bfl", 10: "compat_nee",
+               11: "local_char2", 12: "local_f", 13: "local_f_bfl"}
+
+
+def boundary_reconstruction_id(name: str | int) -> int:
+    if isinstance(name, int):
+        return int(name)
+    key = str(name).strip().lower()
+    if key not in _RECON_BY_NAME:
+        raise ValueError(f"unsupported boundary_reconstruction: {name}")
+    return _RECON_BY_NAME[key]
+
+
+def boundary_reconstruction_name(rid: int) -> str:
+    return _RECON_NAME.get(int(rid), f"unknown_{rid}")
+
+
+def boundary_reconstruction_stencil_radius(rid: int) -> int:
+    """Maximum lattice radius read by one boundary-reconstruction kernel (cbw2d:210-229)."""
+    if rid in (BOUNDARY_RECON_LSQ_BFL, BOUNDARY_RECON_COMPAT_BFL, BOUNDARY_RECON_COMPAT_NEE,
+               BOUNDARY_RECON_LOCAL_CHAR2, BOUNDARY_RECON_LOCAL_F):
+        return 3
+    if rid in (BOUNDARY_RECON_QBFL, BOUNDARY_RECON_POLY):
+        return 2
+    return 1
+
+
+@dataclass(frozen=True)
+class CurvedBoundarySpec:
+    """Boundary kind, value mode and up to eight parameters per boundary id
+    (cbw2d:233-305).  Dirichlet prescribes velocity, Neumann nominal traction."""
+
+    kind: BoundaryKind = "dirichlet"
+    value_mode: int = CURVED_VALUE_ZERO
+    params: tuple[float, ...] = ()
+    id_kinds: dict[int, BoundaryKind] | None = None
+    id_value_modes: dict[int, int] | None = None
+    id_params: dict[int, tuple[float, ...]] | None = None
+    label: str = ""
+
+    @property
+    def kind_id(self) -> int:
+        if self.kind == "dirichlet":
+            return BC_DIRICHLET
+        if self.kind == "neumann":
+            return BC_NEUMANN
+        raise ValueError(f"unsupported curved boundary kind: {self.kind}")
+
+    @property
+    def params8(self) -> np.ndarray:
+        out = np.zeros(8, dtype=np.float64)
+        vals = tuple(self.params)[:8]
+        out[: len(vals)] = [float(v) for v in vals]
+        return out
+
+    def tables(self, max_ids: int = MAX_BOUNDARY_IDS) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        kind = np.full(max_ids, self.kind_id, dtype=np.int32)
+        mode = np.full(max_ids, int(self.value_mode), dtype=np.int32)
+        params = np.tile(self.params8[None, :], (max_ids, 1))
+        for bid, value in (self.id_kinds or {}).items():
+            if 0 <= int(bid) < max_ids:
+                if value == "dirichlet":
+                    kind[int(bid)] = BC_DIRICHLET
+                elif value == "neumann":
+                    kind[int(bid)] = BC_NEUMANN
+                else:
+                    raise ValueError(f"unsupported boundary kind for id {bid}: {value}")
+        for bid, value in (self.id_value_modes or {}).items():
+            if 0 <= int(bid) < max_ids:
+                mode[int(bid)] = int(value)
+        for bid, values in (self.id_params or {}).items():
+            if 0 <= int(bid) < max_ids:
+                row = np.zeros(8)
+                vv = tuple(values)[:8]
+                row[: len(vv)] = [float(v) for v in vv]
+                params[int(bid)] = row
+        return kind, mode, np.ascontiguousarray(params)
+
+
+@dataclass(frozen=True)
+class CircleDomain:
+    center_x: float = 0.5
+    center_y: float = 0.5
+    radius: float = 0.38
+
+    def phi(self, x, y):
+        return np.sqrt((np.asarray(x) - self.center_x) ** 2 + (np.asarray(y) - self.center_y) ** 2) - self.radius
+
+    def normal(self, x, y):
+        rx = np.asarray(x) - self.center_x
+        ry = np.asarray(y) - self.center_y
+        r = np.maximum(np.sqrt(rx * rx + ry * ry), np.finfo(float).eps)
+        return rx / r, ry / r
+
+
+def _local_quadratic_fit(active, x_coords, y_coords, i0, j0, xb, yb, dx, weight_sigma, max_stencil):
+    """Candidates within r^2 <= 9.5 (lattice units) of the cut point in the
+    7x7 box, sorted by distance (stable), and the weighted quadratic LSQ map
+    from sample values to polynomial coefficients; None when rank-deficient
+    or ill-conditioned."""
+    nx, ny = active.shape
+    cand = []
+    for ii in range(max(0, i0 - 3), min(nx, i0 + 4)):
+        for jj in range(max(0, j0 - 3), min(ny, j0 + 4)):
+            if active[ii, jj] == 0:
+                continue
+            rx = (float(x_coords[ii]) - xb) / dx
+            ry = (float(y_coords[jj]) - yb) / dx
+            r2 = rx * rx + ry * ry
+            if r2 <= 9.5:
+                cand.append((r2, ii, jj, rx, ry))
+    cand.sort(key=lambda item: item[0])
+    if len(cand) < 6:
+        return None
+    cand = cand[:max_stencil]
+    A = np.array([(1.0, rx, ry, 0.5 * rx * rx, rx * ry, 0.5 * ry * ry) for _r2, _i, _j, rx, ry in cand])
+    wd = np.array([math.exp(-0.5 * r2 / (weight_sigma * weight_sigma)) + 1.0e-4 for r2, *_ in cand])
+    if np.linalg.matrix_rank(A, tol=1.0e-10) < 6:
+        return None
+    M = A.T @ (A * wd[:, None])
+    cond = np.linalg.cond(M)
+    if not np.isfinite(cond) or cond > 1.0e8:
+        return None
+    try:
+        coeff = np.linalg.solve(M + 1.0e-12 * np.eye(6), A.T * wd[None, :])
+    except np.linalg.LinAlgError:
+        return None
+    return cand, coeff
+
+
+def _build_boundary_lsq_weights(*, active, x_coords, y_coords, link_i, link_j, link_xb, link_yb, link_nx,
+                                link_ny, dx, max_stencil=MAX_LSQ_STENCIL):
+    """Per-link boundary weights (cbw2d:332-606):
+
+    * ``wb``: value at the cut point from two bilinear probes on the inward
+      normal, (h2 U(X1) - h1 U(X2)) / (h2 - h1), with (h1, h2) tried in
+      (0.75,1.75), (1,2), (1.25,2.5), (1.5,3), else one probe at
+      0.75, 1, 1.25, 1.5, 2 cells; entries with |w| > 1e-14 sorted by
+      distance to the link node;
+    * ``wt``: tangential derivative, ``wx``/``wy``: Cartesian derivatives, from
+      a Gaussian-weighted quadratic fit over at most 24 nodes within radius
+      sqrt(9.5), sharing index slots with ``wb`` (new nodes appended).
+    ``valid`` is set when the value weights exist.
+    """
+    n_links = int(link_i.size)
+    nx, ny = active.shape
+    out_i = np.full((n_links, max_stencil), -1, dtype=np.int32)
+    out_j = np.full((n_links, max_stencil), -1, dtype=np.int32)
+    out_w = np.zeros((n_links, max_stencil))
+    out_wt = np.zeros((n_links, max_stencil))
+    out_wx = np.zeros((n_links, max_stencil))
+    out_wy = np.zeros((n_links, max_stencil))
+    valid = np.zeros(n_links, dtype=np.int32)
+    dx = float(dx)
+    x0g = float(x_coords[0]) if len(x_coords) else 0.5 * dx
+    y0g = float(y_coords[0]) if len(y_coords) else 0.5 * dx
+
+    def bilinear(xp, yp):
+        rx = (xp - x0g) / dx
+        ry = (yp - y0g) / dx
+        i0 = math.floor(rx)
+        j0 = math.floor(ry)
+        sx = rx - i0
+        sy = ry - j0
+        if i0 < 0 or j0 < 0 or i0 + 1 >= nx or j0 + 1 >= ny:
+            return None
+        nodes = [(i0, j0, (1.0 - sx) * (1.0 - sy)), (i0 + 1, j0, sx * (1.0 - sy)),
+                 (i0, j0 + 1, (1.0 - sx) * sy), (i0 + 1, j0 + 1, sx * sy)]
+        for ii, jj, _w in nodes:
+            if active[ii, jj] == 0:
+                return None
+        return nodes
+
+    def place(ell, cand, cols):
+        """Assign sample slots (existing slots reused, new nodes appended)."""
+        slot_of = {}
+        for m in range(max_stencil):
+            if out_i[ell, m] >= 0 and out_j[ell, m] >= 0:
+                slot_of[(int(out_i[ell, m]), int(out_j[ell, m]))] = m
+        nxt = len(slot_of)
+        for m, (_r2, ii, jj, _rx, _ry) in enumerate(cand):
+            key = (int(ii), int(jj))
+            slot = slot_of.get(key)
+            if slot is None:
+                if nxt >= max_stencil:
+                    return False
+                slot = nxt
+                nxt += 1
+                slot_of[key] = slot
+                out_i[ell, slot] = int(ii)
+                out_j[ell, slot] = int(jj)
+            for dst, w in cols:
+                dst[ell, slot] = float(w[m])
+        return True
+
+    for ell in range(n_links):
+        xb, yb = float(link_xb[ell]), float(link_yb[ell])
+        nxb, nyb = float(link_nx[ell]), float(link_ny[ell])
+        i0, j0 = int(link_i[ell]), int(link_j[ell])
+        accum: dict = {}
+        ok = False
+        for h1, h2 in ((0.75, 1.75), (1.0, 2.0), (1.25, 2.5), (1.5, 3.0)):
+            w1 = bilinear(xb - h1 * dx * nxb, yb - h1 * dx * nyb)
+            w2 = bilinear(xb - h2 * dx * nxb, yb - h2 * dx * nyb)
+            if w1 is None or w2 is None:
+                continue
+            accum.clear()
+            c1, c2 = h2 / (h2 - h1), -h1 / (h2 - h1)
+            for ii, jj, ww in w1:
+                accum[(int(ii), int(jj))] = accum.get((int(ii), int(jj)), 0.0) + c1 * float(ww)
+            for ii, jj, ww in w2:
+                accum[(int(ii), int(jj))] = accum.get((int(ii), int(jj)), 0.0) + c2 * float(ww)
+            ok = True
+            break
+        if not ok:
+            for h in (0.75, 1.0, 1.25, 1.5, 2.0):
+                w1 = bilinear(xb - h * dx * nxb, yb - h * dx * nyb)
+                if w1 is None:
+                    continue
+                accum.clear()
+                for ii, jj, ww in w1:
+                    accum[(int(ii), int(jj))] = accum.get((int(ii), int(jj)), 0.0) + float(ww)
+                ok = True
+                break
+        if not ok:
+            continue
+        items = [(ij, w) for ij, w in accum.items() if abs(w) > 1.0e-14]
+        items.sort(key=lambda it: (it[0][0] - i0) ** 2 + (it[0][1] - j0) ** 2)
+        if len(items) > max_stencil:
+            continue
+        if not items or not np.isfinite([w for _ij, w in items]).all() or sum(abs(w) for _ij, w in items) > 3.5:
+            continue
+        for m, ((ii, jj), ww) in enumerate(items):
+            out_i[ell, m], out_j[ell, m], out_w[ell, m] = int(ii), int(jj), float(ww)
+        fit = _local_quadratic_fit(active, x_coords, y_coords, i0, j0, xb, yb, dx, 1.5, max_stencil)
+        if fit is not None:
+            cand, coeff = fit
+            tx, ty = -nyb, nxb
+            wt = np.array([0.0, tx / dx, ty / dx, 0.0, 0.0, 0.0]) @ coeff
+            if np.all(np.isfinite(wt)) and abs(np.sum(wt)) * dx <= 1.0e-6 and np.sum(np.abs(wt)) * dx <= 8.0:
+                if not place(ell, cand, ((out_wt, wt),)):
+                    out_wt[ell, :] = 0.0
+            wx = np.array([0.0, 1.0 / dx, 0.0, 0.0, 0.0, 0.0]) @ coeff
+            wy = np.array([0.0, 0.0, 1.0 / dx, 0.0, 0.0, 0.0]) @ coeff
+            if (np.all(np.isfinite(wx)) and np.all(np.isfinite(wy)) and abs(np.sum(wx)) * dx <= 1.0e-6
+                    and abs(np.sum(wy)) * dx <= 1.0e-6 and np.sum(np.abs(wx)) * dx <= 12.0
+                    and np.sum(np.abs(wy)) * dx <= 12.0):
+                if not place(ell, cand, ((out_wx, wx), (out_wy, wy))):
+                    out_wx[ell, :] = 0.0
+                    out_wy[ell, :] = 0.0
+        valid[ell] = 1
+    return tuple(np.ascontiguousarray(a) for a in (out_i, out_j, out_w, out_wt, out_wx, out_wy, valid))
+
+
+@dataclass
+class CurvedBoundaryGeometry:
+    """Active mask, cut links and boundary stencils of a 2D level-set body."""
+
+    nx: int
+    ny: int
+    length_x: float
+    length_y: float
+    active: np.ndarray
+    link_i: np.ndarray
+    link_j: np.ndarray
+    link_q: np.ndarray
+    link_boundary_id: np.ndarray
+    link_eta: np.ndarray
+    link_xb: np.ndarray
+    link_yb: np.ndarray
+    link_nx: np.ndarray
+    link_ny: np.ndarray
+    label: str = "implicit"
+    lsq_i: np.ndarray | None = None
+    lsq_j: np.ndarray | None = None
+    lsq_wb: np.ndarray | None = None
+    lsq_wt: np.ndarray | None = None
+    lsq_wx: np.ndarray | None = None
+    lsq_wy: np.ndarray | None = None
+    lsq_valid: np.ndarray | None = None
+
+    @property
+    def dx(self) -> float:
+        return self.length_x / float(self.nx)
+
+    @property
+    def dy(self) -> float:
+        return self.length_y / float(self.ny)
+
+    @property
+    def n_active(self) -> int:
+        return int(np.count_nonzero(self.active))
+
+    @property
+    def n_links(self) -> int:
+        return int(self.link_i.size)
+
+    def as_metadata(self):
+        return {
+            "label": self.label, "nx": self.nx, "ny": self.ny, "length_x": self.length_x,
+            "length_y": self.length_y, "dx": self.dx, "dy": self.dy, "n_active": self.n_active,
+            "n_cut_links": self.n_links,
+            "boundary_ids": sorted(int(v) for v in np.unique(self.link_boundary_id)) if self.n_links else [],
+        }
+
+    @classmethod
+    def from_level_set(cls, *, nx: int, ny: int, length_x: float, length_y: float,
+                       phi: Callable[[float, float], float],
+                       normal: Callable[[float, float], tuple[float, float]] | None = None,
+                       boundary_id: Callable[[float, float, float, float], int] | None = None,
+                       label: str = "level_set", root_iterations: int = 40) -> "CurvedBoundaryGeometry":
+        """Active mask phi <= 0 and cut links (cbw2d:669-858): every (active
+        node, direction) whose neighbour is inactive or off the lattice, in
+        node-major then direction order.  The cut fraction comes from
+        ``root_iterations`` bisections (after a 17-sample sign search when the
+        link end is not outside), the cut point from the unclipped fraction,
+        and eta is then clipped to [1e-6, 1 - 1e-6]."""
+        if nx < 4 or ny < 4:
+            raise ValueError("nx and ny must be at least four")
+        dx = float(length_x) / float(nx)
+        dy = float(length_y) / float(ny)
+        if abs(dx - dy) > 1.0e-14:
+            raise ValueError("D2Q4 curved-boundary implementation requires dx=dy")
+        x = (np.arange(nx, dtype=np.float64) + 0.5) * dx
+        y = (np.arange(ny, dtype=np.float64) + 0.5) * dy
+        xx, yy = np.meshgrid(x, y, indexing="ij")
+        ph = np.empty((nx, ny))
+        for i in range(nx):
+            for j in range(ny):
+                ph[i, j] = float(phi(float(xx[i, j]), float(yy[i, j])))
+        active = ph <= 0.0
+        if not np.any(active):
+            raise ValueError("level-set contains no active material nodes")
+
+        def phis(xp, yp):
+            return float(phi(float(xp), float(yp)))
+
+        def norms(xp, yp):
+            if normal is not None:
+                a, b = normal(float(xp), float(yp))
+                a, b = float(a), float(b)
+            else:
+                h = 0.5 * min(dx, dy)
+                a = (phis(xp + h, yp) - phis(xp - h, yp)) / (2.0 * h)
+                b = (phis(xp, yp + h) - phis(xp, yp - h)) / (2.0 * h)
+            nrm = math.hypot(a, b)
+            if nrm <= 1.0e-30:
+                raise ValueError("zero level-set normal encountered")
+            return a / nrm, b / nrm
+
+        def bisect(x0, y0, x1, y1, lo, hi):
+            for _ in range(root_iterations):
+                mid = 0.5 * (lo + hi)
+                if phis(x0 + mid * (x1 - x0), y0 + mid * (y1 - y0)) <= 0.0:
+                    lo = mid
+                else:
+                    hi = mid
+            return 0.5 * (lo + hi)
+
+        rows = []
+        for i in range(nx):
+            for j in range(ny):
+                if not active[i, j]:
+                    continue
+                x0, y0, p0 = float(xx[i, j]), float(yy[i, j]), float(ph[i, j])
+                for q in range(Q):
+                    ii, jj = i + int(_CX[q]), j + int(_CY[q])
+                    if 0 <= ii < nx and 0 <= jj < ny and bool(active[ii, jj]):
+                        continue
+                    x1 = x0 + float(_CX[q]) * dx
+                    y1 = y0 + float(_CY[q]) * dy
+                    p1 = phis(x1, y1)
+                    if not (p0 <= 0.0 and p1 >= 0.0):
+                        samples = np.linspace(0.0, 1.0, 17)
+                        vals = [phis(x0 + s * (x1 - x0), y0 + s * (y1 - y0)) for s in samples]
+                        idx = next((m for m in range(len(samples) - 1) if vals[m] <= 0.0 and vals[m + 1] >= 0.0), None)
+                        if idx is None:
+                            eta = 0.5
+                        else:
+                            eta = bisect(x0, y0, x1, y1, float(samples[idx]), float(samples[idx + 1]))
+                    else:
+                        eta = bisect(x0, y0, x1, y1, 0.0, 1.0)
+                    xb = x0 + eta * (x1 - x0)
+                    yb = y0 + eta * (y1 - y0)
+                    nxv, nyv = norms(xb, yb)
+                    eta = float(np.clip(eta, 1.0e-6, 1.0 - 1.0e-6))
+                    bid = 1 if boundary_id is None else int(boundary_id(xb, yb, nxv, nyv))
+                    rows.append((i, j, q, max(0, min(MAX_BOUNDARY_IDS - 1, bid)), eta, xb, yb, nxv, nyv))
+        cols = list(zip(*rows)) if rows else [()] * 9
+        li, lj, lq, lb = (np.ascontiguousarray(np.asarray(c, dtype=np.int32)) for c in cols[:4])
+        le, lxb, lyb, lnx, lny = (np.ascontiguousarray(np.asarray(c, dtype=np.float64)) for c in cols[4:])
+        lsq = _build_boundary_lsq_weights(active=active.astype(np.int32), x_coords=x, y_coords=y, link_i=li,
+                                          link_j=lj, link_xb=lxb, link_yb=lyb, link_nx=lnx, link_ny=lny, dx=dx)
+        return cls(nx=int(nx), ny=int(ny), length_x=float(length_x), length_y=float(length_y),
+                   active=np.ascontiguousarray(active.astype(np.int32)), link_i=li, link_j=lj, link_q=lq,
+                   link_boundary_id=lb, link_eta=le, link_xb=lxb, link_yb=lyb, link_nx=lnx, link_ny=lny,
+                   label=label, lsq_i=lsq[0], lsq_j=lsq[1], lsq_wb=lsq[2], lsq_wt=lsq[3], lsq_wx=lsq[4],
+                   lsq_wy=lsq[5], lsq_valid=lsq[6])
+
+    @classmethod
+    def circle(cls, *, nx: int, ny: int, length_x: float = 1.0, length_y: float = 1.0, center_x: float = 0.5,
+               center_y: float = 0.5, radius: float = 0.38) -> "CurvedBoundaryGeometry":
+        return cls.from_level_set(
+            nx=nx, ny=ny, length_x=length_x, length_y=length_y,
+            phi=lambda x, y: float(math.hypot(x - center_x, y - center_y) - radius),
+            normal=lambda x, y: ((x - center_x) / max(math.hypot(x - center_x, y - center_y), 1.0e-30),
+                                 (y - center_y) / max(math.hypot(x - center_x, y - center_y), 1.0e-30)),
+            label=f"circle(cx={center_x},cy={center_y},r={radius})")
+
+    @classmethod
+    def annulus(cls, *, nx: int, ny: int, length_x: float = 1.0, length_y: float = 1.0, center_x: float = 0.5,
+                center_y: float = 0.5, inner_radius: float = 0.18, outer_radius: float = 0.42):
+        if not 0.0 < inner_radius < outer_radius:
+            raise ValueError("annulus requires 0 < inner_radius < outer_radius")
+
+        def radius(x, y):
+            return math.hypot(x - center_x, y - center_y)
+
+        def phi(x, y):
+            r = radius(x, y)
+            return max(r - outer_radius, inner_radius - r)
+
+        def normal(x, y):
+            r = max(radius(x, y), 1.0e-30)
+            ex, ey = (x - center_x) / r, (y - center_y) / r
+            if abs(r - inner_radius) <= abs(r - outer_radius):
+                return -ex, -ey
+            return ex, ey
+
+        def bid(x, y, _nx, _ny):
+            r = radius(x, y)
+            return BOUNDARY_ID_INNER if abs(r - inner_radius) <= abs(r - outer_radius) else BOUNDARY_ID_OUTER
+
+        return cls.from_level_set(
+            nx=nx, ny=ny, length_x=length_x, length_y=length_y, phi=phi, normal=normal, boundary_id=bid,
+            label=f"annulus(cx={center_x},cy={center_y},ri={inner_radius},ro={outer_radius})")
+
+
+def relative_l2(error, reference, mask) -> float:
+    err, ref, m = np.asarray(error), np.asarray(reference), np.asarray(mask, dtype=bool)
+    num = float(np.sqrt(np.sum(err[..., m] ** 2)))
+    den = float(np.sqrt(np.sum(ref[..., m] ** 2)))
+    return num / max(den, np.finfo(float).eps)
+
+
+def constant_piola_params(material: WarpHyperelasticMaterial, F: np.ndarray) -> tuple[float, float, float, float]:
+    P = _first_piola_np(np.asarray(F, dtype=np.float64).reshape(2, 2), material)
+    return float(P[0, 0]), float(P[0, 1]), float(P[1, 0]), float(P[1, 1])
+
+
+__all__ = [
+    "WarpHyperelasticMaterial", "CurvedBoundarySpec", "CurvedBoundaryGeometry", "CircleDomain",
+    "WarpCurvedBoundaryLBM2D", "relative_l2", "constant_piola_params",
+    "CURVED_VALUE_ZERO", "CURVED_VALUE_CONSTANT_VECTOR", "CURVED_VALUE_AFFINE_VELOCITY",
+    "CURVED_VALUE_PIOLA_NORMAL", "CURVED_VALUE_RADIAL_TRACTION", "CURVED_VALUE_RADIAL_TRACTION_SINE2_HOLD",
+    "CURVED_VALUE_RADIAL_TRACTION_SINE2_PULSE", "CURVED_VALUE_RADIAL_TRACTION_LINEAR_RAMP",
+    "BOUNDARY_ID_OUTER", "BOUNDARY_ID_INNER",
+]
+
+
+class WarpCurvedBoundaryLBM2D:
+    """D2Q4x6 hyperelastic LBM on an implicit 2-D domain, stepped by libvlbm.so.
+
+    Constructor, methods and attributes follow the reference class
+    (cbw2d:3618-4472).  Device arrays are ``curved_lbm.device.DeviceArray``
+    objects with the reference shapes ``(components, nx, ny, 1)``.  Extra
+    keyword ``precision="fp32"`` stores the populations as float deviations
+    from the rest state (all other state and arithmetic stay FP64).
+    Provided: the BGK collision and the ``bfl``, ``lsq_bfl``
+    (``normal_probe_bfl``), ``local_f`` and ``local_f_bfl`` reconstructions;
+    other choices raise ``ValueError``.
+    """
+
+    def __init__(
+        self,
+        geometry: CurvedBoundaryGeometry,
+        *,
+        material: WarpHyperelasticMaterial,
+        boundary: CurvedBoundarySpec,
+        lattice_speed: float = 5.0,
+        collision_omega: float = 2.0,
+        collision_model: str = "bgk",
+        ghost_omega: float | None = None,
+        kinetic_filter_strength: float = 0.0,
+        boundary_reconstruction: str = "bfl",
+        device: str | None = None,
+        check_linearized_cfl: bool = True,
+        jacobian_floor: float = 1.0e-12,
+        source_mode: str = "none",
+        source_vector: tuple[float, ...] = (0.0, 0.0, 0.0, 0.0, 0.0, 0.0),
+        damping_gamma: float = 0.0,
+        compatibility_projection_interval: int = 0,
+        compatibility_projection_iterations: int = 160,
+        compatibility_projection_relax: float = 0.85,
+        compatibility_projection_weight: float = 20.0,
+        compatibility_projection_repair_F: bool = False,
+        compatibility_projection_repair_F_band_width: int = -1,
+        compatibility_projection_repair_F_blend: float = 1.0,
+        compatibility_projection_boundary_id: int = BOUNDARY_ID_INNER,
+        local_compatibility_interval: int = 0,
+        local_compatibility_blend: float = 0.05,
+        local_compatibility_interior_only: bool = True,
+        local_displacement_interval: int = 0,
+        local_displacement_sweeps: int = 1,
+        local_displacement_relax: float = 0.85,
+        local_displacement_boundary_weight: float = 20.0,
+        local_displacement_boundary_id: int = BOUNDARY_ID_INNER,
+        normal_jacobian: str | None = None,
+        neumann_solver: str | None = None,
+        precision: str = "fp64",
+    ) -> None:
+        self.geometry = geometry
+        self.nx, self.ny = int(geometry.nx), int(geometry.ny)
+        self.length_x, self.length_y = float(geometry.length_x), float(geometry.length_y)
+        self.dx, self.dy = geometry.dx, geometry.dy
+        self.material = material
+        self.boundary = boundary
+        self.lattice_speed = float(lattice_speed)
+        self.collision_omega = float(collision_omega)
+        if not 0.0 < self.collision_omega <= 2.0:
+            raise ValueError("collision_omega must be in (0,2]")
+        if collision_model not in ("bgk", "mrt_ghost"):
+            raise ValueError("collision_model must be 'bgk' or 'mrt_ghost'")
+        if collision_model != "bgk":
+            raise ValueError("collision_model 'mrt_ghost' is not provided by the HIP solver")
+        self.collision_model = "bgk"
+        self.ghost_omega = float(self.collision_omega if ghost_omega is None else ghost_omega)
+        if not 0.0 < self.ghost_omega <= 2.0:
+            raise ValueError("ghost_omega must be in (0,2]")
+        self.kinetic_filter_strength = float(kinetic_filter_strength)
+        if not 0.0 <= self.kinetic_filter_strength <= 1.0:
+            raise ValueError("kinetic_filter_strength must be in [0,1]")
+        if self.kinetic_filter_strength > 0.0:
+            raise ValueError("the kinetic ghost filter is not provided by the HIP solver")
+        self.boundary_reconstruction = boundary_reconstruction_id(boundary_reconstruction)
+        self.boundary_reconstruction_name = boundary_reconstruction_name(self.boundary_reconstruction)
+        if self.boundary_reconstruction not in PROVIDED_RECONSTRUCTIONS:
+            raise ValueError(
+                f"boundary_reconstruction {self.boundary_reconstruction_name!r} is not provided "
+                "(bfl, lsq_bfl/normal_probe_bfl, local_f, local_f_bfl)")
+        self.boundary_stencil_radius = boundary_reconstruction_stencil_radius(self.boundary_reconstruction)
+        self.normal_jacobian = normal_jacobian_id(normal_jacobian)
+        self.normal_jacobian_name = normal_jacobian_name(self.normal_jacobian)
+        self.neumann_solver_name = neumann_solver_name(neumann_solver, material)
+        self._normal_solver_mode = NORMAL_SOLVER_ENERGY if self.neumann_solver_name == "energy" else self.normal_jacobian
+        if check_linearized_cfl and self.material.linearized_stability_ratio(self.lattice_speed) >= 1.0:
+            raise ValueError("linearized D2Q4 stability ratio must be below one")
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        self.precision = precision
+        self.dt = self.dx / self.lattice_speed
+        self.time = 0.0
+        self.steps = 0
+        self.shape = (self.nx, self.ny, 1)
+        self.jacobian_floor = float(jacobian_floor)
+        self._source_name_func = source_name
+        self.source_mode = source_id(source_mode)
+        self.source_params = np.zeros(NCOMP, dtype=np.float64)
+        if self.source_mode == SOURCE_CONSTANT:
+            for idx, value in enumerate(tuple(source_vector)[:NCOMP]):
+                self.source_params[idx] = float(value)
+        elif self.source_mode == SOURCE_DAMPING:
+            if damping_gamma < 0.0:
+                raise ValueError("damping_gamma must be non-negative")
+            self.source_params[0] = float(damping_gamma)
+        if int(compatibility_projection_interval) != 0:
+            raise ValueError("the fully-local solver forbids compatibility_projection_interval != 0")
+        self.compatibility_projection_interval = 0
+        self.compatibility_projection_iterations = int(max(1, compatibility_projection_iterations))
+        self.compatibility_projection_relax = float(compatibility_projection_relax)
+        self.compatibility_projection_weight = float(compatibility_projection_weight)
+        self.compatibility_projection_repair_F = False
+        self.compatibility_projection_repair_F_band_width = int(compatibility_projection_repair_F_band_width)
+        self.compatibility_projection_repair_F_blend = float(compatibility_projection_repair_F_blend)
+        self.compatibility_projection_boundary_id = int(compatibility_projection_boundary_id)
+        self._compatibility_projector = None
+        self.local_compatibility_interval = int(max(0, local_compatibility_interval))
+        self.local_compatibility_blend = float(local_compatibility_blend)
+        self.local_compatibility_interior_only = bool(local_compatibility_interior_only)
+        if not 0.0 <= self.local_compatibility_blend <= 1.0:
+            raise ValueError("local_compatibility_blend must be in [0,1]")
+        self.local_displacement_interval = int(max(0, local_displacement_interval))
+        self.local_displacement_sweeps = int(local_displacement_sweeps)
+        self.local_displacement_relax = float(local_displacement_relax)
+        self.local_displacement_boundary_weight = float(local_displacement_boundary_weight)
+        self.local_displacement_boundary_id = int(local_displacement_boundary_id)
+        if not 0.0 <= self.local_displacement_relax <= 1.0:
+            raise ValueError("local_displacement_relax must be in [0,1]")
+        if not 1 <= self.local_displacement_sweeps <= 8:
+            raise ValueError("local_displacement_sweeps must be in [1,8] to keep a bounded local dependency radius")
+        if self.local_displacement_boundary_weight < 0.0:
+            raise ValueError("local_displacement_boundary_weight must be non-negative")
+
+        x = (np.arange(self.nx, dtype=np.float64) + 0.5) * self.dx
+        y = (np.arange(self.ny, dtype=np.float64) + 0.5) * self.dy
+        self.x, self.y = np.meshgrid(x, y, indexing="ij")
+        self.active_np = np.ascontiguousarray(geometry.active.astype(np.int32))
+        L = geometry.n_links
+        if geometry.lsq_i is None or geometry.lsq_j is None or geometry.lsq_wb is None or geometry.lsq_valid is None:
+            lsq_i_np = np.full((L, MAX_LSQ_STENCIL), -1, dtype=np.int32)
+            lsq_j_np = np.full((L, MAX_LSQ_STENCIL), -1, dtype=np.int32)
+            lsq_wb_np = np.zeros((L, MAX_LSQ_STENCIL))
+            lsq_wt_np = np.zeros((L, MAX_LSQ_STENCIL))
+            lsq_wx_np = np.zeros((L, MAX_LSQ_STENCIL))
+            lsq_wy_np = np.zeros((L, MAX_LSQ_STENCIL))
+            lsq_valid_np = np.zeros(L, dtype=np.int32)
+        else:
+            lsq_i_np, lsq_j_np, lsq_wb_np = geometry.lsq_i, geometry.lsq_j, geometry.lsq_wb
+            lsq_wt_np = geometry.lsq_wt if geometry.lsq_wt is not None else np.zeros_like(lsq_wb_np)
+            lsq_wx_np = geometry.lsq_wx if geometry.lsq_wx is not None else np.zeros_like(lsq_wb_np)
+            lsq_wy_np = geometry.lsq_wy if geometry.lsq_wy is not None else np.zeros_like(lsq_wb_np)
+            lsq_valid_np = geometry.lsq_valid
+        self.lsq_i_np = np.asarray(lsq_i_np, dtype=np.int32)
+        self.lsq_j_np = np.asarray(lsq_j_np, dtype=np.int32)
+        self.lsq_wb_np = np.asarray(lsq_wb_np, dtype=np.float64)
+        self.lsq_wt_np = np.asarray(lsq_wt_np, dtype=np.float64)
+        self.lsq_wx_np = np.asarray(lsq_wx_np, dtype=np.float64)
+        self.lsq_wy_np = np.asarray(lsq_wy_np, dtype=np.float64)
+        self.lsq_valid_np = np.asarray(lsq_valid_np, dtype=np.int32)
+        self.boundary_params = boundary.params8
+        kind_by_id, mode_by_id, params_by_id = boundary.tables(MAX_BOUNDARY_IDS)
+        self.boundary_kind_by_id_np = kind_by_id
+        self.boundary_mode_by_id_np = mode_by_id
+        self.boundary_params_by_id_np = params_by_id
+        self.use_dynamic_link_traction = False
+        self._setup_device(device)
+        self.initialize_identity(init_order=1)
+
+    # ---- device setup ------------------------------------------------------
+    def _setup_device(self, device) -> None:
+        import torch
+
+        from .. import _native
+        from ..device import DeviceArray, PopulationArray, hip_device
+
+        self._torch = torch
+        self._native = _native
+        self._lib = _native.load_library()
+        self.device = str(hip_device(device))
+        dev = torch.device(self.device)
+        g = self.geometry
+        nx, ny, L = self.nx, self.ny, g.n_links
+        i32, f64 = np.int32, np.float64
+
+        def dev_i(a):
+            return DeviceArray.from_numpy(a, i32, dev)
+
+        def dev_f(a):
+            return DeviceArray.from_numpy(a, f64, dev)
+
+        act = self.active_np.astype(bool)
+        active_list = np.flatnonzero(act.ravel()).astype(np.int32)
+        self.n_active = int(active_list.size)
+        self.active = dev_i(self.active_np[..., None])
+        self.active_list = dev_i(active_list)
+        # Per-link tables; node index p = i*ny + j.
+        link_p = (np.asarray(g.link_i, np.int64) * ny + np.asarray(g.link_j, np.int64)).astype(np.int32)
+        self.link_i, self.link_j, self.link_q = dev_i(g.link_i), dev_i(g.link_j), dev_i(g.link_q)
+        self.link_p = dev_i(link_p)
+        self.link_boundary_id = dev_i(g.link_boundary_id)
+        self.link_eta, self.link_xb, self.link_yb = dev_f(g.link_eta), dev_f(g.link_xb), dev_f(g.link_yb)
+        self.link_nx, self.link_ny = dev_f(g.link_nx), dev_f(g.link_ny)
+        self.lsq_i, self.lsq_j = dev_i(self.lsq_i_np), dev_i(self.lsq_j_np)
+        self.lsq_wb, self.lsq_wt = dev_f(self.lsq_wb_np), dev_f(self.lsq_wt_np)
+        self.lsq_wx, self.lsq_wy = dev_f(self.lsq_wx_np), dev_f(self.lsq_wy_np)
+        self.lsq_valid = dev_i(self.lsq_valid_np)
+        # Stencil tables transposed to (24, L) for coalesced per-link reads.
+        li = self.lsq_i_np.reshape(L, MAX_LSQ_STENCIL)
+        lj = self.lsq_j_np.reshape(L, MAX_LSQ_STENCIL)
+        lsq_p = np.where((li >= 0) & (lj >= 0), li.astype(np.int64) * ny + lj, -1).astype(np.int32)
+        self._lsq_p_t = dev_i(np.ascontiguousarray(lsq_p.T))
+        self._lsq_wb_t = dev_f(np.ascontiguousarray(self.lsq_wb_np.reshape(L, -1).T))
+        self._lsq_wx_t = dev_f(np.ascontiguousarray(self.lsq_wx_np.reshape(L, -1).T))
+        self._lsq_wy_t = dev_f(np.ascontiguousarray(self.lsq_wy_np.reshape(L, -1).T))
+        # Dirichlet displacement anchors (cbw2d:3288-3304): links of the
+        # chosen boundary id, or of Dirichlet ids when that id is negative.
+        bids = np.clip(np.asarray(g.link_boundary_id, np.int64), 0, MAX_BOUNDARY_IDS - 1)
+        if self.local_displacement_boundary_id >= 0:
+            anchored = np.asarray(g.link_boundary_id) == self.local_displacement_boundary_id
+        else:
+            anchored = self.boundary_kind_by_id_np[bids] == BC_DIRICHLET
+        slot_of_node = np.full(nx * ny, -1, np.int64)
+        slot_of_node[active_list] = np.arange(self.n_active)
+        anchor_links = np.flatnonzero(anchored).astype(np.int32)
+        anchor_slot = slot_of_node[link_p[anchor_links]] if anchor_links.size else np.zeros(0, np.int64)
+        order = np.argsort(anchor_slot, kind="stable")
+        anchor_ids = anchor_links[order]
+        counts = np.bincount(anchor_slot, minlength=self.n_active) if anchor_links.size else np.zeros(self.n_active, np.int64)
+        off = np.zeros(self.n_active + 1, np.int32)
+        off[1:] = np.cumsum(counts)
+        self._anchor_off = dev_i(off)
+        self._anchor_ids = dev_i(anchor_ids if anchor_ids.size else np.zeros(1, i32))
+        self.boundary_kind_by_id = dev_i(self.boundary_kind_by_id_np)
+        self.boundary_mode_by_id = dev_i(self.boundary_mode_by_id_np)
+        self.boundary_params_by_id = dev_f(self.boundary_params_by_id_np)
+        self.link_traction_x = DeviceArray.zeros((max(L, 1),), f64, dev)
+        self.link_traction_y = DeviceArray.zeros((max(L, 1),), f64, dev)
+        pop_dtype = np.float64 if self.precision == "fp64" else np.float32
+        shp = (FQ, nx, ny, 1)
+        self._pops = [PopulationArray(DeviceArray.zeros(shp, pop_dtype, dev).tensor, self) for _ in range(2)]
+        self._fpost = PopulationArray(DeviceArray.zeros(shp, pop_dtype, dev).tensor, self)
+        self._fpost_stale = False
+        self.u = DeviceArray.zeros((2, nx, ny, 1), f64, dev)
+        self.u_star = DeviceArray.zeros((2, nx, ny, 1), f64, dev)
+        self.u_local_next = DeviceArray.zeros((2, nx, ny, 1), f64, dev)
+        self.U = DeviceArray.zeros((NCOMP, nx, ny, 1), f64, dev)
+        self._P = DeviceArray.zeros((4, nx, ny, 1), f64, dev)
+        self._sigma = DeviceArray.zeros((4, nx, ny, 1), f64, dev)
+        self.record = DeviceArray.zeros((12, nx, ny, 1), f64, dev)
+        self.invalid = DeviceArray.zeros((1,), i32, dev)
+        self._reduce_scratch = DeviceArray.zeros((_native.RED_SCRATCH,), f64, dev)
+        self._stream = torch.cuda.current_stream(dev)
+
+        d = _native.Vlbm2dDesc()
+        d.struct_size = C.sizeof(_native.Vlbm2dDesc)
+        d.abi_version = _native.ABI2_VERSION
+        d.nx, d.ny, d.n_active, d.n_links = nx, ny, self.n_active, L
+        d.precision = _native.PREC_F64 if self.precision == "fp64" else _native.PREC_F32
+        m = self.material
+        d.model, d.lam, d.mu, d.p0, d.p1, d.p2 = m.id, m.lam, m.mu, m.param0, m.param1, m.param2
+        d.j_floor = self.jacobian_floor
+        d.dt, d.c, d.omega, d.dx, d.dy = self.dt, self.lattice_speed, self.collision_omega, self.dx, self.dy
+        d.source_mode = self.source_mode
+        for k in range(NCOMP):
+            d.source_params[k] = float(self.source_params[k])
+        d.recon = self.boundary_reconstruction
+        d.normal_solver = self._normal_solver_mode
+        d.disp_interval = self.local_displacement_interval
+        d.disp_sweeps = self.local_displacement_sweeps
+        d.disp_relax = self.local_displacement_relax
+        d.disp_weight = self.local_displacement_boundary_weight
+        d.compat_interval = self.local_compatibility_interval
+        d.compat_blend = self.local_compatibility_blend
+        d.compat_interior_only = int(self.local_compatibility_interior_only)
+        d.n_anchor_links = int(anchor_ids.size)
+        ptrs = {
+            "U": self.U, "u": self.u, "u_star": self.u_star, "u_next": self.u_local_next, "record": self.record,
+            "invalid": self.invalid, "reduce_scratch": self._reduce_scratch, "active": self.active,
+            "active_list": self.active_list, "anchor_off": self._anchor_off, "anchor_ids": self._anchor_ids,
+            "link_p": self.link_p, "link_q": self.link_q, "link_bid": self.link_boundary_id,
+            "link_eta": self.link_eta, "link_xb": self.link_xb, "link_yb": self.link_yb, "link_nx": self.link_nx,
+            "link_ny": self.link_ny, "lsq_p": self._lsq_p_t, "lsq_wb": self._lsq_wb_t, "lsq_wx": self._lsq_wx_t,
+            "lsq_wy": self._lsq_wy_t, "lsq_valid": self.lsq_valid, "kind_by_id": self.boundary_kind_by_id,
+            "mode_by_id": self.boundary_mode_by_id, "params_by_id": self.boundary_params_by_id,
+            "dyn_tx": self.link_traction_x, "dyn_ty": self.link_traction_y,
+        }
+        for name, arr in ptrs.items():
+            setattr(d, name, arr.ptr or None)
+        d.use_dyn = 0
+        d.stream = self._stream.cuda_stream or None
+        self._desc = d
+        h = C.c_void_p()
+        _native.check(self._lib.vlbm2d_create(C.byref(d), C.byref(h)), "vlbm2d_create")
+        self._handle = h
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            try:
+                self._lib.vlbm2d_destroy(h)
+            except Exception:
+                pass
+            self._handle = None
+
+    def _check(self, rc, what) -> None:
+        self._native.check(rc, what)
+
+    def _populations_to_host(self, arr) -> np.ndarray:
+        tmp = self._torch.empty(arr.shape, dtype=self._torch.float64, device=self.device)
+        self._check(self._lib.vlbm2d_populations_to_f64(self._handle, arr.ptr, tmp.data_ptr()), "populations_to_f64")
+        self._torch.cuda.synchronize(self.device)
+        return tmp.cpu().numpy()
+
+    def _populations_from_host(self, arr, host: np.ndarray) -> None:
+        if host.shape != arr.shape:
+            raise ValueError(f"populations must have shape {arr.shape}")
+        tmp = self._torch.from_numpy(np.ascontiguousarray(host)).to(self.device)
+        self._check(self._lib.vlbm2d_populations_from_f64(self._handle, tmp.data_ptr(), arr.ptr), "populations_from_f64")
+        self._torch.cuda.synchronize(self.device)
+
+    def _zero_populations(self, arr) -> None:
+        if self.precision == "fp64":
+            arr.zero_()
+        else:
+            arr.assign(np.zeros(arr.shape))
+
+    # ---- state buffers ------------------------------------------------------
+    @property
+    def f0(self):
+        """Current populations (24, nx, ny, 1)."""
+        return self._pops[0]
+
+    @property
+    def f1(self):
+        """Second population buffer; after a step it held the step's start
+        populations, which the lazily materialised ``fpost`` is built from."""
+        self._materialize_fpost()
+        return self._pops[1]
+
+    @property
+    def fpost(self):
+        """Post-collision populations of the last step (after the F repair),
+        or the copy of f0 made by an in-place boundary reconstruction."""
+        self._materialize_fpost()
+        return self._fpost
+
+    def _materialize_fpost(self) -> None:
+        if self._fpost_stale:
+            self._fpost_stale = False
+            self._check(self._lib.vlbm2d_post_collision(self._handle, self._pops[1].ptr, self._fpost.ptr),
+                        "post_collision")
+
+    @property
+    def P(self):
+        """First Piola stress (4, nx, ny, 1) of the current F."""
+        self._check(self._lib.vlbm2d_stress(self._handle, self._P.ptr, self._sigma.ptr), "stress")
+        return self._P
+
+    @property
+    def sigma(self):
+        """Cauchy stress (4, nx, ny, 1) of the current F."""
+        self._check(self._lib.vlbm2d_stress(self._handle, self._P.ptr, self._sigma.ptr), "stress")
+        return self._sigma
+
+    # ---- initialisation (cbw2d:3842-3954) ----------------------------------
+    def initialize_identity(self, *, init_order: int = 1) -> None:
+        U = np.zeros((NCOMP, self.nx, self.ny), dtype=np.float64)
+        U[2] = 1.0
+        U[5] = 1.0
+        self.initialize_from_numpy(U=U, displacement=np.zeros((2, self.nx, self.ny)), init_order=init_order)
+
+    def initialize_from_numpy(self, *, U: np.ndarray, displacement: np.ndarray | None = None, init_order: int = 1,
+                              apply_initial_boundaries: bool = True, initial_boundary_time: float = 0.0) -> None:
+        U = np.asarray(U, dtype=np.float64)
+        if U.shape != (NCOMP, self.nx, self.ny):
+            raise ValueError(f"U must have shape {(NCOMP, self.nx, self.ny)}")
+        displacement = (np.zeros((2, self.nx, self.ny)) if displacement is None
+                        else np.asarray(displacement, dtype=np.float64))
+        if displacement.shape != (2, self.nx, self.ny):
+            raise ValueError(f"displacement must have shape {(2, self.nx, self.ny)}")
+        if init_order not in (1, 2, 3):
+            raise ValueError("init_order must be 1, 2, or 3")
+        act = self.active_np.astype(bool)
+        Um = np.array(U, copy=True)
+        Um[:, ~act] = 0.0
+        disp = np.array(displacement, copy=True)
+        disp[:, ~act] = 0.0
+        for arr in (self._pops[0], self._pops[1], self._fpost):
+            self._zero_populations(arr)
+        for arr in (self.u, self.u_star, self.u_local_next):
+            arr.zero_()
+        self._fpost_stale = False
+        self.U.assign(Um[..., None])
+        f0 = self._pops[0]
+        if init_order == 1:
+            self._check(self._lib.vlbm2d_init_populations(self._handle, f0.ptr), "init_populations")
+        else:
+            kw = dict(dx=self.dx, dy=self.dy, dt=self.dt, periodic_x=False, periodic_y=False,
+                      source_mode=self.source_mode, source_params=self.source_params)
+            if init_order == 2:
+                f = _second_order_populations_np(U, self.material, self.lattice_speed, **kw)
+            else:
+                f = _second_order_populations_masked_np(U, act, self.material, self.lattice_speed, **kw)
+            f = np.ascontiguousarray(f.reshape((FQ, self.nx, self.ny)))
+            f[:, ~act] = 0.0
+            f0.assign(f[..., None])
+        d_dev = self._torch.from_numpy(np.ascontiguousarray(disp)).to(self.device)
+        self._check(self._lib.vlbm2d_set_ustar(self._handle, d_dev.data_ptr()), "set_ustar")
+        self._torch.cuda.synchronize(self.device)
+        self.time = 0.0
+        self.steps = 0
+        self.refresh_current()
+        if apply_initial_boundaries:
+            self.apply_boundary_reconstruction(float(initial_boundary_time), in_place=True)
+            self.refresh_current()
+
+    # ---- time stepping (cbw2d:4020-4306) -------------------------------------
+    def enable_dynamic_link_traction(self, enabled: bool = True) -> None:
+        """Use the per-link nominal tractions ``link_traction_x/y`` on Neumann links."""
+        self.use_dynamic_link_traction = bool(enabled)
+        self._check(self._lib.vlbm2d_set_dynamic_traction(self._handle, int(self.use_dynamic_link_traction)),
+                    "set_dynamic_traction")
+
+    def apply_boundary_reconstruction(self, boundary_time: float, *, in_place: bool = False) -> None:
+        if self.geometry.n_links == 0:
+            return
+        if in_place:
+            # Opposing cut links must read the same initial population state.
+            self._fpost_stale = False
+            self._fpost.tensor.copy_(self._pops[0].tensor)
+            src, dst = self._fpost, self._pops[0]
+        else:
+            src, dst = self.fpost, self.f1
+        self._check(self._lib.vlbm2d_apply_boundary(self._handle, src.ptr, dst.ptr, float(boundary_time)),
+                    "apply_boundary")
+
+    def refresh_current(self) -> None:
+        self._check(self._lib.vlbm2d_refresh(self._handle, self._pops[0].ptr), "refresh")
+
+    def run(self, nsteps: int) -> None:
+        """Advance ``nsteps`` steps in one library call (same result as
+        calling ``step()`` nsteps times)."""
+        nsteps = int(nsteps)
+        if nsteps <= 0:
+            return
+        a, b = self._pops
+        self._check(self._lib.vlbm2d_step(self._handle, a.ptr, b.ptr, float(self.time), int(self.steps), nsteps),
+                    "step")
+        for _ in range(nsteps):
+            self.time += self.dt
+        self.steps += nsteps
+        if nsteps % 2 == 1:
+            self._pops = [b, a]
+        self._fpost_stale = True
+
+    def step(self) -> None:
+        self.run(1)
+
+    def steps_until(self, t_end: float) -> int:
+        t, k = self.time, 0
+        while t < float(t_end) - 0.5 * self.dt:
+            t += self.dt
+            k += 1
+        return k
+
+    def run_until(self, t_end: float, *, stop_on_invalid: bool = True, check_interval: int = 0,
+                  chunk: int = 1024) -> None:
+        remaining = self.steps_until(t_end)
+        ci = int(check_interval)
+        while remaining > 0:
+            k = min(chunk, remaining)
+            if stop_on_invalid and ci > 0:
+                k = min(k, ci - self.steps % ci)
+            self.run(k)
+            remaining -= k
+            if stop_on_invalid and ci > 0 and self.steps % ci == 0 and self.invalid_state():
+                break
+        self.refresh_current()
+        self._torch.cuda.synchronize(self.device)
+
+    def configure_compatibility_projection(self, *, interval=1, iterations=160, relax=0.85, weight=20.0,
+                                           repair_F=False, repair_F_band_width=-1, repair_F_blend=1.0,
+                                           boundary_id=BOUNDARY_ID_INNER) -> None:
+        if int(interval) != 0 or bool(repair_F):
+            raise ValueError("global compatibility projection is not available in the fully-local solver")
+
+    def project_compatible_displacement(self, *, iterations=None, repair_F=None) -> None:
+        raise RuntimeError("global compatibility projection is disabled by design")
+
+    def _compatibility_projection_due(self) -> bool:
+        return False
+
+    def _local_compatibility_due(self) -> bool:
+        return self.local_compatibility_interval > 0 and self.steps % self.local_compatibility_interval == 0
+
+    def _local_displacement_due(self) -> bool:
+        return self.local_displacement_interval > 0 and self.steps % self.local_displacement_interval == 0
+
+    # ---- measurement ----------------------------------------------------------
+    def timing(self, enable: bool = True) -> None:
+        """Start (or stop) per-kernel HIP-event timing on the solver stream."""
+        self._check(self._lib.vlbm2d_timing_enable(self._handle, 1 if enable else 0), "timing_enable")
+
+    def kernel_times(self) -> dict[str, dict[str, float]]:
+        ns = len(self._native.SLOTS)
+        ms = (C.c_double * ns)()
+        cnt = (C.c_int32 * ns)()
+        self._check(self._lib.vlbm2d_timing_read(self._handle, ms, cnt, ns), "timing_read")
+        return {name: {"ms": float(ms[k]), "count": int(cnt[k])} for k, name in enumerate(self._native.SLOTS)}
+
+    def reductions(self, u_prev=None) -> dict[str, float]:
+        """Device reductions over the active nodes of the current U and u
+        (no refresh); with ``u_prev`` (a device tensor copy of u) also the
+        rms change of u."""
+        out = (C.c_double * 8)()
+        prev = None if u_prev is None else int(u_prev.data_ptr())
+        self._check(self._lib.vlbm2d_reduce(self._handle, prev, out), "reduce")
+        n = max(self.n_active, 1)
+        return {
+            "rms_v": math.sqrt(out[0] / n), "max_abs_v": out[1], "min_J": out[2], "max_J": out[3],
+            "max_abs_u": out[4], "nonfinite": int(out[5]), "invalid": int(out[6]),
+            "rms_du": math.sqrt(out[7] / n),
+        }
+
+    def residual(self) -> float:
+        """rms_v = sqrt(mean_active |v|^2) after a refresh."""
+        self.refresh_current()
+        return self.reductions()["rms_v"]
+
+    def run_to_equilibrium(self, tol: float, *, check_every: int = 200, hold_checks: int = 1,
+                           min_time: float = 0.0, max_steps: int = 10**8,
+                           criterion: str = "displacement_rate") -> dict:
+        """Step until the equilibrium residual is <= tol on ``hold_checks``
+        consecutive checks.  ``displacement_rate`` is the rms over active
+        nodes of |u(t) - u(t - T)| / T with T = check_every * dt; ``rms_v``
+        is the velocity moment."""
+        if criterion not in ("displacement_rate", "rms_v"):
+            raise ValueError("criterion must be 'displacement_rate' or 'rms_v'")
+        held = 0
+        u_prev = self.u.tensor.clone()
+        history = []
+        red: dict = {}
+        while self.steps < max_steps:
+            k = min(check_every, max_steps - self.steps)
+            self.run(k)
+            red = self.reductions(u_prev)
+            u_prev.copy_(self.u.tensor)
+            rate = red["rms_du"] / (k * self.dt)
+            self.refresh_current()
+            red.update({key: v for key, v in self.reductions().items() if key != "rms_du"})
+            red["displacement_rate"] = rate
+            history.append((self.steps, rate, red["rms_v"]))
+            if red["nonfinite"] or red["min_J"] <= 0.0:
+                return {"converged": False, "steps": self.steps, "history": history, **red}
+            value = rate if criterion == "displacement_rate" else red["rms_v"]
+            held = held + 1 if (self.time >= min_time and value <= tol) else 0
+            if held >= hold_checks:
+                return {"converged": True, "steps": self.steps, "history": history, **red}
+        return {"converged": False, "steps": self.steps, "history": history, **red}
+
+    # ---- accessors (cbw2d:4290-4472) -----------------------------------------
+    def invalid_state(self) -> bool:
+        self._torch.cuda.synchronize(self.device)
+        if int(np.asarray(self.invalid.numpy())[0]) != 0:
+            return True
+        U = self.numpy_field("U")
+        return not np.isfinite(U[:, self.active_np.astype(bool)]).all()
+
+    def numpy_field(self, name: str) -> np.ndarray:
+        if name not in ("u", "u_star", "U", "P", "sigma", "f0", "fpost", "f1", "u_local_next", "record"):
+            raise ValueError(f"unknown field {name!r}")
+        field = getattr(self, name)
+        return np.asarray(field.numpy())[..., 0]
+
+    def displacement(self) -> np.ndarray:
+        return self.numpy_field("u")
+
+    def system_state(self) -> np.ndarray:
+        return self.numpy_field("U")
+
+    def deformation_gradient(self) -> np.ndarray:
+        U = self.system_state()
+        F = np.empty((2, 2, self.nx, self.ny), dtype=np.float64)
+        F[0, 0], F[0, 1], F[1, 0], F[1, 1] = U[2], U[3], U[4], U[5]
+        return F
+
+    def deformation_jacobian(self) -> np.ndarray:
+        F = self.deformation_gradient()
+        return F[0, 0] * F[1, 1] - F[0, 1] * F[1, 0]
+
+    def active_mask(self) -> np.ndarray:
+        return self.active_np.astype(bool)
+
+    def summary(self) -> dict:
+        self.refresh_current()
+        red = self.reductions()
+        mask = self.active_mask()
+        displacement_radius = self.local_displacement_sweeps if self.local_displacement_interval > 0 else 0
+        composed_radius = displacement_radius + 2 if self.local_compatibility_interval > 0 else displacement_radius
+        any_active = bool(np.any(mask))
+        return {
+            "device": self.device, "geometry": self.geometry.label, "boundary_label": self.boundary.label,
+            "boundary_kind": self.boundary.kind, "boundary_mode": int(self.boundary.value_mode),
+            "model": self.material.model, "lambda": float(self.material.lam), "mu": float(self.material.mu),
+            "nx": self.nx, "ny": self.ny, "n_active": int(np.count_nonzero(mask)),
+            "n_cut_links": self.geometry.n_links, "steps": self.steps, "time": float(self.time),
+            "dx": float(self.dx), "dt": float(self.dt), "lattice_speed": float(self.lattice_speed),
+            "collision_omega": float(self.collision_omega), "collision_model": self.collision_model,
+            "ghost_omega": float(self.ghost_omega), "kinetic_filter_strength": float(self.kinetic_filter_strength),
+            "boundary_reconstruction": self.boundary_reconstruction_name,
+            "boundary_stencil_radius": self.boundary_stencil_radius,
+            "normal_jacobian": self.normal_jacobian_name, "neumann_solver": self.neumann_solver_name,
+            "local_compatibility_interval": self.local_compatibility_interval,
+            "local_compatibility_blend": self.local_compatibility_blend,
+            "local_compatibility_interior_only": self.local_compatibility_interior_only,
+            "local_displacement_interval": self.local_displacement_interval,
+            "local_displacement_sweeps": self.local_displacement_sweeps,
+            "local_displacement_relax": self.local_displacement_relax,
+            "local_displacement_boundary_weight": self.local_displacement_boundary_weight,
+            "local_displacement_boundary_id": self.local_displacement_boundary_id,
+            "local_kernel_stencil_radius": 2 if self.local_compatibility_interval > 0 else (1 if displacement_radius > 0 else 0),
+            "local_displacement_dependency_radius": displacement_radius,
+            "local_composed_compatibility_radius": composed_radius,
+            "linearized_stability_ratio": float(self.material.linearized_stability_ratio(self.lattice_speed)),
+            "max_abs_u": float(red["max_abs_u"]) if any_active else 0.0,
+            "max_abs_v": float(red["max_abs_v"]) if any_active else 0.0,
+            "min_J": float(red["min_J"]) if any_active else 0.0,
+            "max_J": float(red["max_J"]) if any_active else 0.0,
+            "finite": bool(red["nonfinite"] == 0 and red["min_J"] > 0.0) if any_active else False,
+            "precision": self.precision,
+        }
+
+    def save_npz(self, path: str | Path) -> None:
+        self.refresh_current()
+        path = Path(path)
+        path.parent.mkdir(parents=True, exist_ok=True)
+        np.savez_compressed(
+            path, x=self.x, y=self.y, active=self.active_np, u=self.displacement(), U=self.system_state(),
+            F=self.deformation_gradient(), P=self.numpy_field("P"), sigma=self.numpy_field("sigma"),
+            f=self.numpy_field("f0"), link_i=self.geometry.link_i, link_j=self.geometry.link_j,
+            link_q=self.geometry.link_q, link_boundary_id=self.geometry.link_boundary_id,
+            link_eta=self.geometry.link_eta, link_xb=self.geometry.link_xb, link_yb=self.geometry.link_yb,
+            link_nx=self.geometry.link_nx, link_ny=self.geometry.link_ny, lsq_valid=self.lsq_valid_np,
+            time=np.array(self.time), dx=np.array(self.dx), dt=np.array(self.dt),
+            lattice_speed=np.array(self.lattice_speed), collision_omega=np.array(self.collision_omega),
+            collision_model=np.array(self.collision_model), ghost_omega=np.array(self.ghost_omega),
+            kinetic_filter_strength=np.array(self.kinetic_filter_strength),
+            boundary_reconstruction=np.array(self.boundary_reconstruction_name),
+            normal_jacobian=np.array(self.normal_jacobian_name), neumann_solver=np.array(self.neumann_solver_name),
+            local_compatibility_interval=np.array(self.local_compatibility_interval),
+            local_compatibility_blend=np.array(self.local_compatibility_blend),
+            local_compatibility_interior_only=np.array(self.local_compatibility_interior_only),
+            local_displacement_interval=np.array(self.local_displacement_interval),
+            local_displacement_sweeps=np.array(self.local_displacement_sweeps),
+            local_displacement_relax=np.array(self.local_displacement_relax),
+            local_displacement_boundary_weight=np.array(self.local_displacement_boundary_weight),
+            local_displacement_boundary_id=np.array(self.local_displacement_boundary_id),
+            boundary_kind=np.array(self.boundary.kind), boundary_mode=np.array(int(self.boundary.value_mode)),
+            boundary_params=self.boundary_params, boundary_kind_by_id=self.boundary_kind_by_id_np,
+            boundary_mode_by_id=self.boundary_mode_by_id_np, boundary_params_by_id=self.boundary_params_by_id_np,
+            model=np.array(self.material.model), lam=np.array(self.material.lam), mu=np.array(self.material.mu),
+            material_params=np.asarray(self.material.params, dtype=np.float64),
+            metadata=np.array(json.dumps(self.summary())),
+        )

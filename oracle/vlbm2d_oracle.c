@@ -1,0 +1,920 @@
+/* CPU FP64 restatement of the reference D2Q4x6 dense curved-boundary step.
+ *
+ * TEST INFRASTRUCTURE ONLY (parity oracle and CPU baseline); see the header.
+ * cbw2d = src/curved_lbm/two_d/curved_boundary_warp.py
+ * vnew2d = src/curved_lbm/two_d/vector_nonlinear_elastic_warp.py
+ *
+ * The launch sequence of WarpCurvedBoundaryLBM2D.step (cbw2d:4180-4256) is
+ * kept: collide into fpost, local displacement sweeps, F repair on fpost,
+ * clear + push stream, cut-link reconstruction from fpost.  Arithmetic
+ * follows the reference expression order; the constitutive map is
+ * evaluated once per state instead of once per (q, a) pair, which yields
+ * identical values because it is a pure function of F.  The Dirichlet
+ * anchor sums (float atomics in the reference) are accumulated in
+ * ascending link order.
+ */
+#include "vlbm2d_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NC 6
+#define NQ 4
+#define FQ 24
+#define MAX_LSQ 24
+#define MAX_IDS 8
+
+enum { M_SVK = 0, M_NH = 1, M_LOGNH = 2, M_MR = 3, M_YEOH = 4, M_GENT = 5, M_SUGI = 6 };
+enum { BC_DIR = 1, BC_NEU = 2 };
+enum { SRC_NONE = 0, SRC_CONST = 1, SRC_DAMP = 2 };
+enum { R_BFL = 0, R_LSQ_BFL = 2, R_REG_BFL = 3, R_CHAR2 = 11, R_LOCALF = 12, R_LOCALF_BFL = 13 };
+enum { NJ_FD = 0, NJ_AC = 1, NS_ENERGY = 2 };
+enum {
+    V_ZERO = 0, V_CONST = 1, V_RAMP = 2, V_SHEAR_X = 3, V_RICKER_Y = 4, V_SINE2_HOLD = 5,
+    V_AFFINE_VEL = 6, CV_PIOLA_N = 101, CV_RADIAL = 102, CV_RADIAL_SINE2_HOLD = 103,
+    CV_RADIAL_SINE2_PULSE = 104, CV_RADIAL_LINEAR = 105
+};
+
+static const double PI_D = 3.141592653589793238462643383279502884;
+static const int CX[4] = {1, 0, -1, 0};
+static const int CY[4] = {0, 1, 0, -1};
+
+#define IDX(s, c, i, j) ((size_t)(c) * (size_t)(s)->nx * (size_t)(s)->ny + (size_t)(i) * (size_t)(s)->ny + (size_t)(j))
+
+static inline double clamp_j(double j, double floor) { return j < floor ? floor : j; }
+
+/* vnew2d:938-1052 (_piola_vec), plane strain, P row-major 2x2. */
+void or2d_piola(const double *F, int model, double lam, double mu, double p0, double p1, double p2,
+                double jf, double *P) {
+    const double F11 = F[0], F12 = F[1], F21 = F[2], F22 = F[3];
+    if (model == M_SVK) {
+        double C00 = F11 * F11 + F21 * F21, C01 = F11 * F12 + F21 * F22, C11 = F12 * F12 + F22 * F22;
+        double E00 = 0.5 * (C00 - 1.0), E01 = 0.5 * C01, E11 = 0.5 * (C11 - 1.0);
+        double trE = E00 + E11;
+        double S00 = lam * trE + 2.0 * mu * E00, S01 = 2.0 * mu * E01, S11 = lam * trE + 2.0 * mu * E11;
+        P[0] = F11 * S00 + F12 * S01;
+        P[1] = F11 * S01 + F12 * S11;
+        P[2] = F21 * S00 + F22 * S01;
+        P[3] = F21 * S01 + F22 * S11;
+        return;
+    }
+    const double J = clamp_j(F11 * F22 - F12 * F21, jf);
+    const double H00 = F22 / J, H01 = -F21 / J, H10 = -F12 / J, H11 = F11 / J;
+    const double logJ = log(J);
+    double vol = lam * logJ;
+    if (model == M_NH || model == M_LOGNH || (model != M_MR && model != M_SUGI && model != M_YEOH &&
+                                              model != M_GENT)) {
+        if (model == M_NH) vol = 0.5 * lam * (J * J - 1.0);
+        P[0] = mu * (F11 - H00) + vol * H00;
+        P[1] = mu * (F12 - H01) + vol * H01;
+        P[2] = mu * (F21 - H10) + vol * H10;
+        P[3] = mu * (F22 - H11) + vol * H11;
+        return;
+    }
+    const double F2 = F11 * F11 + F12 * F12 + F21 * F21 + F22 * F22;
+    const double I1 = F2 + 1.0, I2 = J * J + F2;
+    const double jm23 = exp((-2.0 / 3.0) * logJ), jm43 = exp((-4.0 / 3.0) * logJ);
+    const double h1 = (-2.0 / 3.0) * I1, h2 = 2.0 * J * J - (4.0 / 3.0) * I2;
+    const double H[4] = {H00, H01, H10, H11};
+    double G1[4], G2[4];
+    for (int k = 0; k < 4; ++k) {
+        G1[k] = jm23 * (2.0 * F[k] + h1 * H[k]);
+        G2[k] = jm43 * (2.0 * F[k] + h2 * H[k]);
+    }
+    if (model == M_MR || model == M_SUGI) {
+        if (model == M_SUGI) {
+            const double A = jm23 * I1 - 3.0;
+            const double c2 = -0.5 * mu, c3 = (p0 + 2.0 * mu) / 8.0;
+            const double sc = mu + 2.0 * c3 * A;
+            for (int k = 0; k < 4; ++k) P[k] = sc * G1[k] + c2 * G2[k] + vol * H[k];
+            return;
+        }
+        for (int k = 0; k < 4; ++k) P[k] = p0 * G1[k] + p1 * G2[k] + vol * H[k];
+        return;
+    }
+    const double A = jm23 * I1 - 3.0;
+    double sc;
+    if (model == M_YEOH) {
+        sc = p0 + 2.0 * p1 * A + 3.0 * p2 * A * A;
+    } else { /* Gent */
+        double den = 1.0 - A / p0;
+        if (den < jf) den = jf;
+        sc = 0.5 * mu / den;
+    }
+    for (int k = 0; k < 4; ++k) P[k] = sc * G1[k] + vol * H[k];
+}
+
+static inline void piola_s(const or2d_state *s, const double *F, double *P) {
+    or2d_piola(F, s->model, s->lam, s->mu, s->p0, s->p1, s->p2, s->j_floor, P);
+}
+
+/* vnew2d:1078-1137: flux components along x and y of state U with Piola P. */
+static inline double flux_x(int a, const double *U, const double *P) {
+    switch (a) {
+    case 0: return -P[0];
+    case 1: return -P[2];
+    case 2: return -U[0];
+    case 3: return 0.0;
+    case 4: return -U[1];
+    default: return 0.0;
+    }
+}
+static inline double flux_y(int a, const double *U, const double *P) {
+    switch (a) {
+    case 0: return -P[1];
+    case 1: return -P[3];
+    case 2: return 0.0;
+    case 3: return -U[0];
+    case 4: return 0.0;
+    default: return -U[1];
+    }
+}
+
+/* vnew2d:1140-1168 */
+static inline double feq(int q, int a, const double *U, const double *P, double c) {
+    const double cx = (double)CX[q], cy = (double)CY[q];
+    return 0.25 * (U[a] + (2.0 / c) * (cx * flux_x(a, U, P) + cy * flux_y(a, U, P)));
+}
+
+/* vnew2d:1250-1280 */
+static inline double source_comp(int a, int mode, const double *sp, double vx, double vy) {
+    if (mode == SRC_CONST) return sp[a];
+    if (mode == SRC_DAMP) {
+        if (a == 0) return -sp[0] * vx;
+        if (a == 1) return -sp[0] * vy;
+    }
+    return 0.0;
+}
+
+static inline double smooth5(double z) {
+    if (z < 0.0) z = 0.0;
+    if (z > 1.0) z = 1.0;
+    return z * z * z * (10.0 - 15.0 * z + 6.0 * z * z);
+}
+
+/* vnew2d:1171-1247 (_boundary_value_comp) and cbw2d:936-984 (_curved_boundary_value_comp);
+ * mode 105 (nominal traction amp * min(t / t_ramp, 1) * n) is this build's extension. */
+double or2d_boundary_value(int comp, int mode, const double *p, double x, double y, double nxb,
+                           double nyb, double t) {
+    switch (mode) {
+    case CV_PIOLA_N:
+        return comp == 0 ? p[0] * nxb + p[1] * nyb : p[2] * nxb + p[3] * nyb;
+    case CV_RADIAL:
+        return comp == 0 ? p[0] * nxb : p[0] * nyb;
+    case CV_RADIAL_SINE2_HOLD: {
+        double scale = 1.0;
+        if (p[1] > 0.0 && t < p[1]) {
+            double den = p[2];
+            if (den <= 0.0) den = 2.0 * p[1];
+            double sn = sin(PI_D * t / den);
+            scale = sn * sn;
+        }
+        return comp == 0 ? p[0] * scale * nxb : p[0] * scale * nyb;
+    }
+    case CV_RADIAL_SINE2_PULSE: {
+        double scale = 0.0;
+        if (p[1] > 0.0 && t < p[1]) {
+            double sn = sin(PI_D * t / p[1]);
+            scale = sn * sn;
+        }
+        return comp == 0 ? p[0] * scale * nxb : p[0] * scale * nyb;
+    }
+    case CV_RADIAL_LINEAR: {
+        double scale = 1.0;
+        if (p[1] > 0.0 && t < p[1]) scale = t / p[1];
+        return comp == 0 ? p[0] * scale * nxb : p[0] * scale * nyb;
+    }
+    case V_ZERO:
+        return 0.0;
+    case V_CONST:
+        return comp == 0 ? p[0] : p[1];
+    case V_RAMP: {
+        double scale = 1.0;
+        if (p[2] > 0.0 && t < p[2]) {
+            double sn = sin(PI_D * t / p[2]);
+            scale = sn * sn;
+        }
+        return comp == 0 ? p[0] * scale : p[1] * scale;
+    }
+    case V_SINE2_HOLD: {
+        double scale = 1.0;
+        if (p[2] > 0.0 && t < p[2]) {
+            double den = p[3];
+            if (den <= 0.0) den = 2.0 * p[2];
+            double sn = sin(PI_D * t / den);
+            scale = sn * sn;
+        }
+        return comp == 0 ? p[0] * scale : p[1] * scale;
+    }
+    case V_SHEAR_X:
+        if (comp == 0 && t < p[1]) return p[0] * sin(PI_D * t / p[1]);
+        return 0.0;
+    case V_RICKER_Y: {
+        if (comp != 1 || t > p[3]) return 0.0;
+        double z = (t - p[2]) / p[1];
+        double pref = 2.0 * p[0] / (sqrt(3.0 * p[1]) * sqrt(sqrt(PI_D)));
+        double value = pref * (1.0 - z * z) * exp(-0.5 * z * z);
+        if (p[6] > 0.0) {
+            double layer = p[6] * p[5];
+            if (layer < 1.0e-30) layer = 1.0e-30;
+            double window = smooth5(y / layer) * smooth5((p[4] - y) / layer);
+            if (p[7] > 1.0e-14) window = window / p[7];
+            value = value * window;
+        }
+        return value;
+    }
+    case V_AFFINE_VEL:
+        if (p[6] > 0.0 && t > p[6]) return 0.0;
+        return comp == 0 ? p[0] * x + p[1] * y + p[2] : p[3] * x + p[4] * y + p[5];
+    default:
+        return 0.0;
+    }
+}
+
+/* cbw2d:987-1013 */
+static void normal_residual(const or2d_state *s, double hn0, double hn1, double gt0, double gt1,
+                            double nxb, double nyb, double tx, double ty, double *r) {
+    const double t1 = -nyb, t2 = nxb;
+    double F[4] = {hn0 * nxb + gt0 * t1, hn0 * nyb + gt0 * t2, hn1 * nxb + gt1 * t1,
+                   hn1 * nyb + gt1 * t2};
+    double P[4];
+    piola_s(s, F, P);
+    r[0] = P[0] * nxb + P[1] * nyb - tx;
+    r[1] = P[2] * nxb + P[3] * nyb - ty;
+}
+
+/* cbw2d:1016-1043 */
+static void acoustic_normal(const double *F, double nxb, double nyb, double lam, double mu,
+                            double jf, double *Qn) {
+    double J = clamp_j(F[0] * F[3] - F[1] * F[2], jf);
+    double H00 = F[3] / J, H01 = -F[2] / J, H10 = -F[1] / J, H11 = F[0] / J;
+    double beta = 0.5 * lam * (J * J - 1.0) - mu;
+    double coeff = lam * J * J - beta;
+    double n2 = nxb * nxb + nyb * nyb;
+    double v0 = H00 * nxb + H01 * nyb, v1 = H10 * nxb + H11 * nyb;
+    Qn[0] = mu * n2 + coeff * v0 * v0;
+    Qn[1] = coeff * v0 * v1;
+    Qn[2] = coeff * v1 * v0;
+    Qn[3] = mu * n2 + coeff * v1 * v1;
+}
+
+/* cbw2d:1046-1192: normal image of F from P(F) n = T with the tangential image fixed. */
+void or2d_solve_normal(const or2d_state *s, const double *Fg, double nxb, double nyb, double tx,
+                       double ty, double *Fb) {
+    const double jf = s->j_floor;
+    if (s->normal_solver == NS_ENERGY && s->model == M_NH) {
+        double a0 = Fg[3] * nxb - Fg[2] * nyb;
+        double a1 = -Fg[1] * nxb + Fg[0] * nyb;
+        double a2 = a0 * a0 + a1 * a1;
+        if (a2 > 1.0e-28) {
+            double area = sqrt(a2);
+            double e0 = a0 / area, e1 = a1 / area;
+            double tau = tx * e0 + ty * e1;
+            double aa = s->mu + 0.5 * s->lam * a2;
+            double bb = s->mu + 0.5 * s->lam;
+            double disc = sqrt(tau * tau + 4.0 * aa * bb);
+            double sr = (tau + disc) / (2.0 * aa);
+            if (tau < 0.0) sr = 2.0 * bb / (disc - tau);
+            if (area * sr > jf) {
+                double h0 = (tx - tau * e0) / s->mu + sr * e0;
+                double h1 = (ty - tau * e1) / s->mu + sr * e1;
+                double d0 = h0 - (Fg[0] * nxb + Fg[1] * nyb);
+                double d1 = h1 - (Fg[2] * nxb + Fg[3] * nyb);
+                Fb[0] = Fg[0] + d0 * nxb;
+                Fb[1] = Fg[1] + d0 * nyb;
+                Fb[2] = Fg[2] + d1 * nxb;
+                Fb[3] = Fg[3] + d1 * nyb;
+                return;
+            }
+        }
+    }
+    const double tanx = -nyb, tany = nxb;
+    double hn0 = Fg[0] * nxb + Fg[1] * nyb, hn1 = Fg[2] * nxb + Fg[3] * nyb;
+    const double gt0 = Fg[0] * tanx + Fg[1] * tany, gt1 = Fg[2] * tanx + Fg[3] * tany;
+    const double eps = 1.0e-6;
+    for (int it = 0; it < 16; ++it) {
+        double r[2];
+        normal_residual(s, hn0, hn1, gt0, gt1, nxb, nyb, tx, ty, r);
+        if (sqrt(r[0] * r[0] + r[1] * r[1]) < 1.0e-11) break;
+        double a, b, c, d;
+        if (s->normal_solver != NJ_FD && s->model == M_NH) {
+            double F[4] = {hn0 * nxb + gt0 * tanx, hn0 * nyb + gt0 * tany, hn1 * nxb + gt1 * tanx,
+                           hn1 * nyb + gt1 * tany};
+            double Qn[4];
+            acoustic_normal(F, nxb, nyb, s->lam, s->mu, jf, Qn);
+            a = Qn[0]; b = Qn[1]; c = Qn[2]; d = Qn[3];
+        } else {
+            double rp0[2], rp1[2];
+            normal_residual(s, hn0 + eps, hn1, gt0, gt1, nxb, nyb, tx, ty, rp0);
+            normal_residual(s, hn0, hn1 + eps, gt0, gt1, nxb, nyb, tx, ty, rp1);
+            a = (rp0[0] - r[0]) / eps;
+            c = (rp0[1] - r[1]) / eps;
+            b = (rp1[0] - r[0]) / eps;
+            d = (rp1[1] - r[1]) / eps;
+        }
+        double det = a * d - b * c;
+        if (fabs(det) < 1.0e-14) break;
+        double d0 = (-r[0] * d + b * r[1]) / det;
+        double d1 = (c * r[0] - a * r[1]) / det;
+        double step = 1.0;
+        for (int ls = 0; ls < 12; ++ls) {
+            double c0 = hn0 + step * d0, c1 = hn1 + step * d1;
+            double F11 = c0 * nxb + gt0 * tanx, F12 = c0 * nyb + gt0 * tany;
+            double F21 = c1 * nxb + gt1 * tanx, F22 = c1 * nyb + gt1 * tany;
+            if (F11 * F22 - F12 * F21 > jf) {
+                hn0 = c0;
+                hn1 = c1;
+                break;
+            }
+            step = 0.5 * step;
+        }
+    }
+    Fb[0] = hn0 * nxb + gt0 * tanx;
+    Fb[1] = hn0 * nyb + gt0 * tany;
+    Fb[2] = hn1 * nxb + gt1 * tanx;
+    Fb[3] = hn1 * nyb + gt1 * tany;
+}
+
+/* cbw2d:1224-1247 */
+static inline double jvp_eps6(const double *w) {
+    double sc = fabs(w[0]);
+    for (int k = 1; k < 6; ++k)
+        if (fabs(w[k]) > sc) sc = fabs(w[k]);
+    double eps = 1.0e-6;
+    if (sc > 1.0) eps = eps / sc;
+    return eps;
+}
+
+/* cbw2d:1250-1365: central-difference directional derivative of the x
+ * (axis 0) or y (axis 1) flux of every component at U along W. */
+static void flux_jvp(const or2d_state *s, int axis, const double *U, const double *W, double *out) {
+    const double eps = jvp_eps6(W);
+    double Up[6], Um[6], Pp[4], Pm[4];
+    for (int k = 0; k < 6; ++k) {
+        Up[k] = U[k] + eps * W[k];
+        Um[k] = U[k] - eps * W[k];
+    }
+    piola_s(s, Up + 2, Pp);
+    piola_s(s, Um + 2, Pm);
+    for (int a = 0; a < 6; ++a) {
+        double fp = axis == 0 ? flux_x(a, Up, Pp) : flux_y(a, Up, Pp);
+        double fm = axis == 0 ? flux_x(a, Um, Pm) : flux_y(a, Um, Pm);
+        out[a] = (fp - fm) / (2.0 * eps);
+    }
+}
+
+/* cbw2d:1443-1480 */
+static double interp_U_link(const or2d_state *s, int comp, int i, int j, int q_out, double sv) {
+    const int nx = s->nx, ny = s->ny;
+    double v0 = s->U[IDX(s, comp, i, j)];
+    int i1 = i - CX[q_out], j1 = j - CY[q_out];
+    if (i1 >= 0 && i1 < nx && j1 >= 0 && j1 < ny && s->active[(size_t)i1 * ny + j1] != 0) {
+        double v1 = s->U[IDX(s, comp, i1, j1)];
+        int i2 = i - 2 * CX[q_out], j2 = j - 2 * CY[q_out];
+        if (i2 >= 0 && i2 < nx && j2 >= 0 && j2 < ny && s->active[(size_t)i2 * ny + j2] != 0) {
+            double v2 = s->U[IDX(s, comp, i2, j2)];
+            double l0 = 0.5 * (sv + 1.0) * (sv + 2.0);
+            double l1 = -sv * (sv + 2.0);
+            double l2 = 0.5 * sv * (sv + 1.0);
+            return l0 * v0 + l1 * v1 + l2 * v2;
+        }
+        return (sv + 1.0) * v0 - sv * v1;
+    }
+    return v0;
+}
+
+/* cbw2d:1202-1221 */
+static double lsq_at(const or2d_state *s, const double *w, int ell, int comp, double fallback) {
+    if (s->lsq_valid[ell] == 0) return fallback;
+    double out = 0.0;
+    for (int m = 0; m < MAX_LSQ; ++m) {
+        int ii = s->lsq_i[(size_t)ell * MAX_LSQ + m], jj = s->lsq_j[(size_t)ell * MAX_LSQ + m];
+        if (ii >= 0 && jj >= 0) out += w[(size_t)ell * MAX_LSQ + m] * s->U[IDX(s, comp, ii, jj)];
+    }
+    return out;
+}
+
+static inline int recon_supported(int r) {
+    return r == R_BFL || r == R_LSQ_BFL || r == R_LOCALF || r == R_LOCALF_BFL;
+}
+
+/* cbw2d:1692-1843 (_collide_masked_kernel), f0 -> fpost. */
+static void collide(or2d_state *s) {
+    const int nx = s->nx, ny = s->ny;
+    const double dt = s->dt, om = s->omega;
+    const double *sp = s->source_params;
+    int invalid = 0;
+#pragma omp parallel for schedule(static) reduction(| : invalid)
+    for (int i = 0; i < nx; ++i) {
+        for (int j = 0; j < ny; ++j) {
+            if (s->active[(size_t)i * ny + j] == 0) {
+                for (int k = 0; k < FQ; ++k) s->fpost[IDX(s, k, i, j)] = 0.0;
+                for (int a = 0; a < NC; ++a) s->U[IDX(s, a, i, j)] = 0.0;
+                s->u[IDX(s, 0, i, j)] = 0.0;
+                s->u[IDX(s, 1, i, j)] = 0.0;
+                continue;
+            }
+            double U[6] = {0, 0, 0, 0, 0, 0};
+            for (int q = 0; q < NQ; ++q)
+                for (int a = 0; a < NC; ++a) U[a] += s->f0[IDX(s, q * 6 + a, i, j)];
+            const double hdt = 0.5 * dt;
+            if (s->source_mode == SRC_CONST) {
+                for (int a = 0; a < NC; ++a) U[a] += hdt * sp[a];
+            } else if (s->source_mode == SRC_DAMP) {
+                double den = 1.0 + hdt * sp[0];
+                U[0] = U[0] / den;
+                U[1] = U[1] / den;
+            }
+            double b[6];
+            for (int a = 0; a < NC; ++a) b[a] = source_comp(a, s->source_mode, sp, U[0], U[1]);
+            for (int a = 0; a < NC; ++a) s->U[IDX(s, a, i, j)] = U[a];
+            double ux = s->u_star[IDX(s, 0, i, j)] + 0.5 * dt * U[0];
+            double uy = s->u_star[IDX(s, 1, i, j)] + 0.5 * dt * U[1];
+            s->u[IDX(s, 0, i, j)] = ux;
+            s->u[IDX(s, 1, i, j)] = uy;
+            s->u_star[IDX(s, 0, i, j)] = ux + 0.5 * dt * U[0];
+            s->u_star[IDX(s, 1, i, j)] = uy + 0.5 * dt * U[1];
+            double J = U[2] * U[5] - U[3] * U[4];
+            if (J <= s->j_floor) invalid = 1;
+            double Js = clamp_j(J, s->j_floor), P[4];
+            piola_s(s, U + 2, P);
+            for (int k = 0; k < 4; ++k) s->P[IDX(s, k, i, j)] = P[k];
+            s->sigma[IDX(s, 0, i, j)] = (P[0] * U[2] + P[1] * U[3]) / Js;
+            s->sigma[IDX(s, 1, i, j)] = (P[0] * U[4] + P[1] * U[5]) / Js;
+            s->sigma[IDX(s, 2, i, j)] = (P[2] * U[2] + P[3] * U[3]) / Js;
+            s->sigma[IDX(s, 3, i, j)] = (P[2] * U[4] + P[3] * U[5]) / Js;
+            for (int q = 0; q < NQ; ++q)
+                for (int a = 0; a < NC; ++a) {
+                    double fe = feq(q, a, U, P, s->c);
+                    double old = s->f0[IDX(s, q * 6 + a, i, j)];
+                    s->fpost[IDX(s, q * 6 + a, i, j)] = old + om * (fe - old) + dt * (2.0 - om) * 0.125 * b[a];
+                }
+        }
+    }
+    if (invalid) s->invalid[0] = 1;
+}
+
+/* cbw2d:3128-3210 (_refresh_masked_kernel) on f0. */
+int or2d_refresh(or2d_state *s) {
+    const int nx = s->nx, ny = s->ny;
+    const double dt = s->dt;
+    const double *sp = s->source_params;
+    s->invalid[0] = 0;
+    int invalid = 0;
+#pragma omp parallel for schedule(static) reduction(| : invalid)
+    for (int i = 0; i < nx; ++i) {
+        for (int j = 0; j < ny; ++j) {
+            if (s->active[(size_t)i * ny + j] == 0) {
+                for (int a = 0; a < NC; ++a) s->U[IDX(s, a, i, j)] = 0.0;
+                s->u[IDX(s, 0, i, j)] = 0.0;
+                s->u[IDX(s, 1, i, j)] = 0.0;
+                for (int a = 0; a < 4; ++a) {
+                    s->P[IDX(s, a, i, j)] = 0.0;
+                    s->sigma[IDX(s, a, i, j)] = 0.0;
+                }
+                continue;
+            }
+            double U[6] = {0, 0, 0, 0, 0, 0};
+            for (int q = 0; q < NQ; ++q)
+                for (int a = 0; a < NC; ++a) U[a] += s->f0[IDX(s, q * 6 + a, i, j)];
+            const double hdt = 0.5 * dt;
+            if (s->source_mode == SRC_CONST) {
+                for (int a = 0; a < NC; ++a) U[a] += hdt * sp[a];
+            } else if (s->source_mode == SRC_DAMP) {
+                double den = 1.0 + hdt * sp[0];
+                U[0] = U[0] / den;
+                U[1] = U[1] / den;
+            }
+            for (int a = 0; a < NC; ++a) s->U[IDX(s, a, i, j)] = U[a];
+            s->u[IDX(s, 0, i, j)] = s->u_star[IDX(s, 0, i, j)] + 0.5 * dt * U[0];
+            s->u[IDX(s, 1, i, j)] = s->u_star[IDX(s, 1, i, j)] + 0.5 * dt * U[1];
+            double J = U[2] * U[5] - U[3] * U[4];
+            if (J <= s->j_floor) invalid = 1;
+            double Js = clamp_j(J, s->j_floor), P[4];
+            piola_s(s, U + 2, P);
+            for (int k = 0; k < 4; ++k) s->P[IDX(s, k, i, j)] = P[k];
+            s->sigma[IDX(s, 0, i, j)] = (P[0] * U[2] + P[1] * U[3]) / Js;
+            s->sigma[IDX(s, 1, i, j)] = (P[0] * U[4] + P[1] * U[5]) / Js;
+            s->sigma[IDX(s, 2, i, j)] = (P[2] * U[2] + P[3] * U[3]) / Js;
+            s->sigma[IDX(s, 3, i, j)] = (P[2] * U[4] + P[3] * U[5]) / Js;
+        }
+    }
+    if (invalid) s->invalid[0] = 1;
+    return 0;
+}
+
+/* cbw2d:3276-3417: clear + Dirichlet cut-point anchors (diag w^2, rhs w^2 t). */
+static void build_bc(or2d_state *s, double time) {
+    const int nx = s->nx, ny = s->ny;
+    memset(s->bc_diag, 0, sizeof(double) * (size_t)nx * ny);
+    memset(s->bc_rhs, 0, sizeof(double) * 2 * (size_t)nx * ny);
+    for (int ell = 0; ell < s->n_links; ++ell) {
+        int bid = s->link_bid[ell];
+        if (s->disp_boundary_id >= 0) {
+            if (bid != s->disp_boundary_id) continue;
+        } else if (s->kind_by_id[bid] != BC_DIR) {
+            continue;
+        }
+        int i = s->link_i[ell], j = s->link_j[ell];
+        if (s->active[(size_t)i * ny + j] == 0) continue;
+        const double *p = s->params_by_id + (size_t)bid * 8;
+        int mode = s->mode_by_id[bid];
+        double xb = s->link_xb[ell], yb = s->link_yb[ell];
+        double nxb = s->link_nx[ell], nyb = s->link_ny[ell];
+        double ub0 = 0.0, ub1 = 0.0;
+        if (mode == V_CONST) {
+            ub0 = time * or2d_boundary_value(0, mode, p, xb, yb, nxb, nyb, 0.0);
+            ub1 = time * or2d_boundary_value(1, mode, p, xb, yb, nxb, nyb, 0.0);
+        } else if (mode == V_AFFINE_VEL) {
+            double at = time;
+            if (p[6] > 0.0 && at > p[6]) at = p[6];
+            ub0 = at * or2d_boundary_value(0, mode, p, xb, yb, nxb, nyb, 0.0);
+            ub1 = at * or2d_boundary_value(1, mode, p, xb, yb, nxb, nyb, 0.0);
+        }
+        double dbx = xb - ((double)i + 0.5) * s->dx;
+        double dby = yb - ((double)j + 0.5) * s->dy;
+        double tx = ub0 - ((s->U[IDX(s, 2, i, j)] - 1.0) * dbx + s->U[IDX(s, 3, i, j)] * dby);
+        double ty = ub1 - (s->U[IDX(s, 4, i, j)] * dbx + (s->U[IDX(s, 5, i, j)] - 1.0) * dby);
+        double w2 = s->disp_weight * s->disp_weight;
+        s->bc_diag[(size_t)i * ny + j] += w2;
+        s->bc_rhs[IDX(s, 0, i, j)] += w2 * tx;
+        s->bc_rhs[IDX(s, 1, i, j)] += w2 * ty;
+    }
+    (void)nx;
+}
+
+/* cbw2d:3421-3490 */
+static void disp_relax(or2d_state *s) {
+    const int nx = s->nx, ny = s->ny;
+    const double dx = s->dx, dy = s->dy, relax = s->disp_relax;
+    const double *U = s->U, *u = s->u;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i) {
+        for (int j = 0; j < ny; ++j) {
+            if (s->active[(size_t)i * ny + j] == 0) {
+                s->u_next[IDX(s, 0, i, j)] = 0.0;
+                s->u_next[IDX(s, 1, i, j)] = 0.0;
+                continue;
+            }
+            double diag = s->bc_diag[(size_t)i * ny + j];
+            double rx = s->bc_rhs[IDX(s, 0, i, j)], ry = s->bc_rhs[IDX(s, 1, i, j)];
+            if (i + 1 < nx && s->active[(size_t)(i + 1) * ny + j] != 0) {
+                double gx = 0.5 * dx * ((U[IDX(s, 2, i, j)] - 1.0) + (U[IDX(s, 2, i + 1, j)] - 1.0));
+                double gy = 0.5 * dx * (U[IDX(s, 4, i, j)] + U[IDX(s, 4, i + 1, j)]);
+                rx += u[IDX(s, 0, i + 1, j)] - gx;
+                ry += u[IDX(s, 1, i + 1, j)] - gy;
+                diag += 1.0;
+            }
+            if (i > 0 && s->active[(size_t)(i - 1) * ny + j] != 0) {
+                double gx = 0.5 * dx * ((U[IDX(s, 2, i - 1, j)] - 1.0) + (U[IDX(s, 2, i, j)] - 1.0));
+                double gy = 0.5 * dx * (U[IDX(s, 4, i - 1, j)] + U[IDX(s, 4, i, j)]);
+                rx += u[IDX(s, 0, i - 1, j)] + gx;
+                ry += u[IDX(s, 1, i - 1, j)] + gy;
+                diag += 1.0;
+            }
+            if (j + 1 < ny && s->active[(size_t)i * ny + j + 1] != 0) {
+                double gx = 0.5 * dy * (U[IDX(s, 3, i, j)] + U[IDX(s, 3, i, j + 1)]);
+                double gy = 0.5 * dy * ((U[IDX(s, 5, i, j)] - 1.0) + (U[IDX(s, 5, i, j + 1)] - 1.0));
+                rx += u[IDX(s, 0, i, j + 1)] - gx;
+                ry += u[IDX(s, 1, i, j + 1)] - gy;
+                diag += 1.0;
+            }
+            if (j > 0 && s->active[(size_t)i * ny + j - 1] != 0) {
+                double gx = 0.5 * dy * (U[IDX(s, 3, i, j - 1)] + U[IDX(s, 3, i, j)]);
+                double gy = 0.5 * dy * ((U[IDX(s, 5, i, j - 1)] - 1.0) + (U[IDX(s, 5, i, j)] - 1.0));
+                rx += u[IDX(s, 0, i, j - 1)] + gx;
+                ry += u[IDX(s, 1, i, j - 1)] + gy;
+                diag += 1.0;
+            }
+            double u0 = u[IDX(s, 0, i, j)], u1 = u[IDX(s, 1, i, j)];
+            if (diag > 0.0) {
+                s->u_next[IDX(s, 0, i, j)] = u0 + relax * (rx / diag - u0);
+                s->u_next[IDX(s, 1, i, j)] = u1 + relax * (ry / diag - u1);
+            } else {
+                s->u_next[IDX(s, 0, i, j)] = u0;
+                s->u_next[IDX(s, 1, i, j)] = u1;
+            }
+        }
+    }
+}
+
+/* cbw2d:3494-3518 */
+static void disp_commit(or2d_state *s) {
+    const int nx = s->nx, ny = s->ny;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int c = 0; c < 2; ++c) {
+                size_t k = IDX(s, c, i, j);
+                if (s->active[(size_t)i * ny + j] == 0) {
+                    s->u[k] = 0.0;
+                    s->u_star[k] = 0.0;
+                } else {
+                    double delta = s->u_next[k] - s->u[k];
+                    s->u[k] = s->u_next[k];
+                    s->u_star[k] = s->u_star[k] + delta;
+                }
+            }
+}
+
+/* cbw2d:3213-3272 (_local_dudx / _local_dudy): derivative of u component
+ * `comp` along axis (0 x, 1 y) at (i, j): centered, else second-order
+ * one-sided, else first-order one-sided, else 0. */
+static double du_axis(const or2d_state *s, const double *u, int comp, int i, int j, int axis) {
+    const int nx = s->nx, ny = s->ny;
+    const int n = axis == 0 ? nx : ny;
+    const int k = axis == 0 ? i : j;
+    const double h = axis == 0 ? s->dx : s->dy;
+#define ACT(kk) (axis == 0 ? s->active[(size_t)(kk) * ny + j] : s->active[(size_t)i * ny + (kk)])
+#define UV(kk) (axis == 0 ? u[IDX(s, comp, (kk), j)] : u[IDX(s, comp, i, (kk))])
+    if (k > 0 && k + 1 < n && ACT(k - 1) != 0 && ACT(k + 1) != 0) return (UV(k + 1) - UV(k - 1)) / (2.0 * h);
+    if (k + 1 < n && ACT(k + 1) != 0) {
+        if (k + 2 < n && ACT(k + 2) != 0) return (-3.0 * UV(k) + 4.0 * UV(k + 1) - UV(k + 2)) / (2.0 * h);
+        return (UV(k + 1) - UV(k)) / h;
+    }
+    if (k > 0 && ACT(k - 1) != 0) {
+        if (k > 1 && ACT(k - 2) != 0) return (3.0 * UV(k) - 4.0 * UV(k - 1) + UV(k - 2)) / (2.0 * h);
+        return (UV(k) - UV(k - 1)) / h;
+    }
+    return 0.0;
+#undef ACT
+#undef UV
+}
+
+void or2d_du(const or2d_state *s, const double *u, double *out) {
+    const int nx = s->nx, ny = s->ny;
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+            if (s->active[(size_t)i * ny + j] == 0) {
+                for (int k = 0; k < 4; ++k) out[IDX(s, k, i, j)] = 0.0;
+                continue;
+            }
+            out[IDX(s, 0, i, j)] = du_axis(s, u, 0, i, j, 0);
+            out[IDX(s, 1, i, j)] = du_axis(s, u, 0, i, j, 1);
+            out[IDX(s, 2, i, j)] = du_axis(s, u, 1, i, j, 0);
+            out[IDX(s, 3, i, j)] = du_axis(s, u, 1, i, j, 1);
+        }
+}
+
+/* cbw2d:3522-3615 (_local_compatibility_repair_kernel) on fpost. */
+static void repair_F(or2d_state *s) {
+    const int nx = s->nx, ny = s->ny;
+    const double *u = s->u;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i) {
+        for (int j = 0; j < ny; ++j) {
+            const int32_t *act = s->active;
+            if (act[(size_t)i * ny + j] == 0) continue;
+            if (s->compat_interior_only) {
+                if (i == 0 || i + 1 >= nx || j == 0 || j + 1 >= ny) continue;
+                if (act[(size_t)(i - 1) * ny + j] == 0 || act[(size_t)(i + 1) * ny + j] == 0 ||
+                    act[(size_t)i * ny + j - 1] == 0 || act[(size_t)i * ny + j + 1] == 0)
+                    continue;
+            }
+            double o[4] = {s->U[IDX(s, 2, i, j)], s->U[IDX(s, 3, i, j)], s->U[IDX(s, 4, i, j)],
+                           s->U[IDX(s, 5, i, j)]};
+            double t[4] = {1.0 + du_axis(s, u, 0, i, j, 0), du_axis(s, u, 0, i, j, 1),
+                           du_axis(s, u, 1, i, j, 0), 1.0 + du_axis(s, u, 1, i, j, 1)};
+            double alpha = fmax(0.0, fmin(1.0, s->compat_blend));
+            double n[4] = {o[0], o[1], o[2], o[3]};
+            for (int g = 0; g < 10; ++g) {
+                for (int k = 0; k < 4; ++k) n[k] = o[k] + alpha * (t[k] - o[k]);
+                if (n[0] * n[3] - n[1] * n[2] > s->j_floor) break;
+                alpha = 0.5 * alpha;
+            }
+            double Po[4], Pn[4];
+            piola_s(s, o, Po);
+            piola_s(s, n, Pn);
+            double dP[4] = {Pn[0] - Po[0], Pn[1] - Po[1], Pn[2] - Po[2], Pn[3] - Po[3]};
+            const double ff = 0.5 / s->c, qt = 0.25;
+            for (int q = 0; q < NQ; ++q) {
+                double cx = (double)CX[q], cy = (double)CY[q];
+                double *f = s->fpost;
+                f[IDX(s, q * 6 + 0, i, j)] = f[IDX(s, q * 6 + 0, i, j)] - ff * (cx * dP[0] + cy * dP[1]);
+                f[IDX(s, q * 6 + 1, i, j)] = f[IDX(s, q * 6 + 1, i, j)] - ff * (cx * dP[2] + cy * dP[3]);
+                for (int k = 0; k < 4; ++k)
+                    f[IDX(s, q * 6 + 2 + k, i, j)] = f[IDX(s, q * 6 + 2 + k, i, j)] + qt * (n[k] - o[k]);
+            }
+            for (int k = 0; k < 4; ++k) s->U[IDX(s, 2 + k, i, j)] = n[k];
+            double J = n[0] * n[3] - n[1] * n[2];
+            for (int k = 0; k < 4; ++k) s->P[IDX(s, k, i, j)] = Pn[k];
+            double Js = clamp_j(J, s->j_floor);
+            s->sigma[IDX(s, 0, i, j)] = (Pn[0] * n[0] + Pn[1] * n[1]) / Js;
+            s->sigma[IDX(s, 1, i, j)] = (Pn[0] * n[2] + Pn[1] * n[3]) / Js;
+            s->sigma[IDX(s, 2, i, j)] = (Pn[2] * n[0] + Pn[3] * n[1]) / Js;
+            s->sigma[IDX(s, 3, i, j)] = (Pn[2] * n[2] + Pn[3] * n[3]) / Js;
+        }
+    }
+}
+
+/* cbw2d:2044-2060 with the clear of f1 before it. */
+static void stream_push(or2d_state *s) {
+    const int nx = s->nx, ny = s->ny;
+    memset(s->f1, 0, sizeof(double) * FQ * (size_t)nx * ny);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+            if (s->active[(size_t)i * ny + j] == 0) continue;
+            for (int q = 0; q < NQ; ++q) {
+                int ii = i + CX[q], jj = j + CY[q];
+                if (ii >= 0 && ii < nx && jj >= 0 && jj < ny && s->active[(size_t)ii * ny + jj] != 0)
+                    for (int a = 0; a < NC; ++a)
+                        s->f1[IDX(s, q * 6 + a, ii, jj)] = s->fpost[IDX(s, q * 6 + a, i, j)];
+            }
+        }
+}
+
+/* cbw2d:2064-3098 (_apply_curved_boundary_links_kernel) for the bfl,
+ * lsq_bfl, local_f and local_f_bfl reconstructions: reads fpost, writes the
+ * incoming population of every cut link into f1. */
+static void apply_links(or2d_state *s, const double *fpost, double *f1, double time_mid) {
+    const int nx = s->nx, ny = s->ny, rid = s->recon;
+    const double dt = s->dt, c = s->c, om = s->omega, jf = s->j_floor;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int ell = 0; ell < s->n_links; ++ell) {
+        const int i = s->link_i[ell], j = s->link_j[ell], q_out = s->link_q[ell];
+        int bid = s->link_bid[ell];
+        if (bid < 0 || bid >= MAX_IDS) bid = 0;
+        const int kind = s->kind_by_id[bid], mode = s->mode_by_id[bid];
+        const double *p = s->params_by_id + (size_t)bid * 8;
+        int q_in = q_out + 2;
+        if (q_in >= 4) q_in -= 4;
+        const double eta = s->link_eta[ell], xb = s->link_xb[ell], yb = s->link_yb[ell];
+        const double nxb = s->link_nx[ell], nyb = s->link_ny[ell];
+        const double dx_in = (double)CX[q_in], dy_in = (double)CY[q_in];
+        double bv0 = or2d_boundary_value(0, mode, p, xb, yb, nxb, nyb, time_mid);
+        double bv1 = or2d_boundary_value(1, mode, p, xb, yb, nxb, nyb, time_mid);
+        if (kind == BC_NEU && s->use_dyn) {
+            bv0 = s->dyn_tx[ell];
+            bv1 = s->dyn_ty[ell];
+        }
+        double Fg[4];
+        for (int k = 0; k < 4; ++k) Fg[k] = s->U[IDX(s, 2 + k, i, j)];
+        const int iu = i - CX[q_out], ju = j - CY[q_out];
+        const int up_ok = iu >= 0 && iu < nx && ju >= 0 && ju < ny && s->active[(size_t)iu * ny + ju] != 0;
+        if (rid == R_BFL || rid == R_REG_BFL || rid == R_LSQ_BFL || rid == R_LOCALF_BFL) {
+            if (up_ok)
+                for (int k = 0; k < 4; ++k)
+                    Fg[k] = s->U[IDX(s, 2 + k, i, j)] + eta * (s->U[IDX(s, 2 + k, i, j)] - s->U[IDX(s, 2 + k, iu, ju)]);
+            if (rid == R_LSQ_BFL)
+                for (int k = 0; k < 4; ++k) Fg[k] = lsq_at(s, s->lsq_wb, ell, 2 + k, Fg[k]);
+        } else {
+            for (int k = 0; k < 4; ++k) Fg[k] = interp_U_link(s, 2 + k, i, j, q_out, eta);
+        }
+        if (rid == R_CHAR2 || rid == R_LOCALF)
+            for (int k = 0; k < 4; ++k) Fg[k] = lsq_at(s, s->lsq_wb, ell, 2 + k, Fg[k]);
+        double Fb[4] = {Fg[0], Fg[1], Fg[2], Fg[3]};
+        if (kind == BC_NEU) or2d_solve_normal(s, Fg, nxb, nyb, bv0, bv1, Fb);
+        double Pb[4];
+        piola_s(s, Fb, Pb);
+        const int use_char = (rid == R_CHAR2 || rid == R_LOCALF) && kind == BC_NEU && s->lsq_valid[ell] != 0;
+        double cv[6] = {s->U[IDX(s, 0, i, j)], s->U[IDX(s, 1, i, j)], Fb[0], Fb[1], Fb[2], Fb[3]};
+        if (kind == BC_DIR) {
+            cv[0] = bv0;
+            cv[1] = bv1;
+        }
+        double cw[6] = {0, 0, 0, 0, 0, 0};
+        if (use_char) {
+            double ux[6], uy[6];
+            for (int a = 0; a < 6; ++a) {
+                ux[a] = lsq_at(s, s->lsq_wx, ell, a, 0.0);
+                uy[a] = lsq_at(s, s->lsq_wy, ell, a, 0.0);
+            }
+            const double shx = -eta * dt * c * (double)CX[q_out];
+            const double shy = -eta * dt * c * (double)CY[q_out];
+            double d[6];
+            for (int a = 0; a < 6; ++a) d[a] = shx * ux[a] + shy * uy[a];
+            double sc = 1.0;
+            for (int g = 0; g < 10; ++g) {
+                double t11 = cv[2] + sc * d[2], t12 = cv[3] + sc * d[3];
+                double t21 = cv[4] + sc * d[4], t22 = cv[5] + sc * d[5];
+                if (t11 * t22 - t12 * t21 > 10.0 * jf) break;
+                sc = 0.5 * sc;
+            }
+            for (int a = 0; a < 6; ++a) cv[a] = cv[a] + sc * d[a];
+            double jx[6], jy[6];
+            flux_jvp(s, 0, cv, ux, jx);
+            flux_jvp(s, 1, cv, uy, jy);
+            for (int a = 0; a < 6; ++a) {
+                double b = source_comp(a, s->source_mode, s->source_params, cv[0], cv[1]);
+                cw[a] = b - jx[a] - jy[a] + c * (dx_in * ux[a] + dy_in * uy[a]);
+            }
+        }
+        double Pc[4], Acw[6];
+        int char_ok = 0;
+        if (use_char && cv[2] * cv[5] - cv[3] * cv[4] > jf) {
+            char_ok = 1;
+            piola_s(s, cv + 2, Pc);
+            /* _equilibrium_jvp_comp (cbw2d:1368-1440): both axes along cw */
+            double ax[6], ay[6];
+            flux_jvp(s, 0, cv, cw, ax);
+            flux_jvp(s, 1, cv, cw, ay);
+            const double cxq = (double)CX[q_in], cyq = (double)CY[q_in];
+            for (int a = 0; a < 6; ++a) Acw[a] = 0.25 * (cw[a] + (2.0 / c) * (cxq * ax[a] + cyq * ay[a]));
+        }
+        for (int a = 0; a < 6; ++a) {
+            double D, S;
+            if (kind == BC_DIR) {
+                switch (a) {
+                case 0: D = -1.0; S = 0.5 * bv0; break;
+                case 1: D = -1.0; S = 0.5 * bv1; break;
+                case 2: D = 1.0; S = -dx_in * bv0 / c; break;
+                case 3: D = 1.0; S = -dy_in * bv0 / c; break;
+                case 4: D = 1.0; S = -dx_in * bv1 / c; break;
+                default: D = 1.0; S = -dy_in * bv1 / c; break;
+                }
+            } else {
+                switch (a) {
+                case 0: D = 1.0; S = -(dx_in * Pb[0] + dy_in * Pb[1]) / c; break;
+                case 1: D = 1.0; S = -(dx_in * Pb[2] + dy_in * Pb[3]) / c; break;
+                default: D = -1.0; S = 0.5 * Fb[a - 2]; break;
+                }
+            }
+            const double fo = fpost[IDX(s, q_out * 6 + a, i, j)];
+            const double fi = fpost[IDX(s, q_in * 6 + a, i, j)];
+            double val;
+            if (eta >= 0.5) {
+                double alpha = 1.0 / (2.0 * eta);
+                val = alpha * (D * fo + S) + (1.0 - alpha) * fi;
+            } else {
+                double up = fo;
+                if (up_ok) up = fpost[IDX(s, q_out * 6 + a, iu, ju)];
+                val = D * (2.0 * eta * fo + (1.0 - 2.0 * eta) * up) + S;
+            }
+            if (char_ok) {
+                double geq = feq(q_in, a, cv, Pc, c);
+                double src = source_comp(a, s->source_mode, s->source_params, cv[0], cv[1]);
+                val = geq - dt / om * Acw[a] + dt * (1.0 / om - 0.5) * 0.25 * src;
+            }
+            f1[IDX(s, q_in * 6 + a, i, j)] = val;
+        }
+    }
+}
+
+/* WarpCurvedBoundaryLBM2D.apply_boundary_reconstruction (cbw2d:4020-4034). */
+int or2d_apply_boundary(or2d_state *s, double t, int32_t in_place) {
+    if (!recon_supported(s->recon)) return -1;
+    if (s->n_links == 0) return 0;
+    const size_t n = FQ * (size_t)s->nx * s->ny;
+    if (in_place) {
+        memcpy(s->fpost, s->f0, n * sizeof(double));
+        apply_links(s, s->fpost, s->f0, t);
+    } else {
+        apply_links(s, s->fpost, s->f1, t);
+    }
+    return 0;
+}
+
+/* WarpCurvedBoundaryLBM2D.step (cbw2d:4180-4256) repeated nsteps times from
+ * (time, steps).  The newest populations are left in f0. */
+int or2d_step(or2d_state *s, double time, int64_t steps, int32_t nsteps) {
+    if (!recon_supported(s->recon)) return -1;
+    double *f0 = s->f0, *f1 = s->f1;
+    for (int k = 0; k < nsteps; ++k) {
+        s->f0 = f0;
+        s->f1 = f1;
+        s->invalid[0] = 0;
+        collide(s);
+        const double tb = time + 0.5 * s->dt;
+        if (s->disp_interval > 0 && steps % s->disp_interval == 0) {
+            for (int w = 0; w < s->disp_sweeps; ++w) {
+                build_bc(s, tb);
+                disp_relax(s);
+                disp_commit(s);
+            }
+        }
+        if (s->compat_interval > 0 && steps % s->compat_interval == 0) repair_F(s);
+        stream_push(s);
+        if (s->n_links > 0) apply_links(s, s->fpost, s->f1, tb);
+        double *t = f0;
+        f0 = f1;
+        f1 = t;
+        time += s->dt;
+        steps += 1;
+    }
+    /* After an odd count the newest state is in the caller's f1 buffer; the
+     * descriptor's f0/f1 pointers are left swapped accordingly. */
+    s->f0 = f0;
+    s->f1 = f1;
+    return 0;
+}
+
+/* Equilibrium populations (24, index q*6+a) of one state U (vnew2d:1140-1168). */
+void or2d_equilibrium(const or2d_state *s, const double *U, double *f) {
+    double P[4];
+    piola_s(s, U + 2, P);
+    for (int q = 0; q < NQ; ++q)
+        for (int a = 0; a < NC; ++a) f[q * 6 + a] = feq(q, a, U, P, s->c);
+}
+
+int or2d_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

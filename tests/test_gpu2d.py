@@ -1,0 +1,223 @@
+"""HIP (gfx950) D2Q4x6 step against the CPU FP64 oracle and the reference's
+own 2D tests.  Every test calls libvlbm.so through the C ABI.
+
+Tolerances: populations agree to 1e-12 absolute, and u, v, F to 1e-10
+relative in the l2 norm (the FP64 parity bar) unless stated otherwise; the
+GPU contracts multiply-adds into FMAs where the oracle does not, so results
+differ at rounding level only."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from curved_lbm.two_d import curved_boundary_warp as cb  # noqa: E402
+from curved_lbm.two_d import vector_nonlinear_elastic_warp as ve  # noqa: E402
+from oracle.oracle2d import OracleSolver2D  # noqa: E402
+
+import gpu_cases2d as gc  # noqa: E402
+
+
+def assert_parity(gpu, ora, *, f_atol=1e-12, field_rel=1e-10, floor=1e-14):
+    act = gpu.active_mask()
+    Ug, Uo = gpu.system_state(), ora.U
+    for name, a, b in (("v", Ug[:2], Uo[:2]), ("F", Ug[2:], Uo[2:]), ("u", gpu.displacement(), ora.u),
+                       ("u_star", gpu.numpy_field("u_star"), ora.u_star)):
+        a, b = a[:, act], b[:, act]
+        err = np.linalg.norm(a - b)
+        assert err <= field_rel * np.linalg.norm(b) + floor * np.sqrt(b.size), (name, err, np.linalg.norm(b))
+    fd = np.max(np.abs(gpu.numpy_field("f0") - ora.f0))
+    assert fd <= f_atol, ("f0", fd)
+
+
+CASES = {
+    "annulus_local_f": lambda: gc.annulus(40),
+    "annulus_local_f_interior_only": lambda: gc.annulus(40, local_compatibility_interior_only=True),
+    "annulus_fd_jacobian": lambda: gc.annulus(36, normal_jacobian="finite_difference"),
+    "annulus_energy": lambda: gc.annulus(36, neumann_solver="energy"),
+    "annulus_two_sweeps_intervals": lambda: gc.annulus(36, local_displacement_sweeps=2,
+                                                       local_compatibility_interval=2),
+    "annulus_kind_anchors": lambda: gc.annulus(36, local_displacement_boundary_id=-1),
+    "annulus_local_f_bfl": lambda: gc.annulus(36, boundary_reconstruction="local_f_bfl"),
+    "annulus_lsq_bfl": lambda: gc.annulus(36, boundary_reconstruction="lsq_bfl"),
+    "annulus_mooney": lambda: gc.annulus(32, model="mooney_rivlin"),
+    "annulus_svk": lambda: gc.annulus(32, model="svk"),
+    "annulus_yeoh": lambda: gc.annulus(32, model="yeoh"),
+    "annulus_gent": lambda: gc.annulus(32, model="gent"),
+    "annulus_log_nh": lambda: gc.annulus(32, model="log_neo_hooke"),
+    "annulus_sugiyama": lambda: gc.annulus(32, model="sugiyama_svk"),
+    "annulus_sine2_hold": lambda: gc.annulus(36, mode=cb.CURVED_VALUE_RADIAL_TRACTION_SINE2_HOLD,
+                                             params=(0.1, 0.2, 0.0)),
+    "circle_bfl_constant_velocity": lambda: gc.circle_dirichlet(32, "bfl"),
+    "circle_bfl_affine_velocity": lambda: gc.circle_dirichlet(32, "bfl", affine=True),
+    "circle_piola_traction_constant_source": lambda: gc.circle_traction(32),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_gpu_matches_oracle(case):
+    g, kw, init = CASES[case]()
+    gpu = cb.WarpCurvedBoundaryLBM2D(g, **kw)
+    ora = OracleSolver2D(g, **kw)
+    if init is not None:
+        gpu.initialize_from_numpy(**init)
+        ora.initialize_from_numpy(**init)
+    assert_parity(gpu, ora)
+    gpu.run(40)
+    ora.run(40)
+    assert_parity(gpu, ora)
+    red = gpu.reductions()
+    assert red["nonfinite"] == 0 and red["min_J"] > 0
+
+
+def test_gpu_fpost_stress_and_refresh_match_oracle():
+    g, kw, _ = gc.annulus(32)
+    gpu = cb.WarpCurvedBoundaryLBM2D(g, **kw)
+    ora = OracleSolver2D(g, **kw)
+    gpu.run(17)
+    ora.run(17)
+    act = gpu.active_mask()
+    np.testing.assert_allclose(gpu.numpy_field("fpost")[:, act], ora.fpost[:, act], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(gpu.numpy_field("P")[:, act], ora.P[:, act], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(gpu.numpy_field("sigma")[:, act], ora.sigma[:, act], rtol=0, atol=1e-12)
+    gpu.refresh_current()
+    ora.refresh_current()
+    assert_parity(gpu, ora)
+    # a standalone reconstruction from the last post-collision state
+    gpu.apply_boundary_reconstruction(gpu.time + 0.5 * gpu.dt)
+    ora.apply_boundary_reconstruction(ora.time + 0.5 * ora.dt)
+    links = g.n_links
+    gi, gj, gq = g.link_i, g.link_j, (g.link_q + 2) % 4
+    f1g, f1o = gpu.numpy_field("f1"), ora.f1
+    for a in range(6):
+        np.testing.assert_allclose(f1g[gq * 6 + a, gi, gj], f1o[gq * 6 + a, gi, gj], rtol=0, atol=1e-12)
+    assert links > 0
+
+
+def test_gpu_single_steps_equal_batched_run():
+    g, kw, _ = gc.annulus(32)
+    a = cb.WarpCurvedBoundaryLBM2D(g, **kw)
+    b = cb.WarpCurvedBoundaryLBM2D(g, **kw)
+    for _ in range(11):
+        a.step()
+    b.run(11)
+    for name in ("f0", "U", "u", "u_star"):
+        np.testing.assert_array_equal(a.numpy_field(name), b.numpy_field(name))
+
+
+def test_gpu_run_is_deterministic():
+    g, kw, _ = gc.annulus(40)
+    a = cb.WarpCurvedBoundaryLBM2D(g, **kw)
+    b = cb.WarpCurvedBoundaryLBM2D(g, **kw)
+    a.run(50)
+    b.run(50)
+    np.testing.assert_array_equal(a.numpy_field("f0"), b.numpy_field("f0"))
+    np.testing.assert_array_equal(a.numpy_field("u"), b.numpy_field("u"))
+
+
+def test_gpu_in_place_boundary_matches_frozen_source():
+    """Reference tests/test_initial_boundary_buffers.py, 2D case, on the HIP path (bitwise)."""
+    g = cb.CurvedBoundaryGeometry.circle(nx=8, ny=8, radius=0.12)
+    material = cb.WarpHyperelasticMaterial.from_poisson("neo_hooke", 0.2, mu=1)
+    boundary = cb.CurvedBoundarySpec(kind="dirichlet", value_mode=ve.VALUE_CONSTANT, params=(0.001, -0.002))
+    s = cb.WarpCurvedBoundaryLBM2D(g, material=material, boundary=boundary, boundary_reconstruction="bfl",
+                                   lattice_speed=10, collision_omega=1.8)
+    before = s.f0.numpy() + np.random.default_rng(501).normal(0, 0.001, s.f0.shape)
+    s.f0.assign(before)
+    s.fpost.assign(before)
+    s.f1.assign(np.full(s.f1.shape, np.nan))
+    s.apply_boundary_reconstruction(0.03, in_place=False)
+    separate = s.f1.numpy()
+    written = np.isfinite(separate)
+    assert written.any()
+    expected = before.copy()
+    expected[written] = separate[written]
+    s.f0.assign(before)
+    s.apply_boundary_reconstruction(0.03, in_place=True)
+    np.testing.assert_array_equal(s.f0.numpy(), expected)
+
+
+def test_gpu_quadratic_rest_state():
+    """Reference tests/test_gradients.py (2D rest-state half) on the HIP path."""
+    n = 24
+    g = cb.CurvedBoundaryGeometry.circle(nx=n, ny=n, radius=0.35)
+    m = cb.WarpHyperelasticMaterial.from_poisson("neo_hooke", 0.2, mu=1.0)
+    s = cb.WarpCurvedBoundaryLBM2D(g, material=m, boundary=cb.CurvedBoundarySpec(kind="neumann"),
+                                   boundary_reconstruction="local_f", lattice_speed=10.0, collision_omega=1.8,
+                                   local_displacement_interval=1, local_compatibility_interval=1,
+                                   local_compatibility_blend=0.5)
+    U = np.zeros((6, n, n))
+    U[2] = 1
+    U[5] = 1
+    s.initialize_from_numpy(U=U, displacement=np.zeros((2, n, n)), apply_initial_boundaries=False)
+    for _ in range(5):
+        s.step()
+    s.refresh_current()
+    active = s.active_mask()
+    assert float(np.max(np.abs(s.system_state()[:, active] - U[:, active]))) < 1e-11
+
+
+def test_gpu_dynamic_link_traction_matches_mode():
+    """Per-link tractions equal to the mode's values give the same run."""
+    g, kw, _ = gc.annulus(32, mode=cb.CURVED_VALUE_RADIAL_TRACTION, params=(0.02,))
+    a = cb.WarpCurvedBoundaryLBM2D(g, **kw)
+    b = cb.WarpCurvedBoundaryLBM2D(g, **kw)
+    outer = g.link_boundary_id == cb.BOUNDARY_ID_OUTER
+    b.link_traction_x.assign(np.where(outer, 0.02 * g.link_nx, 0.0))
+    b.link_traction_y.assign(np.where(outer, 0.02 * g.link_ny, 0.0))
+    b.enable_dynamic_link_traction(True)
+    a.run(20)
+    b.run(20)
+    np.testing.assert_allclose(a.numpy_field("f0"), b.numpy_field("f0"), rtol=0, atol=1e-15)
+
+
+def test_gpu_fp32_populations_track_fp64():
+    g, kw, _ = gc.annulus(48)
+    a = cb.WarpCurvedBoundaryLBM2D(g, **kw)
+    b = cb.WarpCurvedBoundaryLBM2D(g, precision="fp32", **kw)
+    a.run(400)
+    b.run(400)
+    a.refresh_current()
+    b.refresh_current()
+    act = a.active_mask()
+    ua, ub = a.displacement()[:, act], b.displacement()[:, act]
+    Fa, Fb = a.system_state()[2:, act], b.system_state()[2:, act]
+    assert np.linalg.norm(ub - ua) <= 1e-4 * np.linalg.norm(ua)
+    assert np.max(np.abs(Fb - Fa)) <= 1e-5
+    assert np.max(np.abs(b.numpy_field("f0")[:, ~act])) == 0.0
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_gpu_second_order_initialisation_matches_oracle(order):
+    g, kw, _ = gc.annulus(32)
+    rng = np.random.default_rng(5)
+    n = g.nx
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    U = np.zeros((6, n, n))
+    U[0] = 0.01 * np.sin(3 * X) * np.cos(2 * Y)
+    U[1] = -0.01 * X * Y
+    U[2] = 1.0 + 0.01 * X
+    U[3] = 0.005 * Y
+    U[4] = -0.004 * X
+    U[5] = 1.0 - 0.01 * Y
+    disp = 1e-3 * rng.standard_normal((2, n, n))
+    init = dict(U=U, displacement=disp, init_order=order)
+    gpu = cb.WarpCurvedBoundaryLBM2D(g, **kw)
+    ora = OracleSolver2D(g, **kw)
+    gpu.initialize_from_numpy(**init)
+    ora.initialize_from_numpy(**init)
+    assert_parity(gpu, ora)
+    gpu.run(10)
+    ora.run(10)
+    assert_parity(gpu, ora)
+
+
+def test_gpu_rejects_unprovided_options():
+    g, kw, _ = gc.annulus(16)
+    for over in (dict(boundary_reconstruction="qbfl"), dict(collision_model="mrt_ghost"),
+                 dict(kinetic_filter_strength=0.1), dict(compatibility_projection_interval=1)):
+        with pytest.raises(ValueError):
+            cb.WarpCurvedBoundaryLBM2D(g, **{**kw, **over})

@@ -1,5 +1,6 @@
 """The C ABI library loads and exports every entry point of
-include/vlbm3d.h; descriptor layout agrees with the library.  No GPU work."""
+include/vlbm3d.h and include/vlbm2d.h; descriptor layouts agree with the
+library.  No GPU work."""
 
 import ctypes as C
 import re
@@ -9,18 +10,22 @@ import pytest
 
 from curved_lbm import _native
 
-HEADER = Path(__file__).resolve().parents[1] / "include" / "vlbm3d.h"
+INCLUDE = Path(__file__).resolve().parents[1] / "include"
+HEADER = INCLUDE / "vlbm3d.h"
+HEADER2 = INCLUDE / "vlbm2d.h"
 
 
 def declared_functions():
-    text = HEADER.read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(vlbm\w*)\s*\(", text, flags=re.M)))
+    names = set()
+    for h in (HEADER, HEADER2):
+        names |= set(re.findall(r"^\s*(?:int|const char \*)\s*(vlbm\w*)\s*\(", h.read_text(), flags=re.M))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
     lib = _native.load_library()
     names = declared_functions()
-    assert len(names) >= 10
+    assert len(names) >= 25
     for name in names:
         assert hasattr(lib, name), name
     assert set(names) == set(_native.EXPORTED)
@@ -44,9 +49,32 @@ def test_descriptor_layout_matches_library():
     assert b"size mismatch" in lib.vlbm_last_error()
 
 
-def test_header_struct_fields_match_binding():
-    text = HEADER.read_text()
-    body = text[text.index("typedef struct {"): text.index("} vlbm3d_desc;")]
+def test_descriptor2_layout_matches_library():
+    lib = _native.load_library()
+    d = _native.Vlbm2dDesc()
+    d.struct_size = C.sizeof(_native.Vlbm2dDesc)
+    d.abi_version = _native.ABI2_VERSION
+    d.nx = d.ny = 8
+    d.n_active = 10
+    d.disp_sweeps = 1
+    d.recon = 12
+    h = C.c_void_p()
+    assert lib.vlbm2d_create(C.byref(d), C.byref(h)) == -1
+    assert b"null" in lib.vlbm_last_error()
+    d.recon = 1
+    assert lib.vlbm2d_create(C.byref(d), C.byref(h)) == -1
+    assert b"not provided" in lib.vlbm_last_error()
+    d.recon = 12
+    d.struct_size -= 8
+    assert lib.vlbm2d_create(C.byref(d), C.byref(h)) == -1
+    assert b"size mismatch" in lib.vlbm_last_error()
+
+
+@pytest.mark.parametrize("header,tag,binding", [(HEADER, "vlbm3d_desc", "Vlbm3dDesc"),
+                                                (HEADER2, "vlbm2d_desc", "Vlbm2dDesc")])
+def test_header_struct_fields_match_binding(header, tag, binding):
+    text = header.read_text()
+    body = text[text.index("typedef struct {"): text.index("} " + tag + ";")]
     body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
     names = []
     for decl in body.split(";"):
@@ -63,7 +91,7 @@ def test_header_struct_fields_match_binding():
             item = item.strip()
             if item:
                 names.append(re.sub(r"\[.*\]", "", item).strip())
-    assert names == [f[0] for f in _native.Vlbm3dDesc._fields_]
+    assert names == [f[0] for f in getattr(_native, binding)._fields_]
 
 
 def test_missing_library_fails_loudly(tmp_path):
