@@ -1,11 +1,18 @@
 #!/usr/bin/env python3
-"""Benchmark of the D3Q6x12 time step on one MI355X.
+"""Benchmark of the vectorial LBM time step on one MI355X.
 
-Workload: the spherical shell on a 128^3 lattice (inner radius 0.2 L clamped,
-outer radius 0.45 L loaded by a nominal traction ramped linearly to 0.03 over
-4000 steps), manuscript settings (c = 10, omega = 1.8, alpha_u = 0.85,
-theta_F = 0.5, w_D = 100, gamma = 10, local_f reconstruction, one
-displacement sweep and one F update per step).  Synthetic: starts from rest.
+Default workload (``--workload shell``): the D3Q6x12 spherical shell on a
+128^3 lattice (inner radius 0.2 L clamped, outer radius 0.45 L loaded by a
+nominal traction ramped linearly to 0.03 over 4000 steps), manuscript
+settings (c = 10, omega = 1.8, alpha_u = 0.85, theta_F = 0.5, w_D = 100,
+gamma = 10, local_f reconstruction, one displacement sweep and one F update
+per step).  Synthetic: starts from rest.
+
+``--workload superellipse``: the D2Q4x6 rotated superellipse shell on
+1024^2 (n 4, a 0.42 L, b 0.30 L, 30 degrees, inner circle 0.12 L clamped,
+outer nominal traction 0.04 ramped linearly over 5000 steps).
+``--workload annulus``: the D2Q4x6 annulus on 96^2 (Ri 0.2 L clamped, Ro
+0.45 L, traction 0.05 ramped over 1000 steps).  Same settings.
 
 Prints one JSON line: MLUPS (active-node updates per second) of the whole
 step with the populations resident in HBM, the roofline of the dominant
@@ -41,7 +48,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--n", type=int, default=128, help="lattice side")
+    ap.add_argument("--workload", choices=("shell", "superellipse", "annulus"), default="shell")
+    ap.add_argument("--n", type=int, default=0, help="lattice side (default 128 / 1024 / 96 by workload)")
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
     ap.add_argument("--cpu-steps", type=int, default=40, help="oracle steps timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -59,18 +67,17 @@ def main() -> int:
               "(a single lattice or batch runs on one device).", file=sys.stderr)
         return 2
 
-    import numpy as np
     import torch
 
-    from curved_lbm.three_d import curved_boundary_warp3d as cb
-    import gpu_cases
-
+    if args.n <= 0:
+        args.n = {"shell": 128, "superellipse": 1024, "annulus": 96}[args.workload]
     if args.equilibrium:
         return equilibrium(args)
 
     t_setup = time.time()
-    g, kw = gpu_cases.traction_shell(args.n, traction=0.03, ramp_steps=4000)
-    s = cb.WarpCurvedBoundaryLBM3D(g, **kw, precision=args.precision)
+    case = make_case(args)
+    g, kw = case["geometry"], case["kwargs"]
+    s = case["solver"](args.precision)
     setup_s = time.time() - t_setup
     print(f"[bench] setup {setup_s:.1f}s n_active={s.n_active} n_links={g.n_links}", file=sys.stderr, flush=True)
 
@@ -97,16 +104,18 @@ def main() -> int:
 
     n = s.n_active
     pop_bytes = 8 if args.precision == "fp64" else 4
+    Qn, dim = case["fq"], case["dim"]
     mlups = n * args.steps / elapsed / 1e6
-    box_mlups = args.n ** 3 * args.steps / elapsed / 1e6
+    box_mlups = args.n ** dim * args.steps / elapsed / 1e6
     kernels = {k: v for k, v in kt.items() if k != "step" and v["count"]}
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
     avg_ms = {k: v["ms"] / v["count"] for k, v in kernels.items()}
     # Algorithmic bytes: the population term of the fused-step figure
-    # (read + write 72 populations per active node) for the stream kernel;
-    # 1200 B (FP64) / 624 B (FP32) per active node for the whole step.
-    per_node = {"stream": 2 * 72 * pop_bytes}
-    step_bytes_node = 2 * 72 * pop_bytes + 2 * 3 * 8
+    # (read + write FQ populations per active node) for the stream kernel;
+    # for the whole step also read + write of the FP64 u* (3D: 1200 B FP64 /
+    # 624 B FP32; 2D: 416 B / 224 B per active node).
+    per_node = {"stream": 2 * Qn * pop_bytes}
+    step_bytes_node = 2 * Qn * pop_bytes + 2 * dim * 8
     if dom in per_node:
         alg = per_node[dom] * n
         achieved = alg / (avg_ms[dom] * 1e-3) / 1e9
@@ -116,7 +125,7 @@ def main() -> int:
     step_gbs = step_bytes_node * n / (step_ms * 1e-3) / 1e9
     roofline = {
         "bound": "hbm",
-        "kernel": f"k_stream<{'double' if pop_bytes == 8 else 'float'}>" if dom == "stream" else dom,
+        "kernel": f"{case['stream_kernel']}<{'double' if pop_bytes == 8 else 'float'}>" if dom == "stream" else dom,
         "achieved": round(achieved, 1) if achieved else None,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -128,7 +137,7 @@ def main() -> int:
                  "achieved": round(step_gbs, 1), "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "kernels_ms_per_step": {k: round(v["ms"] / args.steps, 5) for k, v in kernels.items()},
     }
-    prof = ROOT / "profiles" / f"traffic_k_stream_shell{args.n}_{args.precision}.json"
+    prof = ROOT / "profiles" / f"traffic_{case['stream_kernel']}_{args.workload}{args.n}_{args.precision}.json"
     if prof.exists():
         try:
             roofline["traffic"] = json.loads(prof.read_text()).get("hbm_bytes_per_launch")
@@ -147,11 +156,7 @@ def main() -> int:
     if not args.no_cpu:
         threads = min(16, cpu_count())
         os.environ["OMP_NUM_THREADS"] = str(threads)
-        from oracle.oracle3d import OracleSolver3D, load
-
-        load()
-        ora = OracleSolver3D(g, **kw, threads=threads, tables=s.tables,
-                             characteristic_stencil=s.characteristic_stencil)
+        ora = case["oracle"](s, threads)
         ora.run(1)
         tc = time.perf_counter()
         ora.run(args.cpu_steps)
@@ -161,12 +166,12 @@ def main() -> int:
             "unit": "MLUPS",
             "cores": threads,
             "kind": "port",
-            "sample": f"{args.cpu_steps} FP64 steps of the same {args.n}^3 shell after 1 warm-up step, "
+            "sample": f"{args.cpu_steps} FP64 steps of the same {case['short']} after 1 warm-up step, "
                       f"C oracle with OpenMP ({cpu_s:.2f} s)",
         }
 
     out = {
-        "metric": METRIC,
+        "metric": METRIC if args.workload == "shell" else f"MLUPS at D2Q4x6 {args.n}^2 {args.workload}",
         "value": round(mlups, 2),
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -179,9 +184,8 @@ def main() -> int:
         "dtype": "f64" if args.precision == "fp64" else "f32-populations/f64",
         "data": "synthetic",
         "config": {
-            "workload": f"D3Q6x12 spherical shell {args.n}^3, Ri 0.2L clamped, Ro 0.45L nominal traction 0.03 "
-                        f"ramped linearly over 4000 steps, manuscript settings, {args.precision} populations",
-            "lattice": [args.n] * 3,
+            "workload": f"{case['description']}, {args.precision} populations",
+            "lattice": [args.n] * dim,
             "n_active": n,
             "n_links": int(g.n_links),
             "precision": args.precision,
@@ -195,6 +199,58 @@ def main() -> int:
     return 0
 
 
+def make_case(args) -> dict:
+    """Geometry, solver kwargs, solver and oracle factories of a workload."""
+    if args.workload == "shell":
+        from curved_lbm.three_d import curved_boundary_warp3d as cb
+        import gpu_cases
+
+        g, kw = gpu_cases.traction_shell(args.n, traction=0.03, ramp_steps=4000)
+
+        def oracle(s, threads):
+            from oracle.oracle3d import OracleSolver3D, load
+
+            load()
+            return OracleSolver3D(g, **kw, threads=threads, tables=s.tables,
+                                  characteristic_stencil=s.characteristic_stencil)
+
+        return {
+            "geometry": g, "kwargs": kw, "fq": 72, "dim": 3, "stream_kernel": "k_stream",
+            "solver": lambda prec: cb.WarpCurvedBoundaryLBM3D(g, **kw, precision=prec), "oracle": oracle,
+            "ramp_steps": 4000, "short": f"{args.n}^3 shell",
+            "description": f"D3Q6x12 spherical shell {args.n}^3, Ri 0.2L clamped, Ro 0.45L nominal traction 0.03 "
+                           "ramped linearly over 4000 steps, manuscript settings",
+        }
+    from curved_lbm.two_d import curved_boundary_warp as cb2
+    from curved_lbm.two_d import shapes
+    import gpu_cases2d
+
+    dt = (1.0 / args.n) / 10.0
+    if args.workload == "superellipse":
+        g = shapes.rotated_superellipse_shell(args.n)
+        ramp, amp = 5000, 0.04
+        desc = (f"D2Q4x6 rotated superellipse shell {args.n}^2 (n 4, a 0.42L, b 0.30L, 30 deg), inner circle "
+                f"0.12L clamped, outer nominal traction {amp} ramped linearly over {ramp} steps, manuscript settings")
+    else:
+        g = cb2.CurvedBoundaryGeometry.annulus(nx=args.n, ny=args.n, inner_radius=0.2, outer_radius=0.45)
+        ramp, amp = 1000, 0.05
+        desc = (f"D2Q4x6 annulus {args.n}^2, Ri 0.2L clamped, Ro 0.45L nominal traction {amp} ramped linearly over "
+                f"{ramp} steps, manuscript settings")
+    kw = gpu_cases2d.settings()
+    kw.update(material=gpu_cases2d.material(), boundary=gpu_cases2d.annulus_spec(params=(amp, ramp * dt)))
+
+    def oracle2(s, threads):
+        from oracle.oracle2d import OracleSolver2D
+
+        return OracleSolver2D(g, **kw)
+
+    return {
+        "geometry": g, "kwargs": kw, "fq": 24, "dim": 2, "stream_kernel": "k2_stream",
+        "solver": lambda prec: cb2.WarpCurvedBoundaryLBM2D(g, **kw, precision=prec), "oracle": oracle2,
+        "ramp_steps": ramp, "short": f"{args.n}^2 {args.workload}", "description": desc,
+    }
+
+
 def equilibrium(args) -> int:
     """Wall-seconds from rest to rms_v <= tol (the equilibrium residual),
     FP64 populations vs FP32 populations, on the same shell; FP32 error in
@@ -202,15 +258,14 @@ def equilibrium(args) -> int:
     import numpy as np
     import torch
 
-    from curved_lbm.three_d import curved_boundary_warp3d as cb
-    import gpu_cases
-
-    g, kw = gpu_cases.traction_shell(args.n, traction=0.03, ramp_steps=4000)
-    ramp_time = 4000 * g.dx / kw["lattice_speed"]
+    case = make_case(args)
+    g, kw = case["geometry"], case["kwargs"]
+    dim = case["dim"]
+    ramp_time = case["ramp_steps"] * g.dx / kw["lattice_speed"]
     res = {}
     states = {}
     for prec in ("fp64", "fp32"):
-        s = cb.WarpCurvedBoundaryLBM3D(g, **kw, precision=prec)
+        s = case["solver"](prec)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         out = s.run_to_equilibrium(args.tol, check_every=args.check_every, min_time=ramp_time,
@@ -218,7 +273,11 @@ def equilibrium(args) -> int:
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         s.refresh_current()
-        U, u = s.U.numpy(), s.u.numpy()
+        if dim == 3:
+            U, u = s.U.numpy(), s.u.numpy()
+        else:
+            act = s.active_mask()
+            U, u = s.system_state()[:, act], s.displacement()[:, act]
         states[prec] = (U, u)
         res[prec] = {"converged": out["converged"], "steps": out["steps"], "wall_s": round(wall, 3),
                      "displacement_rate": out["displacement_rate"], "rms_v": out["rms_v"],
@@ -227,22 +286,24 @@ def equilibrium(args) -> int:
         print(f"[equilibrium] {prec}: {res[prec]}", file=sys.stderr, flush=True)
         del s
     (U64, u64), (U32, u32) = states["fp64"], states["fp32"]
-    J64 = cb._det3_np(U64[3:].reshape(3, 3, -1))
-    J32 = cb._det3_np(U32[3:].reshape(3, 3, -1))
+    nv = dim
+    F64, F32 = U64[nv:], U32[nv:]
+    J64 = np.linalg.det(np.moveaxis(F64.reshape(dim, dim, -1), -1, 0))
+    J32 = np.linalg.det(np.moveaxis(F32.reshape(dim, dim, -1), -1, 0))
 
     def rel(a, b):
         return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
-    eye = np.eye(3).reshape(9, 1)
-    err = {"u_rel_l2": rel(u32, u64), "F_rel_l2": rel(U32[3:], U64[3:]),
-           "F_minus_I_rel_l2": rel(U32[3:] - eye, U64[3:] - eye), "J_max_abs": float(np.max(np.abs(J32 - J64)))}
+    eye = np.eye(dim).reshape(dim * dim, 1)
+    err = {"u_rel_l2": rel(u32, u64), "F_rel_l2": rel(F32, F64),
+           "F_minus_I_rel_l2": rel(F32 - eye, F64 - eye), "J_max_abs": float(np.max(np.abs(J32 - J64)))}
     out = {
-        "metric": "wall-s to equilibrium (rms displacement rate <= %.0e), D3Q6x12 shell %d^3" % (args.tol, args.n),
+        "metric": "wall-s to equilibrium (rms displacement rate <= %.0e), %s" % (args.tol, case["short"]),
         "fp64": res["fp64"], "fp32": res["fp32"],
         "speedup_fp32_vs_fp64": round(res["fp64"]["wall_s"] / res["fp32"]["wall_s"], 3),
         "fp32_error_vs_fp64": err,
-        "config": {"lattice": [args.n] * 3, "n_active": int(g.n_active), "check_every": args.check_every,
-                   "min_time": ramp_time},
+        "config": {"workload": case["description"], "lattice": [args.n] * dim, "n_active": int(g.n_active),
+                   "check_every": args.check_every, "min_time": ramp_time},
     }
     print(json.dumps(out))
     return 0

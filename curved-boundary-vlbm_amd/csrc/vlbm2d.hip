@@ -38,7 +38,7 @@ using vlbm_common::xcd_grid;
 
 namespace {
 
-constexpr int LBLOCK = 128;
+constexpr int LBLOCK = 64;
 constexpr int RED_BLOCKS = 1024;
 
 // Record layout (12 planes): A[6], B_v[r*2+axis] (4), B_F[r] (2).
@@ -361,15 +361,46 @@ __device__ inline double interp_link(const double *U, const int32_t *act, size_t
     return v0;
 }
 
-// Normal-probe / LSQ weighted sum of U[comp] over a link's stencil (cbw2d:1202-1221).
-__device__ __forceinline__ double lsq_sum(const double *U, size_t N, const int32_t *lsq_p, const double *w, int L,
-                                          int ell, int comp) {
-    double out = 0.0;
-    for (int m = 0; m < MAX_LSQ; ++m) {
+// Normal-probe value and LSQ gradients of U at every cut point
+// (cbw2d:1202-1221 as used at :2141-2160 and :2491-2503): 32 lanes per
+// link, lane m loads stencil sample m; the sums then run over m in stencil
+// order, one thread per (link, output).  Output cs (16, L): rows 0-3 the
+// value of F at the cut point, rows 4-9 d/dx and 10-15 d/dy of U.
+constexpr int GL = 8;  // links per block of 256 threads
+__global__ __launch_bounds__(256) void k2_gather(const double *__restrict__ U, const int32_t *__restrict__ lsq_p,
+                                                 const double *__restrict__ wb, const double *__restrict__ wx,
+                                                 const double *__restrict__ wy, double *__restrict__ cs, int L,
+                                                 size_t N) {
+    __shared__ double sU[6][GL * 32];
+    __shared__ double sW[3][GL * 32];
+    const int t = threadIdx.x;
+    const int ll = t >> 5, m = t & 31;
+    const int ell = blockIdx.x * GL + ll;
+    double u6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, w3[3] = {0.0, 0.0, 0.0};
+    if (ell < L && m < MAX_LSQ) {
         const int pp = lsq_p[(size_t)m * L + ell];
-        if (pp >= 0) out += w[(size_t)m * L + ell] * U[comp * N + pp];
+        if (pp >= 0) {
+#pragma unroll
+            for (int a = 0; a < 6; ++a) u6[a] = U[a * N + pp];
+            w3[0] = wb[(size_t)m * L + ell];
+            w3[1] = wx[(size_t)m * L + ell];
+            w3[2] = wy[(size_t)m * L + ell];
+        }
     }
-    return out;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) sU[a][t] = u6[a];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) sW[k][t] = w3[k];
+    __syncthreads();
+    if (t >= GL * 16) return;
+    const int l2 = t >> 4, v = t & 15;
+    const int e2 = blockIdx.x * GL + l2;
+    if (e2 >= L) return;
+    const int set = v < 4 ? 0 : (v < 10 ? 1 : 2);
+    const int comp = v < 4 ? 2 + v : (v < 10 ? v - 4 : v - 10);
+    double acc = 0.0;
+    for (int k = 0; k < MAX_LSQ; ++k) acc += sW[set][l2 * 32 + k] * sU[comp][l2 * 32 + k];
+    cs[(size_t)v * L + e2] = acc;
 }
 
 // Incoming populations of every cut link (cbw2d:2064-3098 for the bfl,
@@ -379,7 +410,8 @@ __device__ __forceinline__ double lsq_sum(const double *U, size_t N, const int32
 template <typename T>
 __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T *__restrict__ fdst,
                                                    const double *__restrict__ rec, const double *__restrict__ U,
-                                                   Links2 Tb, int use_rec, double time, Params2 P) {
+                                                   const double *__restrict__ cs, Links2 Tb, int use_rec,
+                                                   double time, Params2 P) {
     const int ell = blockIdx.x * LBLOCK + threadIdx.x;
     if (ell >= P.L) return;
     const size_t N = P.N;
@@ -407,8 +439,7 @@ __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T
     if (P.recon == R_LOCALF) {
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            Fg[k] = valid ? lsq_sum(U, N, Tb.lsq_p, Tb.wb, L, ell, 2 + k)
-                          : interp_link(U, Tb.act, N, 2 + k, p, i, j, qo, eta, P);
+            Fg[k] = valid ? cs[(size_t)k * L + ell] : interp_link(U, Tb.act, N, 2 + k, p, i, j, qo, eta, P);
     } else {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -417,7 +448,7 @@ __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T
         }
         if (P.recon == R_LSQ_BFL && valid)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) Fg[k] = lsq_sum(U, N, Tb.lsq_p, Tb.wb, L, ell, 2 + k);
+            for (int k = 0; k < 4; ++k) Fg[k] = cs[(size_t)k * L + ell];
     }
     double Fb[4] = {Fg[0], Fg[1], Fg[2], Fg[3]};
     if (kind == BC_NEU) solve_normal2(Fg, nxb, nyb, bv0, bv1, P.solver, P.m, Fb);
@@ -436,8 +467,8 @@ __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T
         double gx[6], gy[6];
 #pragma unroll
         for (int a = 0; a < 6; ++a) {
-            gx[a] = lsq_sum(U, N, Tb.lsq_p, Tb.wx, L, ell, a);
-            gy[a] = lsq_sum(U, N, Tb.lsq_p, Tb.wy, L, ell, a);
+            gx[a] = cs[(size_t)(4 + a) * L + ell];
+            gy[a] = cs[(size_t)(10 + a) * L + ell];
         }
         const double shx = -eta * P.dt * P.c * (double)cx(qo);
         const double shy = -eta * P.dt * P.c * (double)cy(qo);
@@ -803,10 +834,18 @@ template <typename T>
 static int run_links2(vlbm2d_handle *h, const T *src, T *dst, double t, int use_rec) {
     const vlbm2d_desc &d = h->d;
     if (d.n_links == 0) return 0;
+    const bool gather = d.recon == R_LOCALF || d.recon == R_LSQ_BFL;
+    if (gather) {
+        TimedSlot tg(h->timing, h->stream, VLBM_SLOT_GATHER);
+        const int gg = (d.n_links + GL - 1) / GL;
+        hipLaunchKernelGGL(k2_gather, dim3(gg), dim3(256), 0, h->stream, (const double *)d.U, d.lsq_p, d.lsq_wb,
+                           d.lsq_wx, d.lsq_wy, d.link_state, d.n_links, h->P.N);
+        CHECK_LAUNCH("k2_gather");
+    }
     TimedSlot ts(h->timing, h->stream, VLBM_SLOT_LINKS);
     const int gl = (d.n_links + LBLOCK - 1) / LBLOCK;
     hipLaunchKernelGGL((k2_links<T>), dim3(gl), dim3(LBLOCK), 0, h->stream, src, dst, (const double *)d.record,
-                       (const double *)d.U, h->T, use_rec, t, h->P);
+                       (const double *)d.U, (const double *)d.link_state, h->T, use_rec, t, h->P);
     CHECK_LAUNCH("k2_links");
     return 0;
 }
@@ -907,7 +946,8 @@ int vlbm2d_create(const vlbm2d_desc *desc, vlbm2d_handle **out) {
     if (desc->n_links > 0) {
         const void *lreq[] = {desc->link_p, desc->link_q, desc->link_bid, desc->link_eta, desc->link_xb,
                               desc->link_yb, desc->link_nx, desc->link_ny, desc->lsq_p, desc->lsq_wb,
-                              desc->lsq_wx, desc->lsq_wy, desc->lsq_valid, desc->dyn_tx, desc->dyn_ty};
+                              desc->lsq_wx, desc->lsq_wy, desc->lsq_valid, desc->dyn_tx, desc->dyn_ty,
+                              desc->link_state};
         for (const void *ptr : lreq)
             if (!ptr) return fail_msg("vlbm2d_create: a required link pointer is null");
     }

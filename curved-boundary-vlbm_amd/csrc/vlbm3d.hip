@@ -206,22 +206,19 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(
     double Fp[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) Fp[k] = U[(3 + k) * n + p];
+    // Only the column of F along the link axis enters: the reference's
+    // off-axis terms are multiplied by a zero spacing and add exact zeros.
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const int pn = nbr[q * n + p];
         if (pn < 0) continue;
-        const double h[3] = {(double)dir_comp(q, 0) * P.dx, (double)dir_comp(q, 1) * P.dy,
-                             (double)dir_comp(q, 2) * P.dz};
+        const int j = q % 3;
+        const double h = (q < 3 ? 1.0 : -1.0) * (j == 0 ? P.dx : (j == 1 ? P.dy : P.dz));
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            double g = 0.0;
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                double s = Fp[3 * c + j] + U[(3 + 3 * c + j) * n + pn];
-                if (j == c) s -= 2.0;
-                g += s * h[j];
-            }
-            g = 0.5 * g;
+            double s = Fp[3 * c + j] + U[(3 + 3 * c + j) * n + pn];
+            if (j == c) s -= 2.0;
+            const double g = 0.5 * (s * h);
             r[c] += u[c * n + pn] - g;
             d[c] += 1.0;
         }

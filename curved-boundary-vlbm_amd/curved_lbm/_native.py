@@ -13,7 +13,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libvlbm.so"
 
 ABI_VERSION = 4
-ABI2_VERSION = 1
+ABI2_VERSION = 2
 PREC_F64 = 0
 PREC_F32 = 1
 
@@ -169,6 +169,7 @@ class Vlbm2dDesc(C.Structure):
         ("lsq_wx", C.c_void_p),
         ("lsq_wy", C.c_void_p),
         ("lsq_valid", C.c_void_p),
+        ("link_state", C.c_void_p),
         ("kind_by_id", C.c_void_p),
         ("mode_by_id", C.c_void_p),
         ("params_by_id", C.c_void_p),

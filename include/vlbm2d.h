@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define VLBM2D_ABI_VERSION 1
+#define VLBM2D_ABI_VERSION 2
 #define VLBM2D_MAX_IDS 8
 #define VLBM2D_MAX_LSQ 24
 #define VLBM2D_RECORD 12
@@ -86,6 +86,7 @@ typedef struct {
     const int32_t *lsq_p;            /* (24, L) node index of each stencil sample, -1 if unused */
     const double *lsq_wb, *lsq_wx, *lsq_wy; /* (24, L) value, d/dx, d/dy weights */
     const int32_t *lsq_valid;        /* (L) */
+    double *link_state;              /* (16, L) scratch: F at the cut point, dU/dx, dU/dy */
     const int32_t *kind_by_id;       /* (8) */
     const int32_t *mode_by_id;       /* (8) */
     const double *params_by_id;      /* (8, 8) */

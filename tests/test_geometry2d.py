@@ -155,3 +155,64 @@ def test_spec_tables_and_names():
     mat = cb.WarpHyperelasticMaterial.from_poisson("neo_hooke", 0.2)
     assert cb.neumann_solver_name("energy", mat) == "energy"
     assert math.isclose(mat.lam, 2.0 / 3.0)
+
+
+def test_array_level_set_path_equals_scalar_path():
+    """The array path of from_level_set (used at 1024^2) yields the same
+    geometry, bit for bit, as the per-point reference path when both see the
+    same elementwise level-set and normal values; with numpy's array
+    arithmetic (SIMD pow) the normals may differ in the last bit."""
+    from curved_lbm.two_d import shapes
+
+    outer = shapes.Superellipse(a=0.42, b=0.30, theta_deg=30.0, p=4.0)
+    inner = shapes.Superellipse(a=0.12, b=0.12, theta_deg=0.0, p=2.0)
+
+    def phi(x, y):
+        return float(np.maximum(outer.phi(x, y), -inner.phi(x, y)))
+
+    def normal(x, y):
+        po, pi = float(outer.phi(x, y)), float(inner.phi(x, y))
+        if abs(pi) <= abs(po):
+            gx, gy = inner.grad(x, y)
+            return -float(gx), -float(gy)
+        gx, gy = outer.grad(x, y)
+        return float(gx), float(gy)
+
+    def bid(x, y, _nx, _ny):
+        return cb.BOUNDARY_ID_INNER if abs(float(inner.phi(x, y))) <= abs(float(outer.phi(x, y))) else cb.BOUNDARY_ID_OUTER
+
+    phi_v = np.vectorize(phi, otypes=[np.float64])
+
+    def normal_v(x, y):
+        out = [normal(float(a), float(b)) for a, b in zip(np.ravel(x), np.ravel(y))]
+        return np.array([o[0] for o in out]), np.array([o[1] for o in out])
+
+    kw = dict(nx=48, ny=48, length_x=1.0, length_y=1.0, phi=phi, boundary_id=bid)
+    slow = cb.CurvedBoundaryGeometry.from_level_set(normal=normal, **kw)
+    fast = cb.CurvedBoundaryGeometry.from_level_set(phi_vectorized=phi_v, normal_vectorized=normal_v, **kw)
+    fast_np = shapes.superellipse_shell_geometry(48, outer, inner)
+    assert fast.n_links == slow.n_links == fast_np.n_links > 0
+    for name in ("active", "link_i", "link_j", "link_q", "link_boundary_id", "link_eta", "link_xb", "link_yb",
+                 "link_nx", "link_ny", "lsq_i", "lsq_j", "lsq_wb", "lsq_wt", "lsq_wx", "lsq_wy", "lsq_valid"):
+        np.testing.assert_array_equal(getattr(fast, name), getattr(slow, name), err_msg=name)
+        np.testing.assert_allclose(getattr(fast_np, name), getattr(slow, name), rtol=0, atol=1e-13, err_msg=name)
+    assert set(np.unique(fast.link_boundary_id)) == {1, 2}
+
+
+def test_superellipse1024_counts():
+    """The 1024^2 rotated superellipse shell (n 4, a 0.42, b 0.30, 30 deg,
+    inner circle 0.12): 442 466 active nodes, 4 424 cut links (3 440 outer,
+    984 inner), by the independent numpy count of the level set."""
+    from curved_lbm.two_d import shapes
+
+    g = shapes.rotated_superellipse_shell(1024)
+    n = 1024
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    ct, st = math.cos(math.radians(30.0)), math.sin(math.radians(30.0))
+    xi, eta = ct * (X - 0.5) + st * (Y - 0.5), -st * (X - 0.5) + ct * (Y - 0.5)
+    act = ((np.abs(xi) / 0.42) ** 4 + (np.abs(eta) / 0.30) ** 4 <= 1.0) & (np.hypot(X - 0.5, Y - 0.5) >= 0.12)
+    assert g.n_active == int(act.sum()) == 442466
+    assert g.n_links == 4424
+    assert np.bincount(g.link_boundary_id, minlength=3)[1:].tolist() == [3440, 984]
+    assert g.lsq_valid.all()
