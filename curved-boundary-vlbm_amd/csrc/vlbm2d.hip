@@ -92,17 +92,34 @@ __device__ __forceinline__ double st_nt(float *f, size_t i, int a, double v) {
     return (double)x + rest(a);
 }
 
+// Records are stored in the precision of the populations; float records
+// hold A - omega f_rest on the A planes (rest state subtracted).
+__device__ __forceinline__ double rld(const double *r, size_t i, int, double) { return r[i]; }
+__device__ __forceinline__ double rld(const float *r, size_t i, int a, double w) {
+    return (double)r[i] + w * rest(a);
+}
+__device__ __forceinline__ double rld_b(const double *r, size_t i) { return r[i]; }
+__device__ __forceinline__ double rld_b(const float *r, size_t i) { return (double)r[i]; }
+__device__ __forceinline__ void rst_a(double *r, size_t i, int, double, double v) { r[i] = v; }
+__device__ __forceinline__ void rst_a(float *r, size_t i, int a, double w, double v) {
+    r[i] = (float)(v - w * rest(a));
+}
+__device__ __forceinline__ void rst_b(double *r, size_t i, double v) { r[i] = v; }
+__device__ __forceinline__ void rst_b(float *r, size_t i, double v) { r[i] = (float)v; }
+
 // fpost[q][a] at node p from its pre-collision value fv and the node's record.
-__device__ __forceinline__ double fpost_v(double fv, const double *rec, size_t N, int q, int a, int p, double om1) {
+template <typename R>
+__device__ __forceinline__ double fpost_v(double fv, const R *rec, size_t N, int q, int a, int p, double om1,
+                                          double w) {
     const int axis = q & 1;
     const double s = q < 2 ? 1.0 : -1.0;
     double b = 0.0;
     if (a < 2) {
-        b = rec[(size_t)(R_BV + 2 * a + axis) * N + p];
+        b = rld_b(rec, (size_t)(R_BV + 2 * a + axis) * N + p);
     } else if (((a - 2) & 1) == axis) {
-        b = rec[(size_t)(R_BF + (a - 2) / 2) * N + p];
+        b = rld_b(rec, (size_t)(R_BF + (a - 2) / 2) * N + p);
     }
-    return om1 * fv + rec[(size_t)(R_A + a) * N + p] + s * b;
+    return om1 * fv + rld(rec, (size_t)(R_A + a) * N + p, a, w) + s * b;
 }
 
 __global__ void k2_reset_invalid(int32_t *invalid) { invalid[0] = 0; }
@@ -271,9 +288,10 @@ __device__ __forceinline__ double du2(const double *uc, const int32_t *act, int 
 // cbw2d:1820-1843 into the per-node record, plus the last displacement
 // commit.  ug is the displacement the repair differentiates (the sweep's
 // u_next when a sweep ran, else u).
+template <typename R>
 __global__ __launch_bounds__(BLOCK) void k2_record(double *__restrict__ U, const double *__restrict__ ug,
                                                    double *__restrict__ u, double *__restrict__ us,
-                                                   double *__restrict__ rec, const int32_t *__restrict__ list,
+                                                   R *__restrict__ rec, const int32_t *__restrict__ list,
                                                    const int32_t *__restrict__ act, int repair, int commit,
                                                    Params2 P) {
     const int t = xcd_block() * BLOCK + threadIdx.x;
@@ -328,15 +346,16 @@ __global__ __launch_bounds__(BLOCK) void k2_record(double *__restrict__ U, const
     const double w = P.omega;
     const double st = P.dt * (2.0 - w) * 0.125;
     const double w4 = 0.25 * w, w2c = w / (2.0 * P.c), h2c = 0.5 / P.c;
-    rec[(R_A + 0) * N + p] = w4 * v0 + st * source2(0, P.src_mode, P.src, v0, v1);
-    rec[(R_A + 1) * N + p] = w4 * v1 + st * source2(1, P.src_mode, P.src, v0, v1);
+    rst_a(rec, (R_A + 0) * N + p, 0, w, w4 * v0 + st * source2(0, P.src_mode, P.src, v0, v1));
+    rst_a(rec, (R_A + 1) * N + p, 1, w, w4 * v1 + st * source2(1, P.src_mode, P.src, v0, v1));
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-        rec[(R_A + 2 + k) * N + p] = w4 * Fo[k] + st * source2(2 + k, P.src_mode, P.src, v0, v1) + 0.25 * dF[k];
+        rst_a(rec, (R_A + 2 + k) * N + p, 2 + k, w,
+              w4 * Fo[k] + st * source2(2 + k, P.src_mode, P.src, v0, v1) + 0.25 * dF[k]);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) rec[(R_BV + k) * N + p] = -w2c * Po[k] - h2c * dP[k];
-    rec[(R_BF + 0) * N + p] = -w2c * v0;
-    rec[(R_BF + 1) * N + p] = -w2c * v1;
+    for (int k = 0; k < 4; ++k) rst_b(rec, (R_BV + k) * N + p, -w2c * Po[k] - h2c * dP[k]);
+    rst_b(rec, (R_BF + 0) * N + p, -w2c * v0);
+    rst_b(rec, (R_BF + 1) * N + p, -w2c * v1);
 }
 
 // Quadratic one-sided reconstruction of U[comp] along the inward link
@@ -409,7 +428,7 @@ __global__ __launch_bounds__(256) void k2_gather(const double *__restrict__ U, c
 // records) or populations read as they are (standalone reconstruction).
 template <typename T>
 __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T *__restrict__ fdst,
-                                                   const double *__restrict__ rec, const double *__restrict__ U,
+                                                   const T *__restrict__ rec, const double *__restrict__ U,
                                                    const double *__restrict__ cs, Links2 Tb, int use_rec,
                                                    double time, Params2 P) {
     const int ell = blockIdx.x * LBLOCK + threadIdx.x;
@@ -530,8 +549,8 @@ __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T
             }
             double fo, fi;
             if (use_rec) {
-                fo = fpost_v(ld(fsrc, (size_t)(qo * 6 + a) * N + p, a), rec, N, qo, a, p, om1);
-                fi = fpost_v(ld(fsrc, (size_t)(qi * 6 + a) * N + p, a), rec, N, qi, a, p, om1);
+                fo = fpost_v(ld(fsrc, (size_t)(qo * 6 + a) * N + p, a), rec, N, qo, a, p, om1, P.omega);
+                fi = fpost_v(ld(fsrc, (size_t)(qi * 6 + a) * N + p, a), rec, N, qi, a, p, om1, P.omega);
             } else {
                 fo = ld(fsrc, (size_t)(qo * 6 + a) * N + p, a);
                 fi = ld(fsrc, (size_t)(qi * 6 + a) * N + p, a);
@@ -542,7 +561,7 @@ __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T
             } else {
                 double upv = fo;
                 if (up_ok)
-                    upv = use_rec ? fpost_v(ld(fsrc, (size_t)(qo * 6 + a) * N + pu, a), rec, N, qo, a, pu, om1)
+                    upv = use_rec ? fpost_v(ld(fsrc, (size_t)(qo * 6 + a) * N + pu, a), rec, N, qo, a, pu, om1, P.omega)
                                   : ld(fsrc, (size_t)(qo * 6 + a) * N + pu, a);
                 val = D * (2.0 * eta * fo + (1.0 - 2.0 * eta) * upv) + S;
             }
@@ -557,7 +576,7 @@ __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T
 // next step's collide (cbw2d:1731-1790).
 template <typename T, bool NEXT>
 __global__ __launch_bounds__(BLOCK) void k2_stream(const T *__restrict__ f0, T *__restrict__ f1,
-                                                   const double *__restrict__ rec, const int32_t *__restrict__ list,
+                                                   const T *__restrict__ rec, const int32_t *__restrict__ list,
                                                    const int32_t *__restrict__ act, double *__restrict__ U,
                                                    double *__restrict__ u, double *__restrict__ us,
                                                    int32_t *__restrict__ invalid, Params2 P) {
@@ -582,7 +601,8 @@ __global__ __launch_bounds__(BLOCK) void k2_stream(const T *__restrict__ f0, T *
             for (int a = 0; a < NC; ++a) fv[a] = ld_nt(f0, (size_t)(q * 6 + a) * N + src, a);
 #pragma unroll
             for (int a = 0; a < NC; ++a) {
-                const double x = st_nt(f1, (size_t)(q * 6 + a) * N + p, a, fpost_v(fv[a], rec, N, q, a, src, om1));
+                const double x =
+                    st_nt(f1, (size_t)(q * 6 + a) * N + p, a, fpost_v(fv[a], rec, N, q, a, src, om1, P.omega));
                 if (NEXT) Um[a] += x;
             }
         } else if (NEXT) {
@@ -664,7 +684,7 @@ __global__ __launch_bounds__(BLOCK) void k2_stress(const double *__restrict__ U,
 
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k2_post(const T *__restrict__ f, T *__restrict__ fp,
-                                                 const double *__restrict__ rec, const int32_t *__restrict__ list,
+                                                 const T *__restrict__ rec, const int32_t *__restrict__ list,
                                                  Params2 P) {
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= P.na) return;
@@ -675,7 +695,7 @@ __global__ __launch_bounds__(BLOCK) void k2_post(const T *__restrict__ f, T *__r
 #pragma unroll
         for (int a = 0; a < NC; ++a) {
             const size_t k = (size_t)(q * 6 + a) * P.N + p;
-            st(fp, k, a, fpost_v(ld(f, k, a), rec, P.N, q, a, p, om1));
+            st(fp, k, a, fpost_v(ld(f, k, a), rec, P.N, q, a, p, om1, P.omega));
         }
 }
 
@@ -844,7 +864,7 @@ static int run_links2(vlbm2d_handle *h, const T *src, T *dst, double t, int use_
     }
     TimedSlot ts(h->timing, h->stream, VLBM_SLOT_LINKS);
     const int gl = (d.n_links + LBLOCK - 1) / LBLOCK;
-    hipLaunchKernelGGL((k2_links<T>), dim3(gl), dim3(LBLOCK), 0, h->stream, src, dst, (const double *)d.record,
+    hipLaunchKernelGGL((k2_links<T>), dim3(gl), dim3(LBLOCK), 0, h->stream, src, dst, (const T *)d.record,
                        (const double *)d.U, (const double *)d.link_state, h->T, use_rec, t, h->P);
     CHECK_LAUNCH("k2_links");
     return 0;
@@ -889,8 +909,8 @@ static int run_steps2(vlbm2d_handle *h, T *f0, T *f1, double time, int64_t steps
         }
         {
             TimedSlot ts(h->timing, h->stream, VLBM_SLOT_RECORD);
-            hipLaunchKernelGGL(k2_record, dim3(gn), dim3(BLOCK), 0, h->stream, d.U,
-                               disp ? (const double *)d.u_next : (const double *)d.u, d.u, d.u_star, d.record,
+            hipLaunchKernelGGL(k2_record<T>, dim3(gn), dim3(BLOCK), 0, h->stream, d.U,
+                               disp ? (const double *)d.u_next : (const double *)d.u, d.u, d.u_star, (T *)d.record,
                                d.active_list, d.active, (int)compat, (int)disp, P);
             CHECK_LAUNCH("k2_record");
         }
@@ -902,11 +922,11 @@ static int run_steps2(vlbm2d_handle *h, T *f0, T *f1, double time, int64_t steps
             TimedSlot ts(h->timing, h->stream, VLBM_SLOT_STREAM);
             if (next)
                 hipLaunchKernelGGL((k2_stream<T, true>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, f1,
-                                   (const double *)d.record, d.active_list, d.active, d.U, d.u, d.u_star, d.invalid,
+                                   (const T *)d.record, d.active_list, d.active, d.U, d.u, d.u_star, d.invalid,
                                    P);
             else
                 hipLaunchKernelGGL((k2_stream<T, false>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, f1,
-                                   (const double *)d.record, d.active_list, d.active, d.U, d.u, d.u_star, d.invalid,
+                                   (const T *)d.record, d.active_list, d.active, d.U, d.u, d.u_star, d.invalid,
                                    P);
             CHECK_LAUNCH("k2_stream");
         }
@@ -1047,7 +1067,7 @@ int vlbm2d_post_collision(vlbm2d_handle *h, const void *f_prev, void *fpost) {
                            (double *)fpost, (const double *)h->d.record, h->d.active_list, h->P);
     else
         hipLaunchKernelGGL(k2_post<float>, dim3(g), dim3(BLOCK), 0, h->stream, (const float *)f_prev,
-                           (float *)fpost, (const double *)h->d.record, h->d.active_list, h->P);
+                           (float *)fpost, (const float *)h->d.record, h->d.active_list, h->P);
     CHECK_LAUNCH("k2_post");
     return 0;
 }

@@ -89,34 +89,43 @@ __device__ __forceinline__ double st_nt(float *f, size_t i, int a, double v) {
     return (double)x + rest(a);
 }
 
+// Records are stored in the precision of the populations.  double records
+// hold the values themselves; float records hold A - omega f_rest on the A
+// planes (rest state subtracted, as for float populations), so that their
+// rounding error scales with the deviation from rest.
+__device__ __forceinline__ double rld(const double *r, size_t i, int, double) { return r[i]; }
+__device__ __forceinline__ double rld(const float *r, size_t i, int a, double w) {
+    return (double)r[i] + w * rest(a);
+}
+__device__ __forceinline__ double rld_b(const double *r, size_t i) { return r[i]; }
+__device__ __forceinline__ double rld_b(const float *r, size_t i) { return (double)r[i]; }
+__device__ __forceinline__ void rst_a(double *r, size_t i, int, double, double v) { r[i] = v; }
+__device__ __forceinline__ void rst_a(float *r, size_t i, int a, double w, double v) {
+    r[i] = (float)(v - w * rest(a));
+}
+__device__ __forceinline__ void rst_b(double *r, size_t i, double v) { r[i] = v; }
+__device__ __forceinline__ void rst_b(float *r, size_t i, double v) { r[i] = (float)v; }
+
 // fpost[q][a] at `node` from its pre-collision value fv and the node's record.
-__device__ __forceinline__ double fpost_v(double fv, const double *rec, size_t n, int q, int a, int node,
-                                          double om1) {
+template <typename R>
+__device__ __forceinline__ double fpost_v(double fv, const R *rec, size_t n, int q, int a, int node, double om1,
+                                          double w) {
     const int axis = q % 3;
     const double s = q < 3 ? 1.0 : -1.0;
     double b = 0.0;
     if (a < 3) {
-        b = rec[(size_t)(R_BV + 3 * a + axis) * n + node];
+        b = rld_b(rec, (size_t)(R_BV + 3 * a + axis) * n + node);
     } else if ((a - 3) % 3 == axis) {
-        b = rec[(size_t)(R_BF + (a - 3) / 3) * n + node];
+        b = rld_b(rec, (size_t)(R_BF + (a - 3) / 3) * n + node);
     }
-    return om1 * fv + rec[(size_t)a * n + node] + s * b;
+    return om1 * fv + rld(rec, (size_t)a * n + node, a, w) + s * b;
 }
 
 // fpost[q][a] at node m from pre-collision f and the node's record.
 template <typename T>
-__device__ __forceinline__ double fpost(const T *f, const double *rec, size_t n, int q, int a, int node,
-                                        double om1) {
-    const double fv = ld(f, (size_t)(q * 12 + a) * n + node, a);
-    const int axis = q % 3;
-    const double s = q < 3 ? 1.0 : -1.0;
-    double b = 0.0;
-    if (a < 3) {
-        b = rec[(size_t)(R_BV + 3 * a + axis) * n + node];
-    } else if ((a - 3) % 3 == axis) {
-        b = rec[(size_t)(R_BF + (a - 3) / 3) * n + node];
-    }
-    return om1 * fv + rec[(size_t)a * n + node] + s * b;
+__device__ __forceinline__ double fpost(const T *f, const T *rec, size_t n, int q, int a, int node, double om1,
+                                        double w) {
+    return fpost_v(ld(f, (size_t)(q * 12 + a) * n + node, a), rec, n, q, a, node, om1, w);
 }
 
 __global__ void k_reset_invalid(int32_t *invalid) { invalid[0] = 0; }
@@ -268,9 +277,10 @@ __device__ __forceinline__ double du_axis(const double *u, const int32_t *nbr, s
 // into the per-node record, plus the last displacement commit.
 // The post-repair state goes to the U planes when write_U is set and, for
 // the characteristic sample nodes, straight into the gather's packed records.
+template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const double *__restrict__ ugrad,
                                                   double *__restrict__ u, double *__restrict__ us,
-                                                  double *__restrict__ rec, const int32_t *__restrict__ nbr,
+                                                  R *__restrict__ rec, const int32_t *__restrict__ nbr,
                                                   const int32_t *__restrict__ pack_slot, double *__restrict__ pack,
                                                   int repair, int commit, int write_U, Params P) {
     const int p = xcd_block() * BLOCK + threadIdx.x;
@@ -339,14 +349,17 @@ __global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const 
     const double st = P.dt * (2.0 - w) * (1.0 / 12.0);
     const double w6 = w / 6.0, w2c = w / (2.0 * P.c), h2c = 0.5 / P.c;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) rec[(R_AV + a) * n + p] = w6 * v[a] + st * source_comp(a, P.src_mode, P.src, v[0], v[1], v[2]);
+    for (int a = 0; a < 3; ++a)
+        rst_a(rec, (R_AV + a) * n + p, R_AV + a, w,
+              w6 * v[a] + st * source_comp(a, P.src_mode, P.src, v[0], v[1], v[2]));
 #pragma unroll
     for (int k = 0; k < 9; ++k)
-        rec[(R_AF + k) * n + p] = w6 * Fo[k] + st * source_comp(3 + k, P.src_mode, P.src, v[0], v[1], v[2]) + dF[k] / 6.0;
+        rst_a(rec, (R_AF + k) * n + p, R_AF + k, w,
+              w6 * Fo[k] + st * source_comp(3 + k, P.src_mode, P.src, v[0], v[1], v[2]) + dF[k] / 6.0);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) rec[(R_BV + k) * n + p] = -w2c * Po[k] - h2c * dP[k];
+    for (int k = 0; k < 9; ++k) rst_b(rec, (R_BV + k) * n + p, -w2c * Po[k] - h2c * dP[k]);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) rec[(R_BF + i) * n + p] = -w2c * v[i];
+    for (int i = 0; i < 3; ++i) rst_b(rec, (R_BF + i) * n + p, -w2c * v[i]);
 }
 
 // Pull stream of the post-collision populations; the entries whose upstream
@@ -356,7 +369,7 @@ __global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const 
 // source, u = u* + dt/2 v, u* = u + dt/2 v, and the J check.
 template <typename T, bool NEXT>
 __global__ __launch_bounds__(BLOCK) void k_stream(const T *__restrict__ f0, T *__restrict__ f1,
-                                                  const double *__restrict__ rec,
+                                                  const T *__restrict__ rec,
                                                   const int32_t *__restrict__ nbr, double *__restrict__ U,
                                                   double *__restrict__ u, double *__restrict__ us,
                                                   int32_t *__restrict__ invalid, Params P) {
@@ -376,7 +389,8 @@ __global__ __launch_bounds__(BLOCK) void k_stream(const T *__restrict__ f0, T *_
             for (int a = 0; a < NC; ++a) fv[a] = ld_nt(f0, (size_t)(q * 12 + a) * n + src, a);
 #pragma unroll
             for (int a = 0; a < NC; ++a) {
-                const double x = st_nt(f1, (size_t)(q * 12 + a) * n + p, a, fpost_v(fv[a], rec, n, q, a, src, om1));
+                const double x =
+                    st_nt(f1, (size_t)(q * 12 + a) * n + p, a, fpost_v(fv[a], rec, n, q, a, src, om1, P.omega));
                 if (NEXT) Um[a] += x;
             }
         } else if (NEXT) {
@@ -550,7 +564,7 @@ __global__ __launch_bounds__(128) void k_prepare(const double *__restrict__ U, L
 // (time step) or the raw populations (standalone reconstruction).
 template <typename T>
 __global__ __launch_bounds__(128) void k_links(const T *__restrict__ fsrc, T *__restrict__ fdst,
-                                               const double *__restrict__ rec, const double *__restrict__ U,
+                                               const T *__restrict__ rec, const double *__restrict__ U,
                                                LinkTables Tb, int32_t *__restrict__ invalid, int use_rec,
                                                int has_char, Params P) {
     const int ell = blockIdx.x * 128 + threadIdx.x;
@@ -592,8 +606,8 @@ __global__ __launch_bounds__(128) void k_links(const T *__restrict__ fsrc, T *__
         }
         double fo, fi;
         if (use_rec) {
-            fo = fpost(fsrc, rec, n, qo, a, p, om1);
-            fi = fpost(fsrc, rec, n, qi, a, p, om1);
+            fo = fpost(fsrc, rec, n, qo, a, p, om1, P.omega);
+            fi = fpost(fsrc, rec, n, qi, a, p, om1, P.omega);
         } else {
             fo = ld(fsrc, (size_t)(qo * 12 + a) * n + p, a);
             fi = ld(fsrc, (size_t)(qi * 12 + a) * n + p, a);
@@ -604,7 +618,8 @@ __global__ __launch_bounds__(128) void k_links(const T *__restrict__ fsrc, T *__
             val = alpha * (D * fo + S) + (1.0 - alpha) * fi;
         } else {
             double upv = fo;
-            if (up >= 0) upv = use_rec ? fpost(fsrc, rec, n, qo, a, up, om1) : ld(fsrc, (size_t)(qo * 12 + a) * n + up, a);
+            if (up >= 0)
+                upv = use_rec ? fpost(fsrc, rec, n, qo, a, up, om1, P.omega) : ld(fsrc, (size_t)(qo * 12 + a) * n + up, a);
             val = D * (2.0 * eta * fo + (1.0 - 2.0 * eta) * upv) + S;
         }
         outv[a] = val;
@@ -851,7 +866,7 @@ static int run_boundary(vlbm3d_handle *h, const T *src, T *dst, double t, int us
         CHECK_LAUNCH("k_prepare");
     }
     TimedSlot ts(h->timing, h->stream, VLBM_SLOT_LINKS);
-    hipLaunchKernelGGL((k_links<T>), dim3(gl), dim3(128), 0, h->stream, src, dst, (const double *)d.record,
+    hipLaunchKernelGGL((k_links<T>), dim3(gl), dim3(128), 0, h->stream, src, dst, (const T *)d.record,
                        (const double *)d.U, h->T, d.invalid, use_rec, d.has_char, h->P);
     CHECK_LAUNCH("k_links");
     return 0;
@@ -906,9 +921,9 @@ static int run_steps(vlbm3d_handle *h, T *f0, T *f1, double time, int64_t steps,
             const bool last = k + 1 == nsteps;
             const bool pack = d.has_char && d.n_char_links > 0;
             const int write_U = (last || d.recon != R_LOCALF || !pack) ? 1 : 0;
-            hipLaunchKernelGGL(k_record, dim3(gn), dim3(BLOCK), 0, h->stream, d.U,
+            hipLaunchKernelGGL(k_record<T>, dim3(gn), dim3(BLOCK), 0, h->stream, d.U,
                                disp ? (const double *)d.u_next : (const double *)d.u, d.u, d.u_star,
-                               (double *)d.record, d.neighbor, pack ? d.char_pack_slot : (const int32_t *)nullptr,
+                               (T *)d.record, d.neighbor, pack ? d.char_pack_slot : (const int32_t *)nullptr,
                                d.char_pack, (int)compat, (int)disp, write_U, P);
             CHECK_LAUNCH("k_record");
         }
@@ -920,10 +935,10 @@ static int run_steps(vlbm3d_handle *h, T *f0, T *f1, double time, int64_t steps,
             TimedSlot ts(h->timing, h->stream, VLBM_SLOT_STREAM);
             if (next)
                 hipLaunchKernelGGL((k_stream<T, true>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, f1,
-                                   (const double *)d.record, d.neighbor, d.U, d.u, d.u_star, d.invalid, P);
+                                   (const T *)d.record, d.neighbor, d.U, d.u, d.u_star, d.invalid, P);
             else
                 hipLaunchKernelGGL((k_stream<T, false>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, f1,
-                                   (const double *)d.record, d.neighbor, d.U, d.u, d.u_star, d.invalid, P);
+                                   (const T *)d.record, d.neighbor, d.U, d.u, d.u_star, d.invalid, P);
             CHECK_LAUNCH("k_stream");
         }
         T *tmp = f0;

@@ -929,7 +929,7 @@ class WarpSparseCurvedBoundaryLBM3D:
         self.u = DeviceArray.zeros((3, n), f64, dev)
         self.u_star = DeviceArray.zeros((3, n), f64, dev)
         self.u_local_next = DeviceArray.zeros((3, n), f64, dev)
-        self.record = DeviceArray.zeros((24, n), f64, dev)
+        self.record = DeviceArray.zeros((24, n), pop_dtype, dev)
         self.invalid = DeviceArray.zeros((1,), i32, dev)
         self._reduce_scratch = DeviceArray.zeros((_native.RED_SCRATCH,), f64, dev)
         self.source_params_wp = dev_f(self.source_params)
@@ -1058,7 +1058,7 @@ class WarpSparseCurvedBoundaryLBM3D:
         L = self.geometry.n_links
         return {
             "populations": int(2 * FQ * self.n_active * pop_bytes),
-            "fields": int((NCOMP + 3 + 3 + 3 + 24) * self.n_active * 8),
+            "fields": int((NCOMP + 3 + 3 + 3) * self.n_active * 8 + 24 * self.n_active * pop_bytes),
             "connectivity": int((Q * self.n_active + self.n_active + 1 + 3 * L) * 4),
             "link_data": int((1 + 3 + 9 + 9 + 3) * L * 8 + L * 4),
         }

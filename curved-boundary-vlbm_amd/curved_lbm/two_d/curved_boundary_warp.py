@@ -946,7 +946,7 @@ class WarpCurvedBoundaryLBM2D:
         self.U = DeviceArray.zeros((NCOMP, nx, ny, 1), f64, dev)
         self._P = DeviceArray.zeros((4, nx, ny, 1), f64, dev)
         self._sigma = DeviceArray.zeros((4, nx, ny, 1), f64, dev)
-        self.record = DeviceArray.zeros((12, nx, ny, 1), f64, dev)
+        self.record = DeviceArray.zeros((12, nx, ny, 1), pop_dtype, dev)
         self.link_state = DeviceArray.zeros((16, max(L, 1)), f64, dev)
         self.invalid = DeviceArray.zeros((1,), i32, dev)
         self._reduce_scratch = DeviceArray.zeros((_native.RED_SCRATCH,), f64, dev)

@@ -73,7 +73,8 @@ typedef struct {
     int32_t n_anchor_links; /* length of anchor_ids */
     /* state (device) */
     double *U, *u, *u_star, *u_next; /* (6|2, nx, ny) */
-    double *record;                  /* (12, nx, ny) deferred-collision record */
+    void *record;                    /* (12, nx, ny) deferred-collision record, in the population
+                                        precision (float: A - omega f_rest on the A planes) */
     int32_t *invalid;                /* (1) */
     double *reduce_scratch;          /* (>= 8 * 1025) */
     /* tables (device) */

@@ -70,7 +70,8 @@ typedef struct {
     int32_t n_char_nodes;   /* number of distinct stencil sample nodes */
     /* state (device) */
     double *U, *u, *u_star, *u_next; /* (12,n), (3,n), (3,n), (3,n) */
-    void *record;                    /* (24,n) deferred-collision record, double */
+    void *record;                    /* (24,n) deferred-collision record, in the population precision
+                                        (float records hold A - omega f_rest on the A planes) */
     int32_t *invalid;                /* (1) */
     double *reduce_scratch;          /* (>= 8 * 1024) */
     /* tables (device) */
