@@ -40,6 +40,15 @@ class Superellipse:
         xi, eta = self.local(x, y)
         return (np.abs(xi) / self.a) ** self.p + (np.abs(eta) / self.b) ** self.p - 1.0
 
+    def grad_scalar(self, x: float, y: float) -> tuple[float, float]:
+        """Gradient of phi at one point in Python float arithmetic."""
+        xi, eta = self.local(float(x), float(y))
+        xi_f, eta_f = float(xi), float(eta)
+        ct, st = math.cos(self.theta), math.sin(self.theta)
+        gx_l = self.p * math.copysign(abs(xi_f) ** (self.p - 1.0), xi_f) / (self.a ** self.p)
+        gy_l = self.p * math.copysign(abs(eta_f) ** (self.p - 1.0), eta_f) / (self.b ** self.p)
+        return ct * gx_l - st * gy_l, st * gx_l + ct * gy_l
+
     def grad(self, x, y):
         """Gradient of phi (array form)."""
         xi, eta = self.local(x, y)
@@ -50,10 +59,13 @@ class Superellipse:
 
 
 def superellipse_shell_geometry(n: int, outer: Superellipse, inner: Superellipse,
-                                label: str | None = None) -> CurvedBoundaryGeometry:
+                                label: str | None = None, vectorized: bool = True) -> CurvedBoundaryGeometry:
     """Material inside ``outer`` and outside ``inner`` on an n x n unit box;
     boundary id 2 (inner) where |phi_inner| <= |phi_outer|, else 1 (outer);
-    outward normals from the level-set gradients."""
+    outward normals from the level-set gradients.  ``vectorized=False``
+    evaluates every point with the per-point callables (the reference's
+    path, bit for bit); the array path can differ in the last bit of a
+    normal (numpy's vector pow)."""
 
     def phi_v(x, y):
         return np.maximum(outer.phi(x, y), -inner.phi(x, y))
@@ -68,11 +80,21 @@ def superellipse_shell_geometry(n: int, outer: Superellipse, inner: Superellipse
     def bid(x, y, _nx, _ny):
         return BOUNDARY_ID_INNER if abs(float(inner.phi(x, y))) <= abs(float(outer.phi(x, y))) else BOUNDARY_ID_OUTER
 
+    def normal_s(x, y):
+        if abs(float(inner.phi(x, y))) <= abs(float(outer.phi(x, y))):
+            gx, gy = inner.grad_scalar(x, y)
+            return -gx, -gy
+        return outer.grad_scalar(x, y)
+
+    label = label or (f"superellipse_shell(outer_a={outer.a},outer_b={outer.b},outer_theta={outer.theta_deg},"
+                      f"inner_a={inner.a},inner_b={inner.b},inner_theta={inner.theta_deg},p={outer.p})")
+    if not vectorized:
+        return CurvedBoundaryGeometry.from_level_set(
+            nx=n, ny=n, length_x=1.0, length_y=1.0, phi=lambda x, y: float(phi_v(x, y)), normal=normal_s,
+            boundary_id=bid, label=label)
     return CurvedBoundaryGeometry.from_level_set(
         nx=n, ny=n, length_x=1.0, length_y=1.0, phi=lambda x, y: float(phi_v(x, y)), boundary_id=bid,
-        phi_vectorized=phi_v, normal_vectorized=normal_v,
-        label=label or (f"superellipse_shell(outer_a={outer.a},outer_b={outer.b},outer_theta={outer.theta_deg},"
-                        f"inner_a={inner.a},inner_b={inner.b},inner_theta={inner.theta_deg},p={outer.p})"))
+        phi_vectorized=phi_v, normal_vectorized=normal_v, label=label)
 
 
 def rotated_superellipse_shell(n: int, *, power: float = 4.0, a: float = 0.42, b: float = 0.30,
