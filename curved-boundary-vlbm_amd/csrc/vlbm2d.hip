@@ -56,7 +56,14 @@ struct Params2 {
     double blend;
     int interior_only;
     int use_dyn;
+    const double *clock;  // device time of the step start (captured step loops), or nullptr
 };
+
+// Boundary time of a step: the host value, or clock + dt/2 when the launch
+// belongs to a captured step loop that keeps the time on the device.
+__device__ __forceinline__ double step_time(double tb, const Params2 &P) {
+    return P.clock ? P.clock[0] + 0.5 * P.dt : tb;
+}
 
 struct Links2 {
     const int32_t *lp, *lq, *lbid;
@@ -123,6 +130,9 @@ __device__ __forceinline__ double fpost_v(double fv, const R *rec, size_t N, int
 }
 
 __global__ void k2_reset_invalid(int32_t *invalid) { invalid[0] = 0; }
+__global__ void k2_set_clock(double *clock, double t) { clock[0] = t; }
+// time += dt, the same FP64 operation the host performs after each step.
+__global__ void k2_advance_clock(double *clock, double dt) { clock[0] = clock[0] + dt; }
 
 // Moments with the half-step source; u = u* + dt/2 v and (collide only)
 // u* = u + dt/2 v; J flag.  cbw2d:1692-1790 and :3128-3195.
@@ -180,6 +190,7 @@ __global__ __launch_bounds__(BLOCK) void k2_sweep(const double *__restrict__ U, 
     const double *u0 = u, *u1 = u + N;
     double diag = 0.0, rx = 0.0, ry = 0.0;
     const int l0 = aoff[t], l1 = aoff[t + 1];
+    if (l1 > l0) time = step_time(time, P);
     for (int l = l0; l < l1; ++l) {
         const int ell = aids[l];
         const int bid = T.lbid[ell];
@@ -293,9 +304,12 @@ __global__ __launch_bounds__(BLOCK) void k2_record(double *__restrict__ U, const
                                                    double *__restrict__ u, double *__restrict__ us,
                                                    R *__restrict__ rec, const int32_t *__restrict__ list,
                                                    const int32_t *__restrict__ act, int repair, int commit,
-                                                   Params2 P) {
+                                                   int32_t *__restrict__ reset_invalid, Params2 P) {
     const int t = xcd_block() * BLOCK + threadIdx.x;
     if (t >= P.na) return;
+    // Clears the invalid flag ahead of the stream kernel's fused collide
+    // (the link rows never set it).
+    if (reset_invalid && t == 0) reset_invalid[0] = 0;
     const int p = list[t];
     const int ny = P.ny, nx = P.nx;
     const int i = p / ny, j = p - i * ny;
@@ -444,6 +458,7 @@ __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T
     const double *prm = Tb.prm_by_id + bid * 8;
     const double eta = Tb.eta[ell], xb = Tb.xb[ell], yb = Tb.yb[ell], nxb = Tb.nx[ell], nyb = Tb.ny[ell];
     const double dxi = (double)cx(qi), dyi = (double)cy(qi);
+    time = step_time(time, P);
     double bv0 = value2(0, mode, prm, xb, yb, nxb, nyb, time);
     double bv1 = value2(1, mode, prm, xb, yb, nxb, nyb, time);
     if (kind == BC_NEU && P.use_dyn) {
@@ -791,13 +806,31 @@ __global__ __launch_bounds__(BLOCK) void k2_reduce_final(double *__restrict__ pa
 
 }  // namespace
 
+// A captured loop of `steps` fused steps (an even count, so the population
+// buffers end where they started) for one (f0, f1) pair.
+struct StepGraph {
+    const void *f0 = nullptr, *f1 = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
 struct vlbm2d_handle {
     vlbm2d_desc d;
     Params2 P;
     Links2 T;
     hipStream_t stream;
     Timing timing;
+    int graph_steps = 0;
+    double *clock = nullptr;
+    hipStream_t cap = nullptr;
+    StepGraph graphs[2];
 };
+
+static void drop_graphs(vlbm2d_handle *h) {
+    for (auto &g : h->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g = StepGraph{};
+    }
+}
 
 static Params2 make_params2(const vlbm2d_desc &d) {
     Params2 P{};
@@ -870,66 +903,174 @@ static int run_links2(vlbm2d_handle *h, const T *src, T *dst, double t, int use_
     return 0;
 }
 
+// Gather and cut-link kernels of one step on stream `st`.
+template <typename T>
+static int launch_links2(vlbm2d_handle *h, const Params2 &P, hipStream_t st, const T *f0, T *f1, double tb) {
+    const vlbm2d_desc &d = h->d;
+    if (d.n_links == 0) return 0;
+    const bool gather = d.recon == R_LOCALF || d.recon == R_LSQ_BFL;
+    if (gather) {
+        TimedSlot tg(h->timing, st, VLBM_SLOT_GATHER);
+        const int gg = (d.n_links + GL - 1) / GL;
+        hipLaunchKernelGGL(k2_gather, dim3(gg), dim3(256), 0, st, (const double *)d.U, d.lsq_p, d.lsq_wb,
+                           d.lsq_wx, d.lsq_wy, d.link_state, d.n_links, P.N);
+        CHECK_LAUNCH("k2_gather");
+    }
+    TimedSlot tl(h->timing, st, VLBM_SLOT_LINKS);
+    const int gl = (d.n_links + LBLOCK - 1) / LBLOCK;
+    hipLaunchKernelGGL((k2_links<T>), dim3(gl), dim3(LBLOCK), 0, st, f0, f1, (const T *)d.record,
+                       (const double *)d.U, (const double *)d.link_state, h->T, 1, tb, P);
+    CHECK_LAUNCH("k2_links");
+    return 0;
+}
+
+template <typename T>
+static int launch_stream2(vlbm2d_handle *h, const Params2 &P, bool next, const T *f0, T *f1) {
+    const vlbm2d_desc &d = h->d;
+    const int gn = xcd_grid(d.n_active);
+    hipStream_t st = h->stream;
+    if (next)
+        hipLaunchKernelGGL((k2_stream<T, true>), dim3(gn), dim3(BLOCK), 0, st, f0, f1, (const T *)d.record,
+                           d.active_list, d.active, d.U, d.u, d.u_star, d.invalid, P);
+    else
+        hipLaunchKernelGGL((k2_stream<T, false>), dim3(gn), dim3(BLOCK), 0, st, f0, f1, (const T *)d.record,
+                           d.active_list, d.active, d.U, d.u, d.u_star, d.invalid, P);
+    CHECK_LAUNCH("k2_stream");
+    return 0;
+}
+
+// Launches of one step from (time, steps): sweeps, record, links and the
+// stream (with the next step's collide when `next`).  P carries the clock
+// pointer inside captured step loops.
+template <typename T>
+static int launch_step2(vlbm2d_handle *h, const Params2 &P, T *f0, T *f1, double time, int64_t steps, bool next) {
+    const vlbm2d_desc &d = h->d;
+    const int gn = xcd_grid(d.n_active);
+    TimedSlot whole(h->timing, h->stream, VLBM_SLOT_STEP);
+    const double tb = time + 0.5 * d.dt;
+    const bool disp = d.disp_interval > 0 && steps % d.disp_interval == 0;
+    const bool compat = d.compat_interval > 0 && steps % d.compat_interval == 0;
+    hipStream_t st = h->stream;
+    if (disp) {
+        for (int s = 0; s < d.disp_sweeps; ++s) {
+            TimedSlot ts(h->timing, st, VLBM_SLOT_SWEEP);
+            hipLaunchKernelGGL(k2_sweep, dim3(gn), dim3(BLOCK), 0, st, (const double *)d.U, (const double *)d.u,
+                               d.u_next, d.active_list, d.active, d.anchor_off, d.anchor_ids, h->T, tb, P);
+            CHECK_LAUNCH("k2_sweep");
+            if (s + 1 < d.disp_sweeps) {
+                TimedSlot tc(h->timing, st, VLBM_SLOT_COMMIT);
+                hipLaunchKernelGGL(k2_commit, dim3(gn), dim3(BLOCK), 0, st, (const double *)d.u_next, d.u, d.u_star,
+                                   d.active_list, d.n_active, P.N);
+                CHECK_LAUNCH("k2_commit");
+            }
+        }
+    }
+    {
+        TimedSlot ts(h->timing, st, VLBM_SLOT_RECORD);
+        hipLaunchKernelGGL(k2_record<T>, dim3(gn), dim3(BLOCK), 0, st, d.U,
+                           disp ? (const double *)d.u_next : (const double *)d.u, d.u, d.u_star, (T *)d.record,
+                           d.active_list, d.active, (int)compat, (int)disp, next ? d.invalid : (int32_t *)nullptr,
+                           P);
+        CHECK_LAUNCH("k2_record");
+    }
+    int rc = launch_links2<T>(h, P, st, f0, f1, tb);
+    if (rc) return rc;
+    TimedSlot ts(h->timing, st, VLBM_SLOT_STREAM);
+    return launch_stream2<T>(h, P, next, f0, f1);
+}
+
+// Captures `graph_steps` fused steps with the time kept on the device (the
+// schedule must be the same every step: intervals 0 or 1).
+template <typename T>
+static int capture_steps2(vlbm2d_handle *h, T *f0, T *f1, hipGraphExec_t *out) {
+    hipError_t e;
+    if (!h->cap) {
+        e = hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking);
+        if (e != hipSuccess) return fail("vlbm2d: capture stream", e);
+    }
+    if (!h->clock) {
+        e = hipMalloc((void **)&h->clock, sizeof(double));
+        if (e != hipSuccess) return fail("vlbm2d: clock", e);
+    }
+    Params2 P = h->P;
+    P.clock = h->clock;
+    hipStream_t keep = h->stream;
+    const bool timing = h->timing.on;
+    h->timing.on = false;
+    h->stream = h->cap;
+    e = hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        h->stream = keep;
+        h->timing.on = timing;
+        return fail("vlbm2d: begin capture", e);
+    }
+
+    int rc = 0;
+    for (int k = 0; k < h->graph_steps && rc == 0; ++k) {
+        rc = (k % 2 == 0) ? launch_step2<T>(h, P, f0, f1, 0.0, 0, true) : launch_step2<T>(h, P, f1, f0, 0.0, 0, true);
+        if (rc == 0) hipLaunchKernelGGL(k2_advance_clock, dim3(1), dim3(1), 0, h->cap, h->clock, h->d.dt);
+    }
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(h->cap, &graph);
+    h->stream = keep;
+    h->timing.on = timing;
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail("vlbm2d: end capture", e);
+    e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail("vlbm2d: instantiate", e);
+    return 0;
+}
+
 // One library call runs nsteps reference steps; the collide of every step
 // after the first is fused into the previous step's stream kernel, the cut
 // links run before the stream kernel, and the last step streams without the
-// fused collide so that U, u and u* end as the reference leaves them.
+// fused collide so that U, u and u* end as the reference leaves them.  With
+// graph_steps > 0, blocks of fused steps are replayed from a captured graph.
 template <typename T>
 static int run_steps2(vlbm2d_handle *h, T *f0, T *f1, double time, int64_t steps, int nsteps) {
     const vlbm2d_desc &d = h->d;
-    const Params2 &P = h->P;
     const int gn = xcd_grid(d.n_active);
     if (nsteps <= 0) return 0;
     hipLaunchKernelGGL(k2_reset_invalid, dim3(1), dim3(1), 0, h->stream, d.invalid);
     {
         TimedSlot ts(h->timing, h->stream, VLBM_SLOT_COLLIDE);
         hipLaunchKernelGGL((k2_moments<T, true>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, d.U, d.u,
-                           d.u_star, d.invalid, d.active_list, P);
+                           d.u_star, d.invalid, d.active_list, h->P);
         CHECK_LAUNCH("k2_moments");
     }
-    for (int k = 0; k < nsteps; ++k) {
-        TimedSlot whole(h->timing, h->stream, VLBM_SLOT_STEP);
-        const double tb = time + 0.5 * d.dt;
-        const bool disp = d.disp_interval > 0 && steps % d.disp_interval == 0;
-        const bool compat = d.compat_interval > 0 && steps % d.compat_interval == 0;
-        if (disp) {
-            for (int s = 0; s < d.disp_sweeps; ++s) {
-                TimedSlot ts(h->timing, h->stream, VLBM_SLOT_SWEEP);
-                hipLaunchKernelGGL(k2_sweep, dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)d.U,
-                                   (const double *)d.u, d.u_next, d.active_list, d.active, d.anchor_off,
-                                   d.anchor_ids, h->T, tb, P);
-                CHECK_LAUNCH("k2_sweep");
-                if (s + 1 < d.disp_sweeps) {
-                    TimedSlot tc(h->timing, h->stream, VLBM_SLOT_COMMIT);
-                    hipLaunchKernelGGL(k2_commit, dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)d.u_next, d.u,
-                                       d.u_star, d.active_list, d.n_active, P.N);
-                    CHECK_LAUNCH("k2_commit");
-                }
-            }
+    int k = 0;
+    const bool uniform = d.disp_interval <= 1 && d.compat_interval <= 1;
+    const int K = h->graph_steps;
+    if (K > 0 && uniform && !h->timing.on && nsteps - 1 >= K) {
+        StepGraph *g = nullptr;
+        for (auto &c : h->graphs)
+            if (c.exec && c.f0 == f0 && c.f1 == f1) g = &c;
+        if (!g) {
+            g = !h->graphs[0].exec ? &h->graphs[0] : &h->graphs[1];
+            if (g->exec) (void)hipGraphExecDestroy(g->exec);
+            *g = StepGraph{};
+            int rc = capture_steps2<T>(h, f0, f1, &g->exec);
+            if (rc) return rc;
+            g->f0 = f0;
+            g->f1 = f1;
         }
-        {
-            TimedSlot ts(h->timing, h->stream, VLBM_SLOT_RECORD);
-            hipLaunchKernelGGL(k2_record<T>, dim3(gn), dim3(BLOCK), 0, h->stream, d.U,
-                               disp ? (const double *)d.u_next : (const double *)d.u, d.u, d.u_star, (T *)d.record,
-                               d.active_list, d.active, (int)compat, (int)disp, P);
-            CHECK_LAUNCH("k2_record");
+        hipLaunchKernelGGL(k2_set_clock, dim3(1), dim3(1), 0, h->stream, h->clock, time);
+        CHECK_LAUNCH("k2_set_clock");
+        for (; k + K <= nsteps - 1; k += K) {
+            hipError_t e = hipGraphLaunch(g->exec, h->stream);
+            if (e != hipSuccess) return fail("vlbm2d: graph launch", e);
+            for (int j = 0; j < K; ++j) time += d.dt;
+            steps += K;
         }
-        int rc = run_links2<T>(h, f0, f1, tb, 1);
-        if (rc) return rc;
+    }
+    for (; k < nsteps; ++k) {
         const bool next = k + 1 < nsteps;
-        if (next) hipLaunchKernelGGL(k2_reset_invalid, dim3(1), dim3(1), 0, h->stream, d.invalid);
-        {
-            TimedSlot ts(h->timing, h->stream, VLBM_SLOT_STREAM);
-            if (next)
-                hipLaunchKernelGGL((k2_stream<T, true>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, f1,
-                                   (const T *)d.record, d.active_list, d.active, d.U, d.u, d.u_star, d.invalid,
-                                   P);
-            else
-                hipLaunchKernelGGL((k2_stream<T, false>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, f1,
-                                   (const T *)d.record, d.active_list, d.active, d.U, d.u, d.u_star, d.invalid,
-                                   P);
-            CHECK_LAUNCH("k2_stream");
-        }
+        int rc = launch_step2<T>(h, h->P, f0, f1, time, steps, next);
+        if (rc) return rc;
         T *tmp = f0;
         f0 = f1;
         f1 = tmp;
@@ -983,8 +1124,21 @@ int vlbm2d_create(const vlbm2d_desc *desc, vlbm2d_handle **out) {
 
 int vlbm2d_destroy(vlbm2d_handle *h) {
     if (!h) return 0;
+    drop_graphs(h);
+    if (h->clock) (void)hipFree(h->clock);
+    if (h->cap) (void)hipStreamDestroy(h->cap);
     h->timing.destroy();
     delete h;
+    return 0;
+}
+
+int vlbm2d_set_graph_steps(vlbm2d_handle *h, int32_t steps) {
+    if (!h) return fail_msg("vlbm2d_set_graph_steps: null handle");
+    if (steps < 0 || steps % 2) return fail_msg("vlbm2d_set_graph_steps: steps must be a non-negative even count");
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail("vlbm2d_set_graph_steps: sync", e);
+    drop_graphs(h);
+    h->graph_steps = steps;
     return 0;
 }
 
@@ -1074,6 +1228,9 @@ int vlbm2d_post_collision(vlbm2d_handle *h, const void *f_prev, void *fpost) {
 
 int vlbm2d_set_dynamic_traction(vlbm2d_handle *h, int32_t enabled) {
     if (!h) return fail_msg("vlbm2d_set_dynamic_traction: null handle");
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail("vlbm2d_set_dynamic_traction: sync", e);
+    drop_graphs(h);  // the flag is a captured kernel argument
     h->P.use_dyn = enabled != 0;
     h->d.use_dyn = enabled != 0;
     return 0;

@@ -282,9 +282,12 @@ __global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const 
                                                   double *__restrict__ u, double *__restrict__ us,
                                                   R *__restrict__ rec, const int32_t *__restrict__ nbr,
                                                   const int32_t *__restrict__ pack_slot, double *__restrict__ pack,
-                                                  int repair, int commit, int write_U, Params P) {
+                                                  int repair, int commit, int write_U,
+                                                  int32_t *__restrict__ reset_invalid, Params P) {
     const int p = xcd_block() * BLOCK + threadIdx.x;
     if (p >= P.n) return;
+    // Clears the invalid flag ahead of the stream kernel's fused collide.
+    if (reset_invalid && p == 0) reset_invalid[0] = 0;
     const size_t n = P.n;
     double v[3], Fo[9];
 #pragma unroll
@@ -448,37 +451,79 @@ __global__ __launch_bounds__(BLOCK) void k_gather(const double *__restrict__ pac
     double val[6], gx[6], gy[6], gz[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) val[k] = gx[k] = gy[k] = gz[k] = 0.0;
-    for (int m = 0; m < NG; ++m) {
-        const int s = gidx[(size_t)m * Lc + c];
-        if (s < 0) break;
-        const double w1 = gw[((size_t)0 * NG + m) * Lc + c];
-        const double w2 = gw[((size_t)1 * NG + m) * Lc + c];
-        const double w3 = gw[((size_t)2 * NG + m) * Lc + c];
-        const double2 *rp = reinterpret_cast<const double2 *>(pack + (size_t)s * NC + a0);
-        double x[6];
+    // Samples are consumed in stencil order up to the first empty slot; the
+    // indices, weights and records of GU slots are loaded before any is
+    // used, so each lane keeps GU sample loads in flight.
+    constexpr int GU = 8;
+    static_assert(NG % GU == 0, "gradient stencil length must be a multiple of the unroll");
+    bool alive = true;
+    for (int m0 = 0; m0 < NG && alive; m0 += GU) {
+        int sidx[GU];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const double2 v = rp[k];
-            x[2 * k] = v.x;
-            x[2 * k + 1] = v.y;
+        for (int k = 0; k < GU; ++k) sidx[k] = gidx[(size_t)(m0 + k) * Lc + c];
+        bool use[GU];
+#pragma unroll
+        for (int k = 0; k < GU; ++k) {
+            alive = alive && sidx[k] >= 0;
+            use[k] = alive;
+        }
+        double w1[GU], w2[GU], w3[GU], x[GU][6];
+#pragma unroll
+        for (int k = 0; k < GU; ++k) {
+            w1[k] = w2[k] = w3[k] = 0.0;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) x[k][j] = 0.0;
+            if (use[k]) {
+                const int m = m0 + k;
+                w1[k] = gw[((size_t)0 * NG + m) * Lc + c];
+                w2[k] = gw[((size_t)1 * NG + m) * Lc + c];
+                w3[k] = gw[((size_t)2 * NG + m) * Lc + c];
+                const double2 *rp = reinterpret_cast<const double2 *>(pack + (size_t)sidx[k] * NC + a0);
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const double2 v = rp[j];
+                    x[k][2 * j] = v.x;
+                    x[k][2 * j + 1] = v.y;
+                }
+            }
         }
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            gx[k] += w1 * x[k];
-            gy[k] += w2 * x[k];
-            gz[k] += w3 * x[k];
+        for (int k = 0; k < GU; ++k) {
+            if (!use[k]) continue;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                gx[j] += w1[k] * x[k][j];
+                gy[j] += w2[k] * x[k][j];
+                gz[j] += w3[k] * x[k][j];
+            }
         }
     }
-    for (int m = 0; m < 16; ++m) {
-        const int s = pidx[(size_t)m * Lc + c];
-        if (s < 0) continue;
-        const double w0 = pw[(size_t)m * Lc + c];
-        const double2 *rp = reinterpret_cast<const double2 *>(pack + (size_t)s * NC + a0);
+    for (int m0 = 0; m0 < 16; m0 += GU) {
+        int sidx[GU];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const double2 v = rp[k];
-            val[2 * k] += w0 * v.x;
-            val[2 * k + 1] += w0 * v.y;
+        for (int k = 0; k < GU; ++k) sidx[k] = pidx[(size_t)(m0 + k) * Lc + c];
+        double w0[GU], x[GU][6];
+#pragma unroll
+        for (int k = 0; k < GU; ++k) {
+            w0[k] = 0.0;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) x[k][j] = 0.0;
+            if (sidx[k] >= 0) {
+                w0[k] = pw[(size_t)(m0 + k) * Lc + c];
+                const double2 *rp = reinterpret_cast<const double2 *>(pack + (size_t)sidx[k] * NC + a0);
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const double2 v = rp[j];
+                    x[k][2 * j] = v.x;
+                    x[k][2 * j + 1] = v.y;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < GU; ++k) {
+            if (sidx[k] < 0) continue;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) val[j] += w0[k] * x[k][j];
         }
     }
     const int ell = clink[c];
@@ -636,7 +681,7 @@ __global__ __launch_bounds__(128) void k_links(const T *__restrict__ fsrc, T *__
             else
                 Ub[a] = Tb.lFb[(a - 3) * L + ell];
         }
-        if (det_state(Ub) <= P.m.jf) atomicAdd(invalid, 1);
+        if (invalid && det_state(Ub) <= P.m.jf) atomicAdd(invalid, 1);
         char_population(Ub, Gx, Gy, Gz, din, Tb.eta[ell], P.dt, P.c, P.omega, P.src_mode, P.src, P.m, inc);
 #pragma unroll
         for (int a = 0; a < NC; ++a) {
@@ -841,34 +886,51 @@ static LinkTables make_tables(const vlbm3d_desc &d) {
 
 // use_rec: the sources are pre-collision populations plus the per-node
 // records of this step, and k_record has already filled the gather pack.
+// The kernels go to stream `st`; `invalid` (may be null) receives the
+// characteristic closure's J flags.
 template <typename T>
-static int run_boundary(vlbm3d_handle *h, const T *src, T *dst, double t, int use_rec) {
+static int run_boundary(vlbm3d_handle *h, hipStream_t st, const T *src, T *dst, double t, int use_rec,
+                        int32_t *invalid) {
     const vlbm3d_desc &d = h->d;
     if (d.n_links == 0) return 0;
     if (d.has_char && d.n_char_links > 0) {
-        TimedSlot ts(h->timing, h->stream, VLBM_SLOT_GATHER);
+        TimedSlot ts(h->timing, st, VLBM_SLOT_GATHER);
         if (!use_rec) {
             const int gp = (d.n_char_nodes + BLOCK - 1) / BLOCK;
-            hipLaunchKernelGGL(k_pack, dim3(gp), dim3(BLOCK), 0, h->stream, (const double *)d.U, d.char_nodes,
+            hipLaunchKernelGGL(k_pack, dim3(gp), dim3(BLOCK), 0, st, (const double *)d.U, d.char_nodes,
                                d.char_pack, d.n_active, d.n_char_nodes);
             CHECK_LAUNCH("k_pack");
         }
         const int g = xcd_grid((int)((size_t)d.n_char_links * 2));
-        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BLOCK), 0, h->stream, (const double *)d.char_pack, d.char_link,
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BLOCK), 0, st, (const double *)d.char_pack, d.char_link,
                            d.char_gidx, d.char_gw, d.char_pidx, d.char_pw, d.char_state, d.n_links,
                            d.n_char_links);
         CHECK_LAUNCH("k_gather");
     }
     const int gl = (d.n_links + 127) / 128;
     {
-        TimedSlot ts(h->timing, h->stream, VLBM_SLOT_PREPARE);
-        hipLaunchKernelGGL(k_prepare, dim3(gl), dim3(128), 0, h->stream, (const double *)d.U, h->T, t, h->P);
+        TimedSlot ts(h->timing, st, VLBM_SLOT_PREPARE);
+        hipLaunchKernelGGL(k_prepare, dim3(gl), dim3(128), 0, st, (const double *)d.U, h->T, t, h->P);
         CHECK_LAUNCH("k_prepare");
     }
-    TimedSlot ts(h->timing, h->stream, VLBM_SLOT_LINKS);
-    hipLaunchKernelGGL((k_links<T>), dim3(gl), dim3(128), 0, h->stream, src, dst, (const T *)d.record,
-                       (const double *)d.U, h->T, d.invalid, use_rec, d.has_char, h->P);
+    TimedSlot ts(h->timing, st, VLBM_SLOT_LINKS);
+    hipLaunchKernelGGL((k_links<T>), dim3(gl), dim3(128), 0, st, src, dst, (const T *)d.record,
+                       (const double *)d.U, h->T, invalid, use_rec, d.has_char, h->P);
     CHECK_LAUNCH("k_links");
+    return 0;
+}
+
+template <typename T>
+static int launch_stream(vlbm3d_handle *h, bool next, const T *f0, T *f1) {
+    const vlbm3d_desc &d = h->d;
+    const int gn = xcd_grid(d.n_active);
+    if (next)
+        hipLaunchKernelGGL((k_stream<T, true>), dim3(gn), dim3(BLOCK), 0, h->stream, f0, f1, (const T *)d.record,
+                           d.neighbor, d.U, d.u, d.u_star, d.invalid, h->P);
+    else
+        hipLaunchKernelGGL((k_stream<T, false>), dim3(gn), dim3(BLOCK), 0, h->stream, f0, f1, (const T *)d.record,
+                           d.neighbor, d.U, d.u, d.u_star, d.invalid, h->P);
+    CHECK_LAUNCH("k_stream");
     return 0;
 }
 
@@ -921,25 +983,25 @@ static int run_steps(vlbm3d_handle *h, T *f0, T *f1, double time, int64_t steps,
             const bool last = k + 1 == nsteps;
             const bool pack = d.has_char && d.n_char_links > 0;
             const int write_U = (last || d.recon != R_LOCALF || !pack) ? 1 : 0;
+            // The flag is cleared here when the stream kernel performs the
+            // next collide; the links then report only on the call's last
+            // step, where the reference's flag holds the last collide's and
+            // the last closure's checks.
             hipLaunchKernelGGL(k_record<T>, dim3(gn), dim3(BLOCK), 0, h->stream, d.U,
                                disp ? (const double *)d.u_next : (const double *)d.u, d.u, d.u_star,
                                (T *)d.record, d.neighbor, pack ? d.char_pack_slot : (const int32_t *)nullptr,
-                               d.char_pack, (int)compat, (int)disp, write_U, P);
+                               d.char_pack, (int)compat, (int)disp, write_U,
+                               last ? (int32_t *)nullptr : d.invalid, P);
             CHECK_LAUNCH("k_record");
         }
-        int rc = run_boundary<T>(h, f0, f1, tb, 1);
-        if (rc) return rc;
         const bool next = k + 1 < nsteps;
-        if (next) hipLaunchKernelGGL(k_reset_invalid, dim3(1), dim3(1), 0, h->stream, d.invalid);
+        int32_t *link_flag = next ? (int32_t *)nullptr : d.invalid;
+        int rc = run_boundary<T>(h, h->stream, f0, f1, tb, 1, link_flag);
+        if (rc) return rc;
         {
             TimedSlot ts(h->timing, h->stream, VLBM_SLOT_STREAM);
-            if (next)
-                hipLaunchKernelGGL((k_stream<T, true>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, f1,
-                                   (const T *)d.record, d.neighbor, d.U, d.u, d.u_star, d.invalid, P);
-            else
-                hipLaunchKernelGGL((k_stream<T, false>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, f1,
-                                   (const T *)d.record, d.neighbor, d.U, d.u, d.u_star, d.invalid, P);
-            CHECK_LAUNCH("k_stream");
+            rc = launch_stream<T>(h, next, f0, f1);
+            if (rc) return rc;
         }
         T *tmp = f0;
         f0 = f1;
@@ -1050,17 +1112,17 @@ int vlbm3d_apply_boundary(vlbm3d_handle *h, void *f_src, void *f_dst, double t, 
         if (in_place) {
             hipLaunchKernelGGL(k_copy_pop<double>, dim3(gc), dim3(BLOCK), 0, h->stream, s, t2, count);
             CHECK_LAUNCH("k_copy_pop");
-            return run_boundary<double>(h, t2, s, t, 0);
+            return run_boundary<double>(h, h->stream, t2, s, t, 0, d.invalid);
         }
-        return run_boundary<double>(h, s, t2, t, 0);
+        return run_boundary<double>(h, h->stream, s, t2, t, 0, d.invalid);
     }
     float *s = (float *)f_src, *t2 = (float *)f_dst;
     if (in_place) {
         hipLaunchKernelGGL(k_copy_pop<float>, dim3(gc), dim3(BLOCK), 0, h->stream, s, t2, count);
         CHECK_LAUNCH("k_copy_pop");
-        return run_boundary<float>(h, t2, s, t, 0);
+        return run_boundary<float>(h, h->stream, t2, s, t, 0, d.invalid);
     }
-    return run_boundary<float>(h, s, t2, t, 0);
+    return run_boundary<float>(h, h->stream, s, t2, t, 0, d.invalid);
 }
 
 int vlbm3d_init_populations(vlbm3d_handle *h, void *f) {

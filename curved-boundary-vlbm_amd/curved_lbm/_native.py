@@ -203,6 +203,7 @@ EXPORTED = (
     "vlbm2d_stress",
     "vlbm2d_post_collision",
     "vlbm2d_set_dynamic_traction",
+    "vlbm2d_set_graph_steps",
     "vlbm2d_reduce",
     "vlbm2d_populations_to_f64",
     "vlbm2d_populations_from_f64",
@@ -256,6 +257,7 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vlbm2d_stress": [vp, vp, vp],
         "vlbm2d_post_collision": [vp, vp, vp],
         "vlbm2d_set_dynamic_traction": [vp, C.c_int32],
+        "vlbm2d_set_graph_steps": [vp, C.c_int32],
         "vlbm2d_reduce": [vp, vp, _dp],
         "vlbm2d_populations_to_f64": [vp, vp, vp],
         "vlbm2d_populations_from_f64": [vp, vp, vp],

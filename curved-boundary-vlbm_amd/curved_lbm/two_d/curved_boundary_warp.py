@@ -748,6 +748,7 @@ class WarpCurvedBoundaryLBM2D:
         normal_jacobian: str | None = None,
         neumann_solver: str | None = None,
         precision: str = "fp64",
+        graph_steps: int = 0,
     ) -> None:
         self.geometry = geometry
         self.nx, self.ny = int(geometry.nx), int(geometry.ny)
@@ -788,6 +789,9 @@ class WarpCurvedBoundaryLBM2D:
         if precision not in ("fp64", "fp32"):
             raise ValueError("precision must be 'fp64' or 'fp32'")
         self.precision = precision
+        if int(graph_steps) < 0 or int(graph_steps) % 2:
+            raise ValueError("graph_steps must be a non-negative even count")
+        self.graph_steps = int(graph_steps)
         self.dt = self.dx / self.lattice_speed
         self.time = 0.0
         self.steps = 0
@@ -993,6 +997,7 @@ class WarpCurvedBoundaryLBM2D:
         h = C.c_void_p()
         _native.check(self._lib.vlbm2d_create(C.byref(d), C.byref(h)), "vlbm2d_create")
         self._handle = h
+        _native.check(self._lib.vlbm2d_set_graph_steps(h, self.graph_steps), "vlbm2d_set_graph_steps")
 
     def __del__(self):
         h = getattr(self, "_handle", None)

@@ -118,6 +118,13 @@ int vlbm2d_stress(vlbm2d_handle *h, double *P, double *sigma);
 int vlbm2d_post_collision(vlbm2d_handle *h, const void *f_prev, void *fpost);
 /* Enables (1) or disables (0) the per-link traction override on Neumann links. */
 int vlbm2d_set_dynamic_traction(vlbm2d_handle *h, int32_t enabled);
+/* Replays blocks of `steps` fused steps (an even count; 0 disables) from a
+ * captured HIP graph inside vlbm2d_step, with the step time kept on the
+ * device; used when the displacement and F-repair intervals are 0 or 1 and
+ * timing is off.  The graph holds the descriptor's pointers and the f0/f1
+ * pair of the call that captured it (one per buffer order is cached).  No
+ * reference counterpart: the reference launches every kernel from Python. */
+int vlbm2d_set_graph_steps(vlbm2d_handle *h, int32_t steps);
 /* Synchronous reductions over the active nodes (VLBM_RED_* slots). */
 int vlbm2d_reduce(vlbm2d_handle *h, const double *u_prev, double *out);
 /* Population precision conversion over whole (24, nx, ny) planes. */

@@ -243,3 +243,30 @@ def test_gpu_annulus96_3000_steps_matches_oracle():
     act = gpu.active_mask()
     Jg, Jo = gpu.deformation_jacobian()[act], ora.deformation_jacobian()[act]
     assert np.max(np.abs(Jg - Jo)) <= 1e-10 and Jg.min() > 0.0
+
+
+def test_gpu_graph_step_loop_matches_direct_launches():
+    """Captured step graphs replayed across several calls, with eager work
+    (refresh, reductions, reads) between them, give bitwise the populations,
+    fields and flags of direct launches; odd and even call lengths, and a
+    toggle of the per-link traction override (a captured argument)."""
+    g, kw, _ = gc.annulus(40)
+    a = cb.WarpCurvedBoundaryLBM2D(g, **kw, graph_steps=0)
+    b = cb.WarpCurvedBoundaryLBM2D(g, **kw, graph_steps=8)
+    outer = g.link_boundary_id == cb.BOUNDARY_ID_OUTER
+    for s in (a, b):
+        s.link_traction_x.assign(np.where(outer, 0.01 * g.link_nx, 0.0))
+        s.link_traction_y.assign(np.where(outer, 0.01 * g.link_ny, 0.0))
+    for calls, n in enumerate((37, 20, 9, 64, 33)):
+        if calls == 3:
+            a.enable_dynamic_link_traction(True)
+            b.enable_dynamic_link_traction(True)
+        a.run(n)
+        b.run(n)
+        for name in ("f0", "U", "u", "u_star"):
+            np.testing.assert_array_equal(a.numpy_field(name), b.numpy_field(name), err_msg=f"{name} call {calls}")
+        assert a.reductions() == b.reductions()
+        a.refresh_current()
+        b.refresh_current()
+    assert a.time == b.time and a.steps == b.steps
+
