@@ -8,7 +8,10 @@ missing the solver refuses to construct.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
+
+import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libvlbm.so"
 
@@ -209,6 +212,8 @@ EXPORTED = (
     "vlbm2d_populations_from_f64",
     "vlbm2d_timing_enable",
     "vlbm2d_timing_read",
+    "vlbm_geom3d_char_weights",
+    "vlbm_geom2d_lsq_weights",
     "vlbm_last_error",
     "vlbm_abi_version",
 )
@@ -263,6 +268,10 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vlbm2d_populations_from_f64": [vp, vp, vp],
         "vlbm2d_timing_enable": [vp, C.c_int32],
         "vlbm2d_timing_read": [vp, _dp, _ip, C.c_int32],
+        "vlbm_geom3d_char_weights": [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int64] + [vp] * 4 + [vp] * 7
+        + [C.c_double, C.c_int32, vp, vp, vp, vp],
+        "vlbm_geom2d_lsq_weights": [vp, C.c_int32, C.c_int32, vp, vp, C.c_int64] + [vp] * 6
+        + [C.c_double, C.c_int32, C.c_int32] + [vp] * 7,
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)
@@ -281,3 +290,60 @@ def check(rc: int, what: str) -> None:
         if rc == -1:
             raise ValueError(f"{what}: {msg}")
         raise RuntimeError(f"{what}: {msg}")
+
+
+def host_threads() -> int:
+    """Host threads for the geometry builders: the CPUs this process may
+    use, at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data) if a.size else None
+
+
+def geom3d_char_weights(active_id, li, lj, lk, lq, xb, yb, zb, nx, ny, nz, eta, dx):
+    """Native characteristic stencils (see include/vlbm_geom.h); returns
+    (indices (64, L), weights (4, 64, L), modes (L), status (L))."""
+    lib = load_library()
+    aid = np.ascontiguousarray(active_id, dtype=np.int32)
+    ints = [np.ascontiguousarray(a, dtype=np.int32) for a in (li, lj, lk, lq)]
+    dbl = [np.ascontiguousarray(a, dtype=np.float64) for a in (xb, yb, zb, nx, ny, nz, eta)]
+    L = int(ints[0].size)
+    indices = np.full((64, L), -1, np.int32)
+    weights = np.zeros((4, 64, L), np.float64)
+    modes = np.zeros(L, np.int32)
+    status = np.zeros(L, np.uint8)
+    rc = lib.vlbm_geom3d_char_weights(_ptr(aid), *[int(v) for v in aid.shape], L, *[_ptr(a) for a in ints],
+                                      *[_ptr(a) for a in dbl], float(dx), host_threads(), _ptr(indices),
+                                      _ptr(weights), _ptr(modes), _ptr(status))
+    if rc != 0:
+        raise ValueError("vlbm_geom3d_char_weights: invalid arguments")
+    return indices, weights, modes, status
+
+
+def geom2d_lsq_weights(active, x_coords, y_coords, li, lj, xb, yb, nx, ny, dx, max_stencil):
+    """Native 2D boundary stencils (see include/vlbm_geom.h); returns
+    (out_i, out_j, w, wt, wx, wy, valid) with (L, max_stencil) arrays."""
+    lib = load_library()
+    act = np.ascontiguousarray(active, dtype=np.int32)
+    xc = np.ascontiguousarray(x_coords, dtype=np.float64)
+    yc = np.ascontiguousarray(y_coords, dtype=np.float64)
+    ints = [np.ascontiguousarray(a, dtype=np.int32) for a in (li, lj)]
+    dbl = [np.ascontiguousarray(a, dtype=np.float64) for a in (xb, yb, nx, ny)]
+    L = int(ints[0].size)
+    S = int(max_stencil)
+    oi = np.full((L, S), -1, np.int32)
+    oj = np.full((L, S), -1, np.int32)
+    w, wt, wx, wy = (np.zeros((L, S)) for _ in range(4))
+    valid = np.zeros(L, np.int32)
+    rc = lib.vlbm_geom2d_lsq_weights(_ptr(act), int(act.shape[0]), int(act.shape[1]), _ptr(xc), _ptr(yc), L,
+                                     *[_ptr(a) for a in ints], *[_ptr(a) for a in dbl], float(dx), S, host_threads(),
+                                     *[_ptr(a) for a in (oi, oj, w, wt, wx, wy, valid)])
+    if rc != 0:
+        raise ValueError("vlbm_geom2d_lsq_weights: invalid arguments")
+    return oi, oj, w, wt, wx, wy, valid

@@ -146,18 +146,37 @@ def _predictor(ctr, xb, normal, shape, active_id, link_q, link_eta):
     return pp_out, ww_out, mode
 
 
-def build_characteristic_weights(geometry, tables, *, batch_size=1024, link_mask=None):
+def _raise_for_status(status, batch_size):
+    """The errors of the batched NumPy build, from per-link statuses: the
+    first batch holding a failure decides; rank/condition failures (1) are
+    checked before invalid weights (2)."""
+    for start in range(0, status.size, batch_size):
+        st = status[start:start + batch_size]
+        if np.any(st == 1):
+            bad = (np.flatnonzero(st == 1) + start)[:8]
+            raise ValueError(
+                f"local_f needs a full-rank quadratic stencil within three cells; cut links {bad.tolist()}"
+            )
+        if np.any(st == 2):
+            raise ValueError(f"invalid local_f gradient weights near link {start + np.flatnonzero(st == 2)[0]}")
+
+
+def build_characteristic_weights(geometry, tables, *, batch_size=1024, link_mask=None, builder="native"):
     """Characteristic predictor/gradient weights for the cut links.
 
     ``link_mask`` restricts the fit to the selected links; unselected links
-    get index -1 and zero weights.
+    get index -1 and zero weights.  ``builder="native"`` runs the threaded
+    C++ builder of libvlbm (the same samples, slots and modes; weights equal
+    to rounding), ``"numpy"`` the batched NumPy restatement below.
     """
+    if builder not in ("native", "numpy"):
+        raise ValueError("characteristic builder must be 'native' or 'numpy'")
     if link_mask is not None:
         mask = np.asarray(link_mask, dtype=bool)
         if mask.shape != (geometry.n_links,):
             raise ValueError("link_mask must select one entry per cut link")
         ids = np.flatnonzero(mask)
-        fit = build_characteristic_weights(_subset(geometry, ids), tables, batch_size=batch_size)
+        fit = build_characteristic_weights(_subset(geometry, ids), tables, batch_size=batch_size, builder=builder)
         indices = np.full((MAX_CHARACTERISTIC_SAMPLES, geometry.n_links), -1, np.int32)
         weights = np.zeros((4, MAX_CHARACTERISTIC_SAMPLES, geometry.n_links), np.float64)
         modes = np.zeros(geometry.n_links, np.int32)
@@ -181,6 +200,14 @@ def build_characteristic_weights(geometry, tables, *, batch_size=1024, link_mask
     modes = np.zeros(nlinks, np.int32)
     link_q = np.asarray(geometry.link_q)
     link_eta = np.asarray(geometry.link_eta)
+    if builder == "native":
+        from .. import _native
+
+        indices, weights, modes, status = _native.geom3d_char_weights(
+            active_id, geometry.link_i, geometry.link_j, geometry.link_k, link_q, geometry.link_xb,
+            geometry.link_yb, geometry.link_zb, geometry.link_nx, geometry.link_ny, geometry.link_nz, link_eta, dx)
+        _raise_for_status(status, batch_size)
+        nlinks = 0
     for start in range(0, nlinks, batch_size):
         stop = min(start + batch_size, nlinks)
         ctr = centers[start:stop]

@@ -254,7 +254,7 @@ def _local_quadratic_fit(active, x_coords, y_coords, i0, j0, xb, yb, dx, weight_
 
 
 def _build_boundary_lsq_weights(*, active, x_coords, y_coords, link_i, link_j, link_xb, link_yb, link_nx,
-                                link_ny, dx, max_stencil=MAX_LSQ_STENCIL):
+                                link_ny, dx, max_stencil=MAX_LSQ_STENCIL, builder="native"):
     """Per-link boundary weights (cbw2d:332-606):
 
     * ``wb``: value at the cut point from two bilinear probes on the inward
@@ -266,7 +266,19 @@ def _build_boundary_lsq_weights(*, active, x_coords, y_coords, link_i, link_j, l
       a Gaussian-weighted quadratic fit over at most 24 nodes within radius
       sqrt(9.5), sharing index slots with ``wb`` (new nodes appended).
     ``valid`` is set when the value weights exist.
+
+    ``builder="native"`` runs the threaded C++ builder of libvlbm (same
+    selection, slots and validity; weights equal to rounding), ``"numpy"``
+    this Python restatement.
     """
+    if builder == "native":
+        from .. import _native
+
+        out = _native.geom2d_lsq_weights(active, x_coords, y_coords, link_i, link_j, link_xb, link_yb, link_nx,
+                                         link_ny, dx, max_stencil)
+        return tuple(np.ascontiguousarray(a) for a in out)
+    if builder != "numpy":
+        raise ValueError("lsq builder must be 'native' or 'numpy'")
     n_links = int(link_i.size)
     nx, ny = active.shape
     out_i = np.full((n_links, max_stencil), -1, dtype=np.int32)
@@ -435,7 +447,8 @@ class CurvedBoundaryGeometry:
                        boundary_id: Callable[[float, float, float, float], int] | None = None,
                        label: str = "level_set", root_iterations: int = 40,
                        phi_vectorized: Callable | None = None,
-                       normal_vectorized: Callable | None = None) -> "CurvedBoundaryGeometry":
+                       normal_vectorized: Callable | None = None,
+                       lsq_builder: str = "native") -> "CurvedBoundaryGeometry":
         """Active mask phi <= 0 and cut links (cbw2d:669-858): every (active
         node, direction) whose neighbour is inactive or off the lattice, in
         node-major then direction order.  The cut fraction comes from
@@ -460,7 +473,7 @@ class CurvedBoundaryGeometry:
             return cls._from_level_set_arrays(
                 nx=nx, ny=ny, length_x=length_x, length_y=length_y, phi_vectorized=phi_vectorized,
                 normal=normal, normal_vectorized=normal_vectorized, boundary_id=boundary_id, label=label,
-                root_iterations=root_iterations)
+                root_iterations=root_iterations, lsq_builder=lsq_builder)
         ph = np.empty((nx, ny))
         for i in range(nx):
             for j in range(ny):
@@ -527,7 +540,8 @@ class CurvedBoundaryGeometry:
         li, lj, lq, lb = (np.ascontiguousarray(np.asarray(c, dtype=np.int32)) for c in cols[:4])
         le, lxb, lyb, lnx, lny = (np.ascontiguousarray(np.asarray(c, dtype=np.float64)) for c in cols[4:])
         lsq = _build_boundary_lsq_weights(active=active.astype(np.int32), x_coords=x, y_coords=y, link_i=li,
-                                          link_j=lj, link_xb=lxb, link_yb=lyb, link_nx=lnx, link_ny=lny, dx=dx)
+                                          link_j=lj, link_xb=lxb, link_yb=lyb, link_nx=lnx, link_ny=lny, dx=dx,
+                                          builder=lsq_builder)
         return cls(nx=int(nx), ny=int(ny), length_x=float(length_x), length_y=float(length_y),
                    active=np.ascontiguousarray(active.astype(np.int32)), link_i=li, link_j=lj, link_q=lq,
                    link_boundary_id=lb, link_eta=le, link_xb=lxb, link_yb=lyb, link_nx=lnx, link_ny=lny,
@@ -536,7 +550,7 @@ class CurvedBoundaryGeometry:
 
     @classmethod
     def _from_level_set_arrays(cls, *, nx, ny, length_x, length_y, phi_vectorized, normal, normal_vectorized,
-                               boundary_id, label, root_iterations):
+                               boundary_id, label, root_iterations, lsq_builder="native"):
         """Array form of from_level_set: the same sign search, bisection,
         cut point, normal and clipping per link, evaluated for all links at
         once (normals normalised with math.hypot per link, as the scalar
@@ -630,7 +644,8 @@ class CurvedBoundaryGeometry:
         li, lj, lq, lb = i32(li), i32(lj), i32(lq), i32(bid)
         le, lxb, lyb, lnx, lny = f64(eta), f64(xb), f64(yb), f64(lnx), f64(lny)
         lsq = _build_boundary_lsq_weights(active=active.astype(np.int32), x_coords=x, y_coords=y, link_i=li,
-                                          link_j=lj, link_xb=lxb, link_yb=lyb, link_nx=lnx, link_ny=lny, dx=dx)
+                                          link_j=lj, link_xb=lxb, link_yb=lyb, link_nx=lnx, link_ny=lny, dx=dx,
+                                          builder=lsq_builder)
         return cls(nx=int(nx), ny=int(ny), length_x=float(length_x), length_y=float(length_y),
                    active=np.ascontiguousarray(active.astype(np.int32)), link_i=li, link_j=lj, link_q=lq,
                    link_boundary_id=lb, link_eta=le, link_xb=lxb, link_yb=lyb, link_nx=lnx, link_ny=lny,

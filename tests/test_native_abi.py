@@ -1,6 +1,6 @@
 """The C ABI library loads and exports every entry point of
-include/vlbm3d.h and include/vlbm2d.h; descriptor layouts agree with the
-library.  No GPU work."""
+include/vlbm3d.h, include/vlbm2d.h and include/vlbm_geom.h; descriptor
+layouts agree with the library.  No GPU work."""
 
 import ctypes as C
 import re
@@ -13,11 +13,12 @@ from curved_lbm import _native
 INCLUDE = Path(__file__).resolve().parents[1] / "include"
 HEADER = INCLUDE / "vlbm3d.h"
 HEADER2 = INCLUDE / "vlbm2d.h"
+HEADER_GEOM = INCLUDE / "vlbm_geom.h"
 
 
 def declared_functions():
     names = set()
-    for h in (HEADER, HEADER2):
+    for h in (HEADER, HEADER2, HEADER_GEOM):
         names |= set(re.findall(r"^\s*(?:int|const char \*)\s*(vlbm\w*)\s*\(", h.read_text(), flags=re.M))
     return sorted(names)
 
