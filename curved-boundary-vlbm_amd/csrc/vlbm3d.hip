@@ -20,6 +20,7 @@
 // pulling.  Populations therefore cross HBM once per step in each direction.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -169,40 +170,34 @@ __global__ __launch_bounds__(BLOCK) void k_collide(const T *__restrict__ f, doub
 
 // One relaxed nearest-neighbour compatibility sweep of u (cbw3d:2473-2527)
 // with the Dirichlet anchors of the node's own cut links gathered in link
-// order (cbw3d:2413-2469), so the sums are deterministic.
+// order (cbw3d:2413-2469), so the sums are deterministic.  The anchors come
+// from a compacted per-node list built once at set-up (only links that
+// contribute a component; per entry the boundary id with a component mask,
+// the cut point and its offset from the node), so an anchored node loads its
+// entries directly instead of walking link -> id -> coordinates.
 __global__ __launch_bounds__(BLOCK) void k_sweep(
     const double *__restrict__ U, const double *__restrict__ u, double *__restrict__ un,
-    const int32_t *__restrict__ nbr, const int32_t *__restrict__ loff, const int32_t *__restrict__ lids,
-    const int32_t *__restrict__ link_bid, const double *__restrict__ lxb, const double *__restrict__ lyb,
-    const double *__restrict__ lzb, const int32_t *__restrict__ ai, const int32_t *__restrict__ aj,
-    const int32_t *__restrict__ ak, const int32_t *__restrict__ comp_kind, const int32_t *__restrict__ mode_by_id,
-    const double *__restrict__ prm_by_id, double time, Params P) {
+    const int32_t *__restrict__ nbr, const int32_t *__restrict__ aoff, const int32_t *__restrict__ akey,
+    const double *__restrict__ adat, const double *__restrict__ prm_by_id, double time, Params P) {
     const int p = xcd_block() * BLOCK + threadIdx.x;
     if (p >= P.n) return;
     const size_t n = P.n;
     double d[3] = {0.0, 0.0, 0.0}, r[3] = {0.0, 0.0, 0.0};
-    const int l0 = loff[p], l1 = loff[p + 1];
+    const int l0 = aoff[p], l1 = aoff[p + 1];
     if (l1 > l0) {
         double F[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) F[k] = U[(3 + k) * n + p];
-        const double xp = ((double)ai[p] + 0.5) * P.dx, yp = ((double)aj[p] + 0.5) * P.dy,
-                     zp = ((double)ak[p] + 0.5) * P.dz;
         for (int t = l0; t < l1; ++t) {
-            const int ell = lids[t];
-            const int bid = link_bid[ell];
-            bool force_all = false;
-            if (P.disp_bid >= 0) {
-                if (bid != P.disp_bid) continue;
-                force_all = true;
-            }
-            const double x = lxb[ell], y = lyb[ell], z = lzb[ell];
-            const double db[3] = {x - xp, y - yp, z - zp};
-            const int mode = mode_by_id[bid];
+            const int key = akey[t];
+            const int bid = key & 0xff, mask = (key >> 8) & 7, mode = key >> 12;
+            const double *e = adat + (size_t)6 * t;
+            const double x = e[0], y = e[1], z = e[2];
+            const double db[3] = {e[3], e[4], e[5]};
             const double *prm = prm_by_id + bid * MAX_IDS;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                if (!(force_all || comp_kind[c * MAX_IDS + bid] == BC_DIR)) continue;
+                if (!((mask >> c) & 1)) continue;
                 const double ub = dirichlet_disp(mode, prm, c, x, y, z, time);
                 const double g = (F[3 * c] - (c == 0 ? 1.0 : 0.0)) * db[0] +
                                  (F[3 * c + 1] - (c == 1 ? 1.0 : 0.0)) * db[1] +
@@ -253,23 +248,36 @@ __global__ __launch_bounds__(BLOCK) void k_commit(const double *__restrict__ un,
 }
 
 // Centred / second-order one-sided / first-order derivative of u along one
-// axis (cbw3d:2544-2571).
-__device__ __forceinline__ double du_axis(const double *u, const int32_t *nbr, size_t n, int c, int p,
-                                          int axis, double h) {
-    const int pp = nbr[axis * n + p], pm = nbr[(axis + 3) * n + p];
-    const double *uc = u + c * n;
-    if (pp >= 0 && pm >= 0) return (uc[pp] - uc[pm]) / (2.0 * h);
+// axis (cbw3d:2544-2571).  pp/pm are the neighbours along +axis/-axis and
+// pp2/pm2 the next ones (second-neighbour table); every value the branch
+// might use is loaded unconditionally from an index that falls back to p, so
+// the loads of all branches issue together.
+__device__ __forceinline__ double du_axis(const double *uc, int p, int pp, int pm, int pp2, int pm2, double h) {
+    const bool fwd = pp >= 0 && pm < 0 && pp2 >= 0, bwd = pm >= 0 && pp < 0 && pm2 >= 0;
+    const double u0 = uc[p], up = uc[pp >= 0 ? pp : p], um = uc[pm >= 0 ? pm : p];
+    const double up2 = uc[fwd ? pp2 : p], um2 = uc[bwd ? pm2 : p];
+    if (pp >= 0 && pm >= 0) return (up - um) / (2.0 * h);
     if (pp >= 0) {
-        const int pp2 = nbr[axis * n + pp];
-        if (pp2 >= 0) return (-3.0 * uc[p] + 4.0 * uc[pp] - uc[pp2]) / (2.0 * h);
-        return (uc[pp] - uc[p]) / h;
+        if (pp2 >= 0) return (-3.0 * u0 + 4.0 * up - up2) / (2.0 * h);
+        return (up - u0) / h;
     }
     if (pm >= 0) {
-        const int pm2 = nbr[(axis + 3) * n + pm];
-        if (pm2 >= 0) return (3.0 * uc[p] - 4.0 * uc[pm] + uc[pm2]) / (2.0 * h);
-        return (uc[p] - uc[pm]) / h;
+        if (pm2 >= 0) return (3.0 * u0 - 4.0 * um + um2) / (2.0 * h);
+        return (u0 - um) / h;
     }
     return 0.0;
+}
+
+// Second neighbours: nbr2[q][p] = nbr[q][nbr[q][p]] (or -1).
+__global__ __launch_bounds__(BLOCK) void k_nbr2(const int32_t *__restrict__ nbr, int32_t *__restrict__ nbr2,
+                                                int n) {
+    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= n) return;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int a = nbr[(size_t)q * n + p];
+        nbr2[(size_t)q * n + p] = a >= 0 ? nbr[(size_t)q * n + a] : -1;
+    }
 }
 
 // Deformation-gradient update F <- F + theta (I + grad u - F) with the J
@@ -281,6 +289,7 @@ template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const double *__restrict__ ugrad,
                                                   double *__restrict__ u, double *__restrict__ us,
                                                   R *__restrict__ rec, const int32_t *__restrict__ nbr,
+                                                  const int32_t *__restrict__ nbr2,
                                                   const int32_t *__restrict__ pack_slot, double *__restrict__ pack,
                                                   int repair, int commit, int write_U,
                                                   int32_t *__restrict__ reset_invalid, Params P) {
@@ -307,12 +316,19 @@ __global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const 
         double beta = P.blend;
         beta = beta < 0.0 ? 0.0 : (beta > 1.0 ? 1.0 : beta);
         const double hh[3] = {P.dx, P.dy, P.dz};
+        int nb[NQ], nb2[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            nb[q] = nbr[q * n + p];
+            nb2[q] = nbr2[q * n + p];
+        }
         double Fn[9];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                const double t = (i == j ? 1.0 : 0.0) + du_axis(ugrad, nbr, n, i, p, j, hh[j]);
+                const double t =
+                    (i == j ? 1.0 : 0.0) + du_axis(ugrad + i * n, p, nb[j], nb[j + 3], nb2[j], nb2[j + 3], hh[j]);
                 Fn[3 * i + j] = Fo[3 * i + j] + beta * (t - Fo[3 * i + j]);
             }
         if (det3(Fn) > P.m.jf) {
@@ -830,7 +846,93 @@ struct vlbm3d_handle {
     LinkTables T;
     hipStream_t stream;
     Timing timing;
+    // Library-owned tables built at create: the compacted anchor list of
+    // the displacement sweep and the second-neighbour table.
+    int32_t *anc_off = nullptr, *anc_key = nullptr, *nbr2 = nullptr;
+    double *anc_dat = nullptr;
+    void release() {
+        for (void *ptr : {(void *)anc_off, (void *)anc_key, (void *)nbr2, (void *)anc_dat})
+            if (ptr) (void)hipFree(ptr);
+        anc_off = anc_key = nbr2 = nullptr;
+        anc_dat = nullptr;
+    }
 };
+
+// Compacted Dirichlet anchors per node (cbw3d:2413-2469): for every link of
+// the node, in ascending link order, that contributes a component (all three
+// when it carries the sweep's boundary id, else the components whose kind is
+// Dirichlet): key = boundary id | component mask << 8 | value mode << 12 and
+// (x_b, y_b, z_b, x_b - x_p, y_b - y_p, z_b - z_p) with x_p = (i + 1/2) dx as
+// in the sweep.
+static int build_anchor_tables(vlbm3d_handle *h) {
+    const vlbm3d_desc &d = h->d;
+    const size_t n = (size_t)d.n_active, L = (size_t)d.n_links, na = (size_t)std::max(d.n_anchor_links, 0);
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail("vlbm3d_create: sync", e);
+    std::vector<int32_t> off(n + 1), ids(na), bid(L), ai(n), aj(n), ak(n), ck(3 * MAX_IDS), mode(MAX_IDS);
+    std::vector<double> xb(L), yb(L), zb(L);
+    e = hipMemcpy(off.data(), d.node_link_off, (n + 1) * sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && na) e = hipMemcpy(ids.data(), d.node_link_ids, na * sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && L) {
+        e = hipMemcpy(bid.data(), d.link_bid, L * sizeof(int32_t), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(xb.data(), d.link_xb, L * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(yb.data(), d.link_yb, L * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(zb.data(), d.link_zb, L * sizeof(double), hipMemcpyDeviceToHost);
+    }
+    if (e == hipSuccess) e = hipMemcpy(ai.data(), d.active_i, n * sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(aj.data(), d.active_j, n * sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(ak.data(), d.active_k, n * sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+        e = hipMemcpy(ck.data(), d.comp_kind_by_id, ck.size() * sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+        e = hipMemcpy(mode.data(), d.mode_by_id, mode.size() * sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail("vlbm3d_create: anchor table copy", e);
+    std::vector<int32_t> aoff(n + 1, 0), key;
+    std::vector<double> dat;
+    for (size_t p = 0; p < n; ++p) {
+        aoff[p] = (int32_t)key.size();
+        if (off[p] < 0 || off[p + 1] < off[p] || (size_t)off[p + 1] > na)
+            return fail_msg("vlbm3d_create: node_link_off is not a CSR offset table");
+        const double xp = ((double)ai[p] + 0.5) * d.dx, yp = ((double)aj[p] + 0.5) * d.dy,
+                     zp = ((double)ak[p] + 0.5) * d.dz;
+        for (int t = off[p]; t < off[p + 1]; ++t) {
+            const int ell = ids[t];
+            if (ell < 0 || (size_t)ell >= L) return fail_msg("vlbm3d_create: node_link_ids entry out of range");
+            const int b = bid[ell];
+            if (b < 0 || b >= MAX_IDS) return fail_msg("vlbm3d_create: link boundary id out of range");
+            int mask = 0;
+            if (d.disp_boundary_id >= 0) {
+                if (b == d.disp_boundary_id) mask = 7;
+            } else {
+                for (int c = 0; c < 3; ++c)
+                    if (ck[c * MAX_IDS + b] == BC_DIR) mask |= 1 << c;
+            }
+            if (!mask) continue;
+            if (mode[b] < 0 || mode[b] >= (1 << 19)) return fail_msg("vlbm3d_create: value mode out of range");
+            key.push_back(b | (mask << 8) | (mode[b] << 12));
+            const double x = xb[ell], y = yb[ell], z = zb[ell];
+            const double ent[6] = {x, y, z, x - xp, y - yp, z - zp};
+            dat.insert(dat.end(), ent, ent + 6);
+        }
+    }
+    aoff[n] = (int32_t)key.size();
+    const size_t ne = std::max<size_t>(key.size(), 1);
+    e = hipMalloc((void **)&h->anc_off, (n + 1) * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&h->anc_key, ne * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&h->anc_dat, ne * 6 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&h->nbr2, (size_t)NQ * n * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMemcpy(h->anc_off, aoff.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !key.empty())
+        e = hipMemcpy(h->anc_key, key.data(), key.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !dat.empty())
+        e = hipMemcpy(h->anc_dat, dat.data(), dat.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail("vlbm3d_create: anchor tables", e);
+    hipLaunchKernelGGL(k_nbr2, dim3((unsigned)((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, h->stream,
+                       (const int32_t *)d.neighbor, h->nbr2, (int)n);
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail("vlbm3d_create: k_nbr2", e);
+    return 0;
+}
 
 static Params make_params(const vlbm3d_desc &d) {
     Params P{};
@@ -964,9 +1066,9 @@ static int run_steps(vlbm3d_handle *h, T *f0, T *f1, double time, int64_t steps,
             for (int s = 0; s < d.disp_sweeps; ++s) {
                 TimedSlot ts(h->timing, h->stream, VLBM_SLOT_SWEEP);
                 hipLaunchKernelGGL(k_sweep, dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)d.U,
-                                   (const double *)d.u, d.u_next, d.neighbor, d.node_link_off, d.node_link_ids,
-                                   d.link_bid, d.link_xb, d.link_yb, d.link_zb, d.active_i, d.active_j,
-                                   d.active_k, d.comp_kind_by_id, d.mode_by_id, d.params_by_id, tb, P);
+                                   (const double *)d.u, d.u_next, d.neighbor, (const int32_t *)h->anc_off,
+                                   (const int32_t *)h->anc_key, (const double *)h->anc_dat, d.params_by_id, tb,
+                                   P);
                 CHECK_LAUNCH("k_sweep");
                 if (s + 1 < d.disp_sweeps) {
                     TimedSlot tc(h->timing, h->stream, VLBM_SLOT_COMMIT);
@@ -989,7 +1091,8 @@ static int run_steps(vlbm3d_handle *h, T *f0, T *f1, double time, int64_t steps,
             // the last closure's checks.
             hipLaunchKernelGGL(k_record<T>, dim3(gn), dim3(BLOCK), 0, h->stream, d.U,
                                disp ? (const double *)d.u_next : (const double *)d.u, d.u, d.u_star,
-                               (T *)d.record, d.neighbor, pack ? d.char_pack_slot : (const int32_t *)nullptr,
+                               (T *)d.record, d.neighbor, (const int32_t *)h->nbr2,
+                               pack ? d.char_pack_slot : (const int32_t *)nullptr,
                                d.char_pack, (int)compat, (int)disp, write_U,
                                last ? (int32_t *)nullptr : d.invalid, P);
             CHECK_LAUNCH("k_record");
@@ -1057,12 +1160,20 @@ int vlbm3d_create(const vlbm3d_desc *desc, vlbm3d_handle **out) {
     h->P = make_params(*desc);
     h->T = make_tables(*desc);
     h->stream = (hipStream_t)desc->stream;
+    const int rc = build_anchor_tables(h);
+    if (rc) {
+        h->release();
+        delete h;
+        return rc;
+    }
     *out = h;
     return 0;
 }
 
 int vlbm3d_destroy(vlbm3d_handle *h) {
     if (!h) return 0;
+    (void)hipStreamSynchronize(h->stream);
+    h->release();
     h->timing.destroy();
     delete h;
     return 0;
