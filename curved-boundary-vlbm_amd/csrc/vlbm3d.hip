@@ -452,21 +452,24 @@ __global__ __launch_bounds__(BLOCK) void k_pack(const double *__restrict__ U, co
 }
 
 // Characteristic state at each traction link: predictor value and gradient
-// of every state component (cbw3d:2676-2700).  Two lanes share a link, each
-// accumulating six components from the packed sample records; the pair's
-// table loads coalesce and each sample costs one 96-byte line.
+// of every state component (cbw3d:2676-2700).  GLANES lanes share a link,
+// each accumulating NC / GLANES components from the packed sample records;
+// the group's table loads coalesce and each sample costs one 96-byte line.
+constexpr int GLANES = 2;
+constexpr int GC = NC / GLANES;  // components per lane
+static_assert(NC % GLANES == 0, "components must split evenly over the lanes of a link");
 __global__ __launch_bounds__(BLOCK) void k_gather(const double *__restrict__ pack, const int32_t *__restrict__ clink,
                                                   const int32_t *__restrict__ gidx, const double *__restrict__ gw,
                                                   const int32_t *__restrict__ pidx, const double *__restrict__ pw,
                                                   double *__restrict__ cs, int L, int Lc) {
     constexpr int NG = MAX_CHAR - 16;
     const int t = xcd_block() * BLOCK + threadIdx.x;
-    const int c = t >> 1;
-    const int a0 = 6 * (t & 1);
+    const int c = t / GLANES;
+    const int a0 = GC * (t % GLANES);
     if (c >= Lc) return;
-    double val[6], gx[6], gy[6], gz[6];
+    double val[GC], gx[GC], gy[GC], gz[GC];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) val[k] = gx[k] = gy[k] = gz[k] = 0.0;
+    for (int k = 0; k < GC; ++k) val[k] = gx[k] = gy[k] = gz[k] = 0.0;
     // Samples are consumed in stencil order up to the first empty slot; the
     // indices, weights and records of GU slots are loaded before any is
     // used, so each lane keeps GU sample loads in flight.
@@ -483,31 +486,27 @@ __global__ __launch_bounds__(BLOCK) void k_gather(const double *__restrict__ pac
             alive = alive && sidx[k] >= 0;
             use[k] = alive;
         }
-        double w1[GU], w2[GU], w3[GU], x[GU][6];
+        double w1[GU], w2[GU], w3[GU], x[GU][GC];
 #pragma unroll
         for (int k = 0; k < GU; ++k) {
             w1[k] = w2[k] = w3[k] = 0.0;
 #pragma unroll
-            for (int j = 0; j < 6; ++j) x[k][j] = 0.0;
+            for (int j = 0; j < GC; ++j) x[k][j] = 0.0;
             if (use[k]) {
                 const int m = m0 + k;
                 w1[k] = gw[((size_t)0 * NG + m) * Lc + c];
                 w2[k] = gw[((size_t)1 * NG + m) * Lc + c];
                 w3[k] = gw[((size_t)2 * NG + m) * Lc + c];
-                const double2 *rp = reinterpret_cast<const double2 *>(pack + (size_t)sidx[k] * NC + a0);
+                const double *rp = pack + (size_t)sidx[k] * NC + a0;
 #pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const double2 v = rp[j];
-                    x[k][2 * j] = v.x;
-                    x[k][2 * j + 1] = v.y;
-                }
+                for (int j = 0; j < GC; ++j) x[k][j] = rp[j];
             }
         }
 #pragma unroll
         for (int k = 0; k < GU; ++k) {
             if (!use[k]) continue;
 #pragma unroll
-            for (int j = 0; j < 6; ++j) {
+            for (int j = 0; j < GC; ++j) {
                 gx[j] += w1[k] * x[k][j];
                 gy[j] += w2[k] * x[k][j];
                 gz[j] += w3[k] * x[k][j];
@@ -518,33 +517,29 @@ __global__ __launch_bounds__(BLOCK) void k_gather(const double *__restrict__ pac
         int sidx[GU];
 #pragma unroll
         for (int k = 0; k < GU; ++k) sidx[k] = pidx[(size_t)(m0 + k) * Lc + c];
-        double w0[GU], x[GU][6];
+        double w0[GU], x[GU][GC];
 #pragma unroll
         for (int k = 0; k < GU; ++k) {
             w0[k] = 0.0;
 #pragma unroll
-            for (int j = 0; j < 6; ++j) x[k][j] = 0.0;
+            for (int j = 0; j < GC; ++j) x[k][j] = 0.0;
             if (sidx[k] >= 0) {
                 w0[k] = pw[(size_t)(m0 + k) * Lc + c];
-                const double2 *rp = reinterpret_cast<const double2 *>(pack + (size_t)sidx[k] * NC + a0);
+                const double *rp = pack + (size_t)sidx[k] * NC + a0;
 #pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const double2 v = rp[j];
-                    x[k][2 * j] = v.x;
-                    x[k][2 * j + 1] = v.y;
-                }
+                for (int j = 0; j < GC; ++j) x[k][j] = rp[j];
             }
         }
 #pragma unroll
         for (int k = 0; k < GU; ++k) {
             if (sidx[k] < 0) continue;
 #pragma unroll
-            for (int j = 0; j < 6; ++j) val[j] += w0[k] * x[k][j];
+            for (int j = 0; j < GC; ++j) val[j] += w0[k] * x[k][j];
         }
     }
     const int ell = clink[c];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < GC; ++k) {
         const int a = a0 + k;
         cs[((size_t)0 * NC + a) * L + ell] = val[k];
         cs[((size_t)1 * NC + a) * L + ell] = gx[k];
@@ -1003,7 +998,7 @@ static int run_boundary(vlbm3d_handle *h, hipStream_t st, const T *src, T *dst, 
                                d.char_pack, d.n_active, d.n_char_nodes);
             CHECK_LAUNCH("k_pack");
         }
-        const int g = xcd_grid((int)((size_t)d.n_char_links * 2));
+        const int g = xcd_grid((int)((size_t)d.n_char_links * GLANES));
         hipLaunchKernelGGL(k_gather, dim3(g), dim3(BLOCK), 0, st, (const double *)d.char_pack, d.char_link,
                            d.char_gidx, d.char_gw, d.char_pidx, d.char_pw, d.char_state, d.n_links,
                            d.n_char_links);
