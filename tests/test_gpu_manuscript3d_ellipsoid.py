@@ -1,0 +1,133 @@
+"""End-to-end accuracy of the 3D HIP path against the reference's published
+rotated-ellipsoid Dirichlet study (validation/manuscript_metrics.json,
+dirichlet_cases.ellipsoid; case driver cases/affine_ellipsoid.py, run by
+cases/reproduce.py with --omega 1.8 --init-order 1 --target-max-u 0.1
+--boundary-reconstruction local_f and both local kinematics off).
+
+The exact solution is the affine velocity v = A (X - C), u = t v,
+F = I + t A on the rotated ellipsoid (semi-axes 0.34, 0.25, 0.18 rotated by
+Rz(25 deg) Ry(18 deg) about the box centre), imposed as a Dirichlet velocity
+on every cut link.
+
+Bar: steps exact, relative l2 errors of u, v and F within 5e-5 relative of
+the published values and min/max J within 1e-6 (the reference's release
+check uses three significant digits).  Measured at n = 80: 1.8e-5, 3.1e-5,
+2.7e-5 and 1e-8, 2.6e-7.  The remaining difference is not in this path: the
+CPU oracle, run with the reference's current in-place initial
+reconstruction (frozen source populations, validation/
+initialization_buffer_fix.json and tests/test_initial_boundary_buffers.py
+of the reference), gives the HIP result to 12 digits.  The published rows
+of this case start from a non-zero affine velocity, so they depend on the
+order of reads and writes of the in-place initial reconstruction that the
+frozen-source change removed (the traction cases start from rest, where the
+order does not matter, and reproduce to 1e-10); an order-dependent GPU race
+cannot be reproduced exactly."""
+
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from curved_lbm.three_d import curved_boundary_warp3d as cb
+
+pytestmark = pytest.mark.gpu
+
+PUBLISHED = json.loads((Path(__file__).parent / "golden" / "published_metrics.json").read_text())["ellipsoid"]
+
+CENTER = np.array([0.5, 0.5, 0.5])
+AXES = np.array([0.34, 0.25, 0.18])
+A = np.array([[0.00, 0.80, 0.00], [0.00, 0.00, -0.60], [0.45, 0.00, 0.08]])
+
+
+def _rotation():
+    rz, ry = math.radians(25.0), math.radians(18.0)
+    Rz = np.array([[math.cos(rz), -math.sin(rz), 0.0], [math.sin(rz), math.cos(rz), 0.0], [0.0, 0.0, 1.0]])
+    Ry = np.array([[math.cos(ry), 0.0, math.sin(ry)], [0.0, 1.0, 0.0], [-math.sin(ry), 0.0, math.cos(ry)]])
+    return Rz @ Ry
+
+
+ROT = _rotation()
+
+
+def _local(X, Y, Z):
+    X, Y, Z = np.broadcast_arrays(X, Y, Z)
+    d = np.stack([X - CENTER[0], Y - CENTER[1], Z - CENTER[2]], axis=0)
+    return np.einsum("ab,b...->a...", ROT.T, d), X.ndim
+
+
+def phi_v(X, Y, Z):
+    xi, nd = _local(X, Y, Z)
+    return np.sqrt(np.sum((xi / AXES[(slice(None),) + (None,) * nd]) ** 2, axis=0)) - 1.0
+
+
+def normal_v(X, Y, Z):
+    xi, nd = _local(X, Y, Z)
+    ax = AXES[(slice(None),) + (None,) * nd]
+    s = np.maximum(np.sqrt(np.sum((xi / ax) ** 2, axis=0)), 1.0e-30)
+    g = np.einsum("ab,b...->a...", ROT, xi / (ax ** 2 * s[None, ...]))
+    nrm = np.maximum(np.sqrt(np.sum(g * g, axis=0)), 1.0e-30)
+    g = g / nrm[None, ...]
+    return g[0], g[1], g[2]
+
+
+def phi_s(x, y, z):
+    xi = ROT.T @ (np.array([x, y, z]) - CENTER)
+    return float(np.sqrt(np.sum((xi / AXES) ** 2)) - 1.0)
+
+
+def normal_s(x, y, z):
+    xi = ROT.T @ (np.array([x, y, z]) - CENTER)
+    s = max(float(np.sqrt(np.sum((xi / AXES) ** 2))), 1.0e-30)
+    g = ROT @ (xi / (AXES * AXES * s))
+    g /= max(float(np.linalg.norm(g)), 1.0e-30)
+    return float(g[0]), float(g[1]), float(g[2])
+
+
+def max_surface_speed(n_theta=200, n_phi=400):
+    th, ph = np.meshgrid(np.linspace(0.0, np.pi, n_theta), np.linspace(0.0, 2.0 * np.pi, n_phi), indexing="ij")
+    loc = np.stack([AXES[0] * np.sin(th) * np.cos(ph), AXES[1] * np.sin(th) * np.sin(ph), AXES[2] * np.cos(th)])
+    x0 = CENTER[:, None, None] + np.einsum("ab,bmn->amn", ROT, loc)
+    v = np.einsum("ab,bmn->amn", A, x0 - CENTER[:, None, None])
+    return float(np.max(np.sqrt(np.sum(v * v, axis=0))))
+
+
+def rel_l2(err, ref):
+    return float(np.sqrt(np.sum(err ** 2)) / max(np.sqrt(np.sum(ref ** 2)), np.finfo(float).eps))
+
+
+@pytest.mark.parametrize("n", [80, 120])
+def test_gpu_ellipsoid_reproduces_published_metrics(n):
+    pytest.importorskip("torch")
+    pub = PUBLISHED[str(n)]
+    geom = cb.CurvedBoundaryGeometry3D.from_level_set(
+        nx=n, ny=n, nz=n, length_x=1.0, length_y=1.0, length_z=1.0, phi=phi_s, normal=normal_s,
+        phi_vectorized=phi_v, normal_vectorized=normal_v, label="rotated_ellipsoid_affine_dirichlet")
+    params = tuple(A.reshape(-1).tolist() + (-A @ CENTER).tolist() + [0.0, 0.0, 0.0, 0.0])
+    spec = cb.CurvedBoundarySpec3D(kind="dirichlet", value_mode=cb.CURVED3D_VALUE_AFFINE_VELOCITY, params=params)
+    s = cb.WarpCurvedBoundaryLBM3D(
+        geom, material=cb.WarpHyperelasticMaterial3D.from_poisson("neo_hooke", 0.20, mu=1.0), boundary=spec,
+        lattice_speed=10.0, collision_omega=1.8, boundary_reconstruction="local_f",
+        local_displacement_interval=0, local_displacement_sweeps=1, local_displacement_relax=0.85,
+        local_displacement_boundary_weight=20.0, local_displacement_boundary_id=-1,
+        local_compatibility_interval=0, local_compatibility_blend=0.1)
+    X, Y, Z = s.active_coordinates()
+    U0 = np.zeros((12,) + X.shape)
+    U0[0:3] = np.einsum("ab,b...->a...", A, np.stack([X - CENTER[0], Y - CENTER[1], Z - CENTER[2]]))
+    U0[3] = U0[7] = U0[11] = 1.0
+    s.initialize_from_numpy(U=U0, displacement=np.zeros((3,) + X.shape), init_order=1)
+    s.run_until(0.10 / max_surface_speed())
+    assert s.steps == pub["steps"]
+    v_ref = np.einsum("ab,b...->a...", A, np.stack([X - CENTER[0], Y - CENTER[1], Z - CENTER[2]]))
+    u_ref = s.time * v_ref
+    F_ref = np.broadcast_to((np.eye(3) + s.time * A)[:, :, None], (3, 3, X.size))
+    U, u, F, J = s.active_system_state(), s.active_displacement(), s.active_deformation_gradient(), \
+        s.active_deformation_jacobian()
+    got = {"rel_l2_u": rel_l2(u - u_ref, u_ref), "rel_l2_v": rel_l2(U[:3] - v_ref, v_ref),
+           "rel_l2_F": rel_l2(F - F_ref, F_ref), "min_J": float(np.min(J)), "max_J": float(np.max(J))}
+    print(json.dumps({k: (v, pub[k], v / pub[k] - 1.0) for k, v in got.items()}, indent=1))
+    assert not s.invalid_state()
+    for k, v in got.items():
+        tol = 1e-6 if k.endswith("_J") else 5e-5
+        assert abs(v / pub[k] - 1.0) < tol, (k, v, pub[k])
