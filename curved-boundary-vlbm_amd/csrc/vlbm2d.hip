@@ -397,42 +397,57 @@ __device__ inline double interp_link(const double *U, const int32_t *act, size_t
 // Normal-probe value and LSQ gradients of U at every cut point
 // (cbw2d:1202-1221 as used at :2141-2160 and :2491-2503): 32 lanes per
 // link, lane m loads stencil sample m; the sums then run over m in stencil
-// order, one thread per (link, output).  Output cs (16, L): rows 0-3 the
-// value of F at the cut point, rows 4-9 d/dx and 10-15 d/dy of U.
+// order, one thread per (link, output).  Output cs (19, L): rows 0-3 the
+// value of F at the cut point, rows 4-9 d/dx and 10-15 d/dy of U, and with
+// `tangent` rows 16-17 the tangential derivative of u and row 18 the sum of
+// |w_t| (compat_bfl, cbw2d:2221-2230).
 constexpr int GL = 8;  // links per block of 256 threads
-__global__ __launch_bounds__(256) void k2_gather(const double *__restrict__ U, const int32_t *__restrict__ lsq_p,
-                                                 const double *__restrict__ wb, const double *__restrict__ wx,
-                                                 const double *__restrict__ wy, double *__restrict__ cs, int L,
-                                                 size_t N) {
-    __shared__ double sU[6][GL * 32];
-    __shared__ double sW[3][GL * 32];
+__global__ __launch_bounds__(256) void k2_gather(const double *__restrict__ U, const double *__restrict__ u,
+                                                 const int32_t *__restrict__ lsq_p, const double *__restrict__ wb,
+                                                 const double *__restrict__ wx, const double *__restrict__ wy,
+                                                 const double *__restrict__ wt, int tangent, double *__restrict__ cs,
+                                                 int L, size_t N) {
+    __shared__ double sU[8][GL * 32];
+    __shared__ double sW[4][GL * 32];
     const int t = threadIdx.x;
     const int ll = t >> 5, m = t & 31;
     const int ell = blockIdx.x * GL + ll;
-    double u6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, w3[3] = {0.0, 0.0, 0.0};
+    double u8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, w4[4] = {0.0, 0.0, 0.0, 0.0};
     if (ell < L && m < MAX_LSQ) {
         const int pp = lsq_p[(size_t)m * L + ell];
         if (pp >= 0) {
 #pragma unroll
-            for (int a = 0; a < 6; ++a) u6[a] = U[a * N + pp];
-            w3[0] = wb[(size_t)m * L + ell];
-            w3[1] = wx[(size_t)m * L + ell];
-            w3[2] = wy[(size_t)m * L + ell];
+            for (int a = 0; a < 6; ++a) u8[a] = U[a * N + pp];
+            w4[0] = wb[(size_t)m * L + ell];
+            w4[1] = wx[(size_t)m * L + ell];
+            w4[2] = wy[(size_t)m * L + ell];
+            if (tangent) {
+                u8[6] = u[pp];
+                u8[7] = u[N + pp];
+                w4[3] = wt[(size_t)m * L + ell];
+            }
         }
     }
 #pragma unroll
-    for (int a = 0; a < 6; ++a) sU[a][t] = u6[a];
+    for (int a = 0; a < 8; ++a) sU[a][t] = u8[a];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) sW[k][t] = w3[k];
+    for (int k = 0; k < 4; ++k) sW[k][t] = w4[k];
     __syncthreads();
-    if (t >= GL * 16) return;
-    const int l2 = t >> 4, v = t & 15;
+    const int nout = tangent ? 19 : 16;
+    if (t >= GL * nout) return;
+    const int l2 = t / nout, v = t - l2 * nout;
     const int e2 = blockIdx.x * GL + l2;
     if (e2 >= L) return;
-    const int set = v < 4 ? 0 : (v < 10 ? 1 : 2);
-    const int comp = v < 4 ? 2 + v : (v < 10 ? v - 4 : v - 10);
     double acc = 0.0;
-    for (int k = 0; k < MAX_LSQ; ++k) acc += sW[set][l2 * 32 + k] * sU[comp][l2 * 32 + k];
+    if (v < 16) {
+        const int set = v < 4 ? 0 : (v < 10 ? 1 : 2);
+        const int comp = v < 4 ? 2 + v : (v < 10 ? v - 4 : v - 10);
+        for (int k = 0; k < MAX_LSQ; ++k) acc += sW[set][l2 * 32 + k] * sU[comp][l2 * 32 + k];
+    } else if (v < 18) {
+        for (int k = 0; k < MAX_LSQ; ++k) acc += sW[3][l2 * 32 + k] * sU[6 + (v - 16)][l2 * 32 + k];
+    } else {
+        for (int k = 0; k < MAX_LSQ; ++k) acc += fabs(sW[3][l2 * 32 + k]);
+    }
     cs[(size_t)v * L + e2] = acc;
 }
 
@@ -474,6 +489,35 @@ __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             Fg[k] = valid ? cs[(size_t)k * L + ell] : interp_link(U, Tb.act, N, 2 + k, p, i, j, qo, eta, P);
+    } else if (P.recon == R_COMPAT_BFL) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) Fg[k] = interp_link(U, Tb.act, N, 2 + k, p, i, j, qo, eta, P);
+        // Geometry-compatible predictor (cbw2d:2206-2250): F n from the link
+        // predictor, F t = t + d_t u (traction links: the LSQ tangential
+        // derivative of u; Dirichlet links: the time integral of an affine
+        // velocity's tangential derivative, zero for other modes).
+        if (valid) {
+            const double tanx = -nyb, tany = nxb;
+            double dut0 = 0.0, dut1 = 0.0, wt_abs = 1.0;
+            if (kind == BC_NEU) {
+                dut0 = cs[(size_t)16 * L + ell];
+                dut1 = cs[(size_t)17 * L + ell];
+                wt_abs = cs[(size_t)18 * L + ell];
+            } else if (mode == V_AFFINE_VEL) {
+                double at = time;
+                if (prm[6] > 0.0 && at > prm[6]) at = prm[6];
+                dut0 = at * (prm[0] * tanx + prm[1] * tany);
+                dut1 = at * (prm[3] * tanx + prm[4] * tany);
+            }
+            if (wt_abs > 0.0) {
+                const double hn0 = Fg[0] * nxb + Fg[1] * nyb, hn1 = Fg[2] * nxb + Fg[3] * nyb;
+                const double gt0 = tanx + dut0, gt1 = tany + dut1;
+                Fg[0] = hn0 * nxb + gt0 * tanx;
+                Fg[1] = hn0 * nyb + gt0 * tany;
+                Fg[2] = hn1 * nxb + gt1 * tanx;
+                Fg[3] = hn1 * nyb + gt1 * tany;
+            }
+        }
     } else {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -887,12 +931,13 @@ template <typename T>
 static int run_links2(vlbm2d_handle *h, const T *src, T *dst, double t, int use_rec) {
     const vlbm2d_desc &d = h->d;
     if (d.n_links == 0) return 0;
-    const bool gather = d.recon == R_LOCALF || d.recon == R_LSQ_BFL;
+    const bool gather = d.recon == R_LOCALF || d.recon == R_LSQ_BFL || d.recon == R_COMPAT_BFL;
     if (gather) {
         TimedSlot tg(h->timing, h->stream, VLBM_SLOT_GATHER);
         const int gg = (d.n_links + GL - 1) / GL;
-        hipLaunchKernelGGL(k2_gather, dim3(gg), dim3(256), 0, h->stream, (const double *)d.U, d.lsq_p, d.lsq_wb,
-                           d.lsq_wx, d.lsq_wy, d.link_state, d.n_links, h->P.N);
+        hipLaunchKernelGGL(k2_gather, dim3(gg), dim3(256), 0, h->stream, (const double *)d.U, (const double *)d.u,
+                           d.lsq_p, d.lsq_wb, d.lsq_wx, d.lsq_wy, d.lsq_wt, (int)(d.recon == R_COMPAT_BFL),
+                           d.link_state, d.n_links, h->P.N);
         CHECK_LAUNCH("k2_gather");
     }
     TimedSlot ts(h->timing, h->stream, VLBM_SLOT_LINKS);
@@ -908,12 +953,13 @@ template <typename T>
 static int launch_links2(vlbm2d_handle *h, const Params2 &P, hipStream_t st, const T *f0, T *f1, double tb) {
     const vlbm2d_desc &d = h->d;
     if (d.n_links == 0) return 0;
-    const bool gather = d.recon == R_LOCALF || d.recon == R_LSQ_BFL;
+    const bool gather = d.recon == R_LOCALF || d.recon == R_LSQ_BFL || d.recon == R_COMPAT_BFL;
     if (gather) {
         TimedSlot tg(h->timing, st, VLBM_SLOT_GATHER);
         const int gg = (d.n_links + GL - 1) / GL;
-        hipLaunchKernelGGL(k2_gather, dim3(gg), dim3(256), 0, st, (const double *)d.U, d.lsq_p, d.lsq_wb,
-                           d.lsq_wx, d.lsq_wy, d.link_state, d.n_links, P.N);
+        hipLaunchKernelGGL(k2_gather, dim3(gg), dim3(256), 0, st, (const double *)d.U, (const double *)d.u,
+                           d.lsq_p, d.lsq_wb, d.lsq_wx, d.lsq_wy, d.lsq_wt, (int)(d.recon == R_COMPAT_BFL),
+                           d.link_state, d.n_links, P.N);
         CHECK_LAUNCH("k2_gather");
     }
     TimedSlot tl(h->timing, st, VLBM_SLOT_LINKS);
@@ -1096,8 +1142,9 @@ int vlbm2d_create(const vlbm2d_desc *desc, vlbm2d_handle **out) {
     if (desc->n_links < 0) return fail_msg("vlbm2d_create: n_links must be non-negative");
     if (desc->precision != VLBM_PREC_F64 && desc->precision != VLBM_PREC_F32)
         return fail_msg("vlbm2d_create: unknown precision");
-    if (desc->recon != R_BFL && desc->recon != R_LSQ_BFL && desc->recon != R_LOCALF && desc->recon != R_LOCALF_BFL)
-        return fail_msg("vlbm2d_create: reconstruction not provided (bfl, lsq_bfl, local_f, local_f_bfl)");
+    if (desc->recon != R_BFL && desc->recon != R_LSQ_BFL && desc->recon != R_COMPAT_BFL && desc->recon != R_LOCALF &&
+        desc->recon != R_LOCALF_BFL)
+        return fail_msg("vlbm2d_create: reconstruction not provided (bfl, lsq_bfl, compat_bfl, local_f, local_f_bfl)");
     if (desc->disp_sweeps < 1 || desc->disp_sweeps > 8) return fail_msg("vlbm2d_create: disp_sweeps must be in [1,8]");
     const void *req[] = {desc->U, desc->u, desc->u_star, desc->u_next, desc->record, desc->invalid,
                          desc->reduce_scratch, desc->active, desc->active_list, desc->anchor_off,
@@ -1111,6 +1158,7 @@ int vlbm2d_create(const vlbm2d_desc *desc, vlbm2d_handle **out) {
                               desc->link_state};
         for (const void *ptr : lreq)
             if (!ptr) return fail_msg("vlbm2d_create: a required link pointer is null");
+        if (desc->recon == R_COMPAT_BFL && !desc->lsq_wt) return fail_msg("vlbm2d_create: lsq_wt is null");
     }
     if (desc->n_anchor_links > 0 && !desc->anchor_ids) return fail_msg("vlbm2d_create: anchor_ids is null");
     vlbm2d_handle *h = new vlbm2d_handle;

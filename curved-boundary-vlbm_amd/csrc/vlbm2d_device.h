@@ -27,7 +27,7 @@ constexpr int MAX_LSQ = 24;
 enum { M_SVK = 0, M_NH = 1, M_LOGNH = 2, M_MR = 3, M_YEOH = 4, M_GENT = 5, M_SUGI = 6 };
 enum { BC_DIR = 1, BC_NEU = 2 };
 enum { SRC_NONE = 0, SRC_CONST = 1, SRC_DAMP = 2 };
-enum { R_BFL = 0, R_LSQ_BFL = 2, R_LOCALF = 12, R_LOCALF_BFL = 13 };
+enum { R_BFL = 0, R_LSQ_BFL = 2, R_COMPAT_BFL = 9, R_LOCALF = 12, R_LOCALF_BFL = 13 };
 enum { NJ_FD = 0, NJ_AC = 1, NS_ENERGY = 2 };
 enum {
     V_ZERO = 0, V_CONST = 1, V_RAMP = 2, V_SHEAR_X = 3, V_RICKER_Y = 4, V_SINE2_HOLD = 5,

@@ -16,7 +16,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libvlbm.so"
 
 ABI_VERSION = 4
-ABI2_VERSION = 2
+ABI2_VERSION = 3
 PREC_F64 = 0
 PREC_F32 = 1
 
@@ -171,6 +171,7 @@ class Vlbm2dDesc(C.Structure):
         ("lsq_wb", C.c_void_p),
         ("lsq_wx", C.c_void_p),
         ("lsq_wy", C.c_void_p),
+        ("lsq_wt", C.c_void_p),
         ("lsq_valid", C.c_void_p),
         ("link_state", C.c_void_p),
         ("kind_by_id", C.c_void_p),

@@ -76,8 +76,8 @@ BOUNDARY_RECON_COMPAT_NEE = 10
 BOUNDARY_RECON_LOCAL_CHAR2 = 11
 BOUNDARY_RECON_LOCAL_F = 12
 BOUNDARY_RECON_LOCAL_F_BFL = 13
-PROVIDED_RECONSTRUCTIONS = (BOUNDARY_RECON_BFL, BOUNDARY_RECON_LSQ_BFL, BOUNDARY_RECON_LOCAL_F,
-                            BOUNDARY_RECON_LOCAL_F_BFL)
+PROVIDED_RECONSTRUCTIONS = (BOUNDARY_RECON_BFL, BOUNDARY_RECON_LSQ_BFL, BOUNDARY_RECON_COMPAT_BFL,
+                            BOUNDARY_RECON_LOCAL_F, BOUNDARY_RECON_LOCAL_F_BFL)
 
 NORMAL_JACOBIAN_FINITE_DIFFERENCE = 0
 NORMAL_JACOBIAN_ACOUSTIC = 1
@@ -793,7 +793,7 @@ class WarpCurvedBoundaryLBM2D:
         if self.boundary_reconstruction not in PROVIDED_RECONSTRUCTIONS:
             raise ValueError(
                 f"boundary_reconstruction {self.boundary_reconstruction_name!r} is not provided "
-                "(bfl, lsq_bfl/normal_probe_bfl, local_f, local_f_bfl)")
+                "(bfl, lsq_bfl/normal_probe_bfl, compat_bfl, local_f, local_f_bfl)")
         self.boundary_stencil_radius = boundary_reconstruction_stencil_radius(self.boundary_reconstruction)
         self.normal_jacobian = normal_jacobian_id(normal_jacobian)
         self.normal_jacobian_name = normal_jacobian_name(self.normal_jacobian)
@@ -931,6 +931,7 @@ class WarpCurvedBoundaryLBM2D:
         self._lsq_wb_t = dev_f(np.ascontiguousarray(self.lsq_wb_np.reshape(L, -1).T))
         self._lsq_wx_t = dev_f(np.ascontiguousarray(self.lsq_wx_np.reshape(L, -1).T))
         self._lsq_wy_t = dev_f(np.ascontiguousarray(self.lsq_wy_np.reshape(L, -1).T))
+        self._lsq_wt_t = dev_f(np.ascontiguousarray(self.lsq_wt_np.reshape(L, -1).T))
         # Dirichlet displacement anchors (cbw2d:3288-3304): links of the
         # chosen boundary id, or of Dirichlet ids when that id is negative.
         bids = np.clip(np.asarray(g.link_boundary_id, np.int64), 0, MAX_BOUNDARY_IDS - 1)
@@ -966,7 +967,7 @@ class WarpCurvedBoundaryLBM2D:
         self._P = DeviceArray.zeros((4, nx, ny, 1), f64, dev)
         self._sigma = DeviceArray.zeros((4, nx, ny, 1), f64, dev)
         self.record = DeviceArray.zeros((12, nx, ny, 1), pop_dtype, dev)
-        self.link_state = DeviceArray.zeros((16, max(L, 1)), f64, dev)
+        self.link_state = DeviceArray.zeros((19, max(L, 1)), f64, dev)
         self.invalid = DeviceArray.zeros((1,), i32, dev)
         self._reduce_scratch = DeviceArray.zeros((_native.RED_SCRATCH,), f64, dev)
         self._stream = torch.cuda.current_stream(dev)
@@ -1000,7 +1001,7 @@ class WarpCurvedBoundaryLBM2D:
             "link_p": self.link_p, "link_q": self.link_q, "link_bid": self.link_boundary_id,
             "link_eta": self.link_eta, "link_xb": self.link_xb, "link_yb": self.link_yb, "link_nx": self.link_nx,
             "link_ny": self.link_ny, "lsq_p": self._lsq_p_t, "lsq_wb": self._lsq_wb_t, "lsq_wx": self._lsq_wx_t,
-            "lsq_wy": self._lsq_wy_t, "lsq_valid": self.lsq_valid, "link_state": self.link_state, "kind_by_id": self.boundary_kind_by_id,
+            "lsq_wy": self._lsq_wy_t, "lsq_wt": self._lsq_wt_t, "lsq_valid": self.lsq_valid, "link_state": self.link_state, "kind_by_id": self.boundary_kind_by_id,
             "mode_by_id": self.boundary_mode_by_id, "params_by_id": self.boundary_params_by_id,
             "dyn_tx": self.link_traction_x, "dyn_ty": self.link_traction_y,
         }

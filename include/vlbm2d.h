@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define VLBM2D_ABI_VERSION 2
+#define VLBM2D_ABI_VERSION 3
 #define VLBM2D_MAX_IDS 8
 #define VLBM2D_MAX_LSQ 24
 #define VLBM2D_RECORD 12
@@ -62,7 +62,8 @@ typedef struct {
     double dt, c, omega, dx, dy;
     int32_t source_mode;
     double source_params[6];
-    int32_t recon;         /* reference reconstruction id: 0 bfl, 2 lsq_bfl, 12 local_f, 13 local_f_bfl */
+    int32_t recon;         /* reference reconstruction id: 0 bfl, 2 lsq_bfl, 9 compat_bfl, 12 local_f,
+                              13 local_f_bfl */
     int32_t normal_solver; /* 0 finite-difference Newton, 1 acoustic Newton, 2 energy */
     /* local kinematics */
     int32_t disp_interval, disp_sweeps;
@@ -86,8 +87,10 @@ typedef struct {
     const double *link_eta, *link_xb, *link_yb, *link_nx, *link_ny;
     const int32_t *lsq_p;            /* (24, L) node index of each stencil sample, -1 if unused */
     const double *lsq_wb, *lsq_wx, *lsq_wy; /* (24, L) value, d/dx, d/dy weights */
+    const double *lsq_wt;            /* (24, L) tangential-derivative weights (compat_bfl) */
     const int32_t *lsq_valid;        /* (L) */
-    double *link_state;              /* (16, L) scratch: F at the cut point, dU/dx, dU/dy */
+    double *link_state;              /* (19, L) scratch: F at the cut point, dU/dx, dU/dy, and for
+                                        compat_bfl the tangential derivative of u and sum |w_t| */
     const int32_t *kind_by_id;       /* (8) */
     const int32_t *mode_by_id;       /* (8) */
     const double *params_by_id;      /* (8, 8) */

@@ -31,7 +31,7 @@
 enum { M_SVK = 0, M_NH = 1, M_LOGNH = 2, M_MR = 3, M_YEOH = 4, M_GENT = 5, M_SUGI = 6 };
 enum { BC_DIR = 1, BC_NEU = 2 };
 enum { SRC_NONE = 0, SRC_CONST = 1, SRC_DAMP = 2 };
-enum { R_BFL = 0, R_LSQ_BFL = 2, R_REG_BFL = 3, R_CHAR2 = 11, R_LOCALF = 12, R_LOCALF_BFL = 13 };
+enum { R_BFL = 0, R_LSQ_BFL = 2, R_REG_BFL = 3, R_COMPAT_BFL = 9, R_CHAR2 = 11, R_LOCALF = 12, R_LOCALF_BFL = 13 };
 enum { NJ_FD = 0, NJ_AC = 1, NS_ENERGY = 2 };
 enum {
     V_ZERO = 0, V_CONST = 1, V_RAMP = 2, V_SHEAR_X = 3, V_RICKER_Y = 4, V_SINE2_HOLD = 5,
@@ -399,7 +399,7 @@ static double lsq_at(const or2d_state *s, const double *w, int ell, int comp, do
 }
 
 static inline int recon_supported(int r) {
-    return r == R_BFL || r == R_LSQ_BFL || r == R_LOCALF || r == R_LOCALF_BFL;
+    return r == R_BFL || r == R_LSQ_BFL || r == R_COMPAT_BFL || r == R_LOCALF || r == R_LOCALF_BFL;
 }
 
 /* cbw2d:1692-1843 (_collide_masked_kernel), f0 -> fpost. */
@@ -729,8 +729,8 @@ static void stream_push(or2d_state *s) {
 }
 
 /* cbw2d:2064-3098 (_apply_curved_boundary_links_kernel) for the bfl,
- * lsq_bfl, local_f and local_f_bfl reconstructions: reads fpost, writes the
- * incoming population of every cut link into f1. */
+ * lsq_bfl, compat_bfl, local_f and local_f_bfl reconstructions: reads fpost,
+ * writes the incoming population of every cut link into f1. */
 static void apply_links(or2d_state *s, const double *fpost, double *f1, double time_mid) {
     const int nx = s->nx, ny = s->ny, rid = s->recon;
     const double dt = s->dt, c = s->c, om = s->omega, jf = s->j_floor;
@@ -767,6 +767,41 @@ static void apply_links(or2d_state *s, const double *fpost, double *f1, double t
         }
         if (rid == R_CHAR2 || rid == R_LOCALF)
             for (int k = 0; k < 4; ++k) Fg[k] = lsq_at(s, s->lsq_wb, ell, 2 + k, Fg[k]);
+        /* cbw2d:2206-2250: geometry-compatible predictor of compat_bfl.  F n
+         * from the link predictor, F t = t + d_t u (traction links: the LSQ
+         * tangential derivative of u; Dirichlet links: the time integral of
+         * an affine velocity's tangential derivative, zero otherwise). */
+        if (rid == R_COMPAT_BFL && s->lsq_valid[ell] != 0) {
+            const double tanx = -nyb, tany = nxb;
+            double dut0 = 0.0, dut1 = 0.0, wt_abs = 0.0;
+            if (kind == BC_NEU) {
+                for (int m = 0; m < MAX_LSQ; ++m) {
+                    const int ii = s->lsq_i[(size_t)ell * MAX_LSQ + m], jj = s->lsq_j[(size_t)ell * MAX_LSQ + m];
+                    if (ii >= 0 && jj >= 0) {
+                        const double ww = s->lsq_wt[(size_t)ell * MAX_LSQ + m];
+                        dut0 += ww * s->u[IDX(s, 0, ii, jj)];
+                        dut1 += ww * s->u[IDX(s, 1, ii, jj)];
+                        wt_abs += fabs(ww);
+                    }
+                }
+            } else {
+                if (mode == V_AFFINE_VEL) {
+                    double at = time_mid;
+                    if (p[6] > 0.0 && at > p[6]) at = p[6];
+                    dut0 = at * (p[0] * tanx + p[1] * tany);
+                    dut1 = at * (p[3] * tanx + p[4] * tany);
+                }
+                wt_abs = 1.0;
+            }
+            if (wt_abs > 0.0) {
+                const double hn0 = Fg[0] * nxb + Fg[1] * nyb, hn1 = Fg[2] * nxb + Fg[3] * nyb;
+                const double gt0 = tanx + dut0, gt1 = tany + dut1;
+                Fg[0] = hn0 * nxb + gt0 * tanx;
+                Fg[1] = hn0 * nyb + gt0 * tany;
+                Fg[2] = hn1 * nxb + gt1 * tanx;
+                Fg[3] = hn1 * nyb + gt1 * tany;
+            }
+        }
         double Fb[4] = {Fg[0], Fg[1], Fg[2], Fg[3]};
         if (kind == BC_NEU) or2d_solve_normal(s, Fg, nxb, nyb, bv0, bv1, Fb);
         double Pb[4];

@@ -42,6 +42,7 @@ CASES = {
     "annulus_kind_anchors": lambda: gc.annulus(36, local_displacement_boundary_id=-1),
     "annulus_local_f_bfl": lambda: gc.annulus(36, boundary_reconstruction="local_f_bfl"),
     "annulus_lsq_bfl": lambda: gc.annulus(36, boundary_reconstruction="lsq_bfl"),
+    "annulus_compat_bfl": lambda: gc.annulus(36, boundary_reconstruction="compat_bfl"),
     "annulus_mooney": lambda: gc.annulus(32, model="mooney_rivlin"),
     "annulus_svk": lambda: gc.annulus(32, model="svk"),
     "annulus_yeoh": lambda: gc.annulus(32, model="yeoh"),
@@ -52,6 +53,7 @@ CASES = {
                                              params=(0.1, 0.2, 0.0)),
     "circle_bfl_constant_velocity": lambda: gc.circle_dirichlet(32, "bfl"),
     "circle_bfl_affine_velocity": lambda: gc.circle_dirichlet(32, "bfl", affine=True),
+    "circle_compat_bfl_affine_velocity": lambda: gc.circle_dirichlet(32, "compat_bfl", affine=True),
     "circle_piola_traction_constant_source": lambda: gc.circle_traction(32),
 }
 

@@ -12,16 +12,16 @@ on every cut link.
 Bar: steps exact, relative l2 errors of u, v and F within 5e-5 relative of
 the published values and min/max J within 1e-6 (the reference's release
 check uses three significant digits).  Measured at n = 80: 1.8e-5, 3.1e-5,
-2.7e-5 and 1e-8, 2.6e-7.  The remaining difference is not in this path: the
-CPU oracle, run with the reference's current in-place initial
-reconstruction (frozen source populations, validation/
-initialization_buffer_fix.json and tests/test_initial_boundary_buffers.py
-of the reference), gives the HIP result to 12 digits.  The published rows
-of this case start from a non-zero affine velocity, so they depend on the
-order of reads and writes of the in-place initial reconstruction that the
-frozen-source change removed (the traction cases start from rest, where the
-order does not matter, and reproduce to 1e-10); an order-dependent GPU race
-cannot be reproduced exactly."""
+2.7e-5 and 1e-8, 2.6e-7; at n = 120: 1.6e-6, 7.8e-8, 5.7e-6.  The
+CPU oracle gives the HIP result to 12 digits, so the gap is between this
+restatement and the published run, not in the HIP path; its cause is not
+identified.  One candidate is the in-place initial boundary reconstruction
+(this case starts from a non-zero affine velocity, and the reference
+changed that reconstruction to read frozen source populations,
+validation/initialization_buffer_fix.json); a sequential in-place variant
+moves the oracle further away (1e-3), so the published rows do not follow
+that ordering either.  The 2D affine-annulus Dirichlet study reproduces to
+1e-13 (test_gpu_manuscript2d.py)."""
 
 import json
 import math

@@ -212,3 +212,35 @@ def test_unsupported_reconstruction_is_rejected():
     with pytest.raises(ValueError):
         OracleSolver2D(g, material=NH, boundary=cb.CurvedBoundarySpec(kind="neumann"), lattice_speed=10.0,
                        collision_omega=1.8, boundary_reconstruction="qbfl")
+
+
+def test_compat_projection_keeps_a_compatible_affine_state():
+    """compat_bfl replaces the boundary F by F n n^T + (t + d_t u) t^T
+    (cbw2d:2206-2250).  For a uniform F = I + G with u = G (x - c) the state
+    is compatible (F t = t + d_t u), so the projection must return F itself
+    and the traction links must equal those of bfl (same F predictor for a
+    uniform field); a wrong tangent or a swapped component would not."""
+    g = cb.CurvedBoundaryGeometry.annulus(nx=40, ny=40, inner_radius=0.2, outer_radius=0.45)
+    G = np.array([[0.012, 0.021], [-0.017, 0.006]])
+    spec = cb.CurvedBoundarySpec(kind="neumann", value_mode=cb.CURVED_VALUE_PIOLA_NORMAL,
+                                 params=(0.01, 0.004, 0.004, -0.006),
+                                 id_kinds={1: "neumann", 2: "dirichlet"},
+                                 id_value_modes={1: cb.CURVED_VALUE_PIOLA_NORMAL, 2: 0},
+                                 id_params={1: (0.01, 0.004, 0.004, -0.006), 2: ()})
+    x = (np.arange(40) + 0.5) / 40
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    U0 = np.zeros((6, 40, 40))
+    U0[2], U0[3], U0[4], U0[5] = 1.0 + G[0, 0], G[0, 1], G[1, 0], 1.0 + G[1, 1]
+    disp = np.stack([G[0, 0] * (X - 0.5) + G[0, 1] * (Y - 0.5), G[1, 0] * (X - 0.5) + G[1, 1] * (Y - 0.5)])
+    out = {}
+    for recon in ("bfl", "compat_bfl"):
+        s = OracleSolver2D(g, material=NH, boundary=spec, boundary_reconstruction=recon, lattice_speed=10.0,
+                           collision_omega=1.8)
+        s.initialize_from_numpy(U=U0, displacement=disp, apply_initial_boundaries=False)
+        s.fpost[...] = s.f0
+        s.f1[...] = np.nan
+        s.apply_boundary_reconstruction(0.01, in_place=False)
+        out[recon] = s.f1.copy()
+    written = np.isfinite(out["bfl"])
+    assert written.sum() > 100 and np.array_equal(written, np.isfinite(out["compat_bfl"]))
+    assert np.max(np.abs(out["bfl"][written] - out["compat_bfl"][written])) < 1e-12
