@@ -48,7 +48,7 @@ __device__ __forceinline__ int cy(int q) { return q == 1 ? 1 : (q == 3 ? -1 : 0)
 
 __device__ __forceinline__ double clamp_j(double j, double f) { return j < f ? f : j; }
 
-__device__ inline void piola2(const double *F, const Material2 &m, double *P) {
+__device__ __forceinline__ void piola2(const double *F, const Material2 &m, double *P) {
     const double F11 = F[0], F12 = F[1], F21 = F[2], F22 = F[3];
     const double lam = m.lam, mu = m.mu;
     if (m.model == M_SVK) {
@@ -143,7 +143,7 @@ __device__ __forceinline__ double smooth5(double z) {
 // Boundary datum of component comp: a velocity for Dirichlet links, a
 // nominal traction for Neumann links.  Mode 105 is nominal traction
 // amp * min(t / t_ramp, 1) * n with params (amp, t_ramp).
-__device__ inline double value2(int comp, int mode, const double *p, double x, double y, double nxb,
+__device__ __forceinline__ double value2(int comp, int mode, const double *p, double x, double y, double nxb,
                                 double nyb, double t) {
     switch (mode) {
     case CV_PIOLA_N:
@@ -217,7 +217,7 @@ __device__ inline double value2(int comp, int mode, const double *p, double x, d
     }
 }
 
-__device__ inline void normal_residual2(double hn0, double hn1, double gt0, double gt1, double nxb, double nyb,
+__device__ __forceinline__ void normal_residual2(double hn0, double hn1, double gt0, double gt1, double nxb, double nyb,
                                         double tx, double ty, const Material2 &m, double *r) {
     const double t1 = -nyb, t2 = nxb;
     const double F[4] = {hn0 * nxb + gt0 * t1, hn0 * nyb + gt0 * t2, hn1 * nxb + gt1 * t1, hn1 * nyb + gt1 * t2};
@@ -231,7 +231,7 @@ __device__ inline void normal_residual2(double hn0, double hn1, double gt0, doub
 // (closed-form energy minimiser for neo-Hooke when selected, else Newton
 // with the acoustic tensor or forward differences, 16 iterations, tol 1e-11,
 // 12 line-search halvings keeping J above the floor).
-__device__ inline void solve_normal2(const double *Fg, double nxb, double nyb, double tx, double ty, int solver,
+__device__ __forceinline__ void solve_normal2(const double *Fg, double nxb, double nyb, double tx, double ty, int solver,
                                      const Material2 &m, double *Fb) {
     const double jf = m.jf;
     if (solver == NS_ENERGY && m.model == M_NH) {
@@ -316,7 +316,7 @@ __device__ inline void solve_normal2(const double *Fg, double nxb, double nyb, d
 
 // Central-difference directional derivative of the axis flux of every
 // component at U along W, step 1e-6 / max(1, max|W|).
-__device__ inline void flux_jvp2(int axis, const double *U, const double *W, const Material2 &m, double *out) {
+__device__ __forceinline__ void flux_jvp2(int axis, const double *U, const double *W, const Material2 &m, double *out) {
     double sc = fabs(W[0]);
 #pragma unroll
     for (int k = 1; k < 6; ++k) sc = fabs(W[k]) > sc ? fabs(W[k]) : sc;
