@@ -6,6 +6,10 @@
 //                             of the reference): weighted quadratic LSQ
 //                             gradients over <= 48 nodes and a normal-probe /
 //                             link-extrapolation predictor over <= 16 nodes.
+//   vlbm_geom3d_tangent_weights  the two surface-tangent derivative stencils
+//                             of the D3Q6x12 compat_bfl closure
+//                             (_build_sparse_tangent_lsq_weights,
+//                             curved_boundary_warp3d.py:3725-3849).
 //   vlbm_geom2d_lsq_weights   normal-probe value weights and quadratic LSQ
 //                             tangential / Cartesian derivative weights of the
 //                             D2Q4x6 solver (curved_boundary_warp.py:332-606).
@@ -396,6 +400,103 @@ int build_link3(const Grid3 &g, const Links3 &Lk, int64_t ell, int64_t L, double
     return 0;
 }
 
+// Tangent-derivative stencil of one 3D cut link: weighted quadratic LSQ
+// over at most `S` active nodes within r^2 <= (radius + 1/4)^2 of the cut
+// point (radius 3, then 4, then 5), Gaussian weights exp(-r^2/(2*1.75^2)) +
+// 1e-4, rank >= 10 (singular values above 1e-10), cond <= 1e9, and the
+// derivative rows along t1, t2 of the normal's basis (t1 = n x e_x, or n x e_y
+// when |n_x| >= 0.75; t2 = n x t1).  Returns the sample count, 0 when no
+// radius gives an acceptable fit.
+int tangent_link3(const Grid3 &g, int i0, int j0, int k0, double xb, double yb, double zb, const double nin[3],
+                  double dx, double dy, double dz, int S, int32_t *op, double *ow1, double *ow2) {
+    double nrm = std::sqrt(nin[0] * nin[0] + nin[1] * nin[1] + nin[2] * nin[2]);
+    if (!(nrm > std::numeric_limits<double>::epsilon())) nrm = std::numeric_limits<double>::epsilon();
+    const double n[3] = {nin[0] / nrm, nin[1] / nrm, nin[2] / nrm};
+    const double ref[3] = {std::fabs(n[0]) < 0.75 ? 1.0 : 0.0, std::fabs(n[0]) < 0.75 ? 0.0 : 1.0, 0.0};
+    double t1[3] = {n[1] * ref[2] - n[2] * ref[1], n[2] * ref[0] - n[0] * ref[2], n[0] * ref[1] - n[1] * ref[0]};
+    double r1 = std::sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
+    if (!(r1 > std::numeric_limits<double>::epsilon())) r1 = std::numeric_limits<double>::epsilon();
+    for (double &v : t1) v = v / r1;
+    const double t2[3] = {n[1] * t1[2] - n[2] * t1[1], n[2] * t1[0] - n[0] * t1[2], n[0] * t1[1] - n[1] * t1[0]};
+    struct C {
+        double r2;
+        int32_t id;
+        double rx, ry, rz;
+    };
+    std::vector<C> cand;
+    std::vector<double> A, wd, M(100), B;
+    for (int radius = 3; radius <= 5; ++radius) {
+        cand.clear();
+        const double r2max = (double(radius) + 0.25) * (double(radius) + 0.25);
+        for (int ii = std::max(0, i0 - radius); ii < std::min(g.nx, i0 + radius + 1); ++ii)
+            for (int jj = std::max(0, j0 - radius); jj < std::min(g.ny, j0 + radius + 1); ++jj)
+                for (int kk = std::max(0, k0 - radius); kk < std::min(g.nz, k0 + radius + 1); ++kk) {
+                    const int32_t id = g.at(ii, jj, kk);
+                    if (id < 0) continue;
+                    const double rx = ((double(ii) + 0.5) * dx - xb) / dx;
+                    const double ry = ((double(jj) + 0.5) * dy - yb) / dy;
+                    const double rz = ((double(kk) + 0.5) * dz - zb) / dz;
+                    double r2 = rx * rx + ry * ry;
+                    r2 = r2 + rz * rz;
+                    if (r2 <= r2max) cand.push_back({r2, id, rx, ry, rz});
+                }
+        std::stable_sort(cand.begin(), cand.end(), [](const C &a, const C &b) { return a.r2 < b.r2; });
+        if (cand.size() < 10) continue;
+        const int m = std::min<int>((int)cand.size(), S);
+        A.assign((size_t)m * 10, 0.0);
+        wd.assign(m, 0.0);
+        for (int s = 0; s < m; ++s) {
+            const double x = cand[s].rx, y = cand[s].ry, z = cand[s].rz;
+            const double row[10] = {1.0, x, y, z, 0.5 * x * x, x * y, x * z, 0.5 * y * y, y * z, 0.5 * z * z};
+            for (int b = 0; b < 10; ++b) A[(size_t)s * 10 + b] = row[b];
+            wd[s] = std::exp(-0.5 * cand[s].r2 / (1.75 * 1.75)) + 1.0e-4;
+        }
+        double sv[10];
+        singular_values(A.data(), m, 10, sv);
+        int rank = 0;
+        for (double v : sv) rank += v > 1.0e-10 ? 1 : 0;
+        if (rank < 10) continue;
+        for (int i = 0; i < 10; ++i)
+            for (int j = 0; j < 10; ++j) {
+                double acc = 0.0;
+                for (int s = 0; s < m; ++s) acc += A[(size_t)s * 10 + i] * (A[(size_t)s * 10 + j] * wd[s]);
+                M[i * 10 + j] = acc;
+            }
+        const double cond = sym_cond(M.data(), 10);
+        if (!std::isfinite(cond) || cond > 1.0e9) continue;
+        for (int i = 0; i < 10; ++i) M[i * 10 + i] += 1.0e-12;
+        B.assign((size_t)10 * m, 0.0);
+        for (int i = 0; i < 10; ++i)
+            for (int s = 0; s < m; ++s) B[(size_t)i * m + s] = A[(size_t)s * 10 + i] * wd[s];
+        if (!lu_solve(M.data(), 10, B.data(), m)) continue;
+        const double g1[3] = {t1[0] / dx, t1[1] / dy, t1[2] / dz}, g2[3] = {t2[0] / dx, t2[1] / dy, t2[2] / dz};
+        const double h = std::max(dx, std::max(dy, dz));
+        bool ok = true;
+        double s1 = 0.0, s2 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int s = 0; s < m; ++s) {
+            const double w1 = g1[0] * B[(size_t)1 * m + s] + g1[1] * B[(size_t)2 * m + s] + g1[2] * B[(size_t)3 * m + s];
+            const double w2 = g2[0] * B[(size_t)1 * m + s] + g2[1] * B[(size_t)2 * m + s] + g2[2] * B[(size_t)3 * m + s];
+            ok = ok && std::isfinite(w1) && std::isfinite(w2);
+            s1 += w1;
+            s2 += w2;
+            a1 += std::fabs(w1);
+            a2 += std::fabs(w2);
+            ow1[s] = w1;
+            ow2[s] = w2;
+            op[s] = cand[s].id;
+        }
+        if (!ok || std::fabs(s1) * h > 1.0e-6 || std::fabs(s2) * h > 1.0e-6 || a1 * h > 12.0 || a2 * h > 12.0) {
+            for (int s = 0; s < m; ++s) {
+                op[s] = -1;
+                ow1[s] = ow2[s] = 0.0;
+            }
+            continue;
+        }
+        return m;
+    }
+    return 0;
+}
+
 // ---------------------------------------------------------------- 2D -----
 
 struct Grid2 {
@@ -643,6 +744,34 @@ int vlbm_geom3d_char_weights(const int32_t *active_id, int32_t nx, int32_t ny, i
             }
             modes[l] = 0;
             status[l] = (uint8_t)build_link3(g, Lk, l, n_links, dx, indices, weights, modes);
+        }
+    });
+    return 0;
+}
+
+int vlbm_geom3d_tangent_weights(const int32_t *active_id, int32_t nx, int32_t ny, int32_t nz, int64_t n_links,
+                                const int32_t *li, const int32_t *lj, const int32_t *lk, const double *xb,
+                                const double *yb, const double *zb, const double *lnx, const double *lny,
+                                const double *lnz, double dx, double dy, double dz, int32_t max_stencil,
+                                int32_t nthreads, int32_t *out_p, double *wt1, double *wt2, int32_t *valid) {
+    if (!active_id || nx <= 0 || ny <= 0 || nz <= 0 || n_links < 0 || max_stencil < 10 || !(dx > 0.0) ||
+        !(dy > 0.0) || !(dz > 0.0))
+        return -1;
+    if (n_links > 0 && (!li || !lj || !lk || !xb || !yb || !zb || !lnx || !lny || !lnz || !out_p || !wt1 || !wt2 ||
+                        !valid))
+        return -1;
+    const Grid3 g{active_id, nx, ny, nz};
+    const int S = max_stencil;
+    parallel_for(n_links, thread_count(nthreads, n_links), [&](int64_t b, int64_t e) {
+        for (int64_t l = b; l < e; ++l) {
+            int32_t *op = out_p + l * S;
+            double *w1 = wt1 + l * S, *w2 = wt2 + l * S;
+            for (int s = 0; s < S; ++s) {
+                op[s] = -1;
+                w1[s] = w2[s] = 0.0;
+            }
+            const double nv[3] = {lnx[l], lny[l], lnz[l]};
+            valid[l] = tangent_link3(g, li[l], lj[l], lk[l], xb[l], yb[l], zb[l], nv, dx, dy, dz, S, op, w1, w2) > 0;
         }
     });
     return 0;

@@ -558,11 +558,14 @@ struct LinkTables {
     int32_t *lkind, *lck;
     double *lbv, *lFb, *lPb;
     double *cs;
+    const int32_t *tp, *tvalid;  // compat_bfl tangent stencils (MAX_TAN, L)
+    const double *tw1, *tw2;
 };
 
 // Boundary values, F predictor and the normal-traction solve per link
 // (cbw3d:2888-3151).
-__global__ __launch_bounds__(128) void k_prepare(const double *__restrict__ U, LinkTables T, double t, Params P) {
+__global__ __launch_bounds__(128) void k_prepare(const double *__restrict__ U, const double *__restrict__ uu,
+                                                 LinkTables T, double t, Params P) {
     const int ell = blockIdx.x * 128 + threadIdx.x;
     if (ell >= P.L) return;
     const size_t n = P.n, L = P.L;
@@ -592,12 +595,54 @@ __global__ __launch_bounds__(128) void k_prepare(const double *__restrict__ U, L
         double F[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) F[k] = U[(3 + k) * n + p];
-        if (P.recon == R_LOCALF_BFL) {
+        if (P.recon == R_LOCALF_BFL || P.recon == R_COMPAT) {
             const int up = T.lup[ell];
             if (up >= 0) {
                 const double eta = T.eta[ell];
 #pragma unroll
                 for (int k = 0; k < 9; ++k) F[k] = F[k] + eta * (F[k] - U[(3 + k) * n + up]);
+            }
+        }
+        if (P.recon == R_COMPAT && T.tvalid[ell] != 0) {
+            // Geometry-compatible boundary F (cbw3d:3005-3049): F n kept, the
+            // tangential images replaced by t_i + d u / d t_i from the
+            // tangent stencils.
+            double nn[3] = {nv[0], nv[1], nv[2]};
+            double r = sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+            if (r < 1.0e-30) r = 1.0e-30;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) nn[c] = nn[c] / r;
+            const bool alt = fabs(nn[0]) >= 0.75;
+            const double ref[3] = {alt ? 0.0 : 1.0, alt ? 1.0 : 0.0, 0.0};
+            double t1[3] = {nn[1] * ref[2] - nn[2] * ref[1], nn[2] * ref[0] - nn[0] * ref[2],
+                            nn[0] * ref[1] - nn[1] * ref[0]};
+            double r1 = sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
+            if (r1 < 1.0e-30) r1 = 1.0e-30;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) t1[c] = t1[c] / r1;
+            const double t2[3] = {nn[1] * t1[2] - nn[2] * t1[1], nn[2] * t1[0] - nn[0] * t1[2],
+                                  nn[0] * t1[1] - nn[1] * t1[0]};
+            double du1[3] = {0.0, 0.0, 0.0}, du2[3] = {0.0, 0.0, 0.0}, wt_abs = 0.0;
+            for (int m = 0; m < MAX_TAN; ++m) {
+                const int pp = T.tp[(size_t)m * L + ell];
+                if (pp < 0) continue;
+                const double w1 = T.tw1[(size_t)m * L + ell], w2 = T.tw2[(size_t)m * L + ell];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double uc = uu[(size_t)c * n + pp];
+                    du1[c] += w1 * uc;
+                    du2[c] += w2 * uc;
+                }
+                wt_abs += fabs(w1) + fabs(w2);
+            }
+            if (wt_abs > 0.0) {
+#pragma unroll
+                for (int row = 0; row < 3; ++row) {
+                    const double h = F[3 * row] * nn[0] + F[3 * row + 1] * nn[1] + F[3 * row + 2] * nn[2];
+                    const double g1 = t1[row] + du1[row], g2 = t2[row] + du2[row];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) F[3 * row + c] = h * nn[c] + g1 * t1[c] + g2 * t2[c];
+                }
             }
         }
         if (P.recon == R_LOCALF)
@@ -976,6 +1021,10 @@ static LinkTables make_tables(const vlbm3d_desc &d) {
     T.lFb = d.lFb;
     T.lPb = d.lPb;
     T.cs = d.char_state;
+    T.tp = d.tan_p;
+    T.tw1 = d.tan_w1;
+    T.tw2 = d.tan_w2;
+    T.tvalid = d.tan_valid;
     return T;
 }
 
@@ -1007,7 +1056,8 @@ static int run_boundary(vlbm3d_handle *h, hipStream_t st, const T *src, T *dst, 
     const int gl = (d.n_links + 127) / 128;
     {
         TimedSlot ts(h->timing, st, VLBM_SLOT_PREPARE);
-        hipLaunchKernelGGL(k_prepare, dim3(gl), dim3(128), 0, st, (const double *)d.U, h->T, t, h->P);
+        hipLaunchKernelGGL(k_prepare, dim3(gl), dim3(128), 0, st, (const double *)d.U, (const double *)d.u, h->T, t,
+                           h->P);
         CHECK_LAUNCH("k_prepare");
     }
     TimedSlot ts(h->timing, st, VLBM_SLOT_LINKS);
@@ -1125,7 +1175,8 @@ int vlbm3d_create(const vlbm3d_desc *desc, vlbm3d_handle **out) {
     if (desc->n_links < 0) return fail_msg("vlbm3d_create: n_links must be non-negative");
     if (desc->precision != VLBM_PREC_F64 && desc->precision != VLBM_PREC_F32)
         return fail_msg("vlbm3d_create: unknown precision");
-    if (desc->recon == R_COMPAT) return fail_msg("vlbm3d_create: compat_bfl reconstruction is not provided");
+    if (desc->recon == R_COMPAT && desc->n_links > 0 && (!desc->tan_p || !desc->tan_w1 || !desc->tan_w2 || !desc->tan_valid))
+        return fail_msg("vlbm3d_create: compat_bfl needs the tangent stencil tables");
     if (desc->recon < 0 || desc->recon > R_LOCALF_BFL) return fail_msg("vlbm3d_create: unknown reconstruction");
     if (desc->disp_sweeps < 1 || desc->disp_sweeps > 8) return fail_msg("vlbm3d_create: disp_sweeps must be in [1,8]");
     if ((size_t)desc->n_active * FQ >= (size_t)1 << 40) return fail_msg("vlbm3d_create: lattice too large");

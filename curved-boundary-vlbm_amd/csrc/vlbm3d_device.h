@@ -22,6 +22,7 @@ constexpr int NQ = 6;
 constexpr int FQ = 72;
 constexpr int MAX_IDS = 16;
 constexpr int MAX_CHAR = 64;
+constexpr int MAX_TAN = 32;  // compat_bfl tangent stencil length
 
 enum { M_SVK = 0, M_NH = 1, M_LOGNH = 2, M_MR = 3, M_YEOH = 4, M_GENT = 5, M_SPHNH = 6 };
 enum { BC_DIR = 1, BC_NEU = 2 };

@@ -15,7 +15,7 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libvlbm.so"
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 ABI2_VERSION = 3
 PREC_F64 = 0
 PREC_F32 = 1
@@ -98,6 +98,10 @@ class Vlbm3dDesc(C.Structure):
         ("char_pidx", C.c_void_p),
         ("char_pw", C.c_void_p),
         ("char_state", C.c_void_p),
+        ("tan_p", C.c_void_p),
+        ("tan_w1", C.c_void_p),
+        ("tan_w2", C.c_void_p),
+        ("tan_valid", C.c_void_p),
         ("kind_by_id", C.c_void_p),
         ("comp_kind_by_id", C.c_void_p),
         ("mode_by_id", C.c_void_p),
@@ -215,6 +219,7 @@ EXPORTED = (
     "vlbm2d_timing_read",
     "vlbm_geom3d_char_weights",
     "vlbm_geom2d_lsq_weights",
+    "vlbm_geom3d_tangent_weights",
     "vlbm_last_error",
     "vlbm_abi_version",
 )
@@ -273,6 +278,8 @@ def load_library(path: Path | None = None) -> C.CDLL:
         + [C.c_double, C.c_int32, vp, vp, vp, vp],
         "vlbm_geom2d_lsq_weights": [vp, C.c_int32, C.c_int32, vp, vp, C.c_int64] + [vp] * 6
         + [C.c_double, C.c_int32, C.c_int32] + [vp] * 7,
+        "vlbm_geom3d_tangent_weights": [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int64] + [vp] * 9
+        + [C.c_double] * 3 + [C.c_int32, C.c_int32] + [vp] * 4,
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)
@@ -348,3 +355,22 @@ def geom2d_lsq_weights(active, x_coords, y_coords, li, lj, xb, yb, nx, ny, dx, m
     if rc != 0:
         raise ValueError("vlbm_geom2d_lsq_weights: invalid arguments")
     return oi, oj, w, wt, wx, wy, valid
+
+
+def geom3d_tangent_weights(active_id, li, lj, lk, xb, yb, zb, nx, ny, nz, dx, dy, dz, max_stencil):
+    """Native compat_bfl tangent stencils (see include/vlbm_geom.h); returns
+    (lsq_p, lsq_wt1, lsq_wt2) as (L, max_stencil) arrays and lsq_valid (L)."""
+    lib = load_library()
+    aid = np.ascontiguousarray(active_id, dtype=np.int32)
+    ints = [np.ascontiguousarray(a, dtype=np.int32) for a in (li, lj, lk)]
+    dbl = [np.ascontiguousarray(a, dtype=np.float64) for a in (xb, yb, zb, nx, ny, nz)]
+    L, S = int(ints[0].size), int(max_stencil)
+    out_p = np.full((L, S), -1, np.int32)
+    w1, w2 = np.zeros((L, S)), np.zeros((L, S))
+    valid = np.zeros(L, np.int32)
+    rc = lib.vlbm_geom3d_tangent_weights(_ptr(aid), *[int(v) for v in aid.shape], L, *[_ptr(a) for a in ints],
+                                         *[_ptr(a) for a in dbl], float(dx), float(dy), float(dz), S,
+                                         host_threads(), _ptr(out_p), _ptr(w1), _ptr(w2), _ptr(valid))
+    if rc != 0:
+        raise ValueError("vlbm_geom3d_tangent_weights: invalid arguments")
+    return out_p, w1, w2, valid

@@ -12,8 +12,7 @@ deformation-gradient update couple the kinematic fields.
 Host work here is geometry preprocessing (level set -> compacted cut-link
 lists, neighbour tables, characteristic weights) and orchestration.  Every
 time step runs in libvlbm.so (hand-written HIP for gfx950) on device arrays
-held as torch tensors.  The ``compat_bfl`` reconstruction (tangent least
-squares) is not provided.
+held as torch tensors.
 """
 
 from __future__ import annotations
@@ -572,6 +571,94 @@ def _sparse_tables_from_geometry(geometry: CurvedBoundaryGeometry3D) -> dict[str
     }
 
 
+def build_tangent_lsq_weights(geometry, tables, *, max_stencil: int = MAX_LSQ_STENCIL, builder: str = "native"):
+    """Surface-tangent derivative stencils of the compat_bfl closure
+    (_build_sparse_tangent_lsq_weights, cbw3d:3725-3849): per cut link the
+    nearest active nodes within (r + 1/4) cells of the cut point, r = 3, 4, 5
+    in turn, a Gaussian-weighted quadratic least-squares fit (rank 10,
+    cond <= 1e9), and the derivative weights along t1, t2 of the normal's
+    basis.  ``builder="native"`` runs the threaded C++ builder of libvlbm,
+    ``"numpy"`` the restatement below (same samples and validity; weights
+    equal to rounding).  Returns ``lsq_p`` (active-node index), ``lsq_wt1``,
+    ``lsq_wt2`` as (L, max_stencil) and ``lsq_valid`` (L)."""
+    g = geometry
+    if builder == "native":
+        from .. import _native
+
+        p, w1, w2, valid = _native.geom3d_tangent_weights(
+            tables["active_id"], g.link_i, g.link_j, g.link_k, g.link_xb, g.link_yb, g.link_zb, g.link_nx,
+            g.link_ny, g.link_nz, g.dx, g.dy, g.dz, max_stencil)
+        return {"lsq_p": p, "lsq_wt1": w1, "lsq_wt2": w2, "lsq_valid": valid}
+    if builder != "numpy":
+        raise ValueError("tangent builder must be 'native' or 'numpy'")
+    L = int(g.n_links)
+    out_p = np.full((L, max_stencil), -1, dtype=np.int32)
+    wt1 = np.zeros((L, max_stencil))
+    wt2 = np.zeros((L, max_stencil))
+    valid = np.zeros(L, dtype=np.int32)
+    active_id = np.asarray(tables["active_id"], dtype=np.int32)
+    nx, ny, nz = g.nx, g.ny, g.nz
+    dx, dy, dz = float(g.dx), float(g.dy), float(g.dz)
+    xs = (np.arange(nx, dtype=np.float64) + 0.5) * dx
+    ys = (np.arange(ny, dtype=np.float64) + 0.5) * dy
+    zs = (np.arange(nz, dtype=np.float64) + 0.5) * dz
+
+    def fit(ell, radius):
+        xb, yb, zb = float(g.link_xb[ell]), float(g.link_yb[ell]), float(g.link_zb[ell])
+        t1, t2 = _basis_np(np.array([g.link_nx[ell], g.link_ny[ell], g.link_nz[ell]], dtype=np.float64))
+        i0, j0, k0 = int(g.link_i[ell]), int(g.link_j[ell]), int(g.link_k[ell])
+        r2max = (float(radius) + 0.25) ** 2
+        cand = []
+        for ii in range(max(0, i0 - radius), min(nx, i0 + radius + 1)):
+            for jj in range(max(0, j0 - radius), min(ny, j0 + radius + 1)):
+                for kk in range(max(0, k0 - radius), min(nz, k0 + radius + 1)):
+                    pid = int(active_id[ii, jj, kk])
+                    if pid < 0:
+                        continue
+                    rx, ry, rz = (float(xs[ii]) - xb) / dx, (float(ys[jj]) - yb) / dy, (float(zs[kk]) - zb) / dz
+                    r2 = rx * rx + ry * ry + rz * rz
+                    if r2 <= r2max:
+                        cand.append((r2, pid, rx, ry, rz))
+        cand.sort(key=lambda c: c[0])
+        if len(cand) < 10:
+            return None
+        cand = cand[:max_stencil]
+        A = np.array([(1.0, rx, ry, rz, 0.5 * rx * rx, rx * ry, rx * rz, 0.5 * ry * ry, ry * rz, 0.5 * rz * rz)
+                      for _r2, _p, rx, ry, rz in cand])
+        wd = np.array([math.exp(-0.5 * c[0] / (1.75 * 1.75)) + 1.0e-4 for c in cand])
+        if np.linalg.matrix_rank(A, tol=1.0e-10) < 10:
+            return None
+        M = A.T @ (A * wd[:, None])
+        cond = np.linalg.cond(M)
+        if not np.isfinite(cond) or cond > 1.0e9:
+            return None
+        coeff = np.linalg.solve(M + 1.0e-12 * np.eye(10), A.T * wd[None, :])
+        w1 = np.array([0.0, t1[0] / dx, t1[1] / dy, t1[2] / dz] + [0.0] * 6) @ coeff
+        w2 = np.array([0.0, t2[0] / dx, t2[1] / dy, t2[2] / dz] + [0.0] * 6) @ coeff
+        h = max(dx, dy, dz)
+        if not (np.all(np.isfinite(w1)) and np.all(np.isfinite(w2))):
+            return None
+        if abs(float(np.sum(w1))) * h > 1.0e-6 or abs(float(np.sum(w2))) * h > 1.0e-6:
+            return None
+        if float(np.sum(np.abs(w1))) * h > 12.0 or float(np.sum(np.abs(w2))) * h > 12.0:
+            return None
+        return cand, w1, w2
+
+    for ell in range(L):
+        for radius in (3, 4, 5):
+            res = fit(ell, radius)
+            if res is not None:
+                break
+        if res is None:
+            continue
+        cand, w1, w2 = res
+        for m, c in enumerate(cand):
+            out_p[ell, m] = c[1]
+            wt1[ell, m], wt2[ell, m] = float(w1[m]), float(w2[m])
+        valid[ell] = 1
+    return {"lsq_p": out_p, "lsq_wt1": wt1, "lsq_wt2": wt2, "lsq_valid": valid}
+
+
 # ---- host boundary-value helpers (cbw3d:802-960) -----------------------------
 def _affine_neo_hooke_F_P_np(p, t):
     pp = np.zeros(16)
@@ -780,8 +867,6 @@ class WarpSparseCurvedBoundaryLBM3D:
             raise ValueError("collision_omega must be in (0,2]")
         self.boundary_reconstruction = _recon_id(boundary_reconstruction)
         self.boundary_reconstruction_name = _recon_name(self.boundary_reconstruction)
-        if self.boundary_reconstruction == BOUNDARY_RECON_COMPAT_BFL:
-            raise ValueError("boundary_reconstruction='compat_bfl' is not provided by this solver")
         self.normal_jacobian = _normal_jacobian_for_material(material, normal_jacobian)
         self.normal_jacobian_name = normal_jacobian_name(self.normal_jacobian)
         self.neumann_solver_name = neumann_solver_name(neumann_solver, material)
@@ -838,7 +923,9 @@ class WarpSparseCurvedBoundaryLBM3D:
         self.active_id_np = tables["active_id"]
         self.n_active = int(self.active_i.size)
         self.n_nodes = self.n_active
-        self.tangent_lsq_active = False
+        # The tangent stencils belong only to the compat_bfl closure.
+        self.tangent_lsq_active = self.boundary_reconstruction == BOUNDARY_RECON_COMPAT_BFL
+        self.tangent_lsq = build_tangent_lsq_weights(geometry, tables) if self.tangent_lsq_active else None
         self.kind_by_id, self.mode_by_id, self.params_by_id = boundary.tables(MAX_BOUNDARY_IDS)
         self.component_kind_by_id = boundary.component_kind_table(MAX_BOUNDARY_IDS)
         link_ids = np.clip(np.asarray(geometry.link_boundary_id, dtype=np.int32), 0, MAX_BOUNDARY_IDS - 1)
@@ -911,6 +998,15 @@ class WarpSparseCurvedBoundaryLBM3D:
             self.char_pack_slot = DeviceArray.zeros((1,), i32, dev)
             self.char_gw = self.char_pw = DeviceArray.zeros((1,), f64, dev)
             self.char_state = DeviceArray.zeros((1, 1, 1), f64, dev)
+        if self.tangent_lsq is not None and L:
+            tl = self.tangent_lsq
+            self.tan_p = dev_i(np.ascontiguousarray(tl["lsq_p"].T))
+            self.tan_w1 = dev_f(np.ascontiguousarray(tl["lsq_wt1"].T))
+            self.tan_w2 = dev_f(np.ascontiguousarray(tl["lsq_wt2"].T))
+            self.tan_valid = dev_i(tl["lsq_valid"])
+        else:
+            self.tan_p = self.tan_valid = DeviceArray.zeros((1,), i32, dev)
+            self.tan_w1 = self.tan_w2 = DeviceArray.zeros((1,), f64, dev)
         self.kind_by_id_wp = dev_i(self.kind_by_id)
         self.component_kind_by_id_wp = dev_i(self.component_kind_by_id)
         self.mode_by_id_wp = dev_i(self.mode_by_id)
@@ -972,7 +1068,8 @@ class WarpSparseCurvedBoundaryLBM3D:
             "link_nz": self.link_nz, "char_link": self.char_link, "char_nodes": self.char_nodes,
             "char_pack": self.char_pack, "char_pack_slot": self.char_pack_slot, "char_gidx": self.char_gidx,
             "char_gw": self.char_gw, "char_pidx": self.char_pidx, "char_pw": self.char_pw,
-            "char_state": self.char_state, "kind_by_id": self.kind_by_id_wp,
+            "char_state": self.char_state, "tan_p": self.tan_p, "tan_w1": self.tan_w1, "tan_w2": self.tan_w2,
+            "tan_valid": self.tan_valid, "kind_by_id": self.kind_by_id_wp,
             "comp_kind_by_id": self.component_kind_by_id_wp, "mode_by_id": self.mode_by_id_wp,
             "params_by_id": self.params_by_id_wp, "dyn_enabled": self.dynamic_traction_override_enabled,
             "dyn_traction": self.dynamic_traction_override, "lkind": self.link_kind,

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define VLBM3D_ABI_VERSION 4
+#define VLBM3D_ABI_VERSION 5
 #define VLBM3D_MAX_IDS 16
 #define VLBM3D_MAX_CHAR 64
 #define VLBM3D_RECORD 24
@@ -92,6 +92,10 @@ typedef struct {
     const int32_t *char_pidx;        /* (16,Lc) predictor samples (into char_nodes) */
     const double *char_pw;           /* (16,Lc) predictor weights */
     double *char_state;              /* (4,12,L) predictor and gradient of U */
+    /* compat_bfl surface-tangent stencils (may be null for other reconstructions) */
+    const int32_t *tan_p;            /* (32,L) active node of each sample, -1 if unused */
+    const double *tan_w1, *tan_w2;   /* (32,L) d/dt1 and d/dt2 weights */
+    const int32_t *tan_valid;        /* (L) */
     const int32_t *kind_by_id;       /* (16) */
     const int32_t *comp_kind_by_id;  /* (3,16) */
     const int32_t *mode_by_id;       /* (16) */

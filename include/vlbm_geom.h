@@ -6,6 +6,8 @@
  * Reference interfaces replaced (reference = JingsenFeng/curved-boundary-vlbm):
  *   vlbm_geom3d_char_weights  build_characteristic_weights
  *                             (src/curved_lbm/three_d/characteristic_stencil3d.py:19-171)
+ *   vlbm_geom3d_tangent_weights  _build_sparse_tangent_lsq_weights
+ *                             (src/curved_lbm/three_d/curved_boundary_warp3d.py:3725-3849)
  *   vlbm_geom2d_lsq_weights   _build_boundary_lsq_weights with _local_quadratic_fit
  *                             (src/curved_lbm/two_d/curved_boundary_warp.py:332-606)
  *
@@ -41,6 +43,19 @@ int vlbm_geom3d_char_weights(const int32_t *active_id, int32_t nx, int32_t ny, i
                              const double *xb, const double *yb, const double *zb, const double *lnx,
                              const double *lny, const double *lnz, const double *eta, double dx, int32_t nthreads,
                              int32_t *indices, double *weights, int32_t *modes, uint8_t *status);
+
+/* Surface-tangent derivative stencils of the 3D compat_bfl closure
+ * (replaces _build_sparse_tangent_lsq_weights, curved_boundary_warp3d.py:
+ * 3725-3849): per link, up to max_stencil active nodes (ascending distance)
+ * and the weights of d/dt1 and d/dt2; (n_links, max_stencil) row-major
+ * outputs out_p (active-node index, -1 unused), wt1, wt2, and valid
+ * (n_links).  dx, dy, dz the spacings.  Returns 0, or -1 for invalid
+ * arguments. */
+int vlbm_geom3d_tangent_weights(const int32_t *active_id, int32_t nx, int32_t ny, int32_t nz, int64_t n_links,
+                                const int32_t *li, const int32_t *lj, const int32_t *lk, const double *xb,
+                                const double *yb, const double *zb, const double *lnx, const double *lny,
+                                const double *lnz, double dx, double dy, double dz, int32_t max_stencil,
+                                int32_t nthreads, int32_t *out_p, double *wt1, double *wt2, int32_t *valid);
 
 /* Boundary stencils of the n_links cut links of a D2Q4 lattice.  active:
  * (nx, ny) 0/1; x_coords (nx), y_coords (ny) node coordinates; per link:

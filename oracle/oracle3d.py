@@ -47,6 +47,7 @@ class _State(C.Structure):
         ("link_eta", _dp), ("link_xb", _dp), ("link_yb", _dp), ("link_zb", _dp),
         ("link_nx", _dp), ("link_ny", _dp), ("link_nz", _dp),
         ("char_indices", _dp), ("char_weights", _dp), ("char_state", _dp),
+        ("tan_p", _dp), ("tan_w1", _dp), ("tan_w2", _dp), ("tan_valid", _dp),
         ("kind_by_id", _dp), ("comp_kind_by_id", _dp), ("mode_by_id", _dp), ("params_by_id", _dp),
         ("dyn_enabled", _dp), ("dyn_traction", _dp),
         ("lkind", _dp), ("lcomp_kind", _dp), ("lbv", _dp), ("lFb", _dp), ("lPb", _dp),
@@ -111,7 +112,7 @@ class OracleSolver3D:
                  local_displacement_boundary_weight=20.0, local_displacement_boundary_id=-1,
                  local_compatibility_interval=0, local_compatibility_blend=0.1, normal_jacobian=None,
                  neumann_solver=None, check_linearized_cfl=True, threads: int | None = None,
-                 tables=None, characteristic_stencil=None):
+                 tables=None, characteristic_stencil=None, tangent_lsq=None):
         from curved_lbm.three_d import curved_boundary_warp3d as cb
 
         self.cb = cb
@@ -123,8 +124,6 @@ class OracleSolver3D:
         self.dt = self.dx / self.lattice_speed
         self.jacobian_floor = float(jacobian_floor)
         self.recon = cb._recon_id(boundary_reconstruction)
-        if self.recon == cb.BOUNDARY_RECON_COMPAT_BFL:
-            raise ValueError("compat_bfl is not restated by the oracle")
         if check_linearized_cfl and material.linearized_stability_ratio(self.lattice_speed) >= 1.0:
             raise ValueError("linearized D3Q6 stability ratio must be below one")
         nj = cb._normal_jacobian_for_material(material, normal_jacobian)
@@ -185,6 +184,23 @@ class OracleSolver3D:
             keep("char_indices", np.zeros(1), i32)
             keep("char_weights", np.zeros(1), f64)
             self.char_state = keep("char_state", np.zeros(1), f64)
+        self.tangent_lsq = None
+        if self.recon == cb.BOUNDARY_RECON_COMPAT_BFL:
+            if tangent_lsq is None:
+                tangent_lsq = cb.build_tangent_lsq_weights(geometry, tables, builder="numpy")
+            self.tangent_lsq = tangent_lsq
+            for k in ("lsq_p", "lsq_wt1", "lsq_wt2"):
+                if np.asarray(tangent_lsq[k]).shape != (L, 32):
+                    raise ValueError(f"{k} must have shape (n_links, 32)")
+            keep("tan_p", tangent_lsq["lsq_p"], i32)
+            keep("tan_w1", tangent_lsq["lsq_wt1"], f64)
+            keep("tan_w2", tangent_lsq["lsq_wt2"], f64)
+            keep("tan_valid", tangent_lsq["lsq_valid"], i32)
+        else:
+            for k in ("tan_p", "tan_valid"):
+                keep(k, np.zeros(1), i32)
+            for k in ("tan_w1", "tan_w2"):
+                keep(k, np.zeros(1), f64)
         keep("kind_by_id", kind, i32)
         keep("comp_kind_by_id", ck, i32)
         keep("mode_by_id", mode, i32)

@@ -798,8 +798,8 @@ static void solve_Fb(const or3d_state *s, const double *Fin, double nx, double n
     make_F(h, n, t1, t2, g1, g2, Fb);
 }
 
-/* cbw3d.py:2888-3151 (_sparse_prepare_links_gpu) for the bfl/halfway/local_f
- * reconstructions (compat_bfl's tangent least squares is not restated). */
+/* cbw3d.py:2888-3151 (_sparse_prepare_links_gpu) for the bfl, halfway,
+ * compat_bfl, local_f and local_f_bfl reconstructions. */
 static void prepare_links(or3d_state *s, double t) {
     const int n = s->n_active, L = s->n_links;
 #pragma omp parallel for schedule(dynamic, 64)
@@ -824,13 +824,49 @@ static void prepare_links(or3d_state *s, double t) {
             int p = s->link_p[ell];
             double F[9];
             for (int k = 0; k < 9; ++k) F[k] = s->U[IDX(3 + k, p, n)];
-            if (s->recon == R_LOCALF_BFL) {
+            if (s->recon == R_LOCALF_BFL || s->recon == R_COMPAT) {
                 int up = s->link_up[ell];
                 if (up >= 0) {
                     double eta = s->link_eta[ell];
                     for (int k = 0; k < 9; ++k)
                         F[k] = s->U[IDX(3 + k, p, n)] + eta * (s->U[IDX(3 + k, p, n)] - s->U[IDX(3 + k, up, n)]);
                 }
+            }
+            /* cbw3d.py:3005-3049: geometry-compatible F of compat_bfl. */
+            if (s->recon == R_COMPAT && s->tan_valid[ell] != 0) {
+                double nv[3] = {nx, ny, nz};
+                double r = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+                if (r < 1.0e-30) r = 1.0e-30;
+                for (int c = 0; c < 3; ++c) nv[c] /= r;
+                double ref[3] = {1.0, 0.0, 0.0};
+                if (fabs(nv[0]) >= 0.75) {
+                    ref[0] = 0.0;
+                    ref[1] = 1.0;
+                }
+                double t1[3] = {nv[1] * ref[2] - nv[2] * ref[1], nv[2] * ref[0] - nv[0] * ref[2],
+                                nv[0] * ref[1] - nv[1] * ref[0]};
+                double r1 = sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
+                if (r1 < 1.0e-30) r1 = 1.0e-30;
+                for (int c = 0; c < 3; ++c) t1[c] /= r1;
+                double t2[3] = {nv[1] * t1[2] - nv[2] * t1[1], nv[2] * t1[0] - nv[0] * t1[2],
+                                nv[0] * t1[1] - nv[1] * t1[0]};
+                double du1[3] = {0, 0, 0}, du2[3] = {0, 0, 0}, wt_abs = 0.0;
+                for (int m = 0; m < OR_MAX_TAN; ++m) {
+                    int pp = s->tan_p[(size_t)ell * OR_MAX_TAN + m];
+                    if (pp < 0) continue;
+                    double w1 = s->tan_w1[(size_t)ell * OR_MAX_TAN + m], w2 = s->tan_w2[(size_t)ell * OR_MAX_TAN + m];
+                    for (int c = 0; c < 3; ++c) {
+                        du1[c] += w1 * s->u[IDX(c, pp, n)];
+                        du2[c] += w2 * s->u[IDX(c, pp, n)];
+                    }
+                    wt_abs += fabs(w1) + fabs(w2);
+                }
+                if (wt_abs > 0.0)
+                    for (int row = 0; row < 3; ++row) {
+                        double h = F[3 * row] * nv[0] + F[3 * row + 1] * nv[1] + F[3 * row + 2] * nv[2];
+                        double g1 = t1[row] + du1[row], g2 = t2[row] + du2[row];
+                        for (int c = 0; c < 3; ++c) F[3 * row + c] = h * nv[c] + g1 * t1[c] + g2 * t2[c];
+                    }
             }
             if (s->recon == R_LOCALF)
                 for (int k = 0; k < 9; ++k) F[k] = s->char_state[((size_t)0 * NC + 3 + k) * L + ell];

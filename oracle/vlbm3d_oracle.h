@@ -17,6 +17,7 @@
 
 #define OR_MAX_IDS 16
 #define OR_MAX_CHAR 64
+#define OR_MAX_TAN 32
 
 typedef struct {
     int32_t n_active;
@@ -48,6 +49,9 @@ typedef struct {
     const int32_t *char_indices; /* (64, L) */
     const double *char_weights;  /* (4, 64, L) */
     double *char_state;          /* (4, 12, L) */
+    const int32_t *tan_p;        /* (L, 32) compat_bfl tangent stencil nodes, -1 unused */
+    const double *tan_w1, *tan_w2; /* (L, 32) d/dt1, d/dt2 weights */
+    const int32_t *tan_valid;    /* (L) */
     const int32_t *kind_by_id;   /* (16) */
     const int32_t *comp_kind_by_id; /* (3, 16) */
     const int32_t *mode_by_id;   /* (16) */

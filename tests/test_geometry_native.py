@@ -139,3 +139,50 @@ def test_unknown_builder_is_rejected():
     g3 = gpu_cases.shell_geometry(12, 0.2, 0.45)
     with pytest.raises(ValueError):
         cs3.build_characteristic_weights(g3, cb3._sparse_tables_from_geometry(g3), builder="lapack")
+
+
+@pytest.mark.parametrize("shell", [(0.2, 0.45, 16), (0.3, 0.36, 16), (0.25, 0.45, 12)])
+def test_native_tangent_weights_match_numpy(shell):
+    g = gpu_cases.shell_geometry(shell[2], shell[0], shell[1])
+    tables = cb3._sparse_tables_from_geometry(g)
+    ref = cb3.build_tangent_lsq_weights(g, tables, builder="numpy")
+    got = cb3.build_tangent_lsq_weights(g, tables, builder="native")
+    np.testing.assert_array_equal(got["lsq_p"], ref["lsq_p"])
+    np.testing.assert_array_equal(got["lsq_valid"], ref["lsq_valid"])
+    assert 0 < int(ref["lsq_valid"].sum()) < g.n_links  # both the fitted and the rejected branch
+    for k in ("lsq_wt1", "lsq_wt2"):
+        assert _rel(got[k], ref[k]) < LSQ_REL, k
+
+
+def test_native_tangent_weights_are_exact_for_quadratics():
+    """Independent of any LSQ implementation: the weights of every valid
+    link give the derivatives of a quadratic along the two tangents of the
+    normal's basis (t1 = normalize(n x e), e = e_x unless |n_x| >= 3/4, then
+    e_y; t2 = n x t1) at the cut point."""
+    g = gpu_cases.shell_geometry(20, 0.2, 0.45)
+    tables = cb3._sparse_tables_from_geometry(g)
+    fit = cb3.build_tangent_lsq_weights(g, tables)
+    ai, aj, ak = tables["active_i"], tables["active_j"], tables["active_k"]
+    x, y, z = ((ai + 0.5) * g.dx, (aj + 0.5) * g.dy, (ak + 0.5) * g.dz)
+    c = np.random.default_rng(11).standard_normal(10)
+    f = (c[0] + c[1] * x + c[2] * y + c[3] * z + c[4] * x * x + c[5] * x * y + c[6] * x * z + c[7] * y * y
+         + c[8] * y * z + c[9] * z * z)
+    xb, yb, zb = np.asarray(g.link_xb), np.asarray(g.link_yb), np.asarray(g.link_zb)
+    grad = np.stack([c[1] + 2 * c[4] * xb + c[5] * yb + c[6] * zb, c[2] + c[5] * xb + 2 * c[7] * yb + c[8] * zb,
+                     c[3] + c[6] * xb + c[8] * yb + 2 * c[9] * zb])
+    n = np.stack([g.link_nx, g.link_ny, g.link_nz]).astype(np.float64)
+    n /= np.linalg.norm(n, axis=0)
+    e = np.where(np.abs(n[0]) >= 0.75, np.array([[0.0], [1.0], [0.0]]), np.array([[1.0], [0.0], [0.0]]))
+    t1 = np.cross(n, e, axis=0)
+    t1 /= np.linalg.norm(t1, axis=0)
+    t2 = np.cross(n, t1, axis=0)
+    ok = fit["lsq_valid"] != 0
+    assert ok.sum() > 0.9 * g.n_links
+    idx = fit["lsq_p"]
+    vals = np.where(idx >= 0, f[np.clip(idx, 0, None)], 0.0)
+    scale = np.max(np.abs(grad))
+    for key, t in (("lsq_wt1", t1), ("lsq_wt2", t2)):
+        est = np.sum(fit[key] * vals, axis=1)
+        want = np.sum(grad * t, axis=0)
+        assert np.max(np.abs(est - want)[ok]) < 1e-8 * scale, key
+    assert np.all(idx[~ok] == -1) and np.all(fit["lsq_wt1"][~ok] == 0.0)

@@ -70,6 +70,11 @@ def test_gpu_boundary_mechanics_and_population_formula(bfl_fixture, kinds):
     check_boundary_mechanics(cb.WarpCurvedBoundaryLBM3D, set_f0, cb, bfl_fixture, kinds)
 
 
+def _with(case, **over):
+    g, kw = case
+    return g, dict(kw, **over)
+
+
 CASES = {
     "sphere_traction_acoustic": lambda: gpu_cases.sphere_case(16),
     "sphere_traction_fd_mooney": lambda: gpu_cases.sphere_case(16, model="mooney_rivlin", normal_jacobian="finite_difference"),
@@ -87,6 +92,12 @@ CASES = {
     "sphere_gent": lambda: gpu_cases.sphere_case(14, model="gent"),
     "sphere_sph_nh": lambda: gpu_cases.sphere_case(14, model="sph_neo_hooke"),
     "sphere_local_f_bfl": lambda: gpu_cases.sphere_case(16, boundary_reconstruction="local_f_bfl"),
+    "sphere_compat_bfl": lambda: gpu_cases.sphere_case(16, boundary_reconstruction="compat_bfl"),
+    "sphere_compat_bfl_mixed": lambda: gpu_cases.sphere_case(
+        16, kinds=("neumann", "dirichlet", "neumann"), mode=cb.CURVED3D_VALUE_CONSTANT_VECTOR,
+        params=(0.02, 0.001, -0.01), boundary_reconstruction="compat_bfl"),
+    "shell_compat_bfl": lambda: _with(gpu_cases.traction_shell(16, ramp_steps=40),
+                                      boundary_reconstruction="compat_bfl"),
     "sphere_bfl_affine_dirichlet": lambda: gpu_cases.sphere_case(
         16, kinds=("dirichlet",) * 3, mode=cb.CURVED3D_VALUE_AFFINE_VELOCITY,
         params=(0.01, 0.0, 0.002, 0.0, -0.01, 0.0, 0.001, 0.0, 0.0, 0.001, 0.0, 0.0, 0.0),

@@ -122,3 +122,37 @@ def test_oracle_piola_matches_energy_derivative(model):
                 E[i, j] = h
                 dW[i, j] = (W(F + E) - W(F - E)) / (2 * h)
         np.testing.assert_allclose(piola(F, mat), dW, rtol=1e-7, atol=1e-8)
+
+
+@pytest.mark.parametrize("kinds", [("neumann",) * 3, ("neumann", "dirichlet", "neumann")])
+def test_compat_projection_keeps_a_compatible_affine_state(kinds):
+    """compat_bfl replaces the boundary F by (F n) n^T + sum_k (t_k + d_tk u)
+    t_k^T over the two tangents (cbw3d:3005-3049).  For a uniform F = I + G
+    with u = G (x - c) the state is compatible, so the projection must give
+    back F itself and every written population must equal that of bfl (the
+    extrapolated F of a uniform field is the field); a wrong tangent, a
+    swapped component or a transposed G would not."""
+    g = cb.CurvedBoundaryGeometry3D.sphere(nx=20, ny=20, nz=20, radius=0.35)
+    G = np.array([[0.012, 0.021, -0.004], [-0.017, 0.006, 0.009], [0.003, -0.011, 0.015]])
+    spec = cb.CurvedBoundarySpec3D(kind="neumann", component_kinds=kinds,
+                                   value_mode=cb.CURVED3D_VALUE_SINE2_NORMAL_TRACTION, params=(0.05, 0.1, 0.0))
+    out = {}
+    for recon, scale in (("bfl", 1.0), ("compat_bfl", 1.0), ("compat_bfl", 0.0)):
+        s = OracleSolver3D(g, material=NH, boundary=spec, boundary_reconstruction=recon, lattice_speed=10.0,
+                           collision_omega=1.8)
+        x = np.stack(s.active_coordinates())
+        U0 = np.zeros((12, s.n_active))
+        U0[3:] = (np.eye(3) + G).reshape(9, 1)
+        disp = scale * (G @ (x - 0.5))
+        s.initialize_from_numpy(U=U0, displacement=disp, apply_initial_boundaries=False)
+        if recon == "compat_bfl":
+            assert int(s.tangent_lsq["lsq_valid"].sum()) > 0.9 * g.n_links
+        s.f1[...] = np.nan
+        s.apply_boundary_reconstruction(0.03, in_place=False)
+        out[recon, scale] = s.f1.copy()
+    bfl, compat, incompatible = out["bfl", 1.0], out["compat_bfl", 1.0], out["compat_bfl", 0.0]
+    written = np.isfinite(bfl)
+    assert written.sum() > 1000 and np.array_equal(written, np.isfinite(compat))
+    assert np.max(np.abs(bfl[written] - compat[written])) < 1e-12
+    # with u = 0 the tangential rows are projected back to the identity
+    assert np.max(np.abs(bfl[written] - incompatible[written])) > 1e-6
