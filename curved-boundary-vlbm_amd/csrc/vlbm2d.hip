@@ -56,6 +56,8 @@ struct Params2 {
     double blend;
     int interior_only;
     int use_dyn;
+    double gw;    // mrt_ghost: 0.25 (omega - omega_ghost), 0 for bgk
+    double filt;  // kinetic ghost filter strength, 0 = off
     const double *clock;  // device time of the step start (captured step loops), or nullptr
 };
 
@@ -129,6 +131,15 @@ __device__ __forceinline__ double fpost_v(double fv, const R *rec, size_t N, int
     return om1 * fv + rld(rec, (size_t)(R_A + a) * N + p, a, w) + s * b;
 }
 
+// Ghost part of the mrt_ghost collision (cbw2d:1847-2041).  Relaxing the
+// ghost moment m3 = fE - fN + fW - fS with omega_ghost instead of omega adds
+// 0.25 (omega - omega_ghost) m3 to fpost on E, W and subtracts it on N, S;
+// gh holds m3 of the pre-collision populations, (6, N).
+__device__ __forceinline__ double ghost_term(const double *gh, size_t N, int q, int a, int p, double gw) {
+    const double g = gw * gh[(size_t)a * N + p];
+    return (q & 1) ? -g : g;
+}
+
 __global__ void k2_reset_invalid(int32_t *invalid) { invalid[0] = 0; }
 __global__ void k2_set_clock(double *clock, double t) { clock[0] = t; }
 // time += dt, the same FP64 operation the host performs after each step.
@@ -136,22 +147,31 @@ __global__ void k2_advance_clock(double *clock, double dt) { clock[0] = clock[0]
 
 // Moments with the half-step source; u = u* + dt/2 v and (collide only)
 // u* = u + dt/2 v; J flag.  cbw2d:1692-1790 and :3128-3195.
+// With gout (mrt_ghost) also the ghost moment m3 of every component.
 template <typename T, bool COLLIDE>
 __global__ __launch_bounds__(BLOCK) void k2_moments(const T *__restrict__ f, double *__restrict__ U,
                                                     double *__restrict__ u, double *__restrict__ us,
                                                     int32_t *__restrict__ invalid,
-                                                    const int32_t *__restrict__ list, Params2 P) {
+                                                    const int32_t *__restrict__ list, double *__restrict__ gout,
+                                                    Params2 P) {
     const int t = xcd_block() * BLOCK + threadIdx.x;
     if (t >= P.na) return;
     const int p = list[t];
     const size_t N = P.N;
-    double Um[NC];
+    double Um[NC], Gm[NC];
 #pragma unroll
-    for (int a = 0; a < NC; ++a) Um[a] = 0.0;
+    for (int a = 0; a < NC; ++a) Um[a] = Gm[a] = 0.0;
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
-        for (int a = 0; a < NC; ++a) Um[a] += ld(f, (size_t)(q * 6 + a) * N + p, a);
+        for (int a = 0; a < NC; ++a) {
+            const double x = ld(f, (size_t)(q * 6 + a) * N + p, a);
+            Um[a] += x;
+            Gm[a] += (q & 1) ? -x : x;
+        }
+    if (gout)
+#pragma unroll
+        for (int a = 0; a < NC; ++a) gout[(size_t)a * N + p] = Gm[a];
     const double hdt = 0.5 * P.dt;
     if (P.src_mode == SRC_CONST) {
 #pragma unroll
@@ -459,7 +479,7 @@ template <typename T>
 __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T *__restrict__ fdst,
                                                    const T *__restrict__ rec, const double *__restrict__ U,
                                                    const double *__restrict__ cs, Links2 Tb, int use_rec,
-                                                   double time, Params2 P) {
+                                                   const double *__restrict__ gh, double time, Params2 P) {
     const int ell = blockIdx.x * LBLOCK + threadIdx.x;
     if (ell >= P.L) return;
     const size_t N = P.N;
@@ -610,6 +630,10 @@ __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T
             if (use_rec) {
                 fo = fpost_v(ld(fsrc, (size_t)(qo * 6 + a) * N + p, a), rec, N, qo, a, p, om1, P.omega);
                 fi = fpost_v(ld(fsrc, (size_t)(qi * 6 + a) * N + p, a), rec, N, qi, a, p, om1, P.omega);
+                if (gh) {
+                    fo += ghost_term(gh, N, qo, a, p, P.gw);
+                    fi += ghost_term(gh, N, qi, a, p, P.gw);
+                }
             } else {
                 fo = ld(fsrc, (size_t)(qo * 6 + a) * N + p, a);
                 fi = ld(fsrc, (size_t)(qi * 6 + a) * N + p, a);
@@ -619,9 +643,11 @@ __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T
                 val = alpha * (D * fo + S) + (1.0 - alpha) * fi;
             } else {
                 double upv = fo;
-                if (up_ok)
+                if (up_ok) {
                     upv = use_rec ? fpost_v(ld(fsrc, (size_t)(qo * 6 + a) * N + pu, a), rec, N, qo, a, pu, om1, P.omega)
                                   : ld(fsrc, (size_t)(qo * 6 + a) * N + pu, a);
+                    if (use_rec && gh) upv += ghost_term(gh, N, qo, a, pu, P.gw);
+                }
                 val = D * (2.0 * eta * fo + (1.0 - 2.0 * eta) * upv) + S;
             }
         }
@@ -633,12 +659,21 @@ __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T
 // node is inactive or off the lattice belong to cut links and were written
 // by k2_links.  With NEXT the kernel also performs the moment part of the
 // next step's collide (cbw2d:1731-1790).
-template <typename T, bool NEXT>
+//
+// MODE bit 0 (mrt_ghost): the sources carry the ghost term of gin, and with
+// NEXT the ghost moment of the new populations is written to gout.  MODE
+// bit 1: the kinetic ghost filter of cbw2d:3100-3124 (m3 scaled by
+// 1 - strength through +-m3 strength / 4 on E, W and N, S) is applied to all
+// four populations of the node, cut-link ones included, before they are
+// stored and summed.
+template <typename T, bool NEXT, int MODE>
 __global__ __launch_bounds__(BLOCK) void k2_stream(const T *__restrict__ f0, T *__restrict__ f1,
                                                    const T *__restrict__ rec, const int32_t *__restrict__ list,
                                                    const int32_t *__restrict__ act, double *__restrict__ U,
                                                    double *__restrict__ u, double *__restrict__ us,
-                                                   int32_t *__restrict__ invalid, Params2 P) {
+                                                   int32_t *__restrict__ invalid, const double *__restrict__ gin,
+                                                   double *__restrict__ gout, Params2 P) {
+    constexpr bool MRT = (MODE & 1) != 0, FILT = (MODE & 2) != 0;
     const int t = xcd_block() * BLOCK + threadIdx.x;
     if (t >= P.na) return;
     const int p = list[t];
@@ -649,25 +684,72 @@ __global__ __launch_bounds__(BLOCK) void k2_stream(const T *__restrict__ f0, T *
     double Um[NC];
 #pragma unroll
     for (int a = 0; a < NC; ++a) Um[a] = 0.0;
+    if constexpr (MODE == 0) {
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        const int iu = i - cx(q), ju = j - cy(q);
-        const bool ok = iu >= 0 && iu < P.nx && ju >= 0 && ju < ny && act[iu * ny + ju] != 0;
-        if (ok) {
-            const int src = iu * ny + ju;
-            double fv[NC];
+        for (int q = 0; q < NQ; ++q) {
+            const int iu = i - cx(q), ju = j - cy(q);
+            const bool ok = iu >= 0 && iu < P.nx && ju >= 0 && ju < ny && act[iu * ny + ju] != 0;
+            if (ok) {
+                const int src = iu * ny + ju;
+                double fv[NC];
 #pragma unroll
-            for (int a = 0; a < NC; ++a) fv[a] = ld_nt(f0, (size_t)(q * 6 + a) * N + src, a);
+                for (int a = 0; a < NC; ++a) fv[a] = ld_nt(f0, (size_t)(q * 6 + a) * N + src, a);
+#pragma unroll
+                for (int a = 0; a < NC; ++a) {
+                    const double x =
+                        st_nt(f1, (size_t)(q * 6 + a) * N + p, a, fpost_v(fv[a], rec, N, q, a, src, om1, P.omega));
+                    if (NEXT) Um[a] += x;
+                }
+            } else if (NEXT) {
+#pragma unroll
+                for (int a = 0; a < NC; ++a) Um[a] += ld(f1, (size_t)(q * 6 + a) * N + p, a);
+            }
+        }
+    } else {
+        double fq[NQ][NC];
+        bool okq[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int iu = i - cx(q), ju = j - cy(q);
+            okq[q] = iu >= 0 && iu < P.nx && ju >= 0 && ju < ny && act[iu * ny + ju] != 0;
+            if (okq[q]) {
+                const int src = iu * ny + ju;
+#pragma unroll
+                for (int a = 0; a < NC; ++a) {
+                    double x = fpost_v(ld_nt(f0, (size_t)(q * 6 + a) * N + src, a), rec, N, q, a, src, om1, P.omega);
+                    if (MRT) x += ghost_term(gin, N, q, a, src, P.gw);
+                    fq[q][a] = x;
+                }
+            } else {
+#pragma unroll
+                for (int a = 0; a < NC; ++a) fq[q][a] = ld(f1, (size_t)(q * 6 + a) * N + p, a);
+            }
+        }
+        if (FILT) {
 #pragma unroll
             for (int a = 0; a < NC; ++a) {
-                const double x =
-                    st_nt(f1, (size_t)(q * 6 + a) * N + p, a, fpost_v(fv[a], rec, N, q, a, src, om1, P.omega));
-                if (NEXT) Um[a] += x;
+                const double m3 = fq[0][a] - fq[1][a] + fq[2][a] - fq[3][a];
+                const double quarter = 0.25 * (-P.filt * m3);
+                fq[0][a] = fq[0][a] + quarter;
+                fq[2][a] = fq[2][a] + quarter;
+                fq[1][a] = fq[1][a] - quarter;
+                fq[3][a] = fq[3][a] - quarter;
             }
-        } else if (NEXT) {
-#pragma unroll
-            for (int a = 0; a < NC; ++a) Um[a] += ld(f1, (size_t)(q * 6 + a) * N + p, a);
         }
+        double Gm[NC];
+#pragma unroll
+        for (int a = 0; a < NC; ++a) Gm[a] = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int a = 0; a < NC; ++a) {
+                const double x = (FILT || okq[q]) ? st_nt(f1, (size_t)(q * 6 + a) * N + p, a, fq[q][a]) : fq[q][a];
+                Um[a] += x;
+                Gm[a] += (q & 1) ? -x : x;
+            }
+        if (MRT && NEXT)
+#pragma unroll
+            for (int a = 0; a < NC; ++a) gout[(size_t)a * N + p] = Gm[a];
     }
     if (NEXT) {
         const double hdt = 0.5 * P.dt;
@@ -750,12 +832,34 @@ __global__ __launch_bounds__(BLOCK) void k2_post(const T *__restrict__ f, T *__r
     const int p = list[t];
     const double om1 = 1.0 - P.omega;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q)
+    for (int a = 0; a < NC; ++a) {
+        double fv[NQ];
 #pragma unroll
-        for (int a = 0; a < NC; ++a) {
-            const size_t k = (size_t)(q * 6 + a) * P.N + p;
-            st(fp, k, a, fpost_v(ld(f, k, a), rec, P.N, q, a, p, om1, P.omega));
-        }
+        for (int q = 0; q < NQ; ++q) fv[q] = ld(f, (size_t)(q * 6 + a) * P.N + p, a);
+        const double g = P.gw * (fv[0] - fv[1] + fv[2] - fv[3]);  // mrt_ghost term, 0 for bgk
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            st(fp, (size_t)(q * 6 + a) * P.N + p, a,
+               fpost_v(fv[q], rec, P.N, q, a, p, om1, P.omega) + ((q & 1) ? -g : g));
+    }
+}
+
+// Kinetic ghost filter in place (cbw2d:3100-3124).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k2_filter(T *__restrict__ f, const int32_t *__restrict__ list, Params2 P) {
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= P.na) return;
+    const int p = list[t];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) {
+        double fv[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) fv[q] = ld(f, (size_t)(q * 6 + a) * P.N + p, a);
+        const double quarter = 0.25 * (-P.filt * (fv[0] - fv[1] + fv[2] - fv[3]));
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            st(f, (size_t)(q * 6 + a) * P.N + p, a, (q & 1) ? fv[q] - quarter : fv[q] + quarter);
+    }
 }
 
 template <typename T>
@@ -898,6 +1002,8 @@ static Params2 make_params2(const vlbm2d_desc &d) {
     P.blend = d.compat_blend;
     P.interior_only = d.compat_interior_only;
     P.use_dyn = d.use_dyn;
+    P.gw = d.collision_model == 1 ? 0.25 * (d.omega - d.ghost_omega) : 0.0;
+    P.filt = d.kinetic_filter;
     return P;
 }
 
@@ -943,14 +1049,16 @@ static int run_links2(vlbm2d_handle *h, const T *src, T *dst, double t, int use_
     TimedSlot ts(h->timing, h->stream, VLBM_SLOT_LINKS);
     const int gl = (d.n_links + LBLOCK - 1) / LBLOCK;
     hipLaunchKernelGGL((k2_links<T>), dim3(gl), dim3(LBLOCK), 0, h->stream, src, dst, (const T *)d.record,
-                       (const double *)d.U, (const double *)d.link_state, h->T, use_rec, t, h->P);
+                       (const double *)d.U, (const double *)d.link_state, h->T, use_rec, (const double *)nullptr,
+                       t, h->P);
     CHECK_LAUNCH("k2_links");
     return 0;
 }
 
 // Gather and cut-link kernels of one step on stream `st`.
 template <typename T>
-static int launch_links2(vlbm2d_handle *h, const Params2 &P, hipStream_t st, const T *f0, T *f1, double tb) {
+static int launch_links2(vlbm2d_handle *h, const Params2 &P, hipStream_t st, const T *f0, T *f1, double tb,
+                         const double *gin) {
     const vlbm2d_desc &d = h->d;
     if (d.n_links == 0) return 0;
     const bool gather = d.recon == R_LOCALF || d.recon == R_LSQ_BFL || d.recon == R_COMPAT_BFL;
@@ -965,31 +1073,50 @@ static int launch_links2(vlbm2d_handle *h, const Params2 &P, hipStream_t st, con
     TimedSlot tl(h->timing, st, VLBM_SLOT_LINKS);
     const int gl = (d.n_links + LBLOCK - 1) / LBLOCK;
     hipLaunchKernelGGL((k2_links<T>), dim3(gl), dim3(LBLOCK), 0, st, f0, f1, (const T *)d.record,
-                       (const double *)d.U, (const double *)d.link_state, h->T, 1, tb, P);
+                       (const double *)d.U, (const double *)d.link_state, h->T, 1, gin, tb, P);
     CHECK_LAUNCH("k2_links");
     return 0;
 }
 
-template <typename T>
-static int launch_stream2(vlbm2d_handle *h, const Params2 &P, bool next, const T *f0, T *f1) {
+template <typename T, bool NEXT, int MODE>
+static void launch_stream2_t(vlbm2d_handle *h, const Params2 &P, const T *f0, T *f1, const double *gin,
+                             double *gout) {
     const vlbm2d_desc &d = h->d;
-    const int gn = xcd_grid(d.n_active);
-    hipStream_t st = h->stream;
-    if (next)
-        hipLaunchKernelGGL((k2_stream<T, true>), dim3(gn), dim3(BLOCK), 0, st, f0, f1, (const T *)d.record,
-                           d.active_list, d.active, d.U, d.u, d.u_star, d.invalid, P);
-    else
-        hipLaunchKernelGGL((k2_stream<T, false>), dim3(gn), dim3(BLOCK), 0, st, f0, f1, (const T *)d.record,
-                           d.active_list, d.active, d.U, d.u, d.u_star, d.invalid, P);
+    hipLaunchKernelGGL((k2_stream<T, NEXT, MODE>), dim3(xcd_grid(d.n_active)), dim3(BLOCK), 0, h->stream, f0, f1,
+                       (const T *)d.record, d.active_list, d.active, d.U, d.u, d.u_star, d.invalid, gin, gout, P);
+}
+
+// mode bit 0: mrt_ghost (gin/gout ghost moments), bit 1: kinetic filter.
+template <typename T>
+static int launch_stream2(vlbm2d_handle *h, const Params2 &P, bool next, const T *f0, T *f1, const double *gin,
+                          double *gout) {
+    const int mode = (P.gw != 0.0 ? 1 : 0) | (P.filt > 0.0 ? 2 : 0);
+    switch (mode | (next ? 4 : 0)) {
+    case 0: launch_stream2_t<T, false, 0>(h, P, f0, f1, gin, gout); break;
+    case 1: launch_stream2_t<T, false, 1>(h, P, f0, f1, gin, gout); break;
+    case 2: launch_stream2_t<T, false, 2>(h, P, f0, f1, gin, gout); break;
+    case 3: launch_stream2_t<T, false, 3>(h, P, f0, f1, gin, gout); break;
+    case 4: launch_stream2_t<T, true, 0>(h, P, f0, f1, gin, gout); break;
+    case 5: launch_stream2_t<T, true, 1>(h, P, f0, f1, gin, gout); break;
+    case 6: launch_stream2_t<T, true, 2>(h, P, f0, f1, gin, gout); break;
+    default: launch_stream2_t<T, true, 3>(h, P, f0, f1, gin, gout); break;
+    }
     CHECK_LAUNCH("k2_stream");
     return 0;
+}
+
+// Ghost-moment buffer of step parity `par` (two (6, N) halves; mrt_ghost only).
+static double *ghost_buf(const vlbm2d_handle *h, int par) {
+    if (h->P.gw == 0.0) return nullptr;
+    return h->d.ghost + (size_t)(par & 1) * NC * h->P.N;
 }
 
 // Launches of one step from (time, steps): sweeps, record, links and the
 // stream (with the next step's collide when `next`).  P carries the clock
 // pointer inside captured step loops.
 template <typename T>
-static int launch_step2(vlbm2d_handle *h, const Params2 &P, T *f0, T *f1, double time, int64_t steps, bool next) {
+static int launch_step2(vlbm2d_handle *h, const Params2 &P, T *f0, T *f1, double time, int64_t steps, bool next,
+                        int par) {
     const vlbm2d_desc &d = h->d;
     const int gn = xcd_grid(d.n_active);
     TimedSlot whole(h->timing, h->stream, VLBM_SLOT_STEP);
@@ -1019,10 +1146,11 @@ static int launch_step2(vlbm2d_handle *h, const Params2 &P, T *f0, T *f1, double
                            P);
         CHECK_LAUNCH("k2_record");
     }
-    int rc = launch_links2<T>(h, P, st, f0, f1, tb);
+    const double *gin = ghost_buf(h, par);
+    int rc = launch_links2<T>(h, P, st, f0, f1, tb, gin);
     if (rc) return rc;
     TimedSlot ts(h->timing, st, VLBM_SLOT_STREAM);
-    return launch_stream2<T>(h, P, next, f0, f1);
+    return launch_stream2<T>(h, P, next, f0, f1, gin, ghost_buf(h, par + 1));
 }
 
 // Captures `graph_steps` fused steps with the time kept on the device (the
@@ -1053,7 +1181,8 @@ static int capture_steps2(vlbm2d_handle *h, T *f0, T *f1, hipGraphExec_t *out) {
 
     int rc = 0;
     for (int k = 0; k < h->graph_steps && rc == 0; ++k) {
-        rc = (k % 2 == 0) ? launch_step2<T>(h, P, f0, f1, 0.0, 0, true) : launch_step2<T>(h, P, f1, f0, 0.0, 0, true);
+        rc = (k % 2 == 0) ? launch_step2<T>(h, P, f0, f1, 0.0, 0, true, 0)
+                          : launch_step2<T>(h, P, f1, f0, 0.0, 0, true, 1);
         if (rc == 0) hipLaunchKernelGGL(k2_advance_clock, dim3(1), dim3(1), 0, h->cap, h->clock, h->d.dt);
     }
     hipGraph_t graph = nullptr;
@@ -1085,7 +1214,7 @@ static int run_steps2(vlbm2d_handle *h, T *f0, T *f1, double time, int64_t steps
     {
         TimedSlot ts(h->timing, h->stream, VLBM_SLOT_COLLIDE);
         hipLaunchKernelGGL((k2_moments<T, true>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, d.U, d.u,
-                           d.u_star, d.invalid, d.active_list, h->P);
+                           d.u_star, d.invalid, d.active_list, ghost_buf(h, 0), h->P);
         CHECK_LAUNCH("k2_moments");
     }
     int k = 0;
@@ -1115,7 +1244,7 @@ static int run_steps2(vlbm2d_handle *h, T *f0, T *f1, double time, int64_t steps
     }
     for (; k < nsteps; ++k) {
         const bool next = k + 1 < nsteps;
-        int rc = launch_step2<T>(h, h->P, f0, f1, time, steps, next);
+        int rc = launch_step2<T>(h, h->P, f0, f1, time, steps, next, k & 1);
         if (rc) return rc;
         T *tmp = f0;
         f0 = f1;
@@ -1146,6 +1275,15 @@ int vlbm2d_create(const vlbm2d_desc *desc, vlbm2d_handle **out) {
         desc->recon != R_LOCALF_BFL)
         return fail_msg("vlbm2d_create: reconstruction not provided (bfl, lsq_bfl, compat_bfl, local_f, local_f_bfl)");
     if (desc->disp_sweeps < 1 || desc->disp_sweeps > 8) return fail_msg("vlbm2d_create: disp_sweeps must be in [1,8]");
+    if (desc->collision_model != 0 && desc->collision_model != 1)
+        return fail_msg("vlbm2d_create: collision_model must be 0 (bgk) or 1 (mrt_ghost)");
+    if (desc->collision_model == 1) {
+        if (!(desc->ghost_omega > 0.0 && desc->ghost_omega <= 2.0))
+            return fail_msg("vlbm2d_create: ghost_omega must be in (0,2]");
+        if (!desc->ghost) return fail_msg("vlbm2d_create: ghost is null (mrt_ghost)");
+    }
+    if (!(desc->kinetic_filter >= 0.0 && desc->kinetic_filter <= 1.0))
+        return fail_msg("vlbm2d_create: kinetic_filter must be in [0,1]");
     const void *req[] = {desc->U, desc->u, desc->u_star, desc->u_next, desc->record, desc->invalid,
                          desc->reduce_scratch, desc->active, desc->active_list, desc->anchor_off,
                          desc->kind_by_id, desc->mode_by_id, desc->params_by_id};
@@ -1215,10 +1353,10 @@ int vlbm2d_refresh(vlbm2d_handle *h, const void *f) {
     const int gn = xcd_grid(d.n_active);
     if (d.precision == VLBM_PREC_F64)
         hipLaunchKernelGGL((k2_moments<double, false>), dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)f, d.U,
-                           d.u, d.u_star, d.invalid, d.active_list, h->P);
+                           d.u, d.u_star, d.invalid, d.active_list, (double *)nullptr, h->P);
     else
         hipLaunchKernelGGL((k2_moments<float, false>), dim3(gn), dim3(BLOCK), 0, h->stream, (const float *)f, d.U,
-                           d.u, d.u_star, d.invalid, d.active_list, h->P);
+                           d.u, d.u_star, d.invalid, d.active_list, (double *)nullptr, h->P);
     CHECK_LAUNCH("k2_refresh");
     return 0;
 }
@@ -1271,6 +1409,18 @@ int vlbm2d_post_collision(vlbm2d_handle *h, const void *f_prev, void *fpost) {
         hipLaunchKernelGGL(k2_post<float>, dim3(g), dim3(BLOCK), 0, h->stream, (const float *)f_prev,
                            (float *)fpost, (const float *)h->d.record, h->d.active_list, h->P);
     CHECK_LAUNCH("k2_post");
+    return 0;
+}
+
+int vlbm2d_kinetic_filter(vlbm2d_handle *h, void *f) {
+    if (!h || !f) return fail_msg("vlbm2d_kinetic_filter: null argument");
+    if (h->P.filt <= 0.0) return 0;
+    const int g = (h->d.n_active + BLOCK - 1) / BLOCK;
+    if (h->d.precision == VLBM_PREC_F64)
+        hipLaunchKernelGGL(k2_filter<double>, dim3(g), dim3(BLOCK), 0, h->stream, (double *)f, h->d.active_list, h->P);
+    else
+        hipLaunchKernelGGL(k2_filter<float>, dim3(g), dim3(BLOCK), 0, h->stream, (float *)f, h->d.active_list, h->P);
+    CHECK_LAUNCH("k2_filter");
     return 0;
 }
 

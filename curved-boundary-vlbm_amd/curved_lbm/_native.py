@@ -16,7 +16,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libvlbm.so"
 
 ABI_VERSION = 5
-ABI2_VERSION = 3
+ABI2_VERSION = 4
 PREC_F64 = 0
 PREC_F32 = 1
 
@@ -140,6 +140,9 @@ class Vlbm2dDesc(C.Structure):
         ("omega", C.c_double),
         ("dx", C.c_double),
         ("dy", C.c_double),
+        ("collision_model", C.c_int32),
+        ("ghost_omega", C.c_double),
+        ("kinetic_filter", C.c_double),
         ("source_mode", C.c_int32),
         ("source_params", C.c_double * 6),
         ("recon", C.c_int32),
@@ -158,6 +161,7 @@ class Vlbm2dDesc(C.Structure):
         ("u_next", C.c_void_p),
         ("record", C.c_void_p),
         ("invalid", C.c_void_p),
+        ("ghost", C.c_void_p),
         ("reduce_scratch", C.c_void_p),
         ("active", C.c_void_p),
         ("active_list", C.c_void_p),
@@ -210,6 +214,7 @@ EXPORTED = (
     "vlbm2d_set_ustar",
     "vlbm2d_stress",
     "vlbm2d_post_collision",
+    "vlbm2d_kinetic_filter",
     "vlbm2d_set_dynamic_traction",
     "vlbm2d_set_graph_steps",
     "vlbm2d_reduce",
@@ -267,6 +272,7 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vlbm2d_set_ustar": [vp, vp],
         "vlbm2d_stress": [vp, vp, vp],
         "vlbm2d_post_collision": [vp, vp, vp],
+        "vlbm2d_kinetic_filter": [vp, vp],
         "vlbm2d_set_dynamic_traction": [vp, C.c_int32],
         "vlbm2d_set_graph_steps": [vp, C.c_int32],
         "vlbm2d_reduce": [vp, vp, _dp],

@@ -777,17 +777,13 @@ class WarpCurvedBoundaryLBM2D:
             raise ValueError("collision_omega must be in (0,2]")
         if collision_model not in ("bgk", "mrt_ghost"):
             raise ValueError("collision_model must be 'bgk' or 'mrt_ghost'")
-        if collision_model != "bgk":
-            raise ValueError("collision_model 'mrt_ghost' is not provided by the HIP solver")
-        self.collision_model = "bgk"
+        self.collision_model = collision_model
         self.ghost_omega = float(self.collision_omega if ghost_omega is None else ghost_omega)
         if not 0.0 < self.ghost_omega <= 2.0:
             raise ValueError("ghost_omega must be in (0,2]")
         self.kinetic_filter_strength = float(kinetic_filter_strength)
         if not 0.0 <= self.kinetic_filter_strength <= 1.0:
             raise ValueError("kinetic_filter_strength must be in [0,1]")
-        if self.kinetic_filter_strength > 0.0:
-            raise ValueError("the kinetic ghost filter is not provided by the HIP solver")
         self.boundary_reconstruction = boundary_reconstruction_id(boundary_reconstruction)
         self.boundary_reconstruction_name = boundary_reconstruction_name(self.boundary_reconstruction)
         if self.boundary_reconstruction not in PROVIDED_RECONSTRUCTIONS:
@@ -970,6 +966,8 @@ class WarpCurvedBoundaryLBM2D:
         self.link_state = DeviceArray.zeros((19, max(L, 1)), f64, dev)
         self.invalid = DeviceArray.zeros((1,), i32, dev)
         self._reduce_scratch = DeviceArray.zeros((_native.RED_SCRATCH,), f64, dev)
+        # ghost moments m3 = fE - fN + fW - fS of the pre-collision populations, two step parities
+        self._ghost = DeviceArray.zeros((2, NCOMP, nx, ny), f64, dev) if self.collision_model == "mrt_ghost" else None
         self._stream = torch.cuda.current_stream(dev)
 
         d = _native.Vlbm2dDesc()
@@ -981,6 +979,9 @@ class WarpCurvedBoundaryLBM2D:
         d.model, d.lam, d.mu, d.p0, d.p1, d.p2 = m.id, m.lam, m.mu, m.param0, m.param1, m.param2
         d.j_floor = self.jacobian_floor
         d.dt, d.c, d.omega, d.dx, d.dy = self.dt, self.lattice_speed, self.collision_omega, self.dx, self.dy
+        d.collision_model = 1 if self.collision_model == "mrt_ghost" else 0
+        d.ghost_omega = self.ghost_omega
+        d.kinetic_filter = self.kinetic_filter_strength
         d.source_mode = self.source_mode
         for k in range(NCOMP):
             d.source_params[k] = float(self.source_params[k])
@@ -1005,6 +1006,8 @@ class WarpCurvedBoundaryLBM2D:
             "mode_by_id": self.boundary_mode_by_id, "params_by_id": self.boundary_params_by_id,
             "dyn_tx": self.link_traction_x, "dyn_ty": self.link_traction_y,
         }
+        if self._ghost is not None:
+            ptrs["ghost"] = self._ghost
         for name, arr in ptrs.items():
             setattr(d, name, arr.ptr or None)
         d.use_dyn = 0
@@ -1134,6 +1137,8 @@ class WarpCurvedBoundaryLBM2D:
         self.refresh_current()
         if apply_initial_boundaries:
             self.apply_boundary_reconstruction(float(initial_boundary_time), in_place=True)
+            if self.kinetic_filter_strength > 0.0:  # cbw2d:3947-3953
+                self._check(self._lib.vlbm2d_kinetic_filter(self._handle, self._pops[0].ptr), "kinetic_filter")
             self.refresh_current()
 
     # ---- time stepping (cbw2d:4020-4306) -------------------------------------

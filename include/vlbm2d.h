@@ -28,6 +28,8 @@
  *   vlbm2d_post_collision  the fpost array left by _collide_masked_kernel and
  *                          _local_compatibility_repair_kernel (cbw2d:1692-1843, 3522-3615)
  *   vlbm2d_reduce          host reductions of summary()/invalid_state() (cbw2d:4307-4341)
+ *   vlbm2d_kinetic_filter  _kinetic_ghost_filter_kernel (cbw2d:3100-3124) as launched by
+ *                          initialize_from_numpy (cbw2d:3947-3953)
  *   vlbm2d_destroy         object lifetime end
  */
 #ifndef VLBM2D_H
@@ -41,7 +43,7 @@
 extern "C" {
 #endif
 
-#define VLBM2D_ABI_VERSION 3
+#define VLBM2D_ABI_VERSION 4
 #define VLBM2D_MAX_IDS 8
 #define VLBM2D_MAX_LSQ 24
 #define VLBM2D_RECORD 12
@@ -60,6 +62,9 @@ typedef struct {
     double lam, mu, p0, p1, p2, j_floor;
     /* lattice and collision */
     double dt, c, omega, dx, dy;
+    int32_t collision_model; /* 0 bgk, 1 mrt_ghost (moment-space collide, cbw2d:1847-2041) */
+    double ghost_omega;      /* mrt_ghost relaxation rate of the ghost moment, (0, 2] */
+    double kinetic_filter;   /* strength in [0, 1] of the ghost filter after every step (cbw2d:3100-3124) */
     int32_t source_mode;
     double source_params[6];
     int32_t recon;         /* reference reconstruction id: 0 bfl, 2 lsq_bfl, 9 compat_bfl, 12 local_f,
@@ -77,6 +82,8 @@ typedef struct {
     void *record;                    /* (12, nx, ny) deferred-collision record, in the population
                                         precision (float: A - omega f_rest on the A planes) */
     int32_t *invalid;                /* (1) */
+    double *ghost;                   /* (2, 6, nx, ny) ghost moments of the pre-collision populations,
+                                        two step parities (mrt_ghost only; else may be NULL) */
     double *reduce_scratch;          /* (>= 8 * 1025) */
     /* tables (device) */
     const int32_t *active;           /* (nx, ny) 0/1 */
@@ -117,8 +124,13 @@ int vlbm2d_set_ustar(vlbm2d_handle *h, const double *disp);
 /* First Piola stress P and Cauchy stress sigma (4, nx, ny) each from the F of U. */
 int vlbm2d_stress(vlbm2d_handle *h, double *P, double *sigma);
 /* Post-collision populations of the last step: fpost = (1 - omega) f_prev +
- * record terms, with f_prev the populations that step started from. */
+ * record terms (+ the ghost-moment term of mrt_ghost), with f_prev the
+ * populations that step started from. */
 int vlbm2d_post_collision(vlbm2d_handle *h, const void *f_prev, void *fpost);
+/* Kinetic ghost filter of the descriptor's strength, in place on f at the
+ * active nodes (no-op at strength 0).  vlbm2d_step applies it after every
+ * step by itself. */
+int vlbm2d_kinetic_filter(vlbm2d_handle *h, void *f);
 /* Enables (1) or disables (0) the per-link traction override on Neumann links. */
 int vlbm2d_set_dynamic_traction(vlbm2d_handle *h, int32_t enabled);
 /* Replays blocks of `steps` fused steps (an even count; 0 disables) from a

@@ -39,6 +39,7 @@ class _State(C.Structure):
         ("disp_relax", C.c_double), ("disp_weight", C.c_double), ("disp_boundary_id", C.c_int32),
         ("compat_interval", C.c_int32), ("compat_blend", C.c_double), ("compat_interior_only", C.c_int32),
         ("use_dyn", C.c_int32),
+        ("collision_model", C.c_int32), ("ghost_omega", C.c_double), ("filter_strength", C.c_double),
         ("f0", _dp), ("f1", _dp), ("fpost", _dp),
         ("U", _dp), ("u", _dp), ("u_star", _dp), ("u_next", _dp),
         ("bc_diag", _dp), ("bc_rhs", _dp), ("P", _dp), ("sigma", _dp), ("invalid", _dp),
@@ -75,6 +76,8 @@ def load():
         lib.or2d_boundary_value.argtypes = [C.c_int, C.c_int, _dp] + [C.c_double] * 5
         lib.or2d_boundary_value.restype = C.c_double
         lib.or2d_du.argtypes = [S, _dp, _dp]
+        lib.or2d_kinetic_filter.argtypes = [S, _dp]
+        lib.or2d_kinetic_filter.restype = None
         lib.or2d_du.restype = None
         lib.or2d_equilibrium.argtypes = [S, _dp, _dp]
         lib.or2d_equilibrium.restype = None
@@ -113,7 +116,8 @@ class OracleSolver2D:
                  local_compatibility_blend=0.05, local_compatibility_interior_only=True,
                  local_displacement_interval=0, local_displacement_sweeps=1, local_displacement_relax=0.85,
                  local_displacement_boundary_weight=20.0, local_displacement_boundary_id=2,
-                 normal_jacobian=None, neumann_solver=None, threads: int | None = None, **_ignored):
+                 normal_jacobian=None, neumann_solver=None, collision_model="bgk", ghost_omega=None,
+                 kinetic_filter_strength=0.0, threads: int | None = None, **_ignored):
         from curved_lbm.two_d import curved_boundary_warp as cb
         from curved_lbm.two_d.vector_nonlinear_elastic_warp import source_id
 
@@ -178,6 +182,11 @@ class OracleSolver2D:
         s.compat_blend = float(local_compatibility_blend)
         s.compat_interior_only = int(bool(local_compatibility_interior_only))
         s.use_dyn = 0
+        if collision_model not in ("bgk", "mrt_ghost"):
+            raise ValueError("collision_model must be 'bgk' or 'mrt_ghost'")
+        s.collision_model = 1 if collision_model == "mrt_ghost" else 0
+        s.ghost_omega = float(self.collision_omega if ghost_omega is None else ghost_omega)
+        s.filter_strength = float(kinetic_filter_strength)
         for name in ("f0", "f1", "fpost", "U", "u", "u_star", "u_next", "bc_diag", "bc_rhs", "P", "sigma",
                      "invalid"):
             setattr(s, name, _p(getattr(self, name)))
@@ -235,6 +244,8 @@ class OracleSolver2D:
         self.refresh_current()
         if apply_initial_boundaries:
             self.apply_boundary_reconstruction(float(initial_boundary_time), in_place=True)
+            if self._s.filter_strength > 0.0:
+                self.lib.or2d_kinetic_filter(C.byref(self._s), _p(self.f0))
             self.refresh_current()
 
     def _check(self, rc: int) -> None:

@@ -447,6 +447,27 @@ static void collide(or2d_state *s) {
             s->sigma[IDX(s, 1, i, j)] = (P[0] * U[4] + P[1] * U[5]) / Js;
             s->sigma[IDX(s, 2, i, j)] = (P[2] * U[2] + P[3] * U[3]) / Js;
             s->sigma[IDX(s, 3, i, j)] = (P[2] * U[4] + P[3] * U[5]) / Js;
+            if (s->collision_model == 1) {
+                /* cbw2d:1847-2041 (_collide_masked_mrt_ghost_kernel): moments
+                 * m0 = E+N+W+S, m1 = E-W, m2 = N-S, m3 = E-N+W-S; m0 relaxed to
+                 * U_a with the source, m1, m2 to the flux / c with omega, the
+                 * ghost m3 to zero with ghost_omega. */
+                const double og = s->ghost_omega;
+                for (int a = 0; a < NC; ++a) {
+                    double fE = s->f0[IDX(s, 0 * 6 + a, i, j)], fN = s->f0[IDX(s, 1 * 6 + a, i, j)];
+                    double fW = s->f0[IDX(s, 2 * 6 + a, i, j)], fS = s->f0[IDX(s, 3 * 6 + a, i, j)];
+                    double m0 = fE + fN + fW + fS, m1 = fE - fW, m2 = fN - fS, m3 = fE - fN + fW - fS;
+                    double m0p = m0 + om * (U[a] - m0) + dt * (2.0 - om) * 0.5 * b[a];
+                    double m1p = m1 + om * (flux_x(a, U, P) / s->c - m1);
+                    double m2p = m2 + om * (flux_y(a, U, P) / s->c - m2);
+                    double m3p = (1.0 - og) * m3;
+                    s->fpost[IDX(s, 0 * 6 + a, i, j)] = 0.25 * (m0p + m3p + 2.0 * m1p);
+                    s->fpost[IDX(s, 1 * 6 + a, i, j)] = 0.25 * (m0p - m3p + 2.0 * m2p);
+                    s->fpost[IDX(s, 2 * 6 + a, i, j)] = 0.25 * (m0p + m3p - 2.0 * m1p);
+                    s->fpost[IDX(s, 3 * 6 + a, i, j)] = 0.25 * (m0p - m3p - 2.0 * m2p);
+                }
+                continue;
+            }
             for (int q = 0; q < NQ; ++q)
                 for (int a = 0; a < NC; ++a) {
                     double fe = feq(q, a, U, P, s->c);
@@ -456,6 +477,30 @@ static void collide(or2d_state *s) {
         }
     }
     if (invalid) s->invalid[0] = 1;
+}
+
+/* cbw2d:3100-3124 (_kinetic_ghost_filter_kernel): the ghost moment
+ * m3 = E-N+W-S of every component is scaled by (1 - strength) through
+ * +-m3 strength / 4 on E, W and N, S; U and the flux moments are unchanged. */
+void or2d_kinetic_filter(or2d_state *s, double *f) {
+    const int nx = s->nx, ny = s->ny;
+    const double k = s->filter_strength;
+    if (k <= 0.0) return;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+            if (s->active[(size_t)i * ny + j] == 0) continue;
+            for (int a = 0; a < NC; ++a) {
+                const size_t e = IDX(s, 0 * 6 + a, i, j), n = IDX(s, 1 * 6 + a, i, j);
+                const size_t w = IDX(s, 2 * 6 + a, i, j), so = IDX(s, 3 * 6 + a, i, j);
+                double m3 = f[e] - f[n] + f[w] - f[so];
+                double quarter = 0.25 * (-k * m3);
+                f[e] = f[e] + quarter;
+                f[w] = f[w] + quarter;
+                f[n] = f[n] - quarter;
+                f[so] = f[so] - quarter;
+            }
+        }
 }
 
 /* cbw2d:3128-3210 (_refresh_masked_kernel) on f0. */
@@ -925,6 +970,7 @@ int or2d_step(or2d_state *s, double time, int64_t steps, int32_t nsteps) {
         if (s->compat_interval > 0 && steps % s->compat_interval == 0) repair_F(s);
         stream_push(s);
         if (s->n_links > 0) apply_links(s, s->fpost, s->f1, tb);
+        or2d_kinetic_filter(s, s->f1);
         double *t = f0;
         f0 = f1;
         f1 = t;

@@ -29,6 +29,9 @@ typedef struct {
     double compat_blend;
     int32_t compat_interior_only;
     int32_t use_dyn;
+    int32_t collision_model;           /* 0 bgk, 1 mrt_ghost */
+    double ghost_omega;                /* mrt_ghost relaxation of the ghost moment */
+    double filter_strength;            /* kinetic ghost filter after each step, [0, 1] */
     /* state */
     double *f0, *f1, *fpost;           /* (24, nx, ny) */
     double *U, *u, *u_star, *u_next;   /* (6|2, nx, ny) */
@@ -60,5 +63,6 @@ double or2d_boundary_value(int comp, int mode, const double *p, double x, double
 void or2d_du(const or2d_state *s, const double *u, double *out); /* (4, nx, ny): dux/dx, dux/dy, duy/dx, duy/dy */
 void or2d_equilibrium(const or2d_state *s, const double *U, double *f); /* U (6) -> f (24) */
 int or2d_threads(void);
+void or2d_kinetic_filter(or2d_state *s, double *f); /* in place on (24, nx, ny) */
 
 #endif

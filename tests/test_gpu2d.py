@@ -32,6 +32,11 @@ def assert_parity(gpu, ora, *, f_atol=1e-12, field_rel=1e-10, floor=1e-14):
     assert fd <= f_atol, ("f0", fd)
 
 
+def _with(case, **over):
+    g, kw, init = case
+    return g, dict(kw, **over), init
+
+
 CASES = {
     "annulus_local_f": lambda: gc.annulus(40),
     "annulus_local_f_interior_only": lambda: gc.annulus(40, local_compatibility_interior_only=True),
@@ -55,6 +60,13 @@ CASES = {
     "circle_bfl_affine_velocity": lambda: gc.circle_dirichlet(32, "bfl", affine=True),
     "circle_compat_bfl_affine_velocity": lambda: gc.circle_dirichlet(32, "compat_bfl", affine=True),
     "circle_piola_traction_constant_source": lambda: gc.circle_traction(32),
+    "annulus_mrt_ghost": lambda: gc.annulus(36, collision_model="mrt_ghost", ghost_omega=1.2),
+    "annulus_kinetic_filter": lambda: gc.annulus(36, kinetic_filter_strength=0.15),
+    "annulus_mrt_ghost_filter_compat_bfl": lambda: gc.annulus(
+        36, collision_model="mrt_ghost", ghost_omega=1.5, kinetic_filter_strength=0.05,
+        boundary_reconstruction="compat_bfl"),
+    "circle_mrt_ghost_filter_initial_state": lambda: _with(gc.circle_traction(32), collision_model="mrt_ghost",
+                                                           ghost_omega=0.9, kinetic_filter_strength=0.1),
 }
 
 
@@ -74,8 +86,9 @@ def test_gpu_matches_oracle(case):
     assert red["nonfinite"] == 0 and red["min_J"] > 0
 
 
-def test_gpu_fpost_stress_and_refresh_match_oracle():
-    g, kw, _ = gc.annulus(32)
+@pytest.mark.parametrize("over", [{}, {"collision_model": "mrt_ghost", "ghost_omega": 1.1}])
+def test_gpu_fpost_stress_and_refresh_match_oracle(over):
+    g, kw, _ = gc.annulus(32, **over)
     gpu = cb.WarpCurvedBoundaryLBM2D(g, **kw)
     ora = OracleSolver2D(g, **kw)
     gpu.run(17)
@@ -98,8 +111,10 @@ def test_gpu_fpost_stress_and_refresh_match_oracle():
     assert links > 0
 
 
-def test_gpu_single_steps_equal_batched_run():
-    g, kw, _ = gc.annulus(32)
+@pytest.mark.parametrize("over", [{}, {"collision_model": "mrt_ghost", "ghost_omega": 1.3,
+                                      "kinetic_filter_strength": 0.1}])
+def test_gpu_single_steps_equal_batched_run(over):
+    g, kw, _ = gc.annulus(32, **over)
     a = cb.WarpCurvedBoundaryLBM2D(g, **kw)
     b = cb.WarpCurvedBoundaryLBM2D(g, **kw)
     for _ in range(11):
@@ -219,8 +234,9 @@ def test_gpu_second_order_initialisation_matches_oracle(order):
 
 def test_gpu_rejects_unprovided_options():
     g, kw, _ = gc.annulus(16)
-    for over in (dict(boundary_reconstruction="qbfl"), dict(collision_model="mrt_ghost"),
-                 dict(kinetic_filter_strength=0.1), dict(compatibility_projection_interval=1)):
+    for over in (dict(boundary_reconstruction="qbfl"), dict(collision_model="mrt"),
+                 dict(collision_model="mrt_ghost", ghost_omega=2.5), dict(kinetic_filter_strength=1.5),
+                 dict(compatibility_projection_interval=1)):
         with pytest.raises(ValueError):
             cb.WarpCurvedBoundaryLBM2D(g, **{**kw, **over})
 
@@ -247,12 +263,14 @@ def test_gpu_annulus96_3000_steps_matches_oracle():
     assert np.max(np.abs(Jg - Jo)) <= 1e-10 and Jg.min() > 0.0
 
 
-def test_gpu_graph_step_loop_matches_direct_launches():
+@pytest.mark.parametrize("over", [{}, {"collision_model": "mrt_ghost", "ghost_omega": 1.3,
+                                      "kinetic_filter_strength": 0.1}])
+def test_gpu_graph_step_loop_matches_direct_launches(over):
     """Captured step graphs replayed across several calls, with eager work
     (refresh, reductions, reads) between them, give bitwise the populations,
     fields and flags of direct launches; odd and even call lengths, and a
     toggle of the per-link traction override (a captured argument)."""
-    g, kw, _ = gc.annulus(40)
+    g, kw, _ = gc.annulus(40, **over)
     a = cb.WarpCurvedBoundaryLBM2D(g, **kw, graph_steps=0)
     b = cb.WarpCurvedBoundaryLBM2D(g, **kw, graph_steps=8)
     outer = g.link_boundary_id == cb.BOUNDARY_ID_OUTER

@@ -244,3 +244,98 @@ def test_compat_projection_keeps_a_compatible_affine_state():
     written = np.isfinite(out["bfl"])
     assert written.sum() > 100 and np.array_equal(written, np.isfinite(out["compat_bfl"]))
     assert np.max(np.abs(out["bfl"][written] - out["compat_bfl"][written])) < 1e-12
+
+
+def _ghost_case(**over):
+    g = cb.CurvedBoundaryGeometry.annulus(nx=32, ny=32, inner_radius=0.2, outer_radius=0.45)
+    spec = cb.CurvedBoundarySpec(kind="neumann", value_mode=cb.CURVED_VALUE_RADIAL_TRACTION_LINEAR_RAMP,
+                                 params=(0.05, 0.02), id_kinds={1: "neumann", 2: "dirichlet"},
+                                 id_value_modes={1: cb.CURVED_VALUE_RADIAL_TRACTION_LINEAR_RAMP, 2: 0},
+                                 id_params={1: (0.05, 0.02), 2: ()})
+    kw = dict(material=NH, boundary=spec, lattice_speed=10.0, collision_omega=1.8, source_mode="damping",
+              damping_gamma=10.0, boundary_reconstruction="local_f")
+    kw.update(over)
+    return g, kw
+
+
+def _moments(f):
+    """m0..m3 of every component: (4, 6, nx, ny) from (24, nx, ny), q = E, N, W, S."""
+    E, N, W, S = (f[q * 6:(q + 1) * 6] for q in range(4))
+    return np.stack([E + N + W + S, E - W, N - S, E - N + W - S])
+
+
+def _perturbed(s, seed):
+    rng = np.random.default_rng(seed)
+    act = s._tabs["active"] != 0
+    f = s.f0.copy()
+    f[:, act] += 1e-3 * rng.standard_normal((24, int(act.sum())))
+    return f, act
+
+
+def test_mrt_ghost_collide_relaxes_only_the_ghost_moment_differently():
+    """cbw2d:1847-2041 against the moment closed form: from the same
+    pre-collision populations, mrt_ghost and bgk give the same m0, m1, m2
+    (the hydrodynamic moments), and the ghost moment m3 = E - N + W - S
+    (zero in every equilibrium and source term) relaxes to (1 - omega_g) m3
+    versus bgk's (1 - omega) m3.  A swapped E/N sign or a wrong ghost rate
+    fails this."""
+    out = {}
+    for model, og in (("bgk", None), ("mrt_ghost", 0.7)):
+        g, kw = _ghost_case(collision_model=model, ghost_omega=og)
+        s = OracleSolver2D(g, **kw)
+        f, act = _perturbed(s, 31)
+        s.f0[...] = f
+        s.run(1)
+        out[model] = (_moments(f)[:, :, act], _moments(s.fpost)[:, :, act])
+    m_pre, bgk = out["bgk"]
+    _, mrt = out["mrt_ghost"]
+    assert np.max(np.abs(m_pre[3])) > 1e-4
+    np.testing.assert_allclose(mrt[:3], bgk[:3], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(bgk[3], (1.0 - 1.8) * m_pre[3], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(mrt[3], (1.0 - 0.7) * m_pre[3], rtol=0, atol=1e-14)
+
+
+def test_mrt_ghost_with_the_bgk_rate_is_bgk():
+    """omega_ghost = omega recovers the BGK collision (cbw2d:1889-1890), to
+    the rounding of the moment-space form (1e-12 after 30 steps)."""
+    res = []
+    for model in ("bgk", "mrt_ghost"):
+        g, kw = _ghost_case(collision_model=model)
+        s = OracleSolver2D(g, **kw)
+        s.run(30)
+        res.append(s.f0.copy())
+    np.testing.assert_allclose(res[1], res[0], rtol=0, atol=1e-12)
+    assert np.max(np.abs(res[0])) > 0.1
+
+
+def test_kinetic_filter_scales_the_ghost_moment_only():
+    """cbw2d:3100-3124: m3 -> (1 - s) m3 with m0, m1, m2 unchanged, at
+    active nodes only."""
+    g, kw = _ghost_case(kinetic_filter_strength=0.3)
+    s = OracleSolver2D(g, **kw)
+    f, act = _perturbed(s, 5)
+    f[:, ~act] = 7.0
+    h = f.copy()
+    s.lib.or2d_kinetic_filter(C.byref(s._s), o2._p(h))
+    m0, m1 = _moments(f), _moments(h)
+    np.testing.assert_allclose(m1[:3][:, :, act], m0[:3][:, :, act], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(m1[3][:, act], 0.7 * m0[3][:, act], rtol=0, atol=1e-15)
+    assert np.all(h[:, ~act] == 7.0)
+
+
+def test_kinetic_filter_runs_after_the_boundary_links_of_each_step():
+    """One step with the filter equals the unfiltered step followed by the
+    filter (reference order: stream, links, filter; cbw2d:4245-4254)."""
+    res, f = {}, None
+    for k in (0.0, 0.25):
+        g, kw = _ghost_case(kinetic_filter_strength=k)
+        s = OracleSolver2D(g, **kw)
+        if f is None:
+            f, _ = _perturbed(s, 9)
+        s.f0[...] = f
+        s.run(1)
+        res[k] = (s, s.f0.copy())
+    s, plain = res[0.0]
+    s._s.filter_strength = 0.25
+    s.lib.or2d_kinetic_filter(C.byref(s._s), o2._p(plain))
+    np.testing.assert_array_equal(res[0.25][1], plain)
