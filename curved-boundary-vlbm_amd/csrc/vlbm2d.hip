@@ -471,17 +471,20 @@ __global__ __launch_bounds__(256) void k2_gather(const double *__restrict__ U, c
     cs[(size_t)v * L + e2] = acc;
 }
 
-// Incoming populations of every cut link (cbw2d:2064-3098 for the bfl,
-// lsq_bfl, local_f and local_f_bfl reconstructions).  use_rec selects
-// post-collision sources (time step: pre-collision f plus this step's
-// records) or populations read as they are (standalone reconstruction).
-template <typename T>
+// Incoming populations of every cut link (cbw2d:2064-3098).  use_rec
+// selects post-collision sources (time step: pre-collision f plus this
+// step's records and, for mrt_ghost, the ghost term of gh) or populations
+// read as they are (standalone reconstruction).  EXT instantiates the
+// regularized, NEE, quadratic and polynomial variants (recon_ext), so the
+// kernel of the other reconstructions carries none of their code.
+template <typename T, bool EXT>
 __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T *__restrict__ fdst,
                                                    const T *__restrict__ rec, const double *__restrict__ U,
                                                    const double *__restrict__ cs, Links2 Tb, int use_rec,
                                                    const double *__restrict__ gh, double time, Params2 P) {
     const int ell = blockIdx.x * LBLOCK + threadIdx.x;
     if (ell >= P.L) return;
+    const int r = P.recon;
     const size_t N = P.N;
     const int L = P.L;
     const int p = Tb.lp[ell], qo = Tb.lq[ell], qi = (qo + 2) & 3;
@@ -504,19 +507,40 @@ __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T
     const bool up_ok = iu >= 0 && iu < P.nx && ju >= 0 && ju < ny && Tb.act[iu * ny + ju] != 0;
     const int pu = up_ok ? iu * ny + ju : p;
     const bool valid = Tb.valid[ell] != 0;
+    const double om1 = 1.0 - P.omega;
+    // post-collision population q, component a of node nd
+    auto fps = [&](int q, int a, int nd) -> double {
+        const size_t k = (size_t)(q * 6 + a) * N + nd;
+        if (!use_rec) return ld(fsrc, k, a);
+        double v = fpost_v(ld(fsrc, k, a), rec, N, q, a, nd, om1, P.omega);
+        if (gh) v += ghost_term(gh, N, q, a, nd, P.gw);
+        return v;
+    };
+    // Boundary F predictor (cbw2d:2162-2250): one-link extrapolation for the
+    // bfl family, else the quadratic one-sided interpolation at the cut
+    // point; the LSQ value for lsq_bfl, local_f and local_char2.
     double Fg[4];
-    if (P.recon == R_LOCALF) {
+    if (r == R_BFL || r == R_REG_BFL || r == R_LSQ_BFL || r == R_LOCALF_BFL) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double uc = U[(2 + k) * N + p];
+            Fg[k] = up_ok ? uc + eta * (uc - U[(2 + k) * N + pu]) : uc;
+        }
+        if (r == R_LSQ_BFL && valid)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) Fg[k] = cs[(size_t)k * L + ell];
+    } else {
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            Fg[k] = valid ? cs[(size_t)k * L + ell] : interp_link(U, Tb.act, N, 2 + k, p, i, j, qo, eta, P);
-    } else if (P.recon == R_COMPAT_BFL) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) Fg[k] = interp_link(U, Tb.act, N, 2 + k, p, i, j, qo, eta, P);
+            Fg[k] = ((r == R_LOCALF || r == R_CHAR2) && valid) ? cs[(size_t)k * L + ell]
+                                                              : interp_link(U, Tb.act, N, 2 + k, p, i, j, qo, eta, P);
+    }
+    {
         // Geometry-compatible predictor (cbw2d:2206-2250): F n from the link
         // predictor, F t = t + d_t u (traction links: the LSQ tangential
         // derivative of u; Dirichlet links: the time integral of an affine
         // velocity's tangential derivative, zero for other modes).
-        if (valid) {
+        if (recon_compat(r) && valid) {
             const double tanx = -nyb, tany = nxb;
             double dut0 = 0.0, dut1 = 0.0, wt_abs = 1.0;
             if (kind == BC_NEU) {
@@ -538,23 +562,74 @@ __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T
                 Fg[3] = hn1 * nyb + gt1 * tany;
             }
         }
-    } else {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const double uc = U[(2 + k) * N + p];
-            Fg[k] = up_ok ? uc + eta * (uc - U[(2 + k) * N + pu]) : uc;
-        }
-        if (P.recon == R_LSQ_BFL && valid)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) Fg[k] = cs[(size_t)k * L + ell];
     }
     double Fb[4] = {Fg[0], Fg[1], Fg[2], Fg[3]};
     if (kind == BC_NEU) solve_normal2(Fg, nxb, nyb, bv0, bv1, P.solver, P.m, Fb);
+
+    double node[6], Pn[4];
+    if constexpr (EXT) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) node[k] = U[(size_t)k * N + p];
+        piola2(node + 2, P.m, Pn);
+    }
+    // fpost - feq(st) of node nd with the D2Q4 ghost moment removed
+    // (cbw2d:1517-1643).
+    auto hreg = [&](int q, int a, int nd, const double *stt, const double *Ps) -> double {
+        double h[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) h[k] = fps(k, a, nd) - feq2(k, a, stt, Ps, P.c);
+        const double corr = 0.25 * (h[0] - h[1] + h[2] - h[3]);
+        return (q == 0 || q == 2) ? h[q] - corr : h[q] + corr;
+    };
+    if constexpr (EXT) {
+        if (r == R_RNEE) {
+            // cbw2d:2276-2356: ghost state mirrored through the boundary datum.
+            const double sm = 2.0 * eta - 1.0;
+            double m[6], g[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) m[k] = interp_link(U, Tb.act, N, k, p, i, j, qo, sm, P);
+            if (kind == BC_DIR) {
+                g[0] = 2.0 * bv0 - m[0];
+                g[1] = 2.0 * bv1 - m[1];
+            } else {
+                g[0] = 2.0 * interp_link(U, Tb.act, N, 0, p, i, j, qo, eta, P) - m[0];
+                g[1] = 2.0 * interp_link(U, Tb.act, N, 1, p, i, j, qo, eta, P) - m[1];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) g[2 + k] = 2.0 * Fb[k] - m[2 + k];
+            if (g[2] * g[5] - g[3] * g[4] <= P.m.jf)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) g[2 + k] = Fb[k];
+            double Pg[4];
+            piola2(g + 2, P.m, Pg);
+            for (int a = 0; a < 6; ++a)
+                st(fdst, (size_t)(qi * 6 + a) * N + p, a, feq2(qi, a, g, Pg, P.c) + hreg(qi, a, p, node, Pn));
+            return;
+        }
+        if (r == R_PAIR_NEE || r == R_SAME_NEE || r == R_COMPAT_NEE) {
+            // cbw2d:2358-2454: boundary equilibrium + regularized neq of the
+            // node, the outgoing direction's with the half-way parity (pair,
+            // compat) or the incoming direction's (same).
+            double b[6] = {bv0, bv1, Fb[0], Fb[1], Fb[2], Fb[3]};
+            if (kind == BC_NEU) {
+                b[0] = interp_link(U, Tb.act, N, 0, p, i, j, qo, eta, P);
+                b[1] = interp_link(U, Tb.act, N, 1, p, i, j, qo, eta, P);
+            }
+            double Pbb[4];
+            piola2(b + 2, P.m, Pbb);
+            for (int a = 0; a < 6; ++a) {
+                const double Dp = (kind == BC_DIR) ? (a < 2 ? -1.0 : 1.0) : (a >= 2 ? -1.0 : 1.0);
+                const double h = r == R_SAME_NEE ? hreg(qi, a, p, node, Pn) : Dp * hreg(qo, a, p, node, Pn);
+                st(fdst, (size_t)(qi * 6 + a) * N + p, a, feq2(qi, a, b, Pbb, P.c) + h);
+            }
+            return;
+        }
+    }
     double Pb[4];
     piola2(Fb, P.m, Pb);
 
     // Characteristic closure of the traction rows (cbw2d:2456-2886, 3034-3097).
-    const bool use_char = P.recon == R_LOCALF && kind == BC_NEU && valid;
+    const bool use_char = !EXT && (r == R_LOCALF || r == R_CHAR2) && kind == BC_NEU && valid;
     bool char_ok = false;
     double cv[6], Acw[6], Pc[4];
     if (use_char) {
@@ -598,7 +673,30 @@ __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T
             for (int a = 0; a < 6; ++a) Acw[a] = 0.25 * (cw[a] + (2.0 / P.c) * (dxi * ax[a] + dyi * ay[a]));
         }
     }
-    const double om1 = 1.0 - P.omega;
+    // regularized_bfl / lowq_reg_bfl sources (traction links)
+    const bool use_reg = EXT && kind == BC_NEU && (r == R_REG_BFL || (r == R_LOWQ_REG_BFL && eta < 0.5));
+    double nodeu[6], Pu[4];
+    if (EXT && use_reg && eta < 0.5 && up_ok) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) nodeu[k] = U[(size_t)k * N + pu];
+        piola2(nodeu + 2, P.m, Pu);
+    }
+    // quadratic one-sided interpolation of fpost[qo][a] along the link
+    // (cbw2d:1484-1513)
+    auto interp_fp = [&](int a, double sv) -> double {
+        const double v0 = fps(qo, a, p);
+        if (!up_ok) return v0;
+        const double v1 = fps(qo, a, pu);
+        const int i2 = i - 2 * cx(qo), j2 = j - 2 * cy(qo);
+        if (i2 >= 0 && i2 < P.nx && j2 >= 0 && j2 < ny && Tb.act[i2 * ny + j2] != 0) {
+            const double v2 = fps(qo, a, i2 * ny + j2);
+            const double l0 = 0.5 * (sv + 1.0) * (sv + 2.0);
+            const double l1 = -sv * (sv + 2.0);
+            const double l2 = 0.5 * sv * (sv + 1.0);
+            return l0 * v0 + l1 * v1 + l2 * v2;
+        }
+        return (sv + 1.0) * v0 - sv * v1;
+    };
 #pragma unroll
     for (int a = 0; a < 6; ++a) {
         double val;
@@ -627,27 +725,31 @@ __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T
                 }
             }
             double fo, fi;
-            if (use_rec) {
-                fo = fpost_v(ld(fsrc, (size_t)(qo * 6 + a) * N + p, a), rec, N, qo, a, p, om1, P.omega);
-                fi = fpost_v(ld(fsrc, (size_t)(qi * 6 + a) * N + p, a), rec, N, qi, a, p, om1, P.omega);
-                if (gh) {
-                    fo += ghost_term(gh, N, qo, a, p, P.gw);
-                    fi += ghost_term(gh, N, qi, a, p, P.gw);
-                }
+            if (EXT && use_reg) {
+                fo = feq2(qo, a, node, Pn, P.c) + hreg(qo, a, p, node, Pn);
+                fi = feq2(qi, a, node, Pn, P.c) + hreg(qi, a, p, node, Pn);
             } else {
-                fo = ld(fsrc, (size_t)(qo * 6 + a) * N + p, a);
-                fi = ld(fsrc, (size_t)(qi * 6 + a) * N + p, a);
+                fo = fps(qo, a, p);
+                fi = fps(qi, a, p);
             }
-            if (eta >= 0.5) {
+            if (EXT && r == R_POLY && up_ok && eta > 0.25 && eta < 0.95) {
+                // cbw2d:2977-2993: incoming polynomial through s = -1, 0, eta at s = 1
+                const double gin_b = D * interp_fp(a, eta) + S;
+                const double g1 = fps(qi, a, pu);
+                const double wb = 2.0 / (eta * (eta + 1.0));
+                const double w0 = -2.0 * (1.0 - eta) / eta;
+                const double w1 = (1.0 - eta) / (1.0 + eta);
+                val = wb * gin_b + w0 * fi + w1 * g1;
+            } else if (EXT && r == R_QBFL && eta < 0.5) {
+                // cbw2d:2994-3000: quadratic interpolation at the BFL point s = 2 eta - 1
+                val = D * interp_fp(a, 2.0 * eta - 1.0) + S;
+            } else if (eta >= 0.5) {
                 const double alpha = 1.0 / (2.0 * eta);
                 val = alpha * (D * fo + S) + (1.0 - alpha) * fi;
             } else {
                 double upv = fo;
-                if (up_ok) {
-                    upv = use_rec ? fpost_v(ld(fsrc, (size_t)(qo * 6 + a) * N + pu, a), rec, N, qo, a, pu, om1, P.omega)
-                                  : ld(fsrc, (size_t)(qo * 6 + a) * N + pu, a);
-                    if (use_rec && gh) upv += ghost_term(gh, N, qo, a, pu, P.gw);
-                }
+                if (up_ok) upv = (EXT && use_reg) ? feq2(qo, a, nodeu, Pu, P.c) + hreg(qo, a, pu, nodeu, Pu)
+                                                  : fps(qo, a, pu);
                 val = D * (2.0 * eta * fo + (1.0 - 2.0 * eta) * upv) + S;
             }
         }
@@ -1037,20 +1139,25 @@ template <typename T>
 static int run_links2(vlbm2d_handle *h, const T *src, T *dst, double t, int use_rec) {
     const vlbm2d_desc &d = h->d;
     if (d.n_links == 0) return 0;
-    const bool gather = d.recon == R_LOCALF || d.recon == R_LSQ_BFL || d.recon == R_COMPAT_BFL;
+    const bool gather = recon_gather(d.recon);
     if (gather) {
         TimedSlot tg(h->timing, h->stream, VLBM_SLOT_GATHER);
         const int gg = (d.n_links + GL - 1) / GL;
         hipLaunchKernelGGL(k2_gather, dim3(gg), dim3(256), 0, h->stream, (const double *)d.U, (const double *)d.u,
-                           d.lsq_p, d.lsq_wb, d.lsq_wx, d.lsq_wy, d.lsq_wt, (int)(d.recon == R_COMPAT_BFL),
+                           d.lsq_p, d.lsq_wb, d.lsq_wx, d.lsq_wy, d.lsq_wt, (int)recon_compat(d.recon),
                            d.link_state, d.n_links, h->P.N);
         CHECK_LAUNCH("k2_gather");
     }
     TimedSlot ts(h->timing, h->stream, VLBM_SLOT_LINKS);
     const int gl = (d.n_links + LBLOCK - 1) / LBLOCK;
-    hipLaunchKernelGGL((k2_links<T>), dim3(gl), dim3(LBLOCK), 0, h->stream, src, dst, (const T *)d.record,
-                       (const double *)d.U, (const double *)d.link_state, h->T, use_rec, (const double *)nullptr,
-                       t, h->P);
+    if (recon_ext(d.recon))
+        hipLaunchKernelGGL((k2_links<T, true>), dim3(gl), dim3(LBLOCK), 0, h->stream, src, dst, (const T *)d.record,
+                           (const double *)d.U, (const double *)d.link_state, h->T, use_rec, (const double *)nullptr,
+                           t, h->P);
+    else
+        hipLaunchKernelGGL((k2_links<T, false>), dim3(gl), dim3(LBLOCK), 0, h->stream, src, dst, (const T *)d.record,
+                           (const double *)d.U, (const double *)d.link_state, h->T, use_rec, (const double *)nullptr,
+                           t, h->P);
     CHECK_LAUNCH("k2_links");
     return 0;
 }
@@ -1061,19 +1168,23 @@ static int launch_links2(vlbm2d_handle *h, const Params2 &P, hipStream_t st, con
                          const double *gin) {
     const vlbm2d_desc &d = h->d;
     if (d.n_links == 0) return 0;
-    const bool gather = d.recon == R_LOCALF || d.recon == R_LSQ_BFL || d.recon == R_COMPAT_BFL;
+    const bool gather = recon_gather(d.recon);
     if (gather) {
         TimedSlot tg(h->timing, st, VLBM_SLOT_GATHER);
         const int gg = (d.n_links + GL - 1) / GL;
         hipLaunchKernelGGL(k2_gather, dim3(gg), dim3(256), 0, st, (const double *)d.U, (const double *)d.u,
-                           d.lsq_p, d.lsq_wb, d.lsq_wx, d.lsq_wy, d.lsq_wt, (int)(d.recon == R_COMPAT_BFL),
+                           d.lsq_p, d.lsq_wb, d.lsq_wx, d.lsq_wy, d.lsq_wt, (int)recon_compat(d.recon),
                            d.link_state, d.n_links, P.N);
         CHECK_LAUNCH("k2_gather");
     }
     TimedSlot tl(h->timing, st, VLBM_SLOT_LINKS);
     const int gl = (d.n_links + LBLOCK - 1) / LBLOCK;
-    hipLaunchKernelGGL((k2_links<T>), dim3(gl), dim3(LBLOCK), 0, st, f0, f1, (const T *)d.record,
-                       (const double *)d.U, (const double *)d.link_state, h->T, 1, gin, tb, P);
+    if (recon_ext(d.recon))
+        hipLaunchKernelGGL((k2_links<T, true>), dim3(gl), dim3(LBLOCK), 0, st, f0, f1, (const T *)d.record,
+                           (const double *)d.U, (const double *)d.link_state, h->T, 1, gin, tb, P);
+    else
+        hipLaunchKernelGGL((k2_links<T, false>), dim3(gl), dim3(LBLOCK), 0, st, f0, f1, (const T *)d.record,
+                           (const double *)d.U, (const double *)d.link_state, h->T, 1, gin, tb, P);
     CHECK_LAUNCH("k2_links");
     return 0;
 }
@@ -1271,9 +1382,8 @@ int vlbm2d_create(const vlbm2d_desc *desc, vlbm2d_handle **out) {
     if (desc->n_links < 0) return fail_msg("vlbm2d_create: n_links must be non-negative");
     if (desc->precision != VLBM_PREC_F64 && desc->precision != VLBM_PREC_F32)
         return fail_msg("vlbm2d_create: unknown precision");
-    if (desc->recon != R_BFL && desc->recon != R_LSQ_BFL && desc->recon != R_COMPAT_BFL && desc->recon != R_LOCALF &&
-        desc->recon != R_LOCALF_BFL)
-        return fail_msg("vlbm2d_create: reconstruction not provided (bfl, lsq_bfl, compat_bfl, local_f, local_f_bfl)");
+    if (desc->recon < R_BFL || desc->recon > R_LOCALF_BFL)
+        return fail_msg("vlbm2d_create: unknown reconstruction id (0..13)");
     if (desc->disp_sweeps < 1 || desc->disp_sweeps > 8) return fail_msg("vlbm2d_create: disp_sweeps must be in [1,8]");
     if (desc->collision_model != 0 && desc->collision_model != 1)
         return fail_msg("vlbm2d_create: collision_model must be 0 (bgk) or 1 (mrt_ghost)");
@@ -1296,7 +1406,7 @@ int vlbm2d_create(const vlbm2d_desc *desc, vlbm2d_handle **out) {
                               desc->link_state};
         for (const void *ptr : lreq)
             if (!ptr) return fail_msg("vlbm2d_create: a required link pointer is null");
-        if (desc->recon == R_COMPAT_BFL && !desc->lsq_wt) return fail_msg("vlbm2d_create: lsq_wt is null");
+        if (recon_compat(desc->recon) && !desc->lsq_wt) return fail_msg("vlbm2d_create: lsq_wt is null");
     }
     if (desc->n_anchor_links > 0 && !desc->anchor_ids) return fail_msg("vlbm2d_create: anchor_ids is null");
     vlbm2d_handle *h = new vlbm2d_handle;

@@ -27,7 +27,22 @@ constexpr int MAX_LSQ = 24;
 enum { M_SVK = 0, M_NH = 1, M_LOGNH = 2, M_MR = 3, M_YEOH = 4, M_GENT = 5, M_SUGI = 6 };
 enum { BC_DIR = 1, BC_NEU = 2 };
 enum { SRC_NONE = 0, SRC_CONST = 1, SRC_DAMP = 2 };
-enum { R_BFL = 0, R_LSQ_BFL = 2, R_COMPAT_BFL = 9, R_LOCALF = 12, R_LOCALF_BFL = 13 };
+enum {
+    R_BFL = 0, R_QBFL = 1, R_LSQ_BFL = 2, R_REG_BFL = 3, R_RNEE = 4, R_PAIR_NEE = 5, R_SAME_NEE = 6, R_POLY = 7,
+    R_LOWQ_REG_BFL = 8, R_COMPAT_BFL = 9, R_COMPAT_NEE = 10, R_CHAR2 = 11, R_LOCALF = 12, R_LOCALF_BFL = 13
+};
+// Reconstructions handled by the extended link kernel (regularized, NEE,
+// quadratic and polynomial bounce-back forms).
+__host__ __device__ inline bool recon_ext(int r) {
+    return r == R_QBFL || r == R_REG_BFL || r == R_RNEE || r == R_PAIR_NEE || r == R_SAME_NEE || r == R_POLY ||
+           r == R_LOWQ_REG_BFL || r == R_COMPAT_NEE;
+}
+// Reconstructions that read the LSQ gather of the cut points, and those of
+// them with the geometry-compatible tangential projection.
+__host__ __device__ inline bool recon_gather(int r) {
+    return r == R_LOCALF || r == R_LSQ_BFL || r == R_COMPAT_BFL || r == R_CHAR2 || r == R_COMPAT_NEE;
+}
+__host__ __device__ inline bool recon_compat(int r) { return r == R_COMPAT_BFL || r == R_COMPAT_NEE || r == R_CHAR2; }
 enum { NJ_FD = 0, NJ_AC = 1, NS_ENERGY = 2 };
 enum {
     V_ZERO = 0, V_CONST = 1, V_RAMP = 2, V_SHEAR_X = 3, V_RICKER_Y = 4, V_SINE2_HOLD = 5,

@@ -76,8 +76,7 @@ BOUNDARY_RECON_COMPAT_NEE = 10
 BOUNDARY_RECON_LOCAL_CHAR2 = 11
 BOUNDARY_RECON_LOCAL_F = 12
 BOUNDARY_RECON_LOCAL_F_BFL = 13
-PROVIDED_RECONSTRUCTIONS = (BOUNDARY_RECON_BFL, BOUNDARY_RECON_LSQ_BFL, BOUNDARY_RECON_COMPAT_BFL,
-                            BOUNDARY_RECON_LOCAL_F, BOUNDARY_RECON_LOCAL_F_BFL)
+PROVIDED_RECONSTRUCTIONS = tuple(range(BOUNDARY_RECON_BFL, BOUNDARY_RECON_LOCAL_F_BFL + 1))
 
 NORMAL_JACOBIAN_FINITE_DIFFERENCE = 0
 NORMAL_JACOBIAN_ACOUSTIC = 1
@@ -787,9 +786,7 @@ class WarpCurvedBoundaryLBM2D:
         self.boundary_reconstruction = boundary_reconstruction_id(boundary_reconstruction)
         self.boundary_reconstruction_name = boundary_reconstruction_name(self.boundary_reconstruction)
         if self.boundary_reconstruction not in PROVIDED_RECONSTRUCTIONS:
-            raise ValueError(
-                f"boundary_reconstruction {self.boundary_reconstruction_name!r} is not provided "
-                "(bfl, lsq_bfl/normal_probe_bfl, compat_bfl, local_f, local_f_bfl)")
+            raise ValueError(f"boundary_reconstruction {self.boundary_reconstruction_name!r} is not provided")
         self.boundary_stencil_radius = boundary_reconstruction_stencil_radius(self.boundary_reconstruction)
         self.normal_jacobian = normal_jacobian_id(normal_jacobian)
         self.normal_jacobian_name = normal_jacobian_name(self.normal_jacobian)

@@ -43,7 +43,7 @@
 extern "C" {
 #endif
 
-#define VLBM2D_ABI_VERSION 4
+#define VLBM2D_ABI_VERSION 5
 #define VLBM2D_MAX_IDS 8
 #define VLBM2D_MAX_LSQ 24
 #define VLBM2D_RECORD 12
@@ -67,8 +67,9 @@ typedef struct {
     double kinetic_filter;   /* strength in [0, 1] of the ghost filter after every step (cbw2d:3100-3124) */
     int32_t source_mode;
     double source_params[6];
-    int32_t recon;         /* reference reconstruction id: 0 bfl, 2 lsq_bfl, 9 compat_bfl, 12 local_f,
-                              13 local_f_bfl */
+    int32_t recon;         /* reference reconstruction id 0..13: 0 bfl, 1 qbfl, 2 lsq_bfl, 3 regularized_bfl,
+                              4 regularized_nee, 5 pair_nee, 6 same_nee, 7 poly_ibb, 8 lowq_reg_bfl,
+                              9 compat_bfl, 10 compat_nee, 11 local_char2, 12 local_f, 13 local_f_bfl */
     int32_t normal_solver; /* 0 finite-difference Newton, 1 acoustic Newton, 2 energy */
     /* local kinematics */
     int32_t disp_interval, disp_sweeps;

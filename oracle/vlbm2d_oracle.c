@@ -31,7 +31,10 @@
 enum { M_SVK = 0, M_NH = 1, M_LOGNH = 2, M_MR = 3, M_YEOH = 4, M_GENT = 5, M_SUGI = 6 };
 enum { BC_DIR = 1, BC_NEU = 2 };
 enum { SRC_NONE = 0, SRC_CONST = 1, SRC_DAMP = 2 };
-enum { R_BFL = 0, R_LSQ_BFL = 2, R_REG_BFL = 3, R_COMPAT_BFL = 9, R_CHAR2 = 11, R_LOCALF = 12, R_LOCALF_BFL = 13 };
+enum {
+    R_BFL = 0, R_QBFL = 1, R_LSQ_BFL = 2, R_REG_BFL = 3, R_RNEE = 4, R_PAIR_NEE = 5, R_SAME_NEE = 6, R_POLY = 7,
+    R_LOWQ_REG_BFL = 8, R_COMPAT_BFL = 9, R_COMPAT_NEE = 10, R_CHAR2 = 11, R_LOCALF = 12, R_LOCALF_BFL = 13
+};
 enum { NJ_FD = 0, NJ_AC = 1, NS_ENERGY = 2 };
 enum {
     V_ZERO = 0, V_CONST = 1, V_RAMP = 2, V_SHEAR_X = 3, V_RICKER_Y = 4, V_SINE2_HOLD = 5,
@@ -398,8 +401,47 @@ static double lsq_at(const or2d_state *s, const double *w, int ell, int comp, do
     return out;
 }
 
-static inline int recon_supported(int r) {
-    return r == R_BFL || r == R_LSQ_BFL || r == R_COMPAT_BFL || r == R_LOCALF || r == R_LOCALF_BFL;
+static inline int recon_supported(int r) { return r >= R_BFL && r <= R_LOCALF_BFL; }
+
+/* cbw2d:1484-1513: quadratic one-sided interpolation of fpost[q][comp]
+ * along -c_{q_out} from node (i, j) (s = 0 at the node, -1, -2 upstream). */
+static double interp_fpost_link(const or2d_state *s, const double *fpost, int q, int comp, int i, int j, int q_out,
+                                double sv) {
+    const int nx = s->nx, ny = s->ny;
+    double v0 = fpost[IDX(s, q * 6 + comp, i, j)];
+    int i1 = i - CX[q_out], j1 = j - CY[q_out];
+    if (i1 >= 0 && i1 < nx && j1 >= 0 && j1 < ny && s->active[(size_t)i1 * ny + j1] != 0) {
+        double v1 = fpost[IDX(s, q * 6 + comp, i1, j1)];
+        int i2 = i - 2 * CX[q_out], j2 = j - 2 * CY[q_out];
+        if (i2 >= 0 && i2 < nx && j2 >= 0 && j2 < ny && s->active[(size_t)i2 * ny + j2] != 0) {
+            double v2 = fpost[IDX(s, q * 6 + comp, i2, j2)];
+            double l0 = 0.5 * (sv + 1.0) * (sv + 2.0);
+            double l1 = -sv * (sv + 2.0);
+            double l2 = 0.5 * sv * (sv + 1.0);
+            return l0 * v0 + l1 * v1 + l2 * v2;
+        }
+        return (sv + 1.0) * v0 - sv * v1;
+    }
+    return v0;
+}
+
+/* cbw2d:1517-1643 (_regularized_post_neq_comp): fpost - feq(state) at node
+ * (i, j) with the D2Q4 ghost moment m3 = hE - hN + hW - hS removed. */
+static double reg_post_neq(const or2d_state *s, const double *fpost, int q, int a, int i, int j, const double *st) {
+    double P[4];
+    piola_s(s, st + 2, P);
+    double h[4];
+    for (int k = 0; k < NQ; ++k) h[k] = fpost[IDX(s, k * 6 + a, i, j)] - feq(k, a, st, P, s->c);
+    const double corr = 0.25 * (h[0] - h[1] + h[2] - h[3]);
+    return (q == 0 || q == 2) ? h[q] - corr : h[q] + corr;
+}
+
+/* cbw2d:1646-1688 (_regularized_fpost_comp): feq(U) + regularized neq at a node. */
+static double reg_fpost(const or2d_state *s, const double *fpost, int q, int a, int i, int j) {
+    double st[6], P[4];
+    for (int k = 0; k < 6; ++k) st[k] = s->U[IDX(s, k, i, j)];
+    piola_s(s, st + 2, P);
+    return feq(q, a, st, P, s->c) + reg_post_neq(s, fpost, q, a, i, j, st);
 }
 
 /* cbw2d:1692-1843 (_collide_masked_kernel), f0 -> fpost. */
@@ -773,9 +815,9 @@ static void stream_push(or2d_state *s) {
         }
 }
 
-/* cbw2d:2064-3098 (_apply_curved_boundary_links_kernel) for the bfl,
- * lsq_bfl, compat_bfl, local_f and local_f_bfl reconstructions: reads fpost,
- * writes the incoming population of every cut link into f1. */
+/* cbw2d:2064-3098 (_apply_curved_boundary_links_kernel), every
+ * reconstruction: reads fpost, writes the incoming population of every cut
+ * link into f1. */
 static void apply_links(or2d_state *s, const double *fpost, double *f1, double time_mid) {
     const int nx = s->nx, ny = s->ny, rid = s->recon;
     const double dt = s->dt, c = s->c, om = s->omega, jf = s->j_floor;
@@ -816,7 +858,7 @@ static void apply_links(or2d_state *s, const double *fpost, double *f1, double t
          * from the link predictor, F t = t + d_t u (traction links: the LSQ
          * tangential derivative of u; Dirichlet links: the time integral of
          * an affine velocity's tangential derivative, zero otherwise). */
-        if (rid == R_COMPAT_BFL && s->lsq_valid[ell] != 0) {
+        if ((rid == R_COMPAT_BFL || rid == R_COMPAT_NEE || rid == R_CHAR2) && s->lsq_valid[ell] != 0) {
             const double tanx = -nyb, tany = nxb;
             double dut0 = 0.0, dut1 = 0.0, wt_abs = 0.0;
             if (kind == BC_NEU) {
@@ -849,6 +891,55 @@ static void apply_links(or2d_state *s, const double *fpost, double *f1, double t
         }
         double Fb[4] = {Fg[0], Fg[1], Fg[2], Fg[3]};
         if (kind == BC_NEU) or2d_solve_normal(s, Fg, nxb, nyb, bv0, bv1, Fb);
+        double node[6];
+        for (int k = 0; k < 6; ++k) node[k] = s->U[IDX(s, k, i, j)];
+        /* cbw2d:2276-2356 (regularized_nee): the ghost state mirrors the
+         * interior state at s = 2 eta - 1 through the boundary datum. */
+        if (rid == R_RNEE) {
+            const double sm = 2.0 * eta - 1.0;
+            double m[6];
+            for (int k = 0; k < 6; ++k) m[k] = interp_U_link(s, k, i, j, q_out, sm);
+            double g[6];
+            if (kind == BC_DIR) {
+                g[0] = 2.0 * bv0 - m[0];
+                g[1] = 2.0 * bv1 - m[1];
+            } else {
+                g[0] = 2.0 * interp_U_link(s, 0, i, j, q_out, eta) - m[0];
+                g[1] = 2.0 * interp_U_link(s, 1, i, j, q_out, eta) - m[1];
+            }
+            for (int k = 0; k < 4; ++k) g[2 + k] = 2.0 * Fb[k] - m[2 + k];
+            if (g[2] * g[5] - g[3] * g[4] <= jf)
+                for (int k = 0; k < 4; ++k) g[2 + k] = Fb[k];
+            double Pgh[4];
+            piola_s(s, g + 2, Pgh);
+            for (int a = 0; a < 6; ++a)
+                f1[IDX(s, q_in * 6 + a, i, j)] = feq(q_in, a, g, Pgh, c) + reg_post_neq(s, fpost, q_in, a, i, j, node);
+            continue;
+        }
+        /* cbw2d:2358-2454 (pair_nee, same_nee, compat_nee): the boundary
+         * equilibrium plus the regularized post-collision non-equilibrium of
+         * the adjacent node, with the half-way parity D for the pair forms. */
+        if (rid == R_PAIR_NEE || rid == R_SAME_NEE || rid == R_COMPAT_NEE) {
+            double b[6] = {bv0, bv1, Fb[0], Fb[1], Fb[2], Fb[3]};
+            if (kind == BC_NEU) {
+                b[0] = interp_U_link(s, 0, i, j, q_out, eta);
+                b[1] = interp_U_link(s, 1, i, j, q_out, eta);
+            }
+            double Pbb[4];
+            piola_s(s, b + 2, Pbb);
+            for (int a = 0; a < 6; ++a) {
+                double Dp = 1.0;
+                if (kind == BC_DIR) {
+                    if (a < 2) Dp = -1.0;
+                } else if (a >= 2) {
+                    Dp = -1.0;
+                }
+                double h = rid == R_SAME_NEE ? reg_post_neq(s, fpost, q_in, a, i, j, node)
+                                             : Dp * reg_post_neq(s, fpost, q_out, a, i, j, node);
+                f1[IDX(s, q_in * 6 + a, i, j)] = feq(q_in, a, b, Pbb, c) + h;
+            }
+            continue;
+        }
         double Pb[4];
         piola_s(s, Fb, Pb);
         const int use_char = (rid == R_CHAR2 || rid == R_LOCALF) && kind == BC_NEU && s->lsq_valid[ell] != 0;
@@ -914,15 +1005,33 @@ static void apply_links(or2d_state *s, const double *fpost, double *f1, double t
                 default: D = -1.0; S = 0.5 * Fb[a - 2]; break;
                 }
             }
-            const double fo = fpost[IDX(s, q_out * 6 + a, i, j)];
-            const double fi = fpost[IDX(s, q_in * 6 + a, i, j)];
+            /* cbw2d:2934-3032: regularized sources (regularized_bfl, and
+             * lowq_reg_bfl below eta = 1/2, traction links only), the quadratic
+             * interpolated bounce-back (poly_ibb), quadratic BFL (qbfl). */
+            const int use_reg = kind == BC_NEU && (rid == R_REG_BFL || (rid == R_LOWQ_REG_BFL && eta < 0.5));
+            double fo = fpost[IDX(s, q_out * 6 + a, i, j)];
+            double fi = fpost[IDX(s, q_in * 6 + a, i, j)];
+            if (use_reg) {
+                fo = reg_fpost(s, fpost, q_out, a, i, j);
+                fi = reg_fpost(s, fpost, q_in, a, i, j);
+            }
             double val;
-            if (eta >= 0.5) {
+            if (rid == R_POLY && up_ok && eta > 0.25 && eta < 0.95) {
+                const double fout_b = interp_fpost_link(s, fpost, q_out, a, i, j, q_out, eta);
+                const double gin_b = D * fout_b + S;
+                const double g0 = fi, g1 = fpost[IDX(s, q_in * 6 + a, iu, ju)];
+                const double wb = 2.0 / (eta * (eta + 1.0));
+                const double w0 = -2.0 * (1.0 - eta) / eta;
+                const double w1 = (1.0 - eta) / (1.0 + eta);
+                val = wb * gin_b + w0 * g0 + w1 * g1;
+            } else if (rid == R_QBFL && eta < 0.5) {
+                val = D * interp_fpost_link(s, fpost, q_out, a, i, j, q_out, 2.0 * eta - 1.0) + S;
+            } else if (eta >= 0.5) {
                 double alpha = 1.0 / (2.0 * eta);
                 val = alpha * (D * fo + S) + (1.0 - alpha) * fi;
             } else {
                 double up = fo;
-                if (up_ok) up = fpost[IDX(s, q_out * 6 + a, iu, ju)];
+                if (up_ok) up = use_reg ? reg_fpost(s, fpost, q_out, a, iu, ju) : fpost[IDX(s, q_out * 6 + a, iu, ju)];
                 val = D * (2.0 * eta * fo + (1.0 - 2.0 * eta) * up) + S;
             }
             if (char_ok) {

@@ -62,10 +62,15 @@ def test_descriptor2_layout_matches_library():
     h = C.c_void_p()
     assert lib.vlbm2d_create(C.byref(d), C.byref(h)) == -1
     assert b"null" in lib.vlbm_last_error()
-    d.recon = 1
+    d.recon = 14
     assert lib.vlbm2d_create(C.byref(d), C.byref(h)) == -1
-    assert b"not provided" in lib.vlbm_last_error()
+    assert b"unknown reconstruction" in lib.vlbm_last_error()
     d.recon = 12
+    d.collision_model = 1
+    d.ghost_omega = 1.0
+    assert lib.vlbm2d_create(C.byref(d), C.byref(h)) == -1
+    assert b"ghost" in lib.vlbm_last_error()
+    d.collision_model = 0
     d.struct_size -= 8
     assert lib.vlbm2d_create(C.byref(d), C.byref(h)) == -1
     assert b"size mismatch" in lib.vlbm_last_error()

@@ -207,11 +207,11 @@ def test_loaded_annulus_mirror_symmetry_and_positive_J():
     np.testing.assert_allclose(u[0], u[1].T, atol=1e-8 * scale)
 
 
-def test_unsupported_reconstruction_is_rejected():
+def test_unknown_reconstruction_is_rejected():
     g = cb.CurvedBoundaryGeometry.circle(nx=8, ny=8, radius=0.3)
     with pytest.raises(ValueError):
         OracleSolver2D(g, material=NH, boundary=cb.CurvedBoundarySpec(kind="neumann"), lattice_speed=10.0,
-                       collision_omega=1.8, boundary_reconstruction="qbfl")
+                       collision_omega=1.8, boundary_reconstruction="cubic_bfl")
 
 
 def test_compat_projection_keeps_a_compatible_affine_state():
