@@ -32,8 +32,13 @@ enum { NJ_FD = 0, NJ_AC = 1, NS_ENERGY = 2 };
 enum {
     V_ZERO = 0, V_CONST = 1, V_AFFINE_VEL = 4, V_RAMP = 200, V_PIOLA_N = 201, V_NORMAL_T = 202,
     V_SINE2_NT = 203, V_SINE2_HOLD = 204, V_AFFINE_NH = 205, V_ZPULL = 206, V_ZTWIST = 207,
-    V_LINEAR_NT = 208
+    V_LINEAR_NT = 208, V_XPULL = 209, V_XTWIST = 210
 };
+
+// The x-axis tube modes are the z-axis ones in the cyclically relabelled
+// frame (a, b, axial) = (y, z, x): component comp of the x form is
+// component tube_comp(comp) of the z form evaluated at (y, z).
+__host__ __device__ __forceinline__ int tube_comp(int comp) { return comp == 0 ? 2 : comp - 1; }
 
 constexpr double PI_D = 3.141592653589793238462643383279502884;
 
@@ -171,6 +176,20 @@ __device__ inline double sine2_integral(double time, double active, double denom
     return prefix + (t - active);
 }
 
+// Exact finite twist about the axis through (prm[0], prm[1]) of the
+// transverse plane (a, b) plus axial stretch, as a displacement: comp 0, 1
+// transverse, 2 axial (cbw3d:2384-2398).
+__device__ inline double twist_disp(const double *prm, int comp, double a, double b, double time) {
+    const double active = prm[4];
+    double sv = 1.0;
+    if (active > 0.0 && time < active) sv = 0.5 * (1.0 - cos(PI_D * time / active));
+    const double theta = prm[3] * sv, da = a - prm[0], db = b - prm[1];
+    const double c = cos(theta), s = sin(theta);
+    if (comp == 0) return (c - 1.0) * da - s * db;
+    if (comp == 1) return s * da + (c - 1.0) * db;
+    return prm[2] * sv;
+}
+
 // Prescribed boundary displacement used by the Dirichlet anchors.
 __device__ inline double dirichlet_disp(int mode, const double *prm, int comp, double x, double y,
                                         double z, double time) {
@@ -189,16 +208,8 @@ __device__ inline double dirichlet_disp(int mode, const double *prm, int comp, d
         const int r = comp * 3;
         return at * (prm[r] * x + prm[r + 1] * y + prm[r + 2] * z + prm[9 + comp]);
     }
-    case V_ZTWIST: {
-        const double active = prm[4];
-        double sv = 1.0;
-        if (active > 0.0 && time < active) sv = 0.5 * (1.0 - cos(PI_D * time / active));
-        const double theta = prm[3] * sv, dx0 = x - prm[0], dy0 = y - prm[1];
-        const double c = cos(theta), s = sin(theta);
-        if (comp == 0) return (c - 1.0) * dx0 - s * dy0;
-        if (comp == 1) return s * dx0 + (c - 1.0) * dy0;
-        return prm[2] * sv;
-    }
+    case V_ZTWIST: return twist_disp(prm, comp, x, y, time);
+    case V_XTWIST: return twist_disp(prm, tube_comp(comp), y, z, time);
     default: return 0.0;
     }
 }
@@ -213,6 +224,27 @@ __device__ inline double sine2_scale(double t, double active, double denom) {
 }
 
 // Boundary value (velocity for Dirichlet rows, nominal traction for Neumann
+// Tube end loads about the axis through (prm[0], prm[1]) of the transverse
+// plane (a, b); comp 0, 1 transverse, 2 axial (cbw3d:1100-1132): the
+// pull-torsion traction and the velocity of the exact finite twist.
+__device__ inline double tube_pull(const double *prm, int comp, double a, double b, double t) {
+    const double scale = sine2_scale(t, prm[4], prm[5]);
+    if (comp == 0) return -scale * prm[3] * (b - prm[1]);
+    if (comp == 1) return scale * prm[3] * (a - prm[0]);
+    return scale * prm[2];
+}
+__device__ inline double twist_velocity(const double *prm, int comp, double a, double b, double t) {
+    const double active = prm[4];
+    if (active <= 0.0 || t >= active) return 0.0;
+    const double phase = PI_D * t / active;
+    const double sv = 0.5 * (1.0 - cos(phase)), sdot = 0.5 * PI_D / active * sin(phase);
+    const double theta = prm[3] * sv, thetadot = prm[3] * sdot;
+    const double da = a - prm[0], db = b - prm[1];
+    if (comp == 0) return thetadot * (-sin(theta) * da - cos(theta) * db);
+    if (comp == 1) return thetadot * (cos(theta) * da - sin(theta) * db);
+    return prm[2] * sdot;
+}
+
 // rows) of component comp at a cut point.
 __device__ inline double curved_value(int mode, const double *prm, int comp, double x, double y,
                                       double z, const double *nv, double t, double jf) {
@@ -253,23 +285,10 @@ __device__ inline double curved_value(int mode, const double *prm, int comp, dou
         if (prm[1] > 0.0 && t < prm[1]) scale = t / prm[1];
         return prm[0] * scale * nv[comp];
     }
-    case V_ZPULL: {
-        const double scale = sine2_scale(t, prm[4], prm[5]);
-        if (comp == 0) return -scale * prm[3] * (y - prm[1]);
-        if (comp == 1) return scale * prm[3] * (x - prm[0]);
-        return scale * prm[2];
-    }
-    case V_ZTWIST: {
-        const double active = prm[4];
-        if (active <= 0.0 || t >= active) return 0.0;
-        const double phase = PI_D * t / active;
-        const double sv = 0.5 * (1.0 - cos(phase)), sdot = 0.5 * PI_D / active * sin(phase);
-        const double theta = prm[3] * sv, thetadot = prm[3] * sdot;
-        const double ddx = x - prm[0], ddy = y - prm[1];
-        if (comp == 0) return thetadot * (-sin(theta) * ddx - cos(theta) * ddy);
-        if (comp == 1) return thetadot * (cos(theta) * ddx - sin(theta) * ddy);
-        return prm[2] * sdot;
-    }
+    case V_ZPULL: return tube_pull(prm, comp, x, y, t);
+    case V_ZTWIST: return twist_velocity(prm, comp, x, y, t);
+    case V_XPULL: return tube_pull(prm, tube_comp(comp), y, z, t);
+    case V_XTWIST: return twist_velocity(prm, tube_comp(comp), y, z, t);
     default: return 0.0;
     }
 }

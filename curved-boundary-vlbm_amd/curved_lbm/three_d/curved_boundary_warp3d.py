@@ -68,6 +68,10 @@ CURVED3D_VALUE_Z_PULL_TORSION_TRACTION = 206
 CURVED3D_VALUE_Z_STRETCH_EXACT_TWIST_VELOCITY = 207
 # Extension: nominal traction amp * min(t / t_ramp, 1) * n, params (amp, t_ramp).
 CURVED3D_VALUE_LINEAR_RAMP_NORMAL_TRACTION = 208
+# x-axis forms of 206 and 207 (same params; transverse plane (y, z) about
+# (params[0], params[1]) = (y_c, z_c), axial component x)
+CURVED3D_VALUE_X_PULL_TORSION_TRACTION = 209
+CURVED3D_VALUE_X_STRETCH_EXACT_TWIST_VELOCITY = 210
 CURVED_VALUE_ZERO = CURVED3D_VALUE_ZERO
 CURVED_VALUE_CONSTANT_VECTOR = CURVED3D_VALUE_CONSTANT_VECTOR
 CURVED_VALUE_AFFINE_VELOCITY = CURVED3D_VALUE_AFFINE_VELOCITY
@@ -707,6 +711,9 @@ def _value_np(mode, p, comp, x, y, z, n, t):
     if mode == CURVED3D_VALUE_LINEAR_RAMP_NORMAL_TRACTION:
         scale = 1.0 if pp[1] <= 0 or t >= pp[1] else t / pp[1]
         return float(pp[0] * scale * n[comp])
+    if mode in (CURVED3D_VALUE_X_PULL_TORSION_TRACTION, CURVED3D_VALUE_X_STRETCH_EXACT_TWIST_VELOCITY):
+        zmode = mode - (CURVED3D_VALUE_X_PULL_TORSION_TRACTION - CURVED3D_VALUE_Z_PULL_TORSION_TRACTION)
+        return _value_np(zmode, p, 2 if comp == 0 else comp - 1, y, z, x, (n[1], n[2], n[0]), t)
     if mode == CURVED3D_VALUE_Z_PULL_TORSION_TRACTION:
         scale = _sine2(t, pp[4], pp[5])
         if comp == 0:
@@ -1542,6 +1549,7 @@ __all__ = [
     "CURVED3D_VALUE_NORMAL_TRACTION", "CURVED3D_VALUE_SINE2_NORMAL_TRACTION",
     "CURVED3D_VALUE_AFFINE_NEO_HOOKE_PIOLA_NORMAL", "CURVED3D_VALUE_Z_PULL_TORSION_TRACTION",
     "CURVED3D_VALUE_Z_STRETCH_EXACT_TWIST_VELOCITY", "CURVED3D_VALUE_LINEAR_RAMP_NORMAL_TRACTION",
+    "CURVED3D_VALUE_X_PULL_TORSION_TRACTION", "CURVED3D_VALUE_X_STRETCH_EXACT_TWIST_VELOCITY",
     "BOUNDARY_RECON_BFL", "BOUNDARY_RECON_HALF_WAY", "BOUNDARY_RECON_COMPAT_BFL", "BOUNDARY_RECON_LOCAL_F",
     "BOUNDARY_ID_OUTER", "BOUNDARY_ID_INNER",
 ]

@@ -30,10 +30,47 @@ enum { NJ_FD = 0, NJ_AC = 1, NS_ENERGY = 2 };
 enum {
     V_ZERO = 0, V_CONST = 1, V_AFFINE_VEL = 4, V_RAMP = 200, V_PIOLA_N = 201, V_NORMAL_T = 202,
     V_SINE2_NT = 203, V_SINE2_HOLD = 204, V_AFFINE_NH = 205, V_ZPULL = 206, V_ZTWIST = 207,
-    V_LINEAR_NT = 208
+    V_LINEAR_NT = 208, V_XPULL = 209, V_XTWIST = 210
 };
 
+
 static const double PI_D = 3.141592653589793238462643383279502884;
+
+/* Tube end loads about the axis through (p[0], p[1]) of the transverse plane
+ * (a, b); comp 0, 1 transverse, 2 axial (cbw3d:1100-1132, 2384-2398).  The
+ * x-axis modes 209, 210 use them in the frame (a, b, axial) = (y, z, x). */
+static double tube_pull(const double *p, int comp, double a, double b, double t) {
+    double active = p[4], denom = p[5];
+    if (denom <= 0.0) denom = 2.0 * active;
+    double scale = 1.0;
+    if (active > 0.0 && t < active) scale = sin(PI_D * t / denom) * sin(PI_D * t / denom);
+    if (comp == 0) return -scale * p[3] * (b - p[1]);
+    if (comp == 1) return scale * p[3] * (a - p[0]);
+    return scale * p[2];
+}
+static double twist_velocity(const double *p, int comp, double a, double b, double t) {
+    double active = p[4];
+    if (active <= 0.0 || t >= active) return 0.0;
+    double phase = PI_D * t / active;
+    double sv = 0.5 * (1.0 - cos(phase));
+    double sdot = 0.5 * PI_D / active * sin(phase);
+    double theta = p[3] * sv, thetadot = p[3] * sdot;
+    double da = a - p[0], db = b - p[1];
+    if (comp == 0) return thetadot * (-sin(theta) * da - cos(theta) * db);
+    if (comp == 1) return thetadot * (cos(theta) * da - sin(theta) * db);
+    return p[2] * sdot;
+}
+static double twist_disp(const double *p, int comp, double a, double b, double time) {
+    double active = p[4], sv = 1.0;
+    if (active > 0.0 && time < active) sv = 0.5 * (1.0 - cos(PI_D * time / active));
+    double theta = p[3] * sv;
+    double da = a - p[0], db = b - p[1];
+    double c = cos(theta), ss = sin(theta);
+    if (comp == 0) return (c - 1.0) * da - ss * db;
+    if (comp == 1) return ss * da + (c - 1.0) * db;
+    return p[2] * sv;
+}
+static inline int tube_comp(int comp) { return comp == 0 ? 2 : comp - 1; }
 
 static inline int dir_x(int q) { return q == 0 ? 1 : (q == 3 ? -1 : 0); }
 static inline int dir_y(int q) { return q == 1 ? 1 : (q == 4 ? -1 : 0); }
@@ -301,16 +338,8 @@ static double dirichlet_disp(const or3d_state *s, int mode, int bid, int comp, d
         else if (comp == 2) out = PRM(bid, 6) * x + PRM(bid, 7) * y + PRM(bid, 8) * z + PRM(bid, 11);
         return at * out;
     }
-    if (mode == V_ZTWIST) {
-        double active = PRM(bid, 4), sv = 1.0;
-        if (active > 0.0 && time < active) sv = 0.5 * (1.0 - cos(PI_D * time / active));
-        double theta = PRM(bid, 3) * sv;
-        double dx0 = x - PRM(bid, 0), dy0 = y - PRM(bid, 1);
-        double c = cos(theta), ss = sin(theta);
-        if (comp == 0) return (c - 1.0) * dx0 - ss * dy0;
-        if (comp == 1) return ss * dx0 + (c - 1.0) * dy0;
-        return PRM(bid, 2) * sv;
-    }
+    if (mode == V_ZTWIST) return twist_disp(&PRM(bid, 0), comp, x, y, time);
+    if (mode == V_XTWIST) return twist_disp(&PRM(bid, 0), tube_comp(comp), y, z, time);
     return 0.0;
 }
 
@@ -568,28 +597,10 @@ static double curved_value(const or3d_state *s, int mode, int bid, int comp, dou
         if (active > 0.0 && t < active) scale = t / active;
         return amp * scale * nv[comp];
     }
-    if (mode == V_ZPULL) {
-        double active = PRM(bid, 4), denom = PRM(bid, 5);
-        if (denom <= 0.0) denom = 2.0 * active;
-        double scale = 1.0;
-        if (active > 0.0 && t < active) scale = sin(PI_D * t / denom) * sin(PI_D * t / denom);
-        double ddx = x - PRM(bid, 0), ddy = y - PRM(bid, 1), axial = PRM(bid, 2), alpha = PRM(bid, 3);
-        if (comp == 0) return -scale * alpha * ddy;
-        if (comp == 1) return scale * alpha * ddx;
-        return scale * axial;
-    }
-    if (mode == V_ZTWIST) {
-        double active = PRM(bid, 4);
-        if (active <= 0.0 || t >= active) return 0.0;
-        double phase = PI_D * t / active;
-        double sv = 0.5 * (1.0 - cos(phase));
-        double sdot = 0.5 * PI_D / active * sin(phase);
-        double theta = PRM(bid, 3) * sv, thetadot = PRM(bid, 3) * sdot;
-        double ddx = x - PRM(bid, 0), ddy = y - PRM(bid, 1);
-        if (comp == 0) return thetadot * (-sin(theta) * ddx - cos(theta) * ddy);
-        if (comp == 1) return thetadot * (cos(theta) * ddx - sin(theta) * ddy);
-        return PRM(bid, 2) * sdot;
-    }
+    if (mode == V_ZPULL) return tube_pull(&PRM(bid, 0), comp, x, y, t);
+    if (mode == V_ZTWIST) return twist_velocity(&PRM(bid, 0), comp, x, y, t);
+    if (mode == V_XPULL) return tube_pull(&PRM(bid, 0), tube_comp(comp), y, z, t);
+    if (mode == V_XTWIST) return twist_velocity(&PRM(bid, 0), tube_comp(comp), y, z, t);
     return 0.0;
 }
 

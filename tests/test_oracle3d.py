@@ -156,3 +156,35 @@ def test_compat_projection_keeps_a_compatible_affine_state(kinds):
     assert np.max(np.abs(bfl[written] - compat[written])) < 1e-12
     # with u = 0 the tangential rows are projected back to the identity
     assert np.max(np.abs(bfl[written] - incompatible[written])) > 1e-6
+
+
+def _relabel_z_to_x(U):
+    """Components of a z-tube state in the x-tube frame (X, Y, Z) = (z, x, y)."""
+    s = [2, 0, 1]
+    if U.shape[0] == 3:
+        return U[s]
+    v = U[:3][s]
+    F = U[3:].reshape(3, 3, -1)[s][:, s]
+    return np.concatenate([v, F.reshape(9, -1)])
+
+
+def test_x_tube_is_the_relabelled_z_tube():
+    """The x-axis tube modes (209, 210) are the z-axis modes in the
+    relabelled frame: an x tube and a z tube related by (x, y, z)_x =
+    (z, x, y)_z give the same v, F and u at corresponding nodes (bfl, so
+    that no least-squares stencil depends on the node order)."""
+    import gpu_cases
+
+    out = {}
+    for axis in ("z", "x"):
+        g, kw = gpu_cases.tube_case(axis, 8, ramp_steps=20, boundary_reconstruction="bfl")
+        s = OracleSolver3D(g, **kw)
+        s.run(30)
+        out[axis] = (g, s)
+    (gz, sz), (gx, sx) = out["z"], out["x"]
+    perm = gpu_cases.tube_permutation(gz, gx)
+    for name in ("U", "u"):
+        a = getattr(sx, name)[:, perm]
+        b = _relabel_z_to_x(getattr(sz, name))
+        assert np.linalg.norm(b) > 1e-3
+        assert np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(b), name
