@@ -58,6 +58,7 @@ struct Params2 {
     int use_dyn;
     double gw;    // mrt_ghost: 0.25 (omega - omega_ghost), 0 for bgk
     double filt;  // kinetic ghost filter strength, 0 = off
+    int per_x, per_y;  // periodic axes (rectangular box solver)
     const double *clock;  // device time of the step start (captured step loops), or nullptr
 };
 
@@ -757,6 +758,103 @@ __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T
     }
 }
 
+// Index along a periodic axis of n nodes (one step off either end wraps).
+__device__ __forceinline__ int wrap_i(int i, int n, int periodic) {
+    if (!periodic) return i;
+    return i < 0 ? i + n : (i >= n ? i - n : i);
+}
+
+// Faces of the rectangular box solver: kind (0 periodic, 1 Dirichlet
+// velocity, 2 Neumann traction), value mode and 8 params per side (left,
+// right, bottom, top), and the box lengths.
+struct Box2 {
+    int kind[4], mode[4];
+    double prm[4][8];
+    double lx, ly;
+};
+
+// Half-way face reconstruction of the rectangular box solver
+// (vnew2d:1685-1820): one thread per (side, face node); Dirichlet faces
+// bounce back with the velocity datum, Neumann faces impose the traction on
+// the velocity rows and the face F of the column solve on the F rows.  The
+// four sides read and write disjoint populations, so f_src may equal f_dst
+// (the in-place initial reconstruction).  use_rec as in k2_links.
+template <typename T>
+__global__ __launch_bounds__(LBLOCK) void k2_box_sides(const T *fsrc, T *fdst, const T *__restrict__ rec,
+                                                       const double *__restrict__ U, int use_rec, Box2 B,
+                                                       double time, Params2 P) {
+    const int m = P.nx > P.ny ? P.nx : P.ny;
+    const int t = blockIdx.x * LBLOCK + threadIdx.x;
+    if (t >= 4 * m) return;
+    const int side = t / m, sidx = t - side * m;
+    const int kind = B.kind[side];
+    if (kind == 0) return;
+    const int nn = side < 2 ? P.ny : P.nx;
+    if (sidx >= nn) return;
+    int i = 0, j = sidx, nxn = -1, nyn = 0, qi = 0, qo = 2, axis = 0;
+    double sign = -1.0;
+    if (side == 1) {
+        i = P.nx - 1;
+        nxn = 1;
+        qi = 2;
+        qo = 0;
+        sign = 1.0;
+    } else if (side == 2) {
+        i = sidx;
+        j = 0;
+        nxn = 0;
+        nyn = -1;
+        qi = 1;
+        qo = 3;
+        axis = 1;
+    } else if (side == 3) {
+        i = sidx;
+        j = P.ny - 1;
+        nxn = 0;
+        nyn = 1;
+        qi = 3;
+        qo = 1;
+        axis = 1;
+        sign = 1.0;
+    }
+    const size_t N = P.N;
+    const int p = i * P.ny + j;
+    double xb = (i + 0.5) * P.dx, yb = (j + 0.5) * P.dy;
+    if (side == 0) xb = 0.0;
+    else if (side == 1) xb = B.lx;
+    else if (side == 2) yb = 0.0;
+    else yb = B.ly;
+    time = step_time(time, P);
+    const double *prm = B.prm[side];
+    const double v0 = value2(0, B.mode[side], prm, xb, yb, 0.0, 0.0, time);
+    const double v1 = value2(1, B.mode[side], prm, xb, yb, 0.0, 0.0, time);
+    const double om1 = 1.0 - P.omega;
+    double fo[6];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+        const size_t k = (size_t)(qo * 6 + a) * N + p;
+        fo[a] = use_rec ? fpost_v(ld(fsrc, k, a), rec, N, qo, a, p, om1, P.omega) : ld(fsrc, k, a);
+    }
+    double val[6];
+    if (kind == BC_DIR) {
+        val[0] = -fo[0] + 0.5 * v0;
+        val[1] = -fo[1] + 0.5 * v1;
+        val[2] = fo[2] + (double)nxn * v0 / P.c;
+        val[3] = fo[3] + (double)nyn * v0 / P.c;
+        val[4] = fo[4] + (double)nxn * v1 / P.c;
+        val[5] = fo[5] + (double)nyn * v1 / P.c;
+    } else {
+        double F[4] = {U[2 * N + p], U[3 * N + p], U[4 * N + p], U[5 * N + p]};
+        solve_normal_column2(F, axis, sign, v0, v1, P.m);
+        val[0] = fo[0] + v0 / P.c;
+        val[1] = fo[1] + v1 / P.c;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) val[2 + k] = -fo[2 + k] + 0.5 * F[k];
+    }
+#pragma unroll
+    for (int a = 0; a < 6; ++a) st(fdst, (size_t)(qi * 6 + a) * N + p, a, val[a]);
+}
+
 // Pull stream of the post-collision populations; entries whose upstream
 // node is inactive or off the lattice belong to cut links and were written
 // by k2_links.  With NEXT the kernel also performs the moment part of the
@@ -789,7 +887,7 @@ __global__ __launch_bounds__(BLOCK) void k2_stream(const T *__restrict__ f0, T *
     if constexpr (MODE == 0) {
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            const int iu = i - cx(q), ju = j - cy(q);
+            const int iu = wrap_i(i - cx(q), P.nx, P.per_x), ju = wrap_i(j - cy(q), ny, P.per_y);
             const bool ok = iu >= 0 && iu < P.nx && ju >= 0 && ju < ny && act[iu * ny + ju] != 0;
             if (ok) {
                 const int src = iu * ny + ju;
@@ -812,7 +910,7 @@ __global__ __launch_bounds__(BLOCK) void k2_stream(const T *__restrict__ f0, T *
         bool okq[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            const int iu = i - cx(q), ju = j - cy(q);
+            const int iu = wrap_i(i - cx(q), P.nx, P.per_x), ju = wrap_i(j - cy(q), ny, P.per_y);
             okq[q] = iu >= 0 && iu < P.nx && ju >= 0 && ju < ny && act[iu * ny + ju] != 0;
             if (okq[q]) {
                 const int src = iu * ny + ju;
@@ -1067,6 +1165,7 @@ struct vlbm2d_handle {
     vlbm2d_desc d;
     Params2 P;
     Links2 T;
+    Box2 box{};  // faces of the rectangular box solver (d.box != 0)
     hipStream_t stream;
     Timing timing;
     int graph_steps = 0;
@@ -1106,7 +1205,30 @@ static Params2 make_params2(const vlbm2d_desc &d) {
     P.use_dyn = d.use_dyn;
     P.gw = d.collision_model == 1 ? 0.25 * (d.omega - d.ghost_omega) : 0.0;
     P.filt = d.kinetic_filter;
+    P.per_x = d.box && d.box_side_kind[0] == 0 && d.box_side_kind[1] == 0;
+    P.per_y = d.box && d.box_side_kind[2] == 0 && d.box_side_kind[3] == 0;
     return P;
+}
+
+static Box2 make_box2(const vlbm2d_desc &d) {
+    Box2 B{};
+    for (int s = 0; s < 4; ++s) {
+        B.kind[s] = d.box_side_kind[s];
+        B.mode[s] = d.box_value_mode[s];
+        for (int k = 0; k < 8; ++k) B.prm[s][k] = d.box_params[s * 8 + k];
+    }
+    B.lx = d.length_x;
+    B.ly = d.length_y;
+    return B;
+}
+
+// Side kinds of a box: 0..2 each, periodic on both sides of an axis or neither.
+static const char *box_sides_error(const int32_t *kind) {
+    for (int s = 0; s < 4; ++s)
+        if (kind[s] < 0 || kind[s] > 2) return "box side kind must be 0 (periodic), 1 (Dirichlet) or 2 (Neumann)";
+    if ((kind[0] == 0) != (kind[1] == 0)) return "x-periodic boundaries must be set on both left and right";
+    if ((kind[2] == 0) != (kind[3] == 0)) return "y-periodic boundaries must be set on both bottom and top";
+    return nullptr;
 }
 
 static Links2 make_links2(const vlbm2d_desc &d) {
@@ -1136,8 +1258,21 @@ static Links2 make_links2(const vlbm2d_desc &d) {
 #define CHECK_LAUNCH(where) VLBM_CHECK_LAUNCH(where)
 
 template <typename T>
+static void launch_box2(vlbm2d_handle *h, const Params2 &P, hipStream_t st, const T *src, T *dst, double t,
+                        int use_rec) {
+    const int m = P.nx > P.ny ? P.nx : P.ny;
+    hipLaunchKernelGGL(k2_box_sides<T>, dim3((4 * m + LBLOCK - 1) / LBLOCK), dim3(LBLOCK), 0, st, src, dst,
+                       (const T *)h->d.record, (const double *)h->d.U, use_rec, h->box, t, P);
+}
+
+template <typename T>
 static int run_links2(vlbm2d_handle *h, const T *src, T *dst, double t, int use_rec) {
     const vlbm2d_desc &d = h->d;
+    if (d.box) {
+        launch_box2<T>(h, h->P, h->stream, src, dst, t, use_rec);
+        CHECK_LAUNCH("k2_box_sides");
+        return 0;
+    }
     if (d.n_links == 0) return 0;
     const bool gather = recon_gather(d.recon);
     if (gather) {
@@ -1167,6 +1302,12 @@ template <typename T>
 static int launch_links2(vlbm2d_handle *h, const Params2 &P, hipStream_t st, const T *f0, T *f1, double tb,
                          const double *gin) {
     const vlbm2d_desc &d = h->d;
+    if (d.box) {
+        TimedSlot tl(h->timing, st, VLBM_SLOT_LINKS);
+        launch_box2<T>(h, P, st, f0, f1, tb, 1);
+        CHECK_LAUNCH("k2_box_sides");
+        return 0;
+    }
     if (d.n_links == 0) return 0;
     const bool gather = recon_gather(d.recon);
     if (gather) {
@@ -1394,6 +1535,12 @@ int vlbm2d_create(const vlbm2d_desc *desc, vlbm2d_handle **out) {
     }
     if (!(desc->kinetic_filter >= 0.0 && desc->kinetic_filter <= 1.0))
         return fail_msg("vlbm2d_create: kinetic_filter must be in [0,1]");
+    if (desc->box) {
+        if (desc->n_links != 0) return fail_msg("vlbm2d_create: a box has no cut links (n_links must be 0)");
+        if (desc->n_active != desc->nx * desc->ny) return fail_msg("vlbm2d_create: every box node must be active");
+        if (const char *e = box_sides_error(desc->box_side_kind)) return fail_msg(std::string("vlbm2d_create: ") + e);
+        if (!(desc->length_x > 0.0 && desc->length_y > 0.0)) return fail_msg("vlbm2d_create: box lengths must be positive");
+    }
     const void *req[] = {desc->U, desc->u, desc->u_star, desc->u_next, desc->record, desc->invalid,
                          desc->reduce_scratch, desc->active, desc->active_list, desc->anchor_off,
                          desc->kind_by_id, desc->mode_by_id, desc->params_by_id};
@@ -1413,6 +1560,7 @@ int vlbm2d_create(const vlbm2d_desc *desc, vlbm2d_handle **out) {
     h->d = *desc;
     h->P = make_params2(*desc);
     h->T = make_links2(*desc);
+    h->box = make_box2(*desc);
     h->stream = (hipStream_t)desc->stream;
     *out = h;
     return 0;
@@ -1473,7 +1621,7 @@ int vlbm2d_refresh(vlbm2d_handle *h, const void *f) {
 
 int vlbm2d_apply_boundary(vlbm2d_handle *h, const void *f_src, void *f_dst, double t) {
     if (!h || !f_src || !f_dst) return fail_msg("vlbm2d_apply_boundary: null argument");
-    if (f_src == f_dst) return fail_msg("vlbm2d_apply_boundary: source and target must differ");
+    if (f_src == f_dst && !h->d.box) return fail_msg("vlbm2d_apply_boundary: source and target must differ");
     if (h->d.precision == VLBM_PREC_F64) return run_links2<double>(h, (const double *)f_src, (double *)f_dst, t, 0);
     return run_links2<float>(h, (const float *)f_src, (float *)f_dst, t, 0);
 }
@@ -1519,6 +1667,25 @@ int vlbm2d_post_collision(vlbm2d_handle *h, const void *f_prev, void *fpost) {
         hipLaunchKernelGGL(k2_post<float>, dim3(g), dim3(BLOCK), 0, h->stream, (const float *)f_prev,
                            (float *)fpost, (const float *)h->d.record, h->d.active_list, h->P);
     CHECK_LAUNCH("k2_post");
+    return 0;
+}
+
+int vlbm2d_set_box_sides(vlbm2d_handle *h, const int32_t *kind, const int32_t *mode, const double *params) {
+    if (!h || !kind || !mode || !params) return fail_msg("vlbm2d_set_box_sides: null argument");
+    if (!h->d.box) return fail_msg("vlbm2d_set_box_sides: not a box solver");
+    if (const char *e = box_sides_error(kind)) return fail_msg(std::string("vlbm2d_set_box_sides: ") + e);
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail("vlbm2d_set_box_sides: sync", e);
+    drop_graphs(h);  // the faces are captured kernel arguments
+    for (int s = 0; s < 4; ++s) {
+        h->d.box_side_kind[s] = kind[s];
+        h->d.box_value_mode[s] = mode[s];
+    }
+    for (int k = 0; k < 32; ++k) h->d.box_params[k] = params[k];
+    const int use_dyn = h->P.use_dyn;
+    h->P = make_params2(h->d);
+    h->P.use_dyn = use_dyn;
+    h->box = make_box2(h->d);
     return 0;
 }
 

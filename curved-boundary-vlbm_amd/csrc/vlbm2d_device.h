@@ -232,6 +232,59 @@ __device__ __forceinline__ double value2(int comp, int mode, const double *p, do
     }
 }
 
+// Traction solve of an axis-aligned face of the rectangular box solver
+// (vnew2d:1284-1448, _normal_residual + _solve_normal_column): Newton with a
+// forward-difference Jacobian (eps 1e-6) on the F column of the normal
+// axis, residual sign * P column - T below 1e-10, at most 12 iterations,
+// each step halved up to 10 times until J > j_floor.  F holds the node's F
+// on entry and the face F on exit.
+__device__ inline void box_residual2(const double *F, int axis, double sign, double tx, double ty,
+                                     const Material2 &m, double *r) {
+    double P[4];
+    piola2(F, m, P);
+    r[0] = sign * P[axis] - tx;
+    r[1] = sign * P[2 + axis] - ty;
+}
+__device__ inline void solve_normal_column2(double *F, int axis, double sign, double tx, double ty,
+                                            const Material2 &m) {
+    const double Fin[4] = {F[0], F[1], F[2], F[3]};
+    double x0 = Fin[axis], x1 = Fin[2 + axis];
+    const double eps = 1.0e-6;
+    for (int it = 0; it < 12; ++it) {
+        F[axis] = x0;
+        F[2 + axis] = x1;
+        double r[2];
+        box_residual2(F, axis, sign, tx, ty, m, r);
+        if (sqrt(r[0] * r[0] + r[1] * r[1]) < 1.0e-10) break;
+        double Fp[4] = {F[0], F[1], F[2], F[3]}, rp0[2], rp1[2];
+        Fp[axis] = F[axis] + eps;
+        box_residual2(Fp, axis, sign, tx, ty, m, rp0);
+        Fp[axis] = F[axis];
+        Fp[2 + axis] = F[2 + axis] + eps;
+        box_residual2(Fp, axis, sign, tx, ty, m, rp1);
+        const double a = (rp0[0] - r[0]) / eps, c = (rp0[1] - r[1]) / eps;
+        const double b = (rp1[0] - r[0]) / eps, d = (rp1[1] - r[1]) / eps;
+        const double det = a * d - b * c;
+        if (fabs(det) < 1.0e-14) break;
+        const double d0 = (-r[0] * d + b * r[1]) / det, d1 = (c * r[0] - a * r[1]) / det;
+        double step = 1.0;
+        for (int ls = 0; ls < 10; ++ls) {
+            const double c0 = x0 + step * d0, c1 = x1 + step * d1;
+            double T[4] = {Fin[0], Fin[1], Fin[2], Fin[3]};
+            T[axis] = c0;
+            T[2 + axis] = c1;
+            if (T[0] * T[3] - T[1] * T[2] > m.jf) {
+                x0 = c0;
+                x1 = c1;
+                break;
+            }
+            step = 0.5 * step;
+        }
+    }
+    F[axis] = x0;
+    F[2 + axis] = x1;
+}
+
 __device__ __forceinline__ void normal_residual2(double hn0, double hn1, double gt0, double gt1, double nxb, double nyb,
                                         double tx, double ty, const Material2 &m, double *r) {
     const double t1 = -nyb, t2 = nxb;

@@ -16,7 +16,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libvlbm.so"
 
 ABI_VERSION = 5
-ABI2_VERSION = 5
+ABI2_VERSION = 6
 PREC_F64 = 0
 PREC_F32 = 1
 
@@ -188,6 +188,12 @@ class Vlbm2dDesc(C.Structure):
         ("dyn_tx", C.c_void_p),
         ("dyn_ty", C.c_void_p),
         ("use_dyn", C.c_int32),
+        ("box", C.c_int32),
+        ("box_side_kind", C.c_int32 * 4),
+        ("box_value_mode", C.c_int32 * 4),
+        ("box_params", C.c_double * 32),
+        ("length_x", C.c_double),
+        ("length_y", C.c_double),
         ("stream", C.c_void_p),
     ]
 
@@ -215,6 +221,7 @@ EXPORTED = (
     "vlbm2d_stress",
     "vlbm2d_post_collision",
     "vlbm2d_kinetic_filter",
+    "vlbm2d_set_box_sides",
     "vlbm2d_set_dynamic_traction",
     "vlbm2d_set_graph_steps",
     "vlbm2d_reduce",
@@ -273,6 +280,7 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vlbm2d_stress": [vp, vp, vp],
         "vlbm2d_post_collision": [vp, vp, vp],
         "vlbm2d_kinetic_filter": [vp, vp],
+        "vlbm2d_set_box_sides": [vp, vp, vp, vp],
         "vlbm2d_set_dynamic_traction": [vp, C.c_int32],
         "vlbm2d_set_graph_steps": [vp, C.c_int32],
         "vlbm2d_reduce": [vp, vp, _dp],

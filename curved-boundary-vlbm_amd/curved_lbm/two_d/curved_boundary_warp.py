@@ -763,7 +763,12 @@ class WarpCurvedBoundaryLBM2D:
         neumann_solver: str | None = None,
         precision: str = "fp64",
         graph_steps: int = 0,
+        _box=None,
     ) -> None:
+        # _box: (side_kind[4], value_mode[4], params[4, 8]) of the rectangular
+        # box solver built on this class (vector_nonlinear_elastic_warp's
+        # WarpNonlinearElasticLBM2D); None for a curved geometry.
+        self._box_sides = _box
         self.geometry = geometry
         self.nx, self.ny = int(geometry.nx), int(geometry.ny)
         self.length_x, self.length_y = float(geometry.length_x), float(geometry.length_y)
@@ -921,10 +926,10 @@ class WarpCurvedBoundaryLBM2D:
         lj = self.lsq_j_np.reshape(L, MAX_LSQ_STENCIL)
         lsq_p = np.where((li >= 0) & (lj >= 0), li.astype(np.int64) * ny + lj, -1).astype(np.int32)
         self._lsq_p_t = dev_i(np.ascontiguousarray(lsq_p.T))
-        self._lsq_wb_t = dev_f(np.ascontiguousarray(self.lsq_wb_np.reshape(L, -1).T))
-        self._lsq_wx_t = dev_f(np.ascontiguousarray(self.lsq_wx_np.reshape(L, -1).T))
-        self._lsq_wy_t = dev_f(np.ascontiguousarray(self.lsq_wy_np.reshape(L, -1).T))
-        self._lsq_wt_t = dev_f(np.ascontiguousarray(self.lsq_wt_np.reshape(L, -1).T))
+        self._lsq_wb_t = dev_f(np.ascontiguousarray(self.lsq_wb_np.reshape(L, MAX_LSQ_STENCIL).T))
+        self._lsq_wx_t = dev_f(np.ascontiguousarray(self.lsq_wx_np.reshape(L, MAX_LSQ_STENCIL).T))
+        self._lsq_wy_t = dev_f(np.ascontiguousarray(self.lsq_wy_np.reshape(L, MAX_LSQ_STENCIL).T))
+        self._lsq_wt_t = dev_f(np.ascontiguousarray(self.lsq_wt_np.reshape(L, MAX_LSQ_STENCIL).T))
         # Dirichlet displacement anchors (cbw2d:3288-3304): links of the
         # chosen boundary id, or of Dirichlet ids when that id is negative.
         bids = np.clip(np.asarray(g.link_boundary_id, np.int64), 0, MAX_BOUNDARY_IDS - 1)
@@ -1008,6 +1013,15 @@ class WarpCurvedBoundaryLBM2D:
         for name, arr in ptrs.items():
             setattr(d, name, arr.ptr or None)
         d.use_dyn = 0
+        if self._box_sides is not None:
+            kinds, modes, params = self._box_sides
+            d.box = 1
+            for k in range(4):
+                d.box_side_kind[k] = int(kinds[k])
+                d.box_value_mode[k] = int(modes[k])
+            for k, v in enumerate(np.asarray(params, dtype=np.float64).reshape(32)):
+                d.box_params[k] = float(v)
+            d.length_x, d.length_y = self.length_x, self.length_y
         d.stream = self._stream.cuda_stream or None
         self._desc = d
         h = C.c_void_p()

@@ -9,7 +9,9 @@ Mirrors the host-side interface of the reference module
 (:176-249), ``_first_piola_np`` (:486-564), fluxes and equilibrium
 (:567-603), source (:606-617), the derivative helpers and second-order
 initial populations (:620-848).  The rectangular-box solver of that module
-is not part of this package.
+(``WarpNonlinearElasticLBM2D``, ``RectangularBoundarySet``,
+``AxisBoundarySpec`` and its helpers) lives in ``box_solver.py`` and is
+importable from here under the reference's names.
 """
 
 from __future__ import annotations
@@ -390,3 +392,17 @@ def _flatten_pop(f: np.ndarray) -> np.ndarray:
 
 def _unflatten_pop(f: np.ndarray) -> np.ndarray:
     return f.reshape((Q, NCOMP) + f.shape[1:])
+
+
+_BOX_NAMES = ("WarpNonlinearElasticLBM2D", "RectangularBoundarySet", "AxisBoundarySpec", "SIDE_LEFT", "SIDE_RIGHT",
+              "SIDE_BOTTOM", "SIDE_TOP", "side_id", "kind_id", "smoothstep5_np", "wave_beam_window_mean",
+              "reference_grid", "_edge_derivative", "_enforce_dirichlet_tangent_columns")
+
+
+def __getattr__(name):
+    # The box solver imports this module, so its names resolve on first use.
+    if name in _BOX_NAMES:
+        from . import box_solver
+
+        return getattr(box_solver, name)
+    raise AttributeError(f"module {__name__!r} has no attribute {name!r}")

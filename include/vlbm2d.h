@@ -30,6 +30,12 @@
  *   vlbm2d_reduce          host reductions of summary()/invalid_state() (cbw2d:4307-4341)
  *   vlbm2d_kinetic_filter  _kinetic_ghost_filter_kernel (cbw2d:3100-3124) as launched by
  *                          initialize_from_numpy (cbw2d:3947-3953)
+ *   vlbm2d_set_box_sides   WarpNonlinearElasticLBM2D.set_boundaries
+ *                          (src/curved_lbm/two_d/vector_nonlinear_elastic_warp.py:1970-1979)
+ * With desc.box the same step, refresh and boundary calls run the box solver
+ * WarpNonlinearElasticLBM2D (vector_nonlinear_elastic_warp.py:1896-2352):
+ * vlbm2d_step its step(), vlbm2d_apply_boundary (f_src may equal f_dst) its
+ * apply_boundary_reconstruction (_apply_boundary_side_kernel, :1685-1820).
  *   vlbm2d_destroy         object lifetime end
  */
 #ifndef VLBM2D_H
@@ -43,7 +49,7 @@
 extern "C" {
 #endif
 
-#define VLBM2D_ABI_VERSION 5
+#define VLBM2D_ABI_VERSION 6
 #define VLBM2D_MAX_IDS 8
 #define VLBM2D_MAX_LSQ 24
 #define VLBM2D_RECORD 12
@@ -104,6 +110,13 @@ typedef struct {
     const double *params_by_id;      /* (8, 8) */
     const double *dyn_tx, *dyn_ty;   /* (L) per-link nominal traction (dynamic override) */
     int32_t use_dyn;                 /* initial state of the dynamic override */
+    /* rectangular box solver (WarpNonlinearElasticLBM2D): every node active, no cut links;
+       faces left, right, bottom, top */
+    int32_t box;                     /* 1 = box solver */
+    int32_t box_side_kind[4];        /* 0 periodic (both sides of an axis), 1 Dirichlet, 2 Neumann */
+    int32_t box_value_mode[4];       /* value mode of each face (0..6) */
+    double box_params[32];           /* 8 params per face */
+    double length_x, length_y;       /* box lengths (face coordinates) */
     void *stream;                    /* hipStream_t; NULL = null stream */
 } vlbm2d_desc;
 
@@ -128,6 +141,8 @@ int vlbm2d_stress(vlbm2d_handle *h, double *P, double *sigma);
  * record terms (+ the ghost-moment term of mrt_ghost), with f_prev the
  * populations that step started from. */
 int vlbm2d_post_collision(vlbm2d_handle *h, const void *f_prev, void *fpost);
+/* Replaces the four faces of a box solver (kinds, value modes, 4 x 8 params). */
+int vlbm2d_set_box_sides(vlbm2d_handle *h, const int32_t *kind, const int32_t *mode, const double *params);
 /* Kinetic ghost filter of the descriptor's strength, in place on f at the
  * active nodes (no-op at strength 0).  vlbm2d_step applies it after every
  * step by itself. */

@@ -40,6 +40,8 @@ class _State(C.Structure):
         ("compat_interval", C.c_int32), ("compat_blend", C.c_double), ("compat_interior_only", C.c_int32),
         ("use_dyn", C.c_int32),
         ("collision_model", C.c_int32), ("ghost_omega", C.c_double), ("filter_strength", C.c_double),
+        ("box", C.c_int32), ("side_kind", C.c_int32 * 4), ("side_mode", C.c_int32 * 4),
+        ("side_params", C.c_double * 32), ("length_x", C.c_double), ("length_y", C.c_double),
         ("f0", _dp), ("f1", _dp), ("fpost", _dp),
         ("U", _dp), ("u", _dp), ("u_star", _dp), ("u_next", _dp),
         ("bc_diag", _dp), ("bc_rhs", _dp), ("P", _dp), ("sigma", _dp), ("invalid", _dp),
@@ -317,3 +319,76 @@ class OracleSolver2D:
 
     def numpy_field(self, name: str) -> np.ndarray:
         return getattr(self, name).copy()
+
+
+class OracleBox2D(OracleSolver2D):
+    """CPU FP64 oracle of the rectangular-box solver
+    (vector_nonlinear_elastic_warp.py:1896-2352): every node active, no cut
+    links, periodic axes wrapped in the stream, half-way faces
+    (_apply_boundary_side_kernel, :1685-1820)."""
+
+    def __init__(self, nx, ny, *, material, boundaries=None, length_x=1.0, length_y=1.0, lattice_speed=5.0,
+                 collision_omega=1.8, jacobian_floor=1.0e-12, source_mode="none", source_vector=(0.0,) * 6,
+                 damping_gamma=0.0, threads: int | None = None, **_ignored):
+        from curved_lbm.two_d import box_solver as bx
+        from curved_lbm.two_d import curved_boundary_warp as cb
+
+        empty_i, empty_f = np.zeros(0, dtype=np.int32), np.zeros(0)
+        g = cb.CurvedBoundaryGeometry(
+            nx=int(nx), ny=int(ny), length_x=float(length_x), length_y=float(length_y),
+            active=np.ones((int(nx), int(ny)), dtype=np.int32), link_i=empty_i, link_j=empty_i, link_q=empty_i,
+            link_boundary_id=empty_i, link_eta=empty_f, link_xb=empty_f, link_yb=empty_f, link_nx=empty_f,
+            link_ny=empty_f, label="rectangle", lsq_i=np.zeros((0, 24), dtype=np.int32),
+            lsq_j=np.zeros((0, 24), dtype=np.int32), lsq_wb=np.zeros((0, 24)), lsq_wt=np.zeros((0, 24)),
+            lsq_wx=np.zeros((0, 24)), lsq_wy=np.zeros((0, 24)), lsq_valid=np.zeros(0, dtype=np.int32))
+        super().__init__(g, material=material, boundary=cb.CurvedBoundarySpec(kind="neumann"),
+                         lattice_speed=lattice_speed, collision_omega=collision_omega,
+                         jacobian_floor=jacobian_floor, source_mode=source_mode, source_vector=source_vector,
+                         damping_gamma=damping_gamma, threads=threads)
+        self.length_x, self.length_y = float(length_x), float(length_y)
+        self._bx = bx
+        self.boundaries = boundaries or bx.RectangularBoundarySet.periodic()
+        self._set_sides(self.boundaries)
+        self.initialize_identity()
+
+    def _set_sides(self, b) -> None:
+        s = self._s
+        s.box = 1
+        for k in range(4):
+            s.side_kind[k] = int(b.side_kind[k])
+            s.side_mode[k] = int(b.value_mode[k])
+        for k, v in enumerate(np.asarray(b.params, dtype=np.float64).reshape(32)):
+            s.side_params[k] = float(v)
+        s.length_x, s.length_y = self.length_x, self.length_y
+
+    def set_boundaries(self, boundaries) -> None:
+        boundaries.validate_periodic_pairs()
+        self.boundaries = boundaries
+        self._set_sides(boundaries)
+
+    def initialize_from_numpy(self, *, U, displacement=None, init_order=1, enforce_dirichlet_tangent=True,
+                              apply_initial_boundaries=True, initial_boundary_time=0.0) -> None:
+        from curved_lbm.two_d import vector_nonlinear_elastic_warp as ve
+
+        if not hasattr(self, "_bx"):  # the base constructor's identity start, before the faces are set
+            return super().initialize_from_numpy(U=U, displacement=displacement, init_order=init_order,
+                                                 apply_initial_boundaries=False)
+        U = np.asarray(U, dtype=np.float64)
+        disp = np.zeros((2, self.nx, self.ny)) if displacement is None else np.asarray(displacement, np.float64)
+        if enforce_dirichlet_tangent:
+            U = self._bx._enforce_dirichlet_tangent_columns(U, disp, self.boundaries, dx=self.dx, dy=self.dy)
+        super().initialize_from_numpy(U=U, displacement=disp, init_order=1, apply_initial_boundaries=False)
+        if init_order == 2:
+            f = ve._second_order_populations_np(
+                U, self.material, self.lattice_speed, dx=self.dx, dy=self.dy, dt=self.dt,
+                periodic_x=self.boundaries.periodic_x, periodic_y=self.boundaries.periodic_y,
+                source_mode=self.source_mode, source_params=self.source_params)
+            self.f0[...] = f.reshape((24, self.nx, self.ny))
+            self.refresh_current()
+        elif init_order != 1:
+            raise ValueError("init_order must be 1 or 2")
+        if apply_initial_boundaries:
+            self.apply_boundary_reconstruction(float(initial_boundary_time), in_place=True)
+            self.refresh_current()
+            self.u_star[...] = disp - 0.5 * self.dt * self.U[:2]
+            self.refresh_current()

@@ -799,8 +799,11 @@ static void repair_F(or2d_state *s) {
 }
 
 /* cbw2d:2044-2060 with the clear of f1 before it. */
+/* cbw2d:2044-2060; with a periodic box axis the target wraps (vnew2d:1646-1681). */
 static void stream_push(or2d_state *s) {
     const int nx = s->nx, ny = s->ny;
+    const int px = s->box && s->side_kind[0] == 0 && s->side_kind[1] == 0;
+    const int py = s->box && s->side_kind[2] == 0 && s->side_kind[3] == 0;
     memset(s->f1, 0, sizeof(double) * FQ * (size_t)nx * ny);
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i)
@@ -808,6 +811,8 @@ static void stream_push(or2d_state *s) {
             if (s->active[(size_t)i * ny + j] == 0) continue;
             for (int q = 0; q < NQ; ++q) {
                 int ii = i + CX[q], jj = j + CY[q];
+                if (px) ii = ii >= nx ? 0 : (ii < 0 ? nx - 1 : ii);
+                if (py) jj = jj >= ny ? 0 : (jj < 0 ? ny - 1 : jj);
                 if (ii >= 0 && ii < nx && jj >= 0 && jj < ny && s->active[(size_t)ii * ny + jj] != 0)
                     for (int a = 0; a < NC; ++a)
                         s->f1[IDX(s, q * 6 + a, ii, jj)] = s->fpost[IDX(s, q * 6 + a, i, j)];
@@ -1045,8 +1050,109 @@ static void apply_links(or2d_state *s, const double *fpost, double *f1, double t
 }
 
 /* WarpCurvedBoundaryLBM2D.apply_boundary_reconstruction (cbw2d:4020-4034). */
+/* vnew2d:1284-1448 (_normal_residual, _solve_normal_column): the face F of
+ * an axis-aligned Neumann face of the box. */
+static void box_residual(const or2d_state *s, const double *F, int axis, double sign, double tx, double ty,
+                         double *r) {
+    double P[4];
+    piola_s(s, F, P);
+    r[0] = sign * P[axis] - tx;
+    r[1] = sign * P[2 + axis] - ty;
+}
+static void solve_normal_column(const or2d_state *s, const double *Fin, int axis, double sign, double tx, double ty,
+                                double *F) {
+    for (int k = 0; k < 4; ++k) F[k] = Fin[k];
+    double x0 = Fin[axis], x1 = Fin[2 + axis];
+    const double eps = 1.0e-6;
+    for (int it = 0; it < 12; ++it) {
+        F[axis] = x0;
+        F[2 + axis] = x1;
+        double r[2], rp0[2], rp1[2], Fp[4];
+        box_residual(s, F, axis, sign, tx, ty, r);
+        if (sqrt(r[0] * r[0] + r[1] * r[1]) < 1.0e-10) break;
+        for (int k = 0; k < 4; ++k) Fp[k] = F[k];
+        Fp[axis] += eps;
+        box_residual(s, Fp, axis, sign, tx, ty, rp0);
+        for (int k = 0; k < 4; ++k) Fp[k] = F[k];
+        Fp[2 + axis] += eps;
+        box_residual(s, Fp, axis, sign, tx, ty, rp1);
+        double a = (rp0[0] - r[0]) / eps, c = (rp0[1] - r[1]) / eps;
+        double b = (rp1[0] - r[0]) / eps, d = (rp1[1] - r[1]) / eps;
+        double det = a * d - b * c;
+        if (fabs(det) < 1.0e-14) break;
+        double d0 = (-r[0] * d + b * r[1]) / det, d1 = (c * r[0] - a * r[1]) / det;
+        double step = 1.0;
+        for (int ls = 0; ls < 10; ++ls) {
+            double c0 = x0 + step * d0, c1 = x1 + step * d1, T[4];
+            for (int k = 0; k < 4; ++k) T[k] = Fin[k];
+            T[axis] = c0;
+            T[2 + axis] = c1;
+            if (T[0] * T[3] - T[1] * T[2] > s->j_floor) {
+                x0 = c0;
+                x1 = c1;
+                break;
+            }
+            step = 0.5 * step;
+        }
+    }
+    F[axis] = x0;
+    F[2 + axis] = x1;
+}
+
+/* vnew2d:1685-1820 (_apply_boundary_side_kernel) for the four faces of the
+ * box, sides in order left, right, bottom, top. */
+static void apply_sides(or2d_state *s, const double *fpost, double *f1, double time_mid) {
+    const int nx = s->nx, ny = s->ny;
+    const double c = s->c;
+    for (int side = 0; side < 4; ++side) {
+        const int kind = s->side_kind[side];
+        if (kind == 0) continue;
+        const int nn = side < 2 ? ny : nx;
+        const double *p = s->side_params + 8 * side;
+        for (int sidx = 0; sidx < nn; ++sidx) {
+            int i = 0, j = sidx, nxn = -1, nyn = 0, q_in = 0, q_out = 2, axis = 0;
+            double sign = -1.0;
+            if (side == 1) { i = nx - 1; nxn = 1; q_in = 2; q_out = 0; sign = 1.0; }
+            else if (side == 2) { i = sidx; j = 0; nxn = 0; nyn = -1; q_in = 1; q_out = 3; axis = 1; }
+            else if (side == 3) { i = sidx; j = ny - 1; nxn = 0; nyn = 1; q_in = 3; q_out = 1; axis = 1; sign = 1.0; }
+            double x = (i + 0.5) * s->dx, y = (j + 0.5) * s->dy;
+            if (side == 0) x = 0.0;
+            else if (side == 1) x = s->length_x;
+            else if (side == 2) y = 0.0;
+            else y = s->length_y;
+            const int mode = s->side_mode[side];
+            const double v0 = or2d_boundary_value(0, mode, p, x, y, 0.0, 0.0, time_mid);
+            const double v1 = or2d_boundary_value(1, mode, p, x, y, 0.0, 0.0, time_mid);
+            double fo[6];
+            for (int a = 0; a < 6; ++a) fo[a] = fpost[IDX(s, q_out * 6 + a, i, j)];
+            double val[6];
+            if (kind == BC_DIR) {
+                val[0] = -fo[0] + 0.5 * v0;
+                val[1] = -fo[1] + 0.5 * v1;
+                val[2] = fo[2] + (double)nxn * v0 / c;
+                val[3] = fo[3] + (double)nyn * v0 / c;
+                val[4] = fo[4] + (double)nxn * v1 / c;
+                val[5] = fo[5] + (double)nyn * v1 / c;
+            } else {
+                double Fn[4], Fb[4];
+                for (int k = 0; k < 4; ++k) Fn[k] = s->U[IDX(s, 2 + k, i, j)];
+                solve_normal_column(s, Fn, axis, sign, v0, v1, Fb);
+                val[0] = fo[0] + v0 / c;
+                val[1] = fo[1] + v1 / c;
+                for (int k = 0; k < 4; ++k) val[2 + k] = -fo[2 + k] + 0.5 * Fb[k];
+            }
+            for (int a = 0; a < 6; ++a) f1[IDX(s, q_in * 6 + a, i, j)] = val[a];
+        }
+    }
+}
+
 int or2d_apply_boundary(or2d_state *s, double t, int32_t in_place) {
     if (!recon_supported(s->recon)) return -1;
+    if (s->box) {
+        if (in_place) apply_sides(s, s->f0, s->f0, t);
+        else apply_sides(s, s->fpost, s->f1, t);
+        return 0;
+    }
     if (s->n_links == 0) return 0;
     const size_t n = FQ * (size_t)s->nx * s->ny;
     if (in_place) {
@@ -1078,7 +1184,8 @@ int or2d_step(or2d_state *s, double time, int64_t steps, int32_t nsteps) {
         }
         if (s->compat_interval > 0 && steps % s->compat_interval == 0) repair_F(s);
         stream_push(s);
-        if (s->n_links > 0) apply_links(s, s->fpost, s->f1, tb);
+        if (s->box) apply_sides(s, s->fpost, s->f1, tb);
+        else if (s->n_links > 0) apply_links(s, s->fpost, s->f1, tb);
         or2d_kinetic_filter(s, s->f1);
         double *t = f0;
         f0 = f1;

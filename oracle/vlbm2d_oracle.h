@@ -32,6 +32,10 @@ typedef struct {
     int32_t collision_model;           /* 0 bgk, 1 mrt_ghost */
     double ghost_omega;                /* mrt_ghost relaxation of the ghost moment */
     double filter_strength;            /* kinetic ghost filter after each step, [0, 1] */
+    int32_t box;                       /* 1: rectangular box solver (no cut links) */
+    int32_t side_kind[4], side_mode[4]; /* left, right, bottom, top: 0 periodic, 1 Dirichlet, 2 Neumann */
+    double side_params[32];
+    double length_x, length_y;
     /* state */
     double *f0, *f1, *fpost;           /* (24, nx, ny) */
     double *U, *u, *u_star, *u_next;   /* (6|2, nx, ny) */
