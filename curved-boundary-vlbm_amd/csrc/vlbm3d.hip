@@ -878,12 +878,100 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_final(double *__restrict__ par
     }
 }
 
+// Faces of the rectangular box solver (left, right, bottom, top, front,
+// back): kind 0 periodic / 1 Dirichlet velocity / 2 Neumann traction, value
+// mode and 16 params each, the lattice and the box lengths.
+struct Box3 {
+    int kind[6], mode[6];
+    double prm[6][16];
+    int n[3];
+    double len[3];
+};
+
+// Half-way face reconstruction of the box solver (vnew3d:1999-2200): one
+// thread per (face, face node); the box's nodes are all active in (i, j, k)
+// order with k fastest.  Dirichlet faces bounce back with the velocity
+// datum (velocity rows -f + v/3, F rows f + n v / c); Neumann faces impose
+// the traction on the velocity rows (f + T / c) and the face F of the column
+// solve on the F rows (-f + F_b / 3).  Faces read and write disjoint
+// populations.
+template <typename T>
+__global__ __launch_bounds__(128) void k_box_sides3(const T *fsrc, T *fdst, const T *__restrict__ rec,
+                                                     const double *__restrict__ U, int use_rec, Box3 B,
+                                                     double time, Params P) {
+    const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
+    int m = ny * nz;
+    if (nx * nz > m) m = nx * nz;
+    if (nx * ny > m) m = nx * ny;
+    const long long t = (long long)blockIdx.x * 128 + threadIdx.x;
+    if (t >= 6LL * m) return;
+    const int side = (int)(t / m), s = (int)(t - (long long)side * m);
+    const int kind = B.kind[side];
+    if (kind == 0) return;
+    const int axis = side / 2;
+    const int nn = axis == 0 ? ny * nz : (axis == 1 ? nx * nz : nx * ny);
+    if (s >= nn) return;
+    int i, j, k;
+    if (axis == 0) {
+        i = (side & 1) ? nx - 1 : 0;
+        j = s % ny;
+        k = s / ny;
+    } else if (axis == 1) {
+        j = (side & 1) ? ny - 1 : 0;
+        i = s % nx;
+        k = s / nx;
+    } else {
+        k = (side & 1) ? nz - 1 : 0;
+        i = s % nx;
+        j = s / nx;
+    }
+    const int outward = (side & 1) ? 1 : -1;     // normal component along the axis
+    const int qo = (side & 1) ? axis : axis + 3; // outgoing direction (towards the face)
+    const int qi = opp(qo);
+    const double sign = (double)outward;
+    const size_t n = P.n;
+    const int p = (i * ny + j) * nz + k;
+    double xb[3] = {(i + 0.5) * P.dx, (j + 0.5) * P.dy, (k + 0.5) * P.dz};
+    xb[axis] = (side & 1) ? B.len[axis] : 0.0;
+    const double *prm = B.prm[side];
+    double v[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = box_value3(c, B.mode[side], prm, xb[0], xb[1], xb[2], time);
+    const double om1 = 1.0 - P.omega;
+    double fo[12];
+#pragma unroll
+    for (int a = 0; a < 12; ++a)
+        fo[a] = use_rec ? fpost(fsrc, rec, n, qo, a, p, om1, P.omega) : ld(fsrc, (size_t)(qo * 12 + a) * n + p, a);
+    double val[12];
+    if (kind == BC_DIR) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) val[c] = -fo[c] + v[c] / 3.0;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                val[3 + 3 * r + c] = fo[3 + 3 * r + c] + (c == axis ? (double)outward : 0.0) * v[r] / P.c;
+    } else {
+        double F[9];
+#pragma unroll
+        for (int a = 0; a < 9; ++a) F[a] = U[(size_t)(3 + a) * n + p];
+        solve_normal_column3(F, axis, sign, v, P.m);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) val[c] = fo[c] + v[c] / P.c;
+#pragma unroll
+        for (int a = 0; a < 9; ++a) val[3 + a] = -fo[3 + a] + F[a] / 3.0;
+    }
+#pragma unroll
+    for (int a = 0; a < 12; ++a) st(fdst, (size_t)(qi * 12 + a) * n + p, a, val[a]);
+}
+
 }  // namespace
 
 struct vlbm3d_handle {
     vlbm3d_desc d;
     Params P;
     LinkTables T;
+    Box3 box{};  // faces of the rectangular box solver (d.box != 0)
     hipStream_t stream;
     Timing timing;
     // Library-owned tables built at create: the compacted anchor list of
@@ -1038,6 +1126,17 @@ template <typename T>
 static int run_boundary(vlbm3d_handle *h, hipStream_t st, const T *src, T *dst, double t, int use_rec,
                         int32_t *invalid) {
     const vlbm3d_desc &d = h->d;
+    if (d.box) {
+        const Box3 &B = h->box;
+        long long m = (long long)B.n[1] * B.n[2];
+        if ((long long)B.n[0] * B.n[2] > m) m = (long long)B.n[0] * B.n[2];
+        if ((long long)B.n[0] * B.n[1] > m) m = (long long)B.n[0] * B.n[1];
+        TimedSlot ts(h->timing, st, VLBM_SLOT_LINKS);
+        hipLaunchKernelGGL((k_box_sides3<T>), dim3((unsigned)((6 * m + 127) / 128)), dim3(128), 0, st, src, dst,
+                           (const T *)d.record, (const double *)d.U, use_rec, B, t, h->P);
+        CHECK_LAUNCH("k_box_sides3");
+        return 0;
+    }
     if (d.n_links == 0) return 0;
     if (d.has_char && d.n_char_links > 0) {
         TimedSlot ts(h->timing, st, VLBM_SLOT_GATHER);
@@ -1178,6 +1277,18 @@ int vlbm3d_create(const vlbm3d_desc *desc, vlbm3d_handle **out) {
     if (desc->recon == R_COMPAT && desc->n_links > 0 && (!desc->tan_p || !desc->tan_w1 || !desc->tan_w2 || !desc->tan_valid))
         return fail_msg("vlbm3d_create: compat_bfl needs the tangent stencil tables");
     if (desc->recon < 0 || desc->recon > R_LOCALF_BFL) return fail_msg("vlbm3d_create: unknown reconstruction");
+    if (desc->box) {
+        const int *bn = desc->box_n;
+        if (bn[0] < 1 || bn[1] < 1 || bn[2] < 1 || (long long)bn[0] * bn[1] * bn[2] != desc->n_active)
+            return fail_msg("vlbm3d_create: a box needs every node of box_n active");
+        if (desc->n_links != 0) return fail_msg("vlbm3d_create: a box has no cut links (n_links must be 0)");
+        for (int s = 0; s < 6; ++s)
+            if (desc->box_side_kind[s] < 0 || desc->box_side_kind[s] > 2)
+                return fail_msg("vlbm3d_create: box side kind must be 0 (periodic), 1 (Dirichlet) or 2 (Neumann)");
+        for (int a = 0; a < 3; ++a)
+            if ((desc->box_side_kind[2 * a] == 0) != (desc->box_side_kind[2 * a + 1] == 0))
+                return fail_msg("vlbm3d_create: periodic box faces must come in pairs");
+    }
     if (desc->disp_sweeps < 1 || desc->disp_sweeps > 8) return fail_msg("vlbm3d_create: disp_sweeps must be in [1,8]");
     if ((size_t)desc->n_active * FQ >= (size_t)1 << 40) return fail_msg("vlbm3d_create: lattice too large");
     const void *req[] = {desc->U, desc->u, desc->u_star, desc->u_next, desc->record, desc->invalid,
@@ -1205,6 +1316,15 @@ int vlbm3d_create(const vlbm3d_desc *desc, vlbm3d_handle **out) {
     h->d = *desc;
     h->P = make_params(*desc);
     h->T = make_tables(*desc);
+    for (int s = 0; s < 6; ++s) {
+        h->box.kind[s] = desc->box_side_kind[s];
+        h->box.mode[s] = desc->box_value_mode[s];
+        for (int k = 0; k < 16; ++k) h->box.prm[s][k] = desc->box_params[16 * s + k];
+    }
+    for (int a = 0; a < 3; ++a) {
+        h->box.n[a] = desc->box_n[a];
+        h->box.len[a] = desc->box_length[a];
+    }
     h->stream = (hipStream_t)desc->stream;
     const int rc = build_anchor_tables(h);
     if (rc) {
@@ -1261,7 +1381,7 @@ int vlbm3d_refresh(vlbm3d_handle *h, const void *f) {
 int vlbm3d_apply_boundary(vlbm3d_handle *h, void *f_src, void *f_dst, double t, int32_t in_place) {
     if (!h || !f_src || !f_dst) return fail_msg("vlbm3d_apply_boundary: null argument");
     const vlbm3d_desc &d = h->d;
-    if (d.n_links == 0) return 0;
+    if (d.n_links == 0 && !d.box) return 0;
     const size_t count = (size_t)FQ * d.n_active;
     const int gc = (int)((count + BLOCK - 1) / BLOCK);
     if (d.precision == VLBM_PREC_F64) {

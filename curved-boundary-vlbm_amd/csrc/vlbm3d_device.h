@@ -293,6 +293,102 @@ __device__ inline double curved_value(int mode, const double *prm, int comp, dou
     }
 }
 
+// Face datum of the rectangular box solver (vnew3d:1393-1452); its value
+// modes are numbered apart from the curved ones: 0 zero, 1 constant,
+// 2 sin^2 ramp, 3 sin^2 hold, 4 affine velocity (params A row-major, b,
+// active time at 12), 5 shear velocity x.
+__device__ inline double box_value3(int comp, int mode, const double *p, double x, double y, double z, double t) {
+    switch (mode) {
+    case 1: return p[comp];
+    case 2: {
+        double scale = 1.0;
+        if (p[3] > 0.0 && t < p[3]) {
+            const double s = sin(PI_D * t / p[3]);
+            scale = s * s;
+        }
+        return p[comp] * scale;
+    }
+    case 3: {
+        double scale = 1.0;
+        if (p[3] > 0.0 && t < p[3]) {
+            double den = p[4];
+            if (den <= 0.0) den = 2.0 * p[3];
+            const double s = sin(PI_D * t / den);
+            scale = s * s;
+        }
+        return p[comp] * scale;
+    }
+    case 4:
+        if (p[12] > 0.0 && t > p[12]) return 0.0;
+        return p[3 * comp] * x + p[3 * comp + 1] * y + p[3 * comp + 2] * z + p[9 + comp];
+    case 5: return (comp == 0 && t < p[1]) ? p[0] * sin(PI_D * t / p[1]) : 0.0;
+    default: return 0.0;
+    }
+}
+
+// Traction solve of an axis-aligned box face (vnew3d:1455-1766): Newton with
+// a forward-difference Jacobian (eps 1e-6) on the F column of the normal
+// axis, residual sign * P column - T below 1e-10, at most 16 iterations,
+// each step halved up to 12 times until det F > j_floor.  F (row-major)
+// holds the node's F on entry and the face F on exit.
+__device__ inline void box_residual3(const double *F, int axis, double sign, const double *T, const Material &m,
+                                     double *r) {
+    double P[9];
+    piola(F, m, P);
+#pragma unroll
+    for (int row = 0; row < 3; ++row) r[row] = sign * P[3 * row + axis] - T[row];
+}
+__device__ inline void solve_normal_column3(double *F, int axis, double sign, const double *T, const Material &m) {
+    double Fin[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Fin[k] = F[k];
+    double x[3] = {Fin[axis], Fin[3 + axis], Fin[6 + axis]};
+    const double eps = 1.0e-6;
+    auto with = [&](const double *col, double *G) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) G[k] = Fin[k];
+#pragma unroll
+        for (int row = 0; row < 3; ++row) G[3 * row + axis] = col[row];
+    };
+    for (int it = 0; it < 16; ++it) {
+        double G[9], r[3], rp[3][3];
+        with(x, G);
+        box_residual3(G, axis, sign, T, m, r);
+        if (sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) < 1.0e-10) break;
+        for (int c = 0; c < 3; ++c) {
+            double xc[3] = {x[0], x[1], x[2]};
+            xc[c] = x[c] + eps;
+            with(xc, G);
+            box_residual3(G, axis, sign, T, m, rp[c]);
+        }
+        // J[row][c] = d r[row] / d x[c]
+        const double a = (rp[0][0] - r[0]) / eps, b = (rp[1][0] - r[0]) / eps, c = (rp[2][0] - r[0]) / eps;
+        const double d = (rp[0][1] - r[1]) / eps, e = (rp[1][1] - r[1]) / eps, f = (rp[2][1] - r[1]) / eps;
+        const double g = (rp[0][2] - r[2]) / eps, h = (rp[1][2] - r[2]) / eps, ii = (rp[2][2] - r[2]) / eps;
+        const double det = a * (e * ii - f * h) - b * (d * ii - f * g) + c * (d * h - e * g);
+        if (fabs(det) < 1.0e-14) break;
+        const double i00 = (e * ii - f * h) / det, i01 = (c * h - b * ii) / det, i02 = (b * f - c * e) / det;
+        const double i10 = (f * g - d * ii) / det, i11 = (a * ii - c * g) / det, i12 = (c * d - a * f) / det;
+        const double i20 = (d * h - e * g) / det, i21 = (b * g - a * h) / det, i22 = (a * e - b * d) / det;
+        const double d0 = -(i00 * r[0] + i01 * r[1] + i02 * r[2]);
+        const double d1 = -(i10 * r[0] + i11 * r[1] + i12 * r[2]);
+        const double d2 = -(i20 * r[0] + i21 * r[1] + i22 * r[2]);
+        double step = 1.0;
+        for (int ls = 0; ls < 12; ++ls) {
+            const double cand[3] = {x[0] + step * d0, x[1] + step * d1, x[2] + step * d2};
+            with(cand, G);
+            if (det3(G) > m.jf) {
+                x[0] = cand[0];
+                x[1] = cand[1];
+                x[2] = cand[2];
+                break;
+            }
+            step = 0.5 * step;
+        }
+    }
+    with(x, F);
+}
+
 __device__ __forceinline__ void normalize3(double *v) {
     double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     if (r < 1.0e-30) r = 1.0e-30;

@@ -15,7 +15,7 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libvlbm.so"
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 ABI2_VERSION = 6
 PREC_F64 = 0
 PREC_F32 = 1
@@ -113,6 +113,12 @@ class Vlbm3dDesc(C.Structure):
         ("lbv", C.c_void_p),
         ("lFb", C.c_void_p),
         ("lPb", C.c_void_p),
+        ("box", C.c_int32),
+        ("box_n", C.c_int32 * 3),
+        ("box_side_kind", C.c_int32 * 6),
+        ("box_value_mode", C.c_int32 * 6),
+        ("box_params", C.c_double * 96),
+        ("box_length", C.c_double * 3),
         ("stream", C.c_void_p),
     ]
 

@@ -544,6 +544,23 @@ class CurvedBoundaryGeometry3D:
         )
 
 
+def periodic_neighbor_table(tables, shape, periodic) -> np.ndarray:
+    """Neighbour table (6, n) of the active nodes with the axes flagged in
+    `periodic` wrapped (rectangular box solver, vnew3d:1947-1995)."""
+    ijk = np.stack((tables["active_i"], tables["active_j"], tables["active_k"]), axis=1).astype(np.int64)
+    dims = np.array(shape, dtype=np.int64)
+    active_id = tables["active_id"]
+    out = np.full((Q, ijk.shape[0]), -1, dtype=np.int32)
+    for q in range(Q):
+        t = ijk + DIRS3[q]
+        for a in range(3):
+            if periodic[a]:
+                t[:, a] %= dims[a]
+        ok = np.all((t >= 0) & (t < dims), axis=1)
+        out[q, ok] = active_id[t[ok, 0], t[ok, 1], t[ok, 2]]
+    return np.ascontiguousarray(out)
+
+
 def _sparse_tables_from_geometry(geometry: CurvedBoundaryGeometry3D) -> dict[str, np.ndarray]:
     """Active-node ids (lexicographic, k fastest), neighbour table (6, n) with
     -1 for missing neighbours, link node and upstream node (cbw3d:3683-3722)."""
@@ -861,7 +878,13 @@ class WarpSparseCurvedBoundaryLBM3D:
         normal_jacobian: Literal["finite_difference", "acoustic"] | None = None,
         neumann_solver: Literal["newton", "energy"] | None = None,
         precision: Precision = "fp64",
+        _box=None,
     ):
+        # _box: dict(side_kind (6), value_mode (6), params (6, 16), lengths (3)) of
+        # the rectangular box solver built on this class
+        # (vector_nonlinear_elastic_warp3d.WarpNonlinearElasticLBM3D); None for
+        # a curved geometry.
+        self._box_sides = _box
         self.geometry = geometry
         self.nx, self.ny, self.nz = geometry.nx, geometry.ny, geometry.nz
         self.length_x, self.length_y, self.length_z = geometry.length_x, geometry.length_y, geometry.length_z
@@ -925,6 +948,10 @@ class WarpSparseCurvedBoundaryLBM3D:
         self._x_dense = self._y_dense = self._z_dense = None
         self.active_np = np.ascontiguousarray(geometry.active.astype(np.int32))
         tables = _sparse_tables_from_geometry(geometry)
+        if _box is not None:
+            kinds = np.asarray(_box["side_kind"])
+            tables["neighbor"] = periodic_neighbor_table(tables, geometry.active.shape,
+                                                         [bool(kinds[2 * a] == 0) for a in range(3)])
         self.tables = tables
         self.active_i, self.active_j, self.active_k = tables["active_i"], tables["active_j"], tables["active_k"]
         self.active_id_np = tables["active_id"]
@@ -1084,12 +1111,40 @@ class WarpSparseCurvedBoundaryLBM3D:
         }
         for name, arr in ptrs.items():
             setattr(d, name, arr.ptr or None)
+        if self._box_sides is not None:
+            self._fill_box_desc(d, self._box_sides)
         d.stream = self._stream.cuda_stream or None
         self._desc = d
         h = C.c_void_p()
         _native.check(self._lib.vlbm3d_create(C.byref(d), C.byref(h)), "vlbm3d_create")
         self._handle = h
         self._native = _native
+
+    def _fill_box_desc(self, d, box) -> None:
+        d.box = 1
+        for a, n in enumerate(self.active_np.shape):
+            d.box_n[a] = int(n)
+            d.box_length[a] = float(box["lengths"][a])
+        for s in range(6):
+            d.box_side_kind[s] = int(box["side_kind"][s])
+            d.box_value_mode[s] = int(box["value_mode"][s])
+        for k, v in enumerate(np.asarray(box["params"], dtype=np.float64).reshape(96)):
+            d.box_params[k] = float(v)
+
+    def _set_box_faces(self, box) -> None:
+        """Replaces the faces of a box solver: the neighbour table is rebuilt
+        for the new periodic axes and the library handle re-created."""
+        self._torch.cuda.synchronize(self.device)
+        kinds = np.asarray(box["side_kind"])
+        nb = periodic_neighbor_table(self.tables, self.active_np.shape, [bool(kinds[2 * a] == 0) for a in range(3)])
+        self.tables["neighbor"] = nb
+        self.neighbor.assign(nb)
+        self._box_sides = box
+        self._fill_box_desc(self._desc, box)
+        self._lib.vlbm3d_destroy(self._handle)
+        h = C.c_void_p()
+        self._check(self._lib.vlbm3d_create(C.byref(self._desc), C.byref(h)), "vlbm3d_create")
+        self._handle = h
 
     def __del__(self):
         h = getattr(self, "_handle", None)
@@ -1261,7 +1316,7 @@ class WarpSparseCurvedBoundaryLBM3D:
 
     # ---- time stepping -------------------------------------------------------
     def apply_boundary_reconstruction(self, t: float, *, in_place: bool = False):
-        if self.geometry.n_links == 0:
+        if self.geometry.n_links == 0 and self._box_sides is None:
             return
         self._check(
             self._lib.vlbm3d_apply_boundary(self._handle, self.f0.ptr, self.f1.ptr, float(t), 1 if in_place else 0),

@@ -337,3 +337,58 @@ def _flux_jvp_np(U: np.ndarray, W: np.ndarray, material, axis: int) -> np.ndarra
         except FloatingPointError:
             eps *= 0.25
     raise FloatingPointError("failed to compute finite-difference flux tangent")
+
+
+def _spatial_derivative(U: np.ndarray, spacing: float, axis: int, periodic: bool) -> np.ndarray:
+    """d/dx_axis of a (c, nx, ny, nz) field: centred, wrapped on a periodic
+    axis, else second-order one-sided at the two ends (vnew3d:563-597)."""
+    U = np.asarray(U, dtype=np.float64)
+    h2 = 2.0 * float(spacing)
+    if periodic:
+        return (np.roll(U, -1, axis=axis) - np.roll(U, 1, axis=axis)) / h2
+    out = np.zeros_like(U)
+    if U.shape[axis] < 3:
+        return out
+
+    def at(sl):
+        idx = [slice(None)] * U.ndim
+        idx[axis] = sl
+        return tuple(idx)
+
+    out[at(slice(1, -1))] = (U[at(slice(2, None))] - U[at(slice(None, -2))]) / h2
+    out[at(0)] = (-3.0 * U[at(0)] + 4.0 * U[at(1)] - U[at(2)]) / h2
+    out[at(-1)] = (3.0 * U[at(-1)] - 4.0 * U[at(-2)] + U[at(-3)]) / h2
+    return out
+
+
+def _second_order_populations_np(U, material, lattice_speed, *, dx, dy, dz, dt, periodic_x, periodic_y, periodic_z,
+                                 source_mode=SOURCE_NONE, source_params=None) -> np.ndarray:
+    """Second-order initial populations of a box state (vnew3d:617-653):
+    f = feq(U) - dt/2 * (W_q + 3 sum_A s_A A_A W_q / c_A) / 6 with
+    W_q = dU/dt + c s_q . grad U and dU/dt = B - sum_A A_A dU/dx_A."""
+    if source_params is None:
+        source_params = np.zeros(NCOMP)
+    speeds = _speed_tuple(lattice_speed)
+    grads = [_spatial_derivative(U, h, axis=a + 1, periodic=p)
+             for a, (h, p) in enumerate(((dx, periodic_x), (dy, periodic_y), (dz, periodic_z)))]
+    Ut = _source_np(U, source_mode, source_params) - sum(_flux_jvp_np(U, grads[a], material, a) for a in range(3))
+    out = _equilibrium_np(U, material, lattice_speed)
+    for q, s in enumerate(_DIRS):
+        Wq = Ut + sum(s[a] * speeds[a] * grads[a] for a in range(3))
+        corr = Wq + 3.0 * sum(s[a] * _flux_jvp_np(U, Wq, material, a) / speeds[a] for a in range(3))
+        out[q] = out[q] - 0.5 * float(dt) * ((1.0 / 6.0) * corr)
+    return out
+
+
+_BOX_NAMES = ("WarpNonlinearElasticLBM3D", "BoxBoundarySet", "BoxBoundarySpec", "SIDE_LEFT", "SIDE_RIGHT",
+              "SIDE_BOTTOM", "SIDE_TOP", "SIDE_FRONT", "SIDE_BACK", "side_id", "kind_id", "reference_grid",
+              "_edge_derivative_along", "_enforce_dirichlet_tangent_columns")
+
+
+def __getattr__(name):
+    # The box solver imports this module, so its names resolve on first use.
+    if name in _BOX_NAMES:
+        from . import box_solver3d
+
+        return getattr(box_solver3d, name)
+    raise AttributeError(f"module {__name__!r} has no attribute {name!r}")

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define VLBM3D_ABI_VERSION 5
+#define VLBM3D_ABI_VERSION 6
 #define VLBM3D_MAX_IDS 16
 #define VLBM3D_MAX_CHAR 64
 #define VLBM3D_RECORD 24
@@ -56,7 +56,7 @@ typedef struct {
     double dt, c, omega, dx, dy, dz;
     int32_t source_mode;
     double source_params[12];
-    int32_t recon;         /* 0 bfl, 1 halfway, 3 local_f, 4 local_f_bfl */
+    int32_t recon;         /* 0 bfl, 1 halfway, 2 compat_bfl, 3 local_f, 4 local_f_bfl */
     int32_t normal_solver; /* 0 finite-difference Newton, 1 acoustic Newton, 2 energy */
     /* local kinematics */
     int32_t disp_interval, disp_sweeps;
@@ -105,6 +105,15 @@ typedef struct {
     int32_t *lkind;                  /* (L) */
     int32_t *lcomp_kind;             /* (3,L) */
     double *lbv, *lFb, *lPb;         /* (3,L), (9,L), (9,L) */
+    /* rectangular box solver (WarpNonlinearElasticLBM3D): every node of a box_n box active in
+       (i, j, k) order with k fastest, n_links = 0, periodic wrap in the neighbour table; faces
+       left, right, bottom, top, front, back */
+    int32_t box;                     /* 1 = box solver */
+    int32_t box_n[3];
+    int32_t box_side_kind[6];        /* 0 periodic (both faces of an axis), 1 Dirichlet, 2 Neumann */
+    int32_t box_value_mode[6];       /* box value modes 0..5 */
+    double box_params[96];           /* 16 params per face */
+    double box_length[3];
     void *stream;                    /* hipStream_t; NULL = null stream */
 } vlbm3d_desc;
 

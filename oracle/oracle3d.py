@@ -51,6 +51,8 @@ class _State(C.Structure):
         ("kind_by_id", _dp), ("comp_kind_by_id", _dp), ("mode_by_id", _dp), ("params_by_id", _dp),
         ("dyn_enabled", _dp), ("dyn_traction", _dp),
         ("lkind", _dp), ("lcomp_kind", _dp), ("lbv", _dp), ("lFb", _dp), ("lPb", _dp),
+        ("box", C.c_int32), ("box_n", C.c_int32 * 3), ("side_kind", C.c_int32 * 6), ("side_mode", C.c_int32 * 6),
+        ("side_params", C.c_double * 96), ("box_length", C.c_double * 3),
     ]
 
 
@@ -293,3 +295,88 @@ class OracleSolver3D:
     def rms_v(self):
         self.refresh_current()
         return float(np.sqrt(np.mean(np.sum(self.U[:3] ** 2, axis=0))))
+
+
+class OracleBox3D(OracleSolver3D):
+    """CPU FP64 oracle of the rectangular-box solver
+    (vector_nonlinear_elastic_warp3d.py:2305-2744): every node active, no
+    cut links, periodic axes wrapped in the neighbour table, half-way faces
+    (_apply_boundary_side_kernel, :1999-2200).  One lattice speed."""
+
+    def __init__(self, nx, ny, nz, *, material, boundaries=None, length_x=1.0, length_y=1.0, length_z=1.0,
+                 lattice_speed=5.0, collision_omega=1.8, jacobian_floor=1e-12, source_mode="none",
+                 source_vector=(0.0,) * 12, damping_gamma=0.0, threads: int | None = None):
+        from curved_lbm.three_d import box_solver3d as bx
+        from curved_lbm.three_d import curved_boundary_warp3d as cb
+
+        ei, ef = np.zeros(0, dtype=np.int32), np.zeros(0)
+        g = cb.CurvedBoundaryGeometry3D(
+            nx=int(nx), ny=int(ny), nz=int(nz), length_x=float(length_x), length_y=float(length_y),
+            length_z=float(length_z), active=np.ones((int(nx), int(ny), int(nz)), dtype=bool), link_i=ei,
+            link_j=ei, link_k=ei, link_q=ei, link_boundary_id=ei, link_eta=ef, link_xb=ef, link_yb=ef, link_zb=ef,
+            link_nx=ef, link_ny=ef, link_nz=ef, label="box")
+        self._bx, self._cb = bx, cb
+        self.boundaries = boundaries or bx.BoxBoundarySet.periodic()
+        self.length = (float(length_x), float(length_y), float(length_z))
+        tables = cb._sparse_tables_from_geometry(g)
+        self._box_shape = (int(nx), int(ny), int(nz))
+        tables["neighbor"] = self._neighbor(tables, self.boundaries)
+        super().__init__(g, material=material, boundary=cb.CurvedBoundarySpec3D(kind="neumann"),
+                         lattice_speed=lattice_speed, collision_omega=collision_omega, boundary_reconstruction="bfl",
+                         jacobian_floor=jacobian_floor, source_mode=source_mode, source_vector=source_vector,
+                         damping_gamma=damping_gamma, threads=threads, tables=tables)
+        self._set_faces(self.boundaries)
+        self.initialize_identity()
+
+    def _neighbor(self, tables, b):
+        return self._cb.periodic_neighbor_table(tables, self._box_shape, [b.periodic_x, b.periodic_y, b.periodic_z])
+
+    def _set_faces(self, b) -> None:
+        s = self._s
+        s.box = 1
+        for a in range(3):
+            s.box_n[a] = self._box_shape[a]
+            s.box_length[a] = self.length[a]
+        for k in range(6):
+            s.side_kind[k] = int(b.side_kind[k])
+            s.side_mode[k] = int(b.value_mode[k])
+        for k, v in enumerate(np.asarray(b.params, dtype=np.float64).reshape(96)):
+            s.side_params[k] = float(v)
+
+    def set_boundaries(self, boundaries) -> None:
+        boundaries.validate_periodic_pairs()
+        self.boundaries = boundaries
+        self.neighbor[...] = self._neighbor(self.tables, boundaries)
+        self._set_faces(boundaries)
+
+    def initialize_from_numpy(self, *, U, displacement=None, init_order=1, enforce_dirichlet_tangent=True,
+                              apply_initial_boundaries=True, initial_boundary_time=0.0):
+        if not hasattr(self, "_s") or not getattr(self._s, "box", 0):
+            return super().initialize_from_numpy(U=U, displacement=displacement, apply_initial_boundaries=False)
+        from curved_lbm.three_d import vector_nonlinear_elastic_warp3d as ve3
+
+        shape = self._box_shape
+        n = self.n_active
+        U = np.asarray(U, dtype=np.float64).reshape((12,) + shape)
+        disp = np.zeros((3,) + shape) if displacement is None else np.asarray(displacement, np.float64).reshape(
+            (3,) + shape)
+        if enforce_dirichlet_tangent:
+            U = self._bx._enforce_dirichlet_tangent_columns(U, disp, self.boundaries, dx=self.dx, dy=self.dy,
+                                                            dz=self.dz)
+        super().initialize_from_numpy(U=U.reshape(12, n), displacement=disp.reshape(3, n),
+                                      apply_initial_boundaries=False)
+        if init_order == 2:
+            b = self.boundaries
+            f = ve3._second_order_populations_np(
+                U, self.material, self.lattice_speed, dx=self.dx, dy=self.dy, dz=self.dz, dt=self.dt,
+                periodic_x=b.periodic_x, periodic_y=b.periodic_y, periodic_z=b.periodic_z,
+                source_mode=self.source_mode, source_params=self.source_params)
+            self.f0[...] = f.reshape(72, n)
+            self.refresh_current()
+        elif init_order != 1:
+            raise ValueError("init_order must be 1 or 2")
+        if apply_initial_boundaries:
+            self.apply_boundary_reconstruction(float(initial_boundary_time), in_place=True)
+            self.refresh_current()
+            self._lib.or3d_set_ustar(C.byref(self._s), _p(np.ascontiguousarray(disp.reshape(3, n))))
+            self.refresh_current()

@@ -1038,7 +1038,137 @@ static void apply_characteristic(or3d_state *s, double *f1) {
 }
 
 /* cbw3d.py:4339-4399 (apply_boundary_reconstruction) */
+/* vnew3d:1393-1452 (_boundary_value_comp): face datum of the box. */
+static double box_value(int comp, int mode, const double *p, double x, double y, double z, double t) {
+    if (mode == 1) return p[comp];
+    if (mode == 2) {
+        double scale = 1.0;
+        if (p[3] > 0.0 && t < p[3]) scale = sin(PI_D * t / p[3]) * sin(PI_D * t / p[3]);
+        return p[comp] * scale;
+    }
+    if (mode == 3) {
+        double scale = 1.0, denom = p[4];
+        if (p[3] > 0.0 && t < p[3]) {
+            if (denom <= 0.0) denom = 2.0 * p[3];
+            scale = sin(PI_D * t / denom) * sin(PI_D * t / denom);
+        }
+        return p[comp] * scale;
+    }
+    if (mode == 4) {
+        if (p[12] > 0.0 && t > p[12]) return 0.0;
+        return p[3 * comp] * x + p[3 * comp + 1] * y + p[3 * comp + 2] * z + p[9 + comp];
+    }
+    if (mode == 5) return (comp == 0 && t < p[1]) ? p[0] * sin(PI_D * t / p[1]) : 0.0;
+    return 0.0;
+}
+
+/* vnew3d:1455-1766 (_normal_residual_from_col, _det_from_col,
+ * _solve_normal_column): face F of an axis-aligned Neumann face. */
+static void col_F(const double *Fin, int axis, const double *x, double *G) {
+    for (int k = 0; k < 9; ++k) G[k] = Fin[k];
+    for (int r = 0; r < 3; ++r) G[3 * r + axis] = x[r];
+}
+static void col_residual(const or3d_state *s, const double *Fin, int axis, double sign, const double *T,
+                         const double *x, double *r) {
+    double G[9], P[9];
+    col_F(Fin, axis, x, G);
+    or3d_piola(G, s->model, s->lam, s->mu, s->p0, s->p1, s->p2, s->j_floor, P);
+    for (int row = 0; row < 3; ++row) r[row] = sign * P[3 * row + axis] - T[row];
+}
+static void solve_normal_column(const or3d_state *s, const double *Fin, int axis, double sign, const double *T,
+                                double *Fout) {
+    double x[3] = {Fin[axis], Fin[3 + axis], Fin[6 + axis]};
+    const double eps = 1.0e-6;
+    for (int it = 0; it < 16; ++it) {
+        double r[3], rp[3][3];
+        col_residual(s, Fin, axis, sign, T, x, r);
+        if (sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) < 1.0e-10) break;
+        for (int c = 0; c < 3; ++c) {
+            double xc[3] = {x[0], x[1], x[2]};
+            xc[c] = x[c] + eps;
+            col_residual(s, Fin, axis, sign, T, xc, rp[c]);
+        }
+        double a = (rp[0][0] - r[0]) / eps, b = (rp[1][0] - r[0]) / eps, c = (rp[2][0] - r[0]) / eps;
+        double d = (rp[0][1] - r[1]) / eps, e = (rp[1][1] - r[1]) / eps, f = (rp[2][1] - r[1]) / eps;
+        double g = (rp[0][2] - r[2]) / eps, h = (rp[1][2] - r[2]) / eps, ii = (rp[2][2] - r[2]) / eps;
+        double det = a * (e * ii - f * h) - b * (d * ii - f * g) + c * (d * h - e * g);
+        if (fabs(det) < 1.0e-14) break;
+        double i00 = (e * ii - f * h) / det, i01 = (c * h - b * ii) / det, i02 = (b * f - c * e) / det;
+        double i10 = (f * g - d * ii) / det, i11 = (a * ii - c * g) / det, i12 = (c * d - a * f) / det;
+        double i20 = (d * h - e * g) / det, i21 = (b * g - a * h) / det, i22 = (a * e - b * d) / det;
+        double d0 = -(i00 * r[0] + i01 * r[1] + i02 * r[2]);
+        double d1 = -(i10 * r[0] + i11 * r[1] + i12 * r[2]);
+        double d2 = -(i20 * r[0] + i21 * r[1] + i22 * r[2]);
+        double step = 1.0;
+        for (int ls = 0; ls < 12; ++ls) {
+            double cand[3] = {x[0] + step * d0, x[1] + step * d1, x[2] + step * d2}, G[9];
+            col_F(Fin, axis, cand, G);
+            if (det3(G) > s->j_floor) {
+                x[0] = cand[0];
+                x[1] = cand[1];
+                x[2] = cand[2];
+                break;
+            }
+            step = 0.5 * step;
+        }
+    }
+    col_F(Fin, axis, x, Fout);
+}
+
+/* vnew3d:1999-2200 (_apply_boundary_side_kernel), faces in order left,
+ * right, bottom, top, front, back. */
+static void apply_sides(or3d_state *s, const double *fpost, double *f1, double tb) {
+    const int nx = s->box_n[0], ny = s->box_n[1], nz = s->box_n[2], n = s->n_active;
+    const double c = s->c;
+    for (int side = 0; side < 6; ++side) {
+        const int kind = s->side_kind[side];
+        if (kind == 0) continue;
+        const int axis = side / 2, hi = side & 1;
+        const int nn = axis == 0 ? ny * nz : (axis == 1 ? nx * nz : nx * ny);
+        const int q_out = hi ? axis : axis + 3, q_in = hi ? axis + 3 : axis;
+        const double outward = hi ? 1.0 : -1.0;
+        const double *p = s->side_params + 16 * side;
+        for (int t = 0; t < nn; ++t) {
+            int i, j, k;
+            if (axis == 0) { i = hi ? nx - 1 : 0; j = t % ny; k = t / ny; }
+            else if (axis == 1) { j = hi ? ny - 1 : 0; i = t % nx; k = t / nx; }
+            else { k = hi ? nz - 1 : 0; i = t % nx; j = t / nx; }
+            const int node = (i * ny + j) * nz + k;
+            double xb[3] = {(i + 0.5) * s->dx, (j + 0.5) * s->dy, (k + 0.5) * s->dz};
+            xb[axis] = hi ? s->box_length[axis] : 0.0;
+            double v[3];
+            for (int cc = 0; cc < 3; ++cc)
+                v[cc] = box_value(cc, s->side_mode[side], p, xb[0], xb[1], xb[2], tb);
+            double fo[12];
+            for (int a = 0; a < 12; ++a) fo[a] = fpost[IDX(q_out * 12 + a, node, n)];
+            double val[12];
+            if (kind == BC_DIR) {
+                for (int cc = 0; cc < 3; ++cc) val[cc] = -fo[cc] + v[cc] / 3.0;
+                for (int r = 0; r < 3; ++r)
+                    for (int cc = 0; cc < 3; ++cc)
+                        val[3 + 3 * r + cc] = fo[3 + 3 * r + cc] + (cc == axis ? outward : 0.0) * v[r] / c;
+            } else {
+                double Fn[9], Fb[9];
+                for (int a = 0; a < 9; ++a) Fn[a] = s->U[IDX(3 + a, node, n)];
+                solve_normal_column(s, Fn, axis, outward, v, Fb);
+                for (int cc = 0; cc < 3; ++cc) val[cc] = fo[cc] + v[cc] / c;
+                for (int a = 0; a < 9; ++a) val[3 + a] = -fo[3 + a] + Fb[a] / 3.0;
+            }
+            for (int a = 0; a < 12; ++a) f1[IDX(q_in * 12 + a, node, n)] = val[a];
+        }
+    }
+}
+
 int or3d_apply_boundary(or3d_state *s, double t, int in_place) {
+    if (s->box) {
+        if (in_place) {
+            memcpy(s->f1, s->f0, sizeof(double) * FQ * (size_t)s->n_active);
+            apply_sides(s, s->f1, s->f0, t);
+        } else {
+            apply_sides(s, s->f0, s->f1, t);
+        }
+        return 0;
+    }
     if (s->n_links == 0) return 0;
     if (s->has_char) char_gather(s);
     prepare_links(s, t);
@@ -1065,7 +1195,9 @@ int or3d_step(or3d_state *s, double time, int64_t steps, int nsteps) {
         }
         if (s->compat_interval > 0 && steps % s->compat_interval == 0) repair_F(s);
         stream_push(s);
-        if (s->n_links > 0) {
+        if (s->box) {
+            apply_sides(s, s->f0, s->f1, tb);
+        } else if (s->n_links > 0) {
             if (s->has_char) char_gather(s);
             prepare_links(s, tb);
             apply_links(s, s->f0, s->f1);

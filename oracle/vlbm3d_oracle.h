@@ -61,6 +61,14 @@ typedef struct {
     int32_t *lkind;              /* (L) */
     int32_t *lcomp_kind;         /* (3, L) */
     double *lbv, *lFb, *lPb;     /* (3, L), (9, L), (9, L) */
+    /* rectangular box solver: every node of box_n active (k fastest), no cut
+     * links, periodic wrap in the neighbour table; faces left, right,
+     * bottom, top, front, back */
+    int32_t box;
+    int32_t box_n[3];
+    int32_t side_kind[6], side_mode[6];
+    double side_params[96];
+    double box_length[3];
 } or3d_state;
 
 int or3d_refresh(or3d_state *s);
