@@ -257,6 +257,12 @@ def load_library(path: Path | None = None) -> C.CDLL:
         raise RuntimeError(
             f"HIP library {p} is missing; build it with `python curved-boundary-vlbm_amd/build_native.py`"
         )
+    # torch first: its HIP runtime is then the one libvlbm binds to (same
+    # soname), so the process holds one HIP runtime and the library works on
+    # torch's device memory and streams.  Loading libvlbm before torch would
+    # pull in a second runtime that finds no device once torch owns it.
+    import torch  # noqa: F401
+
     lib = C.CDLL(str(p))
     vp = C.c_void_p
     lib.vlbm_last_error.restype = C.c_char_p

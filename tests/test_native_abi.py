@@ -103,3 +103,20 @@ def test_header_struct_fields_match_binding(header, tag, binding):
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(RuntimeError):
         _native.load_library(tmp_path / "libvlbm.so")
+
+
+def test_library_binds_to_the_hip_runtime_torch_loads():
+    """load_library() imports torch before dlopen, so libvlbm resolves its
+    HIP runtime to the one torch has loaded (a second runtime loaded first
+    finds no device once torch owns it).  Checked in a fresh interpreter."""
+    import subprocess
+    import sys
+
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from curved_lbm import _native\n"
+            "assert 'torch' not in sys.modules\n"
+            "_native.load_library()\n"
+            "assert 'torch' in sys.modules\n"
+            "libs = {l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l}\n"
+            "assert len(libs) == 1, libs\n" % str(_native.Path(_native.__file__).resolve().parents[1]))
+    subprocess.run([sys.executable, "-c", code], check=True, timeout=300)
