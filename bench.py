@@ -13,6 +13,12 @@ per step).  Synthetic: starts from rest.
 outer nominal traction 0.04 ramped linearly over 5000 steps).
 ``--workload annulus``: the D2Q4x6 annulus on 96^2 (Ri 0.2 L clamped, Ro
 0.45 L, traction 0.05 ramped over 1000 steps).  Same settings.
+``--workload batch``: --batch-count superellipse shells drawn from seed 0
+(exponent U[2,6], a U[0.30,0.45] L, b/a U[0.5,1], rotation U[0,180) deg,
+inner circle 0.1 L clamped, traction U[0.01,0.05] ramped over 5000 steps,
+side from {128,192,256,384,512}), FP32 populations by default, every shell
+on its own HIP stream and step graph; reports the aggregate MLUPS of the
+concurrent batch and of the same shells stepped one after another.
 
 Prints one JSON line: MLUPS (active-node updates per second) of the whole
 step with the populations resident in HBM, the roofline of the dominant
@@ -48,7 +54,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", choices=("shell", "superellipse", "annulus"), default="shell")
+    ap.add_argument("--workload", choices=("shell", "superellipse", "annulus", "batch"), default="shell")
+    ap.add_argument("--batch-count", type=int, default=256, help="shells of the batch workload")
+    ap.add_argument("--graph-steps", type=int, default=50, help="captured steps per graph (batch workload)")
     ap.add_argument("--n", type=int, default=0, help="lattice side (default 128 / 1024 / 96 by workload)")
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
     ap.add_argument("--cpu-steps", type=int, default=40, help="oracle steps timed for the CPU baseline")
@@ -69,6 +77,8 @@ def main() -> int:
 
     import torch
 
+    if args.workload == "batch":
+        return batch(args)
     if args.n <= 0:
         args.n = {"shell": 128, "superellipse": 1024, "annulus": 96}[args.workload]
     if args.equilibrium:
@@ -252,6 +262,99 @@ def make_case(args) -> dict:
         "solver": lambda prec: cb2.WarpCurvedBoundaryLBM2D(g, **kw, precision=prec), "oracle": oracle2,
         "ramp_steps": ramp, "short": f"{args.n}^2 {args.workload}", "description": desc,
     }
+
+
+def batch(args) -> int:
+    """Aggregate throughput of a batch of uneven 2D shells stepped
+    concurrently (one stream and step graph per shell), beside the same
+    shells stepped one after another.  ``value`` counts the node updates of
+    the shells that stayed finite only (a shell that diverges still costs
+    its kernel time).  With --equilibrium: every shell run to the
+    displacement-rate residual --tol after its load ramp, per-shell steps."""
+    import torch
+
+    from curved_lbm.two_d import batch as bt
+
+    prec = args.precision if "--precision" in sys.argv else "fp32"
+    t0 = time.time()
+    inst = bt.batch_instances(args.batch_count, seed=0)
+    b = bt.BatchedCurvedBoundaryLBM2D.build([bt.shell_solver_inputs(i) for i in inst], precision=prec,
+                                            graph_steps=args.graph_steps)
+    setup_s = time.time() - t0
+    n_act = b.n_active
+    print(f"[bench] batch setup {setup_s:.1f}s shells={len(inst)} n_active={n_act}", file=sys.stderr, flush=True)
+    workload = (f"{len(inst)} superellipse shells, seed 0, sides {sorted({i.n for i in inst})}, {prec} populations, "
+                f"one stream and {args.graph_steps}-step graph per shell")
+    if args.equilibrium:
+        return batch_equilibrium(args, b, inst, prec, workload, setup_s)
+    steps = args.steps if "--steps" in sys.argv else 200
+    warm = args.warmup if "--warmup" in sys.argv else 2 * args.graph_steps
+    b.run(warm)
+    t1 = time.perf_counter()
+    b.run(steps)
+    conc = time.perf_counter() - t1
+    # the same shells one after another (each waits for the previous)
+    t2 = time.perf_counter()
+    for s in b.solvers:
+        s.run(steps)
+        torch.cuda.synchronize()
+    serial = time.perf_counter() - t2
+    bad = [k for k, s in enumerate(b.solvers) if s._refreshed_reductions()["nonfinite"]]
+    if bad:
+        print(f"[bench] non-finite shells {bad}", file=sys.stderr, flush=True)
+    n_ok = n_act - sum(b.solvers[k].n_active for k in bad)
+    mlups = n_ok * steps / conc / 1e6
+    out = {
+        "metric": "aggregate MLUPS of a batch of uneven D2Q4x6 shells (one GPU)", "value": round(mlups, 2),
+        "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": round(conc / steps * 1e3, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if prec == "fp32" else "f64", "data": "synthetic",
+        "config": {"workload": workload, "shells": len(inst), "n_active": n_act, "setup_s": round(setup_s, 1),
+                   "all_shells_mlups": round(n_act * steps / conc / 1e6, 2),
+                   "serial_mlups": round(n_ok * steps / serial / 1e6, 2),
+                   "concurrency_speedup": round(serial / conc, 3), "nonfinite_shells": bad},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def batch_equilibrium(args, b, inst, prec, workload, setup_s) -> int:
+    """Every shell of the batch to the displacement-rate residual after its
+    load ramp; per-shell steps, aggregate MLUPS over the whole run."""
+    import numpy as np
+
+    last = [time.perf_counter()]
+
+    def progress(live, steps):
+        now = time.perf_counter()
+        if now - last[0] > 20.0:
+            print(f"[bench] batch equilibrium: {live} shells stepping, {steps} steps", file=sys.stderr, flush=True)
+            last[0] = now
+
+    t0 = time.perf_counter()
+    res = b.run_to_equilibrium(args.tol, check_every=args.check_every, max_steps=args.max_steps,
+                               min_steps=[i.ramp_steps for i in inst], on_check=progress)
+    wall = time.perf_counter() - t0
+    updates = sum(r["steps"] * s.n_active for r, s in zip(res, b.solvers))
+    conv = [k for k, r in enumerate(res) if r["converged"]]
+    div = [k for k, r in enumerate(res) if r["diverged"]]
+    steps = np.array([res[k]["steps"] for k in conv])
+    by_side = {}
+    for k in conv:
+        by_side.setdefault(inst[k].n, []).append(res[k]["steps"])
+    out = {
+        "metric": "batch of uneven D2Q4x6 shells to displacement-rate residual %.0e (one GPU)" % args.tol,
+        "wall_s": round(wall, 2), "aggregate_mlups": round(updates / wall / 1e6, 2), "node_updates": updates,
+        "converged": len(conv), "diverged_shells": div,
+        "not_converged_shells": [k for k, r in enumerate(res) if not r["converged"] and not r["diverged"]],
+        "steps_min_median_max": ([int(steps.min()), int(np.median(steps)), int(steps.max())] if conv else None),
+        "median_steps_by_side": {str(n): int(np.median(v)) for n, v in sorted(by_side.items())},
+        "steps_per_shell": [r["steps"] for r in res],
+        "config": {"workload": workload, "shells": len(inst), "n_active": b.n_active, "setup_s": round(setup_s, 1),
+                   "check_every": args.check_every, "max_steps": args.max_steps, "min_steps": "load ramp"},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def equilibrium(args) -> int:

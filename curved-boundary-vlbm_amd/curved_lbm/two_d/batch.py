@@ -1,0 +1,178 @@
+"""Many independent D2Q4x6 shells stepped concurrently on one GPU.
+
+The batched uneven workload: superellipse shells of different lattice
+sizes, each a ``WarpCurvedBoundaryLBM2D`` with its own HIP stream (and, for
+uniform schedules, its own captured step graph), advanced together so that
+the small per-shell kernels overlap on the device instead of running one
+shell after another.  Results are bitwise those of stepping every shell on
+its own.  There is no reference counterpart (the reference builds one
+solver object per geometry and steps it alone).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+import math
+
+import numpy as np
+
+from . import curved_boundary_warp as cbw
+from . import shapes
+
+SIDES = (128, 192, 256, 384, 512)
+
+
+@dataclass(frozen=True)
+class ShellInstance:
+    """One shell of the batch: lattice side n, outer superellipse (exponent
+    power, semi-axes a and b = a * b_over_a, rotation theta_deg about the
+    box centre), clamped inner circle, outer nominal traction ramped
+    linearly to `traction` over `ramp_steps` steps."""
+
+    n: int
+    power: float
+    a: float
+    b_over_a: float
+    theta_deg: float
+    traction: float
+    inner_radius: float = 0.10
+    ramp_steps: int = 5000
+
+    @property
+    def b(self) -> float:
+        return self.a * self.b_over_a
+
+
+def _extent(power: float, a: float, b: float, theta_deg: float) -> float:
+    """Largest |x - 1/2| or |y - 1/2| over the rotated superellipse boundary."""
+    t = np.linspace(0.0, 2.0 * math.pi, 2049)
+    c, s = np.cos(t), np.sin(t)
+    xi = a * np.sign(c) * np.abs(c) ** (2.0 / power)
+    eta = b * np.sign(s) * np.abs(s) ** (2.0 / power)
+    th = math.radians(theta_deg)
+    x = math.cos(th) * xi - math.sin(th) * eta
+    y = math.sin(th) * xi + math.cos(th) * eta
+    return float(max(np.max(np.abs(x)), np.max(np.abs(y))))
+
+
+def batch_instances(count: int = 256, seed: int = 0, sides=SIDES, max_extent: float = 0.47) -> list[ShellInstance]:
+    """The batch drawn from numpy's default_rng(seed), per shell in this
+    order: exponent ~ U[2, 6], a ~ U[0.30, 0.45], b/a ~ U[0.5, 1], rotation
+    ~ U[0, 180) degrees, traction ~ U[0.01, 0.05], side uniform over
+    `sides`.  A shell whose rotated outline would leave [0.03, 0.97]^2 has
+    a and b scaled down to fit (the cut links must lie inside the box)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(int(count)):
+        power = float(rng.uniform(2.0, 6.0))
+        a = float(rng.uniform(0.30, 0.45))
+        ratio = float(rng.uniform(0.5, 1.0))
+        theta = float(rng.uniform(0.0, 180.0))
+        traction = float(rng.uniform(0.01, 0.05))
+        n = int(sides[int(rng.integers(len(sides)))])
+        ext = _extent(power, a, a * ratio, theta)
+        if ext > max_extent:
+            a *= max_extent / ext
+        out.append(ShellInstance(n=n, power=power, a=a, b_over_a=ratio, theta_deg=theta, traction=traction))
+    return out
+
+
+def manuscript_settings(**over) -> dict:
+    """Traction-case solver settings: c = 10, omega = 1.8, alpha_u = 0.85,
+    theta_F = 0.5, w_D = 100, damping gamma = 10, local_f, anchors on the
+    clamped inner wall."""
+    kw = dict(lattice_speed=10.0, collision_omega=1.8, boundary_reconstruction="local_f", source_mode="damping",
+              damping_gamma=10.0, local_compatibility_interval=1, local_compatibility_blend=0.5,
+              local_compatibility_interior_only=False, local_displacement_interval=1, local_displacement_sweeps=1,
+              local_displacement_relax=0.85, local_displacement_boundary_weight=100.0,
+              local_displacement_boundary_id=cbw.BOUNDARY_ID_INNER)
+    kw.update(over)
+    return kw
+
+
+def shell_solver_inputs(inst: ShellInstance):
+    """(geometry, solver kwargs) of one shell."""
+    g = shapes.superellipse_shell_geometry(
+        inst.n, shapes.Superellipse(a=inst.a, b=inst.b, theta_deg=inst.theta_deg, p=inst.power),
+        shapes.Superellipse(a=inst.inner_radius, b=inst.inner_radius, p=2.0))
+    dt = (1.0 / inst.n) / 10.0
+    mode = cbw.CURVED_VALUE_RADIAL_TRACTION_LINEAR_RAMP
+    params = (inst.traction, inst.ramp_steps * dt)
+    spec = cbw.CurvedBoundarySpec(
+        kind="neumann", value_mode=mode, params=params,
+        id_kinds={cbw.BOUNDARY_ID_OUTER: "neumann", cbw.BOUNDARY_ID_INNER: "dirichlet"},
+        id_value_modes={cbw.BOUNDARY_ID_OUTER: mode, cbw.BOUNDARY_ID_INNER: cbw.CURVED_VALUE_ZERO},
+        id_params={cbw.BOUNDARY_ID_OUTER: params, cbw.BOUNDARY_ID_INNER: ()})
+    mat = cbw.WarpHyperelasticMaterial.from_poisson("neo_hooke", 0.2, mu=1.0)
+    return g, manuscript_settings(material=mat, boundary=spec)
+
+
+class BatchedCurvedBoundaryLBM2D:
+    """Steps a list of 2D solvers concurrently: each was built on its own
+    HIP stream, ``run(n)`` enqueues n steps of every solver before waiting
+    for any, and ``run_to_equilibrium`` keeps stepping the shells that have
+    not yet met the residual tolerance."""
+
+    def __init__(self, solvers: list):
+        if len({id(s._stream) for s in solvers}) != len(solvers):
+            raise ValueError("every solver of a batch needs its own stream")
+        self.solvers = list(solvers)
+
+    @classmethod
+    def build(cls, inputs, *, precision: str = "fp32", graph_steps: int = 0, device: str = "cuda:0"):
+        """One solver per (geometry, kwargs), each constructed on a new
+        default-priority stream."""
+        import torch
+
+        solvers = []
+        for g, kw in inputs:
+            with torch.cuda.stream(torch.cuda.Stream(device=device)):
+                solvers.append(cbw.WarpCurvedBoundaryLBM2D(g, **kw, precision=precision, graph_steps=graph_steps,
+                                                           device=device))
+        return cls(solvers)
+
+    @property
+    def n_active(self) -> int:
+        return int(sum(s.n_active for s in self.solvers))
+
+    def synchronize(self) -> None:
+        import torch
+
+        torch.cuda.synchronize(self.solvers[0].device)
+
+    def run(self, nsteps: int) -> None:
+        for s in self.solvers:
+            s.run(nsteps)
+        self.synchronize()
+
+    def run_to_equilibrium(self, tol: float, *, check_every: int = 200, max_steps: int = 10**6,
+                           min_steps=0, on_check=None) -> list[dict]:
+        """Per shell: steps until the displacement-rate residual (rms of
+        |u(t) - u(t - T)| / T, T = check_every dt) is <= tol once the shell
+        has run min_steps (an int or one per shell, e.g. its load ramp);
+        returns each shell's converged flag, steps and residual.  A shell
+        that turns non-finite or inverts (min J <= 0) stops there with
+        ``diverged`` set.  ``on_check(live, steps)`` is called after every
+        round of checks with the number of shells still stepping."""
+        mins = [int(min_steps)] * len(self.solvers) if np.isscalar(min_steps) else [int(m) for m in min_steps]
+        prev = [s.u.tensor.clone() for s in self.solvers]
+        done = [None] * len(self.solvers)
+        while any(d is None for d in done):
+            live = [k for k, d in enumerate(done) if d is None]
+            for k in live:
+                self.solvers[k].run(check_every)
+            for k in live:
+                s = self.solvers[k]
+                red = s.reductions(prev[k])
+                prev[k].copy_(s.u.tensor)
+                rate = red["rms_du"] / (check_every * s.dt)
+                ramp_done = s.steps >= mins[k]
+                if red["nonfinite"] or red["min_J"] <= 0.0:
+                    done[k] = {"converged": False, "diverged": True, "steps": s.steps, "residual": rate}
+                elif ramp_done and rate <= tol:
+                    done[k] = {"converged": True, "diverged": False, "steps": s.steps, "residual": rate}
+                elif s.steps >= max_steps:
+                    done[k] = {"converged": False, "diverged": False, "steps": s.steps, "residual": rate}
+            if on_check is not None:
+                on_check(sum(d is None for d in done), max(self.solvers[k].steps for k in live))
+        return done
