@@ -39,3 +39,54 @@ def test_gpu_concurrent_batch_equals_shells_stepped_alone(graph_steps):
     assert b.n_active == sum(s.n_active for s in alone)
     res = b.run_to_equilibrium(1e30, check_every=20, min_steps=60)
     assert all(r["converged"] and r["steps"] >= 60 for r in res)
+
+
+class _Tensor:
+    def __init__(self, x):
+        self.x = np.array(x, dtype=float)
+
+    def clone(self):
+        return _Tensor(self.x.copy())
+
+    def copy_(self, other):
+        self.x[...] = other.x
+
+
+class _StubSolver:
+    """Host stand-in with the attributes the batch reads: a displacement
+    that moves by rate * dt per step, optionally turning non-finite."""
+
+    def __init__(self, rate, dt=1e-3, blow_at=None):
+        self._stream = object()
+        self.u = type("Field", (), {})()
+        self.u.tensor = _Tensor([0.0])
+        self.steps, self.dt, self.rate, self.blow_at = 0, dt, rate, blow_at
+        self.n_active = 10
+
+    def run(self, k):
+        for _ in range(k):
+            self.steps += 1
+            self.u.tensor.x += self.rate(self.steps) * self.dt
+
+    def reductions(self, prev):
+        bad = self.blow_at is not None and self.steps >= self.blow_at
+        return {"rms_du": float(abs(self.u.tensor.x[0] - prev.x[0])), "nonfinite": int(bad), "min_J": 1.0}
+
+
+def test_batch_equilibrium_bookkeeping_on_the_host():
+    decay = _StubSolver(lambda s: 10.0 * 0.5 ** (s / 10))
+    steady = _StubSolver(lambda s: 1.0)
+    blows = _StubSolver(lambda s: 1.0, blow_at=30)
+    b = bt.BatchedCurvedBoundaryLBM2D([decay, steady, blows])
+    seen = []
+    res = b.run_to_equilibrium(1e-3, check_every=20, max_steps=400, min_steps=[100, 0, 0],
+                               on_check=lambda live, steps: seen.append(live))
+    assert res[0]["converged"] and not res[0]["diverged"] and res[0]["steps"] >= 100
+    assert res[0]["residual"] <= 1e-3
+    assert not res[1]["converged"] and not res[1]["diverged"] and res[1]["steps"] == 400
+    assert res[2]["diverged"] and not res[2]["converged"] and res[2]["steps"] == 40
+    # stepping stops for a shell once it is decided
+    assert blows.steps == 40 and steady.steps == 400
+    assert seen[0] == 3 and seen[-1] == 0 and seen == sorted(seen, reverse=True)
+    with pytest.raises(ValueError):
+        bt.BatchedCurvedBoundaryLBM2D([decay, decay])
