@@ -1,0 +1,81 @@
+"""The HIP path against the CPU FP64 oracle at the benchmark's full sizes:
+the 128^3 clamped traction shell (730 008 active nodes, D3Q6x12, local_f)
+and the 1024^2 rotated superellipse shell (442 466 active nodes, D2Q4x6,
+local_f), each from a perturbed identity state (F = I + 1e-3 N(0, 1),
+v = 1e-3 N(0, 1), numpy default_rng(0)) so that every node carries
+non-trivial arithmetic, stepped 12 times in two calls (5 + 7, so the
+call-boundary collide and the fused one are both exercised).
+
+FP64 bar: populations to 1e-12 absolute, v, F, u, u* to 1e-10 relative."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import gpu_cases  # noqa: E402
+import gpu_cases2d  # noqa: E402
+
+
+def _close(name, a, b, rel=1e-10, floor=1e-14):
+    err = np.linalg.norm(a - b)
+    assert err <= rel * np.linalg.norm(b) + floor * np.sqrt(b.size), (name, err, np.linalg.norm(b))
+
+
+def _perturbed(shape, dim, rng):
+    U = 1e-3 * rng.standard_normal(shape)
+    for k in range(dim):
+        U[dim + k * (dim + 1)] += 1.0
+    return U
+
+
+def test_gpu_shell128_matches_oracle_at_full_size():
+    from curved_lbm.three_d import curved_boundary_warp3d as cb
+    from oracle.oracle3d import OracleSolver3D
+
+    g, kw = gpu_cases.traction_shell(128, traction=0.03, ramp_steps=4000)
+    gpu = cb.WarpCurvedBoundaryLBM3D(g, **kw)
+    assert gpu.n_active == 730008
+    ora = OracleSolver3D(g, **kw, tables=gpu.tables, characteristic_stencil=gpu.characteristic_stencil)
+    U = _perturbed((12, gpu.n_active), 3, np.random.default_rng(0))
+    gpu.initialize_from_numpy(U=U)
+    ora.initialize_from_numpy(U=U)
+    for k in (5, 7):
+        gpu.run(k)
+        ora.run(k)
+    Ug = gpu.U.numpy()
+    for name, a, b in (("v", Ug[:3], ora.U[:3]), ("F", Ug[3:], ora.U[3:]), ("u", gpu.u.numpy(), ora.u),
+                       ("u_star", gpu.u_star.numpy(), ora.u_star)):
+        _close(name, a, b)
+    assert np.max(np.abs(gpu.f0.numpy() - ora.f0)) <= 1e-12
+    assert np.max(np.abs(ora.U[:3])) > 1e-4
+    assert not gpu.invalid_state()
+
+
+def test_gpu_superellipse1024_matches_oracle_at_full_size():
+    from curved_lbm.two_d import curved_boundary_warp as cb2
+    from curved_lbm.two_d import shapes
+    from oracle.oracle2d import OracleSolver2D
+
+    n = 1024
+    g = shapes.rotated_superellipse_shell(n)
+    kw = gpu_cases2d.settings()
+    kw.update(material=gpu_cases2d.material(), boundary=gpu_cases2d.annulus_spec(params=(0.04, 5000 * 0.1 / n)))
+    gpu = cb2.WarpCurvedBoundaryLBM2D(g, **kw)
+    assert gpu.n_active == 442466
+    ora = OracleSolver2D(g, **kw)
+    U = _perturbed((6, n, n), 2, np.random.default_rng(0))
+    gpu.initialize_from_numpy(U=U)
+    ora.initialize_from_numpy(U=U)
+    for k in (5, 7):
+        gpu.run(k)
+        ora.run(k)
+    act = gpu.active_mask()
+    for name, a, b in (("U", gpu.system_state(), ora.U), ("u", gpu.displacement(), ora.u),
+                       ("u_star", gpu.numpy_field("u_star"), ora.u_star)):
+        _close(name, a[:, act], b[:, act])
+    assert np.max(np.abs(gpu.numpy_field("f0") - ora.f0)) <= 1e-12
+    assert np.max(np.abs(ora.U[:2][:, act])) > 1e-4
+    assert not gpu.invalid_state()
