@@ -1,7 +1,8 @@
 """The HIP path against the CPU FP64 oracle at the benchmark's full sizes:
-the 128^3 clamped traction shell (730 008 active nodes, D3Q6x12, local_f)
-and the 1024^2 rotated superellipse shell (442 466 active nodes, D2Q4x6,
-local_f), each from a perturbed identity state (F = I + 1e-3 N(0, 1),
+the 128^3 clamped traction shell (730 008 active nodes, D3Q6x12, local_f),
+the 1024^2 rotated superellipse shell (442 466 active nodes, D2Q4x6,
+local_f) and the 192 x 96 x 96 x-axis tube (665 856 active nodes), each
+from a perturbed identity state (F = I + 1e-3 N(0, 1),
 v = 1e-3 N(0, 1), numpy default_rng(0)) so that every node carries
 non-trivial arithmetic, stepped 12 times in two calls (5 + 7, so the
 call-boundary collide and the fused one are both exercised).
@@ -78,4 +79,29 @@ def test_gpu_superellipse1024_matches_oracle_at_full_size():
         _close(name, a[:, act], b[:, act])
     assert np.max(np.abs(gpu.numpy_field("f0") - ora.f0)) <= 1e-12
     assert np.max(np.abs(ora.U[:2][:, act])) > 1e-4
+    assert not gpu.invalid_state()
+
+
+def test_gpu_x_tube_192x96x96_matches_oracle_at_full_size():
+    """The 192 x 96 x 96 tube along x (665 856 active nodes), local_f on the
+    traction-free walls, clamped end and exact stretch + twist end velocity
+    (mode 210), perturbed identity start, 12 steps."""
+    from curved_lbm.three_d import curved_boundary_warp3d as cb
+    from oracle.oracle3d import OracleSolver3D
+
+    g, kw = gpu_cases.tube_case("x", 96, ramp_steps=200)
+    gpu = cb.WarpCurvedBoundaryLBM3D(g, **kw)
+    assert gpu.n_active == 665856
+    ora = OracleSolver3D(g, **kw, tables=gpu.tables, characteristic_stencil=gpu.characteristic_stencil)
+    U = _perturbed((12, gpu.n_active), 3, np.random.default_rng(0))
+    gpu.initialize_from_numpy(U=U)
+    ora.initialize_from_numpy(U=U)
+    for k in (5, 7):
+        gpu.run(k)
+        ora.run(k)
+    Ug = gpu.U.numpy()
+    for name, a, b in (("v", Ug[:3], ora.U[:3]), ("F", Ug[3:], ora.U[3:]), ("u", gpu.u.numpy(), ora.u),
+                       ("u_star", gpu.u_star.numpy(), ora.u_star)):
+        _close(name, a, b)
+    assert np.max(np.abs(gpu.f0.numpy() - ora.f0)) <= 1e-12
     assert not gpu.invalid_state()
