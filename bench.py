@@ -13,6 +13,10 @@ per step).  Synthetic: starts from rest.
 outer nominal traction 0.04 ramped linearly over 5000 steps).
 ``--workload annulus``: the D2Q4x6 annulus on 96^2 (Ri 0.2 L clamped, Ro
 0.45 L, traction 0.05 ramped over 1000 steps).  Same settings.
+``--workload tube``: the D3Q6x12 tube along x on (2n) x n x n, n = 96 by
+default (length 2, outer / inner radius 0.4 / 0.2 of the unit cross-section,
+traction-free walls, one end clamped, the other moved by the exact axial
+stretch --stretch plus twist --twist-deg, ramped over 2000 steps), local_f.
 ``--workload batch``: --batch-count superellipse shells drawn from seed 0
 (exponent U[2,6], a U[0.30,0.45] L, b/a U[0.5,1], rotation U[0,180) deg,
 inner circle 0.1 L clamped, traction U[0.01,0.05] ramped over 5000 steps,
@@ -30,6 +34,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -54,7 +59,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", choices=("shell", "superellipse", "annulus", "batch"), default="shell")
+    ap.add_argument("--workload", choices=("shell", "superellipse", "annulus", "tube", "batch"), default="shell")
+    ap.add_argument("--stretch", type=float, default=0.1, help="tube workload: axial stretch lambda - 1")
+    ap.add_argument("--twist-deg", type=float, default=15.0, help="tube workload: end twist in degrees")
     ap.add_argument("--batch-count", type=int, default=256, help="shells of the batch workload")
     ap.add_argument("--graph-steps", type=int, default=50, help="captured steps per graph (batch workload)")
     ap.add_argument("--n", type=int, default=0, help="lattice side (default 128 / 1024 / 96 by workload)")
@@ -80,7 +87,7 @@ def main() -> int:
     if args.workload == "batch":
         return batch(args)
     if args.n <= 0:
-        args.n = {"shell": 128, "superellipse": 1024, "annulus": 96}[args.workload]
+        args.n = {"shell": 128, "superellipse": 1024, "annulus": 96, "tube": 96}[args.workload]
     if args.equilibrium:
         return equilibrium(args)
 
@@ -119,7 +126,7 @@ def main() -> int:
     pop_bytes = 8 if args.precision == "fp64" else 4
     Qn, dim = case["fq"], case["dim"]
     mlups = n * args.steps / elapsed / 1e6
-    box_mlups = args.n ** dim * args.steps / elapsed / 1e6
+    box_mlups = float(math.prod(case["lattice"])) * args.steps / elapsed / 1e6
     kernels = {k: v for k, v in kt.items() if k != "step" and v["count"]}
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
     avg_ms = {k: v["ms"] / v["count"] for k, v in kernels.items()}
@@ -184,7 +191,7 @@ def main() -> int:
         }
 
     out = {
-        "metric": METRIC if args.workload == "shell" else f"MLUPS at D2Q4x6 {args.n}^2 {args.workload}",
+        "metric": METRIC if args.workload == "shell" else case["metric"],
         "value": round(mlups, 2),
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -198,7 +205,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": f"{case['description']}, {args.precision} populations",
-            "lattice": [args.n] * dim,
+            "lattice": case["lattice"],
             "n_active": n,
             "n_links": int(g.n_links),
             "precision": args.precision,
@@ -230,9 +237,35 @@ def make_case(args) -> dict:
         return {
             "geometry": g, "kwargs": kw, "fq": 72, "dim": 3, "stream_kernel": "k_stream",
             "solver": lambda prec: cb.WarpCurvedBoundaryLBM3D(g, **kw, precision=prec), "oracle": oracle,
-            "ramp_steps": 4000, "short": f"{args.n}^3 shell",
+            "ramp_steps": 4000, "short": f"{args.n}^3 shell", "lattice": [args.n] * 3, "metric": METRIC,
             "description": f"D3Q6x12 spherical shell {args.n}^3, Ri 0.2L clamped, Ro 0.45L nominal traction 0.03 "
                            "ramped linearly over 4000 steps, manuscript settings",
+        }
+    if args.workload == "tube":
+        from curved_lbm.three_d import curved_boundary_warp3d as cb
+        import gpu_cases
+
+        ramp = 2000
+        g, kw = gpu_cases.tube_case("x", args.n, stretch=args.stretch, twist=math.radians(args.twist_deg),
+                                    ramp_steps=ramp)
+
+        def oracle_t(s, threads):
+            from oracle.oracle3d import OracleSolver3D, load
+
+            load()
+            return OracleSolver3D(g, **kw, threads=threads, tables=s.tables,
+                                  characteristic_stencil=s.characteristic_stencil)
+
+        shape = [2 * args.n, args.n, args.n]
+        return {
+            "geometry": g, "kwargs": kw, "fq": 72, "dim": 3, "stream_kernel": "k_stream",
+            "solver": lambda prec: cb.WarpCurvedBoundaryLBM3D(g, **kw, precision=prec), "oracle": oracle_t,
+            "ramp_steps": ramp, "short": "x 2:1:1 tube", "lattice": shape,
+            "metric": f"MLUPS at D3Q6x12 {shape[0]}x{shape[1]}x{shape[2]} x-axis tube",
+            "description": f"D3Q6x12 tube along x on {shape[0]}x{shape[1]}x{shape[2]} (length 2, outer / inner radius "
+                           f"0.4 / 0.2), traction-free walls, end x = 0 clamped, end x = 2 moved by the exact stretch "
+                           f"{args.stretch} + twist {args.twist_deg} deg velocity ramped over {ramp} steps, local_f, "
+                           "manuscript settings",
         }
     from curved_lbm.two_d import curved_boundary_warp as cb2
     from curved_lbm.two_d import shapes
@@ -258,6 +291,7 @@ def make_case(args) -> dict:
         return OracleSolver2D(g, **kw)
 
     return {
+        "lattice": [args.n] * 2, "metric": f"MLUPS at D2Q4x6 {args.n}^2 {args.workload}",
         "geometry": g, "kwargs": kw, "fq": 24, "dim": 2, "stream_kernel": "k2_stream",
         "solver": lambda prec: cb2.WarpCurvedBoundaryLBM2D(g, **kw, precision=prec), "oracle": oracle2,
         "ramp_steps": ramp, "short": f"{args.n}^2 {args.workload}", "description": desc,
@@ -358,9 +392,10 @@ def batch_equilibrium(args, b, inst, prec, workload, setup_s) -> int:
 
 
 def equilibrium(args) -> int:
-    """Wall-seconds from rest to rms_v <= tol (the equilibrium residual),
-    FP64 populations vs FP32 populations, on the same shell; FP32 error in
-    u, F and J relative to the FP64 solution at equilibrium."""
+    """Wall-seconds from rest to the equilibrium residual (rms displacement
+    rate <= tol after the load ramp), FP64 populations vs FP32 populations on
+    the same workload; FP32 error in u, F and J relative to the FP64
+    solution at equilibrium."""
     import numpy as np
     import torch
 
@@ -408,7 +443,7 @@ def equilibrium(args) -> int:
         "fp64": res["fp64"], "fp32": res["fp32"],
         "speedup_fp32_vs_fp64": round(res["fp64"]["wall_s"] / res["fp32"]["wall_s"], 3),
         "fp32_error_vs_fp64": err,
-        "config": {"workload": case["description"], "lattice": [args.n] * dim, "n_active": int(g.n_active),
+        "config": {"workload": case["description"], "lattice": case["lattice"], "n_active": int(g.n_active),
                    "check_every": args.check_every, "min_time": ramp_time},
     }
     print(json.dumps(out))
