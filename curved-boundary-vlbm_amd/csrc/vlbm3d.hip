@@ -472,8 +472,11 @@ __global__ __launch_bounds__(BLOCK) void k_gather(const double *__restrict__ pac
     for (int k = 0; k < GC; ++k) val[k] = gx[k] = gy[k] = gz[k] = 0.0;
     // Samples are consumed in stencil order up to the first empty slot; the
     // indices, weights and records of GU slots are loaded before any is
-    // used, so each lane keeps GU sample loads in flight.
-    constexpr int GU = 8;
+    // used, so each lane keeps GU sample loads in flight.  Four (120 VGPRs,
+    // four waves per SIMD) rather than eight (190 VGPRs, two waves): the
+    // short stencils stop earlier and more waves hide the index -> record
+    // load chain.
+    constexpr int GU = 4;
     static_assert(NG % GU == 0, "gradient stencil length must be a multiple of the unroll");
     bool alive = true;
     for (int m0 = 0; m0 < NG && alive; m0 += GU) {
