@@ -57,15 +57,15 @@ def cpu_count() -> int:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (2000; batch 200)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (200; batch two graphs)")
     ap.add_argument("--workload", choices=("shell", "superellipse", "annulus", "tube", "batch"), default="shell")
     ap.add_argument("--stretch", type=float, default=0.1, help="tube workload: axial stretch lambda - 1")
     ap.add_argument("--twist-deg", type=float, default=15.0, help="tube workload: end twist in degrees")
     ap.add_argument("--batch-count", type=int, default=256, help="shells of the batch workload")
     ap.add_argument("--graph-steps", type=int, default=50, help="captured steps per graph (batch workload)")
     ap.add_argument("--n", type=int, default=0, help="lattice side (default 128 / 1024 / 96 by workload)")
-    ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
+    ap.add_argument("--precision", choices=("fp64", "fp32"), default=None, help="populations (fp64; batch fp32)")
     ap.add_argument("--cpu-steps", type=int, default=40, help="oracle steps timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--chunk", type=int, default=500, help="steps per library call")
@@ -77,6 +77,13 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy a MIB-sized device buffer twice (reference kernel for counter scaling)")
     args = ap.parse_args()
+    is_batch = args.workload == "batch"
+    if args.precision is None:
+        args.precision = "fp32" if is_batch else "fp64"
+    if args.steps is None:
+        args.steps = 200 if is_batch else 2000
+    if args.warmup is None:
+        args.warmup = 2 * args.graph_steps if is_batch else 200
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
               "(a single lattice or batch runs on one device).", file=sys.stderr)
@@ -309,7 +316,7 @@ def batch(args) -> int:
 
     from curved_lbm.two_d import batch as bt
 
-    prec = args.precision if "--precision" in sys.argv else "fp32"
+    prec = args.precision
     t0 = time.time()
     inst = bt.batch_instances(args.batch_count, seed=0)
     b = bt.BatchedCurvedBoundaryLBM2D.build([bt.shell_solver_inputs(i) for i in inst], precision=prec,
@@ -321,8 +328,7 @@ def batch(args) -> int:
                 f"one stream and {args.graph_steps}-step graph per shell")
     if args.equilibrium:
         return batch_equilibrium(args, b, inst, prec, workload, setup_s)
-    steps = args.steps if "--steps" in sys.argv else 200
-    warm = args.warmup if "--warmup" in sys.argv else 2 * args.graph_steps
+    steps, warm = args.steps, args.warmup
     b.run(warm)
     t1 = time.perf_counter()
     b.run(steps)
