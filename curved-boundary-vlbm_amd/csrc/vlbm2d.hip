@@ -855,6 +855,27 @@ __global__ __launch_bounds__(LBLOCK) void k2_box_sides(const T *fsrc, T *fdst, c
     for (int a = 0; a < 6; ++a) st(fdst, (size_t)(qi * 6 + a) * N + p, a, val[a]);
 }
 
+// Bit q of upm[t]: the upstream node of population q at the active node
+// list[t] lies on the lattice (after the periodic wrap) and is active.  It
+// depends on the geometry and the periodic axes only, so it is computed once
+// and after the periodicity changes, and the stream kernel loads it beside
+// the node index instead of four dependent mask loads per node.
+__global__ __launch_bounds__(BLOCK) void k2_upmask(const int32_t *__restrict__ list, const int32_t *__restrict__ act,
+                                                   uint8_t *__restrict__ upm, Params2 P) {
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= P.na) return;
+    const int p = list[t];
+    const int ny = P.ny;
+    const int i = p / ny, j = p - i * ny;
+    int bits = 0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int iu = wrap_i(i - cx(q), P.nx, P.per_x), ju = wrap_i(j - cy(q), ny, P.per_y);
+        if (iu >= 0 && iu < P.nx && ju >= 0 && ju < ny && act[iu * ny + ju] != 0) bits |= 1 << q;
+    }
+    upm[t] = (uint8_t)bits;
+}
+
 // Pull stream of the post-collision populations; entries whose upstream
 // node is inactive or off the lattice belong to cut links and were written
 // by k2_links.  With NEXT the kernel also performs the moment part of the
@@ -869,7 +890,7 @@ __global__ __launch_bounds__(LBLOCK) void k2_box_sides(const T *fsrc, T *fdst, c
 template <typename T, bool NEXT, int MODE>
 __global__ __launch_bounds__(BLOCK) void k2_stream(const T *__restrict__ f0, T *__restrict__ f1,
                                                    const T *__restrict__ rec, const int32_t *__restrict__ list,
-                                                   const int32_t *__restrict__ act, double *__restrict__ U,
+                                                   const uint8_t *__restrict__ upm, double *__restrict__ U,
                                                    double *__restrict__ u, double *__restrict__ us,
                                                    int32_t *__restrict__ invalid, const double *__restrict__ gin,
                                                    double *__restrict__ gout, Params2 P) {
@@ -877,6 +898,7 @@ __global__ __launch_bounds__(BLOCK) void k2_stream(const T *__restrict__ f0, T *
     const int t = xcd_block() * BLOCK + threadIdx.x;
     if (t >= P.na) return;
     const int p = list[t];
+    const int up = upm[t];
     const int ny = P.ny;
     const int i = p / ny, j = p - i * ny;
     const size_t N = P.N;
@@ -888,8 +910,7 @@ __global__ __launch_bounds__(BLOCK) void k2_stream(const T *__restrict__ f0, T *
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const int iu = wrap_i(i - cx(q), P.nx, P.per_x), ju = wrap_i(j - cy(q), ny, P.per_y);
-            const bool ok = iu >= 0 && iu < P.nx && ju >= 0 && ju < ny && act[iu * ny + ju] != 0;
-            if (ok) {
+            if ((up >> q) & 1) {
                 const int src = iu * ny + ju;
                 double fv[NC];
 #pragma unroll
@@ -911,7 +932,7 @@ __global__ __launch_bounds__(BLOCK) void k2_stream(const T *__restrict__ f0, T *
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const int iu = wrap_i(i - cx(q), P.nx, P.per_x), ju = wrap_i(j - cy(q), ny, P.per_y);
-            okq[q] = iu >= 0 && iu < P.nx && ju >= 0 && ju < ny && act[iu * ny + ju] != 0;
+            okq[q] = ((up >> q) & 1) != 0;
             if (okq[q]) {
                 const int src = iu * ny + ju;
 #pragma unroll
@@ -1172,6 +1193,8 @@ struct vlbm2d_handle {
     double *clock = nullptr;
     hipStream_t cap = nullptr;
     StepGraph graphs[2];
+    uint8_t *upmask = nullptr;  // (n_active) upstream-active bits, see k2_upmask
+    bool upmask_ok = false;
 };
 
 static void drop_graphs(vlbm2d_handle *h) {
@@ -1335,7 +1358,8 @@ static void launch_stream2_t(vlbm2d_handle *h, const Params2 &P, const T *f0, T 
                              double *gout) {
     const vlbm2d_desc &d = h->d;
     hipLaunchKernelGGL((k2_stream<T, NEXT, MODE>), dim3(xcd_grid(d.n_active)), dim3(BLOCK), 0, h->stream, f0, f1,
-                       (const T *)d.record, d.active_list, d.active, d.U, d.u, d.u_star, d.invalid, gin, gout, P);
+                       (const T *)d.record, d.active_list, (const uint8_t *)h->upmask, d.U, d.u, d.u_star, d.invalid,
+                       gin, gout, P);
 }
 
 // mode bit 0: mrt_ghost (gin/gout ghost moments), bit 1: kinetic filter.
@@ -1556,7 +1580,11 @@ int vlbm2d_create(const vlbm2d_desc *desc, vlbm2d_handle **out) {
         if (recon_compat(desc->recon) && !desc->lsq_wt) return fail_msg("vlbm2d_create: lsq_wt is null");
     }
     if (desc->n_anchor_links > 0 && !desc->anchor_ids) return fail_msg("vlbm2d_create: anchor_ids is null");
+    uint8_t *upmask = nullptr;
+    hipError_t ae = hipMalloc((void **)&upmask, (size_t)desc->n_active);
+    if (ae != hipSuccess) return fail("vlbm2d_create: upstream mask", ae);
     vlbm2d_handle *h = new vlbm2d_handle;
+    h->upmask = upmask;
     h->d = *desc;
     h->P = make_params2(*desc);
     h->T = make_links2(*desc);
@@ -1570,6 +1598,7 @@ int vlbm2d_destroy(vlbm2d_handle *h) {
     if (!h) return 0;
     drop_graphs(h);
     if (h->clock) (void)hipFree(h->clock);
+    if (h->upmask) (void)hipFree(h->upmask);
     if (h->cap) (void)hipStreamDestroy(h->cap);
     h->timing.destroy();
     delete h;
@@ -1599,6 +1628,13 @@ int vlbm2d_timing_read(vlbm2d_handle *h, double *ms, int32_t *counts, int32_t ns
 int vlbm2d_step(vlbm2d_handle *h, void *f0, void *f1, double time, int64_t steps, int32_t nsteps) {
     if (!h || !f0 || !f1) return fail_msg("vlbm2d_step: null argument");
     if (nsteps < 0) return fail_msg("vlbm2d_step: nsteps must be non-negative");
+    if (!h->upmask_ok) {
+        // On the handle's stream ahead of every launch (and graph) that reads it.
+        hipLaunchKernelGGL(k2_upmask, dim3((h->d.n_active + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, h->stream,
+                           h->d.active_list, h->d.active, h->upmask, h->P);
+        CHECK_LAUNCH("k2_upmask");
+        h->upmask_ok = true;
+    }
     if (h->d.precision == VLBM_PREC_F64)
         return run_steps2<double>(h, (double *)f0, (double *)f1, time, steps, nsteps);
     return run_steps2<float>(h, (float *)f0, (float *)f1, time, steps, nsteps);
@@ -1686,6 +1722,7 @@ int vlbm2d_set_box_sides(vlbm2d_handle *h, const int32_t *kind, const int32_t *m
     h->P = make_params2(h->d);
     h->P.use_dyn = use_dyn;
     h->box = make_box2(h->d);
+    h->upmask_ok = false;  // the periodic axes may have changed
     return 0;
 }
 
