@@ -193,12 +193,37 @@ __global__ __launch_bounds__(BLOCK) void k2_moments(const T *__restrict__ f, dou
     if (Um[2] * Um[5] - Um[3] * Um[4] <= P.m.jf) atomicOr(invalid, 1);
 }
 
+// Per active node, which lattice neighbours along the axes are on the
+// lattice and active: first (+-x, +-y) and second (+-2x, +-2y) neighbours,
+// without periodic wrap.  Geometry only; computed once, so the sweep and the
+// record kernels test bits instead of loading the dense mask.
+enum { NB_XP = 1, NB_XM = 2, NB_YP = 4, NB_YM = 8, NB_XP2 = 16, NB_XM2 = 32, NB_YP2 = 64, NB_YM2 = 128,
+       NB_NEAR = 15 };
+__global__ __launch_bounds__(BLOCK) void k2_nbmask(const int32_t *__restrict__ list, const int32_t *__restrict__ act,
+                                                   uint8_t *__restrict__ nbm, Params2 P) {
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= P.na) return;
+    const int p = list[t];
+    const int ny = P.ny, nx = P.nx;
+    const int i = p / ny, j = p - i * ny;
+    int b = 0;
+    if (i + 1 < nx && act[p + ny] != 0) b |= NB_XP;
+    if (i > 0 && act[p - ny] != 0) b |= NB_XM;
+    if (j + 1 < ny && act[p + 1] != 0) b |= NB_YP;
+    if (j > 0 && act[p - 1] != 0) b |= NB_YM;
+    if (i + 2 < nx && act[p + 2 * ny] != 0) b |= NB_XP2;
+    if (i > 1 && act[p - 2 * ny] != 0) b |= NB_XM2;
+    if (j + 2 < ny && act[p + 2] != 0) b |= NB_YP2;
+    if (j > 1 && act[p - 2] != 0) b |= NB_YM2;
+    nbm[t] = (uint8_t)b;
+}
+
 // One relaxed nearest-neighbour compatibility sweep of u (cbw2d:3421-3490)
 // with the Dirichlet cut-point anchors of the node's own links
 // (cbw2d:3288-3417) summed in link order, so the sums are deterministic.
 __global__ __launch_bounds__(BLOCK) void k2_sweep(const double *__restrict__ U, const double *__restrict__ u,
                                                   double *__restrict__ un, const int32_t *__restrict__ list,
-                                                  const int32_t *__restrict__ act, const int32_t *__restrict__ aoff,
+                                                  const uint8_t *__restrict__ nbm, const int32_t *__restrict__ aoff,
                                                   const int32_t *__restrict__ aids, Links2 T, double time,
                                                   Params2 P) {
     const int t = xcd_block() * BLOCK + threadIdx.x;
@@ -237,7 +262,8 @@ __global__ __launch_bounds__(BLOCK) void k2_sweep(const double *__restrict__ U, 
         ry += P.w2 * ty;
     }
     const double dx = P.dx, dy = P.dy;
-    if (i + 1 < P.nx && act[p + ny] != 0) {
+    const int nb = nbm[t];
+    if (nb & NB_XP) {
         const int q = p + ny;
         const double gx = 0.5 * dx * ((U2[p] - 1.0) + (U2[q] - 1.0));
         const double gy = 0.5 * dx * (U4[p] + U4[q]);
@@ -245,7 +271,7 @@ __global__ __launch_bounds__(BLOCK) void k2_sweep(const double *__restrict__ U, 
         ry += u1[q] - gy;
         diag += 1.0;
     }
-    if (i > 0 && act[p - ny] != 0) {
+    if (nb & NB_XM) {
         const int q = p - ny;
         const double gx = 0.5 * dx * ((U2[q] - 1.0) + (U2[p] - 1.0));
         const double gy = 0.5 * dx * (U4[q] + U4[p]);
@@ -253,7 +279,7 @@ __global__ __launch_bounds__(BLOCK) void k2_sweep(const double *__restrict__ U, 
         ry += u1[q] + gy;
         diag += 1.0;
     }
-    if (j + 1 < ny && act[p + 1] != 0) {
+    if (nb & NB_YP) {
         const int q = p + 1;
         const double gx = 0.5 * dy * (U3[p] + U3[q]);
         const double gy = 0.5 * dy * ((U5[p] - 1.0) + (U5[q] - 1.0));
@@ -261,7 +287,7 @@ __global__ __launch_bounds__(BLOCK) void k2_sweep(const double *__restrict__ U, 
         ry += u1[q] - gy;
         diag += 1.0;
     }
-    if (j > 0 && act[p - 1] != 0) {
+    if (nb & NB_YM) {
         const int q = p - 1;
         const double gx = 0.5 * dy * (U3[q] + U3[p]);
         const double gy = 0.5 * dy * ((U5[q] - 1.0) + (U5[p] - 1.0));
@@ -297,19 +323,18 @@ __global__ __launch_bounds__(BLOCK) void k2_commit(const double *__restrict__ un
 // Centred / second-order one-sided / first-order one-sided derivative of one
 // displacement component plane along x (axis 0) or y (axis 1) over active
 // nodes (cbw2d:3213-3272).
-__device__ __forceinline__ double du2(const double *uc, const int32_t *act, int p, int i, int j, int axis,
-                                      const Params2 &P) {
-    const int n = axis == 0 ? P.nx : P.ny;
-    const int k = axis == 0 ? i : j;
+__device__ __forceinline__ double du2(const double *uc, int nb, int p, int axis, const Params2 &P) {
     const int s = axis == 0 ? P.ny : 1;
     const double h = axis == 0 ? P.dx : P.dy;
-    if (k > 0 && k + 1 < n && act[p - s] != 0 && act[p + s] != 0) return (uc[p + s] - uc[p - s]) / (2.0 * h);
-    if (k + 1 < n && act[p + s] != 0) {
-        if (k + 2 < n && act[p + 2 * s] != 0) return (-3.0 * uc[p] + 4.0 * uc[p + s] - uc[p + 2 * s]) / (2.0 * h);
+    const bool pp = (nb & (axis == 0 ? NB_XP : NB_YP)) != 0, pm = (nb & (axis == 0 ? NB_XM : NB_YM)) != 0;
+    const bool pp2 = (nb & (axis == 0 ? NB_XP2 : NB_YP2)) != 0, pm2 = (nb & (axis == 0 ? NB_XM2 : NB_YM2)) != 0;
+    if (pp && pm) return (uc[p + s] - uc[p - s]) / (2.0 * h);
+    if (pp) {
+        if (pp2) return (-3.0 * uc[p] + 4.0 * uc[p + s] - uc[p + 2 * s]) / (2.0 * h);
         return (uc[p + s] - uc[p]) / h;
     }
-    if (k > 0 && act[p - s] != 0) {
-        if (k > 1 && act[p - 2 * s] != 0) return (3.0 * uc[p] - 4.0 * uc[p - s] + uc[p - 2 * s]) / (2.0 * h);
+    if (pm) {
+        if (pm2) return (3.0 * uc[p] - 4.0 * uc[p - s] + uc[p - 2 * s]) / (2.0 * h);
         return (uc[p] - uc[p - s]) / h;
     }
     return 0.0;
@@ -324,7 +349,7 @@ template <typename R>
 __global__ __launch_bounds__(BLOCK) void k2_record(double *__restrict__ U, const double *__restrict__ ug,
                                                    double *__restrict__ u, double *__restrict__ us,
                                                    R *__restrict__ rec, const int32_t *__restrict__ list,
-                                                   const int32_t *__restrict__ act, int repair, int commit,
+                                                   const uint8_t *__restrict__ nbm, int repair, int commit,
                                                    int32_t *__restrict__ reset_invalid, Params2 P) {
     const int t = xcd_block() * BLOCK + threadIdx.x;
     if (t >= P.na) return;
@@ -342,15 +367,12 @@ __global__ __launch_bounds__(BLOCK) void k2_record(double *__restrict__ U, const
     double Po[4], dP[4] = {0.0, 0.0, 0.0, 0.0}, dF[4] = {0.0, 0.0, 0.0, 0.0};
     piola2(Fo, P.m, Po);
     bool do_rep = repair != 0;
-    if (do_rep && P.interior_only) {
-        if (i == 0 || i + 1 >= nx || j == 0 || j + 1 >= ny)
-            do_rep = false;
-        else if (act[p - ny] == 0 || act[p + ny] == 0 || act[p - 1] == 0 || act[p + 1] == 0)
-            do_rep = false;
-    }
+    const int nb = nbm[t];
+    // interior_only: all four nearest neighbours on the lattice and active
+    if (do_rep && P.interior_only && (nb & NB_NEAR) != NB_NEAR) do_rep = false;
     if (do_rep) {
-        const double tt[4] = {1.0 + du2(ug, act, p, i, j, 0, P), du2(ug, act, p, i, j, 1, P),
-                              du2(ug + N, act, p, i, j, 0, P), 1.0 + du2(ug + N, act, p, i, j, 1, P)};
+        const double tt[4] = {1.0 + du2(ug, nb, p, 0, P), du2(ug, nb, p, 1, P), du2(ug + N, nb, p, 0, P),
+                              1.0 + du2(ug + N, nb, p, 1, P)};
         double alpha = P.blend;
         alpha = alpha < 0.0 ? 0.0 : (alpha > 1.0 ? 1.0 : alpha);
         double Fn[4];
@@ -1194,6 +1216,7 @@ struct vlbm2d_handle {
     hipStream_t cap = nullptr;
     StepGraph graphs[2];
     uint8_t *upmask = nullptr;  // (n_active) upstream-active bits, see k2_upmask
+    uint8_t *nbmask = nullptr;  // (n_active) neighbour-active bits, see k2_nbmask
     bool upmask_ok = false;
 };
 
@@ -1404,7 +1427,7 @@ static int launch_step2(vlbm2d_handle *h, const Params2 &P, T *f0, T *f1, double
         for (int s = 0; s < d.disp_sweeps; ++s) {
             TimedSlot ts(h->timing, st, VLBM_SLOT_SWEEP);
             hipLaunchKernelGGL(k2_sweep, dim3(gn), dim3(BLOCK), 0, st, (const double *)d.U, (const double *)d.u,
-                               d.u_next, d.active_list, d.active, d.anchor_off, d.anchor_ids, h->T, tb, P);
+                               d.u_next, d.active_list, (const uint8_t *)h->nbmask, d.anchor_off, d.anchor_ids, h->T, tb, P);
             CHECK_LAUNCH("k2_sweep");
             if (s + 1 < d.disp_sweeps) {
                 TimedSlot tc(h->timing, st, VLBM_SLOT_COMMIT);
@@ -1418,7 +1441,8 @@ static int launch_step2(vlbm2d_handle *h, const Params2 &P, T *f0, T *f1, double
         TimedSlot ts(h->timing, st, VLBM_SLOT_RECORD);
         hipLaunchKernelGGL(k2_record<T>, dim3(gn), dim3(BLOCK), 0, st, d.U,
                            disp ? (const double *)d.u_next : (const double *)d.u, d.u, d.u_star, (T *)d.record,
-                           d.active_list, d.active, (int)compat, (int)disp, next ? d.invalid : (int32_t *)nullptr,
+                           d.active_list, (const uint8_t *)h->nbmask, (int)compat, (int)disp,
+                           next ? d.invalid : (int32_t *)nullptr,
                            P);
         CHECK_LAUNCH("k2_record");
     }
@@ -1580,11 +1604,17 @@ int vlbm2d_create(const vlbm2d_desc *desc, vlbm2d_handle **out) {
         if (recon_compat(desc->recon) && !desc->lsq_wt) return fail_msg("vlbm2d_create: lsq_wt is null");
     }
     if (desc->n_anchor_links > 0 && !desc->anchor_ids) return fail_msg("vlbm2d_create: anchor_ids is null");
-    uint8_t *upmask = nullptr;
+    uint8_t *upmask = nullptr, *nbmask = nullptr;
     hipError_t ae = hipMalloc((void **)&upmask, (size_t)desc->n_active);
     if (ae != hipSuccess) return fail("vlbm2d_create: upstream mask", ae);
+    ae = hipMalloc((void **)&nbmask, (size_t)desc->n_active);
+    if (ae != hipSuccess) {
+        (void)hipFree(upmask);
+        return fail("vlbm2d_create: neighbour mask", ae);
+    }
     vlbm2d_handle *h = new vlbm2d_handle;
     h->upmask = upmask;
+    h->nbmask = nbmask;
     h->d = *desc;
     h->P = make_params2(*desc);
     h->T = make_links2(*desc);
@@ -1599,6 +1629,7 @@ int vlbm2d_destroy(vlbm2d_handle *h) {
     drop_graphs(h);
     if (h->clock) (void)hipFree(h->clock);
     if (h->upmask) (void)hipFree(h->upmask);
+    if (h->nbmask) (void)hipFree(h->nbmask);
     if (h->cap) (void)hipStreamDestroy(h->cap);
     h->timing.destroy();
     delete h;
@@ -1633,6 +1664,9 @@ int vlbm2d_step(vlbm2d_handle *h, void *f0, void *f1, double time, int64_t steps
         hipLaunchKernelGGL(k2_upmask, dim3((h->d.n_active + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, h->stream,
                            h->d.active_list, h->d.active, h->upmask, h->P);
         CHECK_LAUNCH("k2_upmask");
+        hipLaunchKernelGGL(k2_nbmask, dim3((h->d.n_active + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, h->stream,
+                           h->d.active_list, h->d.active, h->nbmask, h->P);
+        CHECK_LAUNCH("k2_nbmask");
         h->upmask_ok = true;
     }
     if (h->d.precision == VLBM_PREC_F64)
