@@ -30,6 +30,9 @@
 
 #include "../../include/vlbm3d.h"
 #include "vlbm3d_device.h"
+
+static_assert(vlbm::MAX_IDS == VLBM3D_MAX_IDS && vlbm::NPRM == VLBM3D_NPRM,
+              "boundary-id table shape must match the C ABI");
 #include "vlbm_common.h"
 
 using namespace vlbm;
@@ -194,7 +197,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(
             const double *e = adat + (size_t)6 * t;
             const double x = e[0], y = e[1], z = e[2];
             const double db[3] = {e[3], e[4], e[5]};
-            const double *prm = prm_by_id + bid * MAX_IDS;
+            const double *prm = prm_by_id + bid * NPRM;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 if (!((mask >> c) & 1)) continue;
@@ -577,7 +580,7 @@ __global__ __launch_bounds__(128) void k_prepare(const double *__restrict__ U, c
     const int kk[3] = {T.comp_kind[0 * MAX_IDS + bid], T.comp_kind[1 * MAX_IDS + bid], T.comp_kind[2 * MAX_IDS + bid]};
     const double x = T.xb[ell], y = T.yb[ell], z = T.zb[ell];
     const double nv[3] = {T.nx[ell], T.ny[ell], T.nz[ell]};
-    const double *prm = T.prm_by_id + bid * MAX_IDS;
+    const double *prm = T.prm_by_id + bid * NPRM;
     double tr[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) tr[c] = curved_value(mode, prm, c, x, y, z, nv, t, P.m.jf);

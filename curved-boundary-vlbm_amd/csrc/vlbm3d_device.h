@@ -20,7 +20,8 @@ namespace vlbm {
 constexpr int NC = 12;
 constexpr int NQ = 6;
 constexpr int FQ = 72;
-constexpr int MAX_IDS = 16;
+constexpr int MAX_IDS = 16;  // boundary ids
+constexpr int NPRM = 16;     // parameters per boundary id (row stride of params_by_id)
 constexpr int MAX_CHAR = 64;
 constexpr int MAX_TAN = 32;  // compat_bfl tangent stencil length
 

@@ -95,6 +95,7 @@ NORMAL_JACOBIAN_ACOUSTIC = 1
 NORMAL_SOLVER_ENERGY = 2
 MAX_LSQ_STENCIL = 32
 MAX_BOUNDARY_IDS = 16
+PARAMS_PER_ID = 16  # row width of the per-id parameter table (VLBM3D_NPRM)
 BOUNDARY_ID_OUTER = 1
 BOUNDARY_ID_INNER = 2
 BoundaryKind = Literal["dirichlet", "neumann"]
@@ -196,8 +197,8 @@ class CurvedBoundarySpec3D:
     def tables(self, max_ids: int = MAX_BOUNDARY_IDS) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
         kind = np.full(max_ids, _kind_id(self.kind), dtype=np.int32)
         mode = np.full(max_ids, int(self.value_mode), dtype=np.int32)
-        base = np.zeros(16, dtype=np.float64)
-        vals = tuple(self.params)[:16]
+        base = np.zeros(PARAMS_PER_ID, dtype=np.float64)
+        vals = tuple(self.params)[:PARAMS_PER_ID]
         base[: len(vals)] = [float(v) for v in vals]
         params = np.tile(base[None, :], (max_ids, 1))
         for i, v in (self.id_kinds or {}).items():
@@ -208,8 +209,8 @@ class CurvedBoundarySpec3D:
                 mode[int(i)] = int(v)
         for i, vals in (self.id_params or {}).items():
             if 0 <= int(i) < max_ids:
-                row = np.zeros(16)
-                vv = tuple(vals)[:16]
+                row = np.zeros(PARAMS_PER_ID)
+                vv = tuple(vals)[:PARAMS_PER_ID]
                 row[: len(vv)] = [float(v) for v in vv]
                 params[int(i)] = row
         return kind, mode, np.ascontiguousarray(params)
@@ -961,6 +962,8 @@ class WarpSparseCurvedBoundaryLBM3D:
         self.tangent_lsq_active = self.boundary_reconstruction == BOUNDARY_RECON_COMPAT_BFL
         self.tangent_lsq = build_tangent_lsq_weights(geometry, tables) if self.tangent_lsq_active else None
         self.kind_by_id, self.mode_by_id, self.params_by_id = boundary.tables(MAX_BOUNDARY_IDS)
+        if self.params_by_id.shape != (MAX_BOUNDARY_IDS, PARAMS_PER_ID):
+            raise ValueError(f"boundary parameter table must be ({MAX_BOUNDARY_IDS}, {PARAMS_PER_ID})")
         self.component_kind_by_id = boundary.component_kind_table(MAX_BOUNDARY_IDS)
         link_ids = np.clip(np.asarray(geometry.link_boundary_id, dtype=np.int32), 0, MAX_BOUNDARY_IDS - 1)
         neumann_rows = self.component_kind_by_id[:, link_ids] == BC_NEUMANN
@@ -1166,6 +1169,12 @@ class WarpSparseCurvedBoundaryLBM3D:
         self._check(self._lib.vlbm3d_create(C.byref(self._desc), C.byref(h)), "vlbm3d_create")
         self._handle = h
 
+    def _to_device(self, host: np.ndarray):
+        """Host array -> device tensor, copied on the solver's stream so the
+        library launches that read it next are ordered after the copy."""
+        with self._torch.cuda.stream(self._stream):
+            return self._torch.from_numpy(np.ascontiguousarray(host)).to(self.device)
+
     def _populations_to_host(self, arr) -> np.ndarray:
         tmp = self._torch.empty((FQ, self.n_active), dtype=self._torch.float64, device=self.device)
         self._check(self._lib.vlbm3d_populations_to_f64(self._handle, arr.ptr, tmp.data_ptr()), "populations_to_f64")
@@ -1175,7 +1184,7 @@ class WarpSparseCurvedBoundaryLBM3D:
     def _populations_from_host(self, arr, host: np.ndarray) -> None:
         if host.shape != (FQ, self.n_active):
             raise ValueError(f"populations must have shape {(FQ, self.n_active)}")
-        tmp = self._torch.from_numpy(np.ascontiguousarray(host)).to(self.device)
+        tmp = self._to_device(host)
         self._check(self._lib.vlbm3d_populations_from_f64(self._handle, tmp.data_ptr(), arr.ptr), "populations_from_f64")
         self._torch.cuda.synchronize(self.device)
 
@@ -1309,7 +1318,7 @@ class WarpSparseCurvedBoundaryLBM3D:
         if apply_initial_boundaries:
             self.apply_boundary_reconstruction(initial_boundary_time, in_place=True)
             self.refresh_current()
-            disp = self._torch.from_numpy(np.ascontiguousarray(disp_active, dtype=np.float64)).to(self.device)
+            disp = self._to_device(np.asarray(disp_active, dtype=np.float64))
             self._check(self._lib.vlbm3d_set_ustar(self._handle, disp.data_ptr()), "set_ustar")
             self._torch.cuda.synchronize(self.device)
             self.refresh_current()
@@ -1395,7 +1404,8 @@ class WarpSparseCurvedBoundaryLBM3D:
         return self.reductions()["rms_v"]
 
     def run_to_equilibrium(self, tol: float, *, check_every: int = 200, hold_checks: int = 1,
-                           min_time: float = 0.0, max_steps: int = 10**7, criterion: str = "displacement_rate") -> dict:
+                           min_time: float = 0.0, max_steps: int = 10**7, criterion: str = "displacement_rate",
+                           first_check_limit: float | None = None) -> dict:
         """Step until the equilibrium residual is <= tol on ``hold_checks``
         consecutive checks.
 
@@ -1405,17 +1415,20 @@ class WarpSparseCurvedBoundaryLBM3D:
         state.  ``criterion="rms_v"``: the velocity moment, which in this
         method settles at a nonzero plateau (the displacement sweep and the F
         update balance a residual flux), so it only suits loose tolerances.
+        ``first_check_limit``: stop after the first check, with
+        ``first_check_failed`` set, when its residual exceeds this value (the
+        warm start's fallback test).
         """
         if criterion not in ("displacement_rate", "rms_v"):
             raise ValueError("criterion must be 'displacement_rate' or 'rms_v'")
         held = 0
-        u_prev = self.u.tensor.clone()
+        u_prev = self.u.snapshot()
         history = []
         while self.steps < max_steps:
             k = min(check_every, max_steps - self.steps)
             self.run(k)
             red = self.reductions(u_prev)
-            u_prev.copy_(self.u.tensor)
+            self.u.snapshot(u_prev)
             rate = red["rms_du"] / (k * self.dt)
             self.refresh_current()
             red.update({k2: v for k2, v in self.reductions().items() if k2 != "rms_du"})
@@ -1424,6 +1437,9 @@ class WarpSparseCurvedBoundaryLBM3D:
             if red["nonfinite"] or red["min_J"] <= 0.0:
                 return {"converged": False, "steps": self.steps, "history": history, **red}
             value = rate if criterion == "displacement_rate" else red["rms_v"]
+            if first_check_limit is not None and len(history) == 1 and value > first_check_limit:
+                return {"converged": False, "first_check_failed": True, "steps": self.steps,
+                        "history": history, **red}
             held = held + 1 if (self.time >= min_time and value <= tol) else 0
             if held >= hold_checks:
                 return {"converged": True, "steps": self.steps, "history": history, **red}

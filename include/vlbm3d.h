@@ -34,6 +34,7 @@ extern "C" {
 
 #define VLBM3D_ABI_VERSION 6
 #define VLBM3D_MAX_IDS 16
+#define VLBM3D_NPRM 16 /* parameters per boundary id: params_by_id is (MAX_IDS, NPRM) */
 #define VLBM3D_MAX_CHAR 64
 #define VLBM3D_RECORD 24
 
@@ -99,7 +100,7 @@ typedef struct {
     const int32_t *kind_by_id;       /* (16) */
     const int32_t *comp_kind_by_id;  /* (3,16) */
     const int32_t *mode_by_id;       /* (16) */
-    const double *params_by_id;      /* (16,16) */
+    const double *params_by_id;      /* (MAX_IDS, NPRM) row-major */
     const int32_t *dyn_enabled;      /* (L) */
     const double *dyn_traction;      /* (3,L) */
     int32_t *lkind;                  /* (L) */

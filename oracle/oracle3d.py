@@ -84,6 +84,8 @@ def load():
         lib.or3d_piola.restype = None
         lib.or3d_du_axis_all.argtypes = [C.POINTER(_State), _dp, _dp]
         lib.or3d_du_axis_all.restype = None
+        lib.or3d_value_probe.argtypes = [_dp, C.c_int, C.c_int] + [C.c_double] * 7 + [C.c_int]
+        lib.or3d_value_probe.restype = C.c_double
         _lib = lib
     return _lib
 
@@ -100,6 +102,17 @@ def piola(F: np.ndarray, material, j_floor: float = 1e-12) -> np.ndarray:
     lib.or3d_piola(_p(Fi), material.id, material.lam, material.mu, material.param0, material.param1,
                    material.param2, j_floor, _p(P))
     return P.reshape(3, 3)
+
+
+def boundary_value(mode, params, comp, x, y, z, n, t, *, displacement=False) -> float:
+    """Oracle boundary value (or, with ``displacement``, the Dirichlet
+    anchor displacement) of one value mode at one point."""
+    prm = np.zeros(16)
+    vals = np.asarray(params, np.float64).ravel()[:16]
+    prm[: vals.size] = vals
+    return float(load().or3d_value_probe(_p(prm), int(mode), int(comp), float(x), float(y), float(z),
+                                         float(n[0]), float(n[1]), float(n[2]), float(t),
+                                         1 if displacement else 0))
 
 
 class OracleSolver3D:

@@ -316,7 +316,7 @@ static double sine2_integral(double time, double active, double denom) {
     return prefix + (t - active);
 }
 
-#define PRM(bid, k) (s->params_by_id[(bid) * OR_MAX_IDS + (k)])
+#define PRM(bid, k) (s->params_by_id[(bid) * OR_NPRM + (k)])
 
 /* cbw3d.py:2351-2399 (_local_dirichlet_displacement_comp); mode 208 (linear
  * ramp, a traction mode) has no displacement anchor and yields zero. */
@@ -602,6 +602,22 @@ static double curved_value(const or3d_state *s, int mode, int bid, int comp, dou
     if (mode == V_XPULL) return tube_pull(&PRM(bid, 0), tube_comp(comp), y, z, t);
     if (mode == V_XTWIST) return twist_velocity(&PRM(bid, 0), tube_comp(comp), y, z, t);
     return 0.0;
+}
+
+/* Test access: one boundary value (want_disp = 0, cbw3d.py:996-1133) or
+ * Dirichlet anchor displacement (want_disp = 1, cbw3d.py:2351-2399) of value
+ * mode `mode` with the 16 parameters `prm`, evaluated at boundary id 0. */
+double or3d_value_probe(const double *prm, int mode, int comp, double x, double y, double z, double nx,
+                        double ny, double nz, double t, int want_disp) {
+    or3d_state st;
+    double table[OR_MAX_IDS * OR_NPRM];
+    memset(&st, 0, sizeof(st));
+    memset(table, 0, sizeof(table));
+    memcpy(table, prm, sizeof(double) * OR_NPRM);
+    st.params_by_id = table;
+    st.j_floor = 1e-12;
+    if (want_disp) return dirichlet_disp(&st, mode, 0, comp, x, y, z, t);
+    return curved_value(&st, mode, 0, comp, x, y, z, nx, ny, nz, t);
 }
 
 static inline void normalize3(double *v) {

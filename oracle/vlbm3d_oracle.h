@@ -16,6 +16,7 @@
 #include <stdint.h>
 
 #define OR_MAX_IDS 16
+#define OR_NPRM 16 /* parameters per boundary id */
 #define OR_MAX_CHAR 64
 #define OR_MAX_TAN 32
 
@@ -83,5 +84,7 @@ int or3d_step(or3d_state *s, double time, int64_t steps, int nsteps);
 void or3d_piola(const double *F, int model, double lam, double mu, double p0, double p1,
                 double p2, double jf, double *P);
 void or3d_du_axis_all(const or3d_state *s, const double *u, double *out); /* (3,3,n) */
+double or3d_value_probe(const double *prm, int mode, int comp, double x, double y, double z, double nx,
+                        double ny, double nz, double t, int want_disp);
 int or3d_threads(void);
 #endif

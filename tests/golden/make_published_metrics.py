@@ -1,6 +1,11 @@
 """Writes tests/golden/published_metrics.json: the subset of the reference's
 published manuscript metrics (validation/manuscript_metrics.json of
-JingsenFeng/curved-boundary-vlbm) that the end-to-end GPU tests reproduce.
+JingsenFeng/curved-boundary-vlbm) that the end-to-end GPU tests reproduce,
+plus, under "release", the values of the reference's release re-run
+(validation/release_verification.json, full_benchmark_verification.cases),
+which it made after its frozen-source initial-boundary fix
+(validation/initialization_buffer_fix.json): the rotated-ellipsoid rows, the
+only published case that fix changes beyond rounding, and the tube rows.
 
 Run from the repository root with the reference checked out at
 /root/reference:  python tests/golden/make_published_metrics.py
@@ -36,6 +41,17 @@ def main() -> None:
         out[case] = row
     for case in ("affine_annulus", "ellipsoid"):
         out[case] = {str(r["n"]): {k: r[k] for k in DIRICHLET_KEYS if k in r} for r in src["dirichlet_cases"][case]}
+    entry = src["traction_cases"]["tube10"]
+    out["tube10"] = {"steps": entry["steps"], "time": entry["time"], **entry["metrics"]}
+    rel = json.loads((REFERENCE / "validation" / "release_verification.json").read_text())
+    cases = rel["full_benchmark_verification"]["cases"]
+    out["release"] = {"source": "validation/release_verification.json of the reference "
+                                "(full_benchmark_verification.cases, re-run after the initial-boundary fix)"}
+    for case in ("ellipsoid", "tube10"):
+        out["release"][case] = {
+            str(r.get("n", r.get("radius_nodes"))): {"steps": r["steps"], "time": r["time"],
+                                                     **{k: v["value"] for k, v in r["metrics"].items()}}
+            for r in cases[case]["rows"]}
     OUT.write_text(json.dumps(out, indent=1) + "\n")
     print(f"wrote {OUT}")
 

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from curved_lbm.three_d import curved_boundary_warp3d as cb
-from oracle.oracle3d import OracleSolver3D, piola
+from oracle.oracle3d import OracleSolver3D, boundary_value, piola
 
 from _charcheck import check_boundary_mechanics
 
@@ -102,26 +102,146 @@ def test_in_place_boundary_matches_frozen_source():
     np.testing.assert_array_equal(s.f0, expected)
 
 
-@pytest.mark.parametrize("model", ["svk", "neo_hooke", "log_neo_hooke", "mooney_rivlin", "yeoh", "gent", "sph_neo_hooke"])
-def test_oracle_piola_matches_energy_derivative(model):
-    """P = dW/dF checked against the host Piola map and, for neo-Hooke,
-    against a central difference of the stated strain energy."""
-    mat = cb.WarpHyperelasticMaterial3D.from_poisson(model, 0.2, mu=1.0)
-    rng = np.random.default_rng(3)
-    F = np.eye(3) + 0.15 * rng.normal(size=(3, 3))
-    np.testing.assert_allclose(piola(F, mat), cb._first_piola_np(F, mat), rtol=1e-12, atol=1e-13)
+def _strain_energy(model, mat, F):
+    """Textbook strain energies of the seven 3D laws (Holzapfel, Nonlinear
+    Solid Mechanics, ch. 6), written from their invariants; the reference
+    names them at vector_nonlinear_elastic_warp3d.py:793-1022.  The
+    compressible neo-Hooke law carries the quadratic volumetric term
+    lam/4 (J^2 - 1 - 2 ln J); the isochoric laws use the distortional
+    invariants Ib1 = J^(-2/3) I1, Ib2 = J^(-4/3) I2 and the volumetric term
+    lam/2 (ln J)^2, except sph_neo_hooke, whose lam slot holds the bulk
+    modulus K with K/4 (J^2 - 1 - 2 ln J)."""
+    lam, mu = mat.lam, mat.mu
+    C = F.T @ F
+    I1 = np.trace(C)
+    I2 = 0.5 * (I1 * I1 - np.trace(C @ C))
+    J = np.linalg.det(F)
+    lnJ = np.log(J)
+    Ib1, Ib2 = J ** (-2.0 / 3.0) * I1, J ** (-4.0 / 3.0) * I2
+    if model == "svk":
+        E = 0.5 * (C - np.eye(3))
+        return 0.5 * lam * np.trace(E) ** 2 + mu * np.sum(E * E)
     if model == "neo_hooke":
-        def W(Fm):
-            J = np.linalg.det(Fm)
-            return 0.5 * mat.mu * (np.sum(Fm * Fm) - 3) - (mat.mu + 0.5 * mat.lam) * np.log(J) + 0.25 * mat.lam * (J * J - 1)
-        h = 1e-6
-        dW = np.zeros((3, 3))
-        for i in range(3):
-            for j in range(3):
-                E = np.zeros((3, 3))
-                E[i, j] = h
-                dW[i, j] = (W(F + E) - W(F - E)) / (2 * h)
-        np.testing.assert_allclose(piola(F, mat), dW, rtol=1e-7, atol=1e-8)
+        return 0.5 * mu * (I1 - 3.0) - mu * lnJ + 0.25 * lam * (J * J - 1.0 - 2.0 * lnJ)
+    if model == "log_neo_hooke":
+        return 0.5 * mu * (I1 - 3.0) - mu * lnJ + 0.5 * lam * lnJ ** 2
+    if model == "sph_neo_hooke":
+        return 0.5 * mu * (Ib1 - 3.0) + 0.25 * lam * (J * J - 1.0 - 2.0 * lnJ)
+    vol = 0.5 * lam * lnJ ** 2
+    if model == "mooney_rivlin":
+        return mat.param0 * (Ib1 - 3.0) + mat.param1 * (Ib2 - 3.0) + vol
+    A = Ib1 - 3.0
+    if model == "yeoh":
+        return mat.param0 * A + mat.param1 * A ** 2 + mat.param2 * A ** 3 + vol
+    if model == "gent":
+        return -0.5 * mu * mat.param0 * np.log(1.0 - A / mat.param0) + vol
+    raise ValueError(model)
+
+
+@pytest.mark.parametrize("model", ["svk", "neo_hooke", "log_neo_hooke", "mooney_rivlin", "yeoh", "gent", "sph_neo_hooke"])
+def test_oracle_piola_is_the_energy_derivative(model):
+    """The oracle's P(F) is dW/dF of each law's textbook strain energy
+    (central differences, h = 1e-6) at three strained states, and agrees
+    with the host Piola map.  Yeoh is also checked with non-zero c2, c3 so
+    that its higher-order terms are exercised."""
+    mats = [cb.WarpHyperelasticMaterial3D.from_poisson(model, 0.2, mu=1.0)]
+    if model == "yeoh":
+        mats.append(cb.WarpHyperelasticMaterial3D.from_poisson(model, 0.3, mu=1.3, yeoh_c2=0.2, yeoh_c3=0.05))
+    rng = np.random.default_rng(3)
+    for mat in mats:
+        for scale in (0.05, 0.15, 0.25):
+            F = np.eye(3) + scale * rng.normal(size=(3, 3))
+            assert np.linalg.det(F) > 0.2
+            P = piola(F, mat)
+            np.testing.assert_allclose(P, cb._first_piola_np(F, mat), rtol=1e-12, atol=1e-13)
+            h = 1e-6
+            dW = np.zeros((3, 3))
+            for i in range(3):
+                for j in range(3):
+                    E = np.zeros((3, 3))
+                    E[i, j] = h
+                    dW[i, j] = (_strain_energy(model, mat, F + E) - _strain_energy(model, mat, F - E)) / (2 * h)
+            np.testing.assert_allclose(P, dW, rtol=1e-6, atol=2e-8)
+            assert np.max(np.abs(P)) > 1e-3
+
+
+def test_oracle_piola_vanishes_at_identity_and_is_objective():
+    """P(I) = 0 and P F^T is symmetric (balance of angular momentum) for
+    every law."""
+    rng = np.random.default_rng(5)
+    F = np.eye(3) + 0.2 * rng.normal(size=(3, 3))
+    for model in ("svk", "neo_hooke", "log_neo_hooke", "mooney_rivlin", "yeoh", "gent", "sph_neo_hooke"):
+        mat = cb.WarpHyperelasticMaterial3D.from_poisson(model, 0.2, mu=1.0)
+        assert np.max(np.abs(piola(np.eye(3), mat))) < 1e-14
+        tau = piola(F, mat) @ F.T
+        assert np.max(np.abs(tau - tau.T)) < 1e-12
+
+
+def test_linear_ramp_normal_traction_mode_closed_form():
+    """Value mode 208 (params amp, t_ramp) is the nominal traction
+    amp * min(t / t_ramp, 1) * n, and, being a traction mode, carries no
+    Dirichlet anchor displacement."""
+    amp, tr = 0.03, 0.4
+    rng = np.random.default_rng(8)
+    for _ in range(6):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        x = rng.uniform(0, 1, 3)
+        for t, s in ((0.0, 0.0), (0.1, 0.25), (0.5 * tr, 0.5), (tr, 1.0), (2.0 * tr, 1.0), (7.0, 1.0)):
+            for comp in range(3):
+                got = boundary_value(208, (amp, tr), comp, *x, n, t)
+                assert got == pytest.approx(amp * s * n[comp], rel=1e-15, abs=1e-18)
+                assert boundary_value(208, (amp, tr), comp, *x, n, t, displacement=True) == 0.0
+    # t_ramp <= 0 means the full load from t = 0
+    assert boundary_value(208, (amp, 0.0), 2, 0, 0, 0, (0, 0, 1.0), 0.0) == amp
+
+
+def _twist_closed_form(p, a, b, t):
+    """Anchor displacement of the exact twist + stretch end (modes 207 /
+    210): rotation by theta = phi s(t) about the axis through (ca, cb)
+    and axial offset d s(t), s = (1 - cos(pi t / T)) / 2 until T, then 1."""
+    ca, cbb, d, phi, T = p
+    s = 0.5 * (1 - np.cos(np.pi * t / T)) if t < T else 1.0
+    th = phi * s
+    R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+    r0 = np.array([a - ca, b - cbb])
+    da, db = R @ r0 - r0
+    return np.array([da, db, d * s])
+
+
+def test_tube_end_modes_closed_forms():
+    """The displacement-controlled tube end (z form 207, x form 210): the
+    anchor displacement is the rigid rotation plus axial offset of the
+    cosine ramp (closed form above), its time derivative (central
+    difference) is the prescribed velocity, and the velocity vanishes after
+    the ramp.  The x form is the z form with (a, b, axial) = (y, z, x).
+    The pull-torsion traction end (206 / 209) is the sine^2-ramped
+    (-alpha db, alpha da, axial)."""
+    p = (0.5, 0.45, 0.3, np.radians(27.0), 0.8)
+    zero = (0.0, 0.0, 0.0)
+    for (a, b, c) in ((0.8, 0.3, 1.1), (0.2, 0.7, 0.4), (0.5, 0.45, 2.0)):
+        for t in (0.05, 0.3, 0.61, 0.8, 1.5):
+            want = _twist_closed_form(p, a, b, t)
+            gz = [boundary_value(207, p, k, a, b, c, zero, t, displacement=True) for k in range(3)]
+            gx = [boundary_value(210, p, k, c, a, b, zero, t, displacement=True) for k in range(3)]
+            np.testing.assert_allclose(gz, want, rtol=1e-13, atol=1e-15)
+            np.testing.assert_allclose(gx, [want[2], want[0], want[1]], rtol=1e-13, atol=1e-15)
+            h = 1e-5
+            vz = [boundary_value(207, p, k, a, b, c, zero, t) for k in range(3)]
+            fd = (_twist_closed_form(p, a, b, t + h) - _twist_closed_form(p, a, b, t - h)) / (2 * h)
+            if abs(t - p[4]) > 2 * h:
+                np.testing.assert_allclose(vz, fd, rtol=1e-7, atol=1e-9)
+            if t >= p[4]:
+                assert vz == [0.0, 0.0, 0.0]
+    q = (0.5, 0.45, 0.02, 0.7, 0.6, 0.0)
+    for t in (0.1, 0.3, 2.0):
+        s = np.sin(np.pi * t / 1.2) ** 2 if t < 0.6 else 1.0
+        a, b = 0.9, 0.2
+        want = s * np.array([-0.7 * (b - 0.45), 0.7 * (a - 0.5), 0.02])
+        got = [boundary_value(206, q, k, a, b, 1.0, zero, t) for k in range(3)]
+        np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-16)
+        gotx = [boundary_value(209, q, k, 1.0, a, b, zero, t) for k in range(3)]
+        np.testing.assert_allclose(gotx, [want[2], want[0], want[1]], rtol=1e-13, atol=1e-16)
 
 
 @pytest.mark.parametrize("kinds", [("neumann",) * 3, ("neumann", "dirichlet", "neumann")])
