@@ -4,6 +4,12 @@ the small array interface the reference solver attributes offer
 
 PyTorch only supplies device memory and the stream here; all arithmetic of
 the time step runs in libvlbm.so.
+
+Every array belongs to the stream that was current when it was created (a
+solver creates its arrays on its own stream), and every torch operation the
+array performs (assignment, zeroing, snapshots) is issued on that stream, so
+it is ordered with the solver's library launches whichever stream the caller
+has current.
 """
 
 from __future__ import annotations
@@ -28,10 +34,16 @@ def hip_device(device: str | None) -> torch.device:
 
 
 class DeviceArray:
-    """A device tensor with numpy-style host access."""
+    """A device tensor with numpy-style host access, bound to a stream."""
 
-    def __init__(self, tensor: torch.Tensor):
+    def __init__(self, tensor: torch.Tensor, stream=None):
         self.tensor = tensor
+        if stream is None and tensor.is_cuda:
+            stream = torch.cuda.current_stream(tensor.device)
+        self.stream = stream
+
+    def _on_stream(self):
+        return torch.cuda.stream(self.stream) if self.stream is not None else _NullContext()
 
     @classmethod
     def zeros(cls, shape, dtype, device) -> "DeviceArray":
@@ -41,6 +53,15 @@ class DeviceArray:
     def from_numpy(cls, arr: np.ndarray, dtype, device) -> "DeviceArray":
         host = np.ascontiguousarray(np.asarray(arr, dtype=dtype))
         return cls(torch.from_numpy(host).to(device))
+
+    def snapshot(self, out: torch.Tensor | None = None) -> torch.Tensor:
+        """A device copy of the array taken on its stream (into ``out`` when
+        given), ordered after the launches already queued there."""
+        with self._on_stream():
+            if out is None:
+                return self.tensor.clone()
+            out.copy_(self.tensor)
+            return out
 
     @property
     def shape(self) -> tuple[int, ...]:
@@ -61,24 +82,35 @@ class DeviceArray:
     def assign(self, arr) -> None:
         if isinstance(arr, DeviceArray):
             arr = arr.tensor
-        if isinstance(arr, torch.Tensor):
-            src = arr.to(device=self.tensor.device, dtype=self.tensor.dtype)
-        else:
-            src = torch.from_numpy(np.ascontiguousarray(np.asarray(arr))).to(
-                device=self.tensor.device, dtype=self.tensor.dtype
-            )
-        if tuple(src.shape) != self.shape:
-            raise ValueError(f"shape mismatch: {tuple(src.shape)} vs {self.shape}")
-        self.tensor.copy_(src)
+        with self._on_stream():
+            if isinstance(arr, torch.Tensor):
+                src = arr.to(device=self.tensor.device, dtype=self.tensor.dtype)
+            else:
+                src = torch.from_numpy(np.ascontiguousarray(np.asarray(arr))).to(
+                    device=self.tensor.device, dtype=self.tensor.dtype
+                )
+            if tuple(src.shape) != self.shape:
+                raise ValueError(f"shape mismatch: {tuple(src.shape)} vs {self.shape}")
+            self.tensor.copy_(src)
 
     def zero_(self) -> None:
-        self.tensor.zero_()
+        with self._on_stream():
+            self.tensor.zero_()
 
     def fill_(self, value) -> None:
-        self.tensor.fill_(value)
+        with self._on_stream():
+            self.tensor.fill_(value)
 
     def __repr__(self) -> str:
         return f"DeviceArray(shape={self.shape}, dtype={self.dtype}, device={self.tensor.device})"
+
+
+class _NullContext:
+    def __enter__(self):
+        return None
+
+    def __exit__(self, *exc):
+        return False
 
 
 class PopulationArray(DeviceArray):
@@ -86,8 +118,8 @@ class PopulationArray(DeviceArray):
     deviations from the rest state; ``numpy()``/``assign`` convert to and from
     the full double-precision populations through the HIP library."""
 
-    def __init__(self, tensor: torch.Tensor, owner):
-        super().__init__(tensor)
+    def __init__(self, tensor: torch.Tensor, owner, stream=None):
+        super().__init__(tensor, stream)
         self._owner = owner
 
     def numpy(self) -> np.ndarray:

@@ -272,7 +272,7 @@ class WarpNonlinearElasticLBM3D:
         if apply_initial_boundaries:
             self.apply_boundary_reconstruction(float(initial_boundary_time))
             s.refresh_current()
-            d_dev = s._torch.from_numpy(np.ascontiguousarray(disp.reshape(3, n))).to(s.device)
+            d_dev = s._to_device(disp.reshape(3, n))
             s._check(s._lib.vlbm3d_set_ustar(s._handle, d_dev.data_ptr()), "set_ustar")
             s._torch.cuda.synchronize(s.device)
             s.refresh_current()

@@ -155,7 +155,7 @@ class BatchedCurvedBoundaryLBM2D:
         ``diverged`` set.  ``on_check(live, steps)`` is called after every
         round of checks with the number of shells still stepping."""
         mins = [int(min_steps)] * len(self.solvers) if np.isscalar(min_steps) else [int(m) for m in min_steps]
-        prev = [s.u.tensor.clone() for s in self.solvers]
+        prev = [s.u.snapshot() for s in self.solvers]
         done = [None] * len(self.solvers)
         while any(d is None for d in done):
             live = [k for k, d in enumerate(done) if d is None]
@@ -164,7 +164,7 @@ class BatchedCurvedBoundaryLBM2D:
             for k in live:
                 s = self.solvers[k]
                 red = s.reductions(prev[k])
-                prev[k].copy_(s.u.tensor)
+                s.u.snapshot(prev[k])
                 rate = red["rms_du"] / (check_every * s.dt)
                 ramp_done = s.steps >= mins[k]
                 if red["nonfinite"] or red["min_J"] <= 0.0:

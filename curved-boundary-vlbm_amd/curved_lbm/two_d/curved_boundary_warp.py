@@ -1041,6 +1041,12 @@ class WarpCurvedBoundaryLBM2D:
     def _check(self, rc, what) -> None:
         self._native.check(rc, what)
 
+    def _to_device(self, host: np.ndarray):
+        """Host array -> device tensor, copied on the solver's stream so the
+        library launches that read it next are ordered after the copy."""
+        with self._torch.cuda.stream(self._stream):
+            return self._torch.from_numpy(np.ascontiguousarray(host)).to(self.device)
+
     def _populations_to_host(self, arr) -> np.ndarray:
         tmp = self._torch.empty(arr.shape, dtype=self._torch.float64, device=self.device)
         self._check(self._lib.vlbm2d_populations_to_f64(self._handle, arr.ptr, tmp.data_ptr()), "populations_to_f64")
@@ -1050,7 +1056,7 @@ class WarpCurvedBoundaryLBM2D:
     def _populations_from_host(self, arr, host: np.ndarray) -> None:
         if host.shape != arr.shape:
             raise ValueError(f"populations must have shape {arr.shape}")
-        tmp = self._torch.from_numpy(np.ascontiguousarray(host)).to(self.device)
+        tmp = self._to_device(host)
         self._check(self._lib.vlbm2d_populations_from_f64(self._handle, tmp.data_ptr(), arr.ptr), "populations_from_f64")
         self._torch.cuda.synchronize(self.device)
 
@@ -1140,7 +1146,7 @@ class WarpCurvedBoundaryLBM2D:
             f = np.ascontiguousarray(f.reshape((FQ, self.nx, self.ny)))
             f[:, ~act] = 0.0
             f0.assign(f[..., None])
-        d_dev = self._torch.from_numpy(np.ascontiguousarray(disp)).to(self.device)
+        d_dev = self._to_device(disp)
         self._check(self._lib.vlbm2d_set_ustar(self._handle, d_dev.data_ptr()), "set_ustar")
         self._torch.cuda.synchronize(self.device)
         self.time = 0.0
@@ -1165,7 +1171,7 @@ class WarpCurvedBoundaryLBM2D:
         if in_place:
             # Opposing cut links must read the same initial population state.
             self._fpost_stale = False
-            self._fpost.tensor.copy_(self._pops[0].tensor)
+            self._pops[0].snapshot(self._fpost.tensor)
             src, dst = self._fpost, self._pops[0]
         else:
             src, dst = self.fpost, self.f1
@@ -1279,14 +1285,14 @@ class WarpCurvedBoundaryLBM2D:
         if criterion not in ("displacement_rate", "rms_v"):
             raise ValueError("criterion must be 'displacement_rate' or 'rms_v'")
         held = 0
-        u_prev = self.u.tensor.clone()
+        u_prev = self.u.snapshot()
         history = []
         red: dict = {}
         while self.steps < max_steps:
             k = min(check_every, max_steps - self.steps)
             self.run(k)
             red = self.reductions(u_prev)
-            u_prev.copy_(self.u.tensor)
+            self.u.snapshot(u_prev)
             rate = red["rms_du"] / (k * self.dt)
             self.refresh_current()
             red.update({key: v for key, v in self.reductions().items() if key != "rms_du"})

@@ -41,6 +41,41 @@ def test_gpu_concurrent_batch_equals_shells_stepped_alone(graph_steps):
     assert all(r["converged"] and r["steps"] >= 60 for r in res)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("graph_steps", [0, 10])
+def test_gpu_batch_residual_checks_equal_shells_alone(graph_steps):
+    """With a finite tolerance the batch's per-check residuals and stopping
+    steps are those of each shell stepped alone with the same checks (the
+    snapshots of u run on each shell's own stream)."""
+    pytest.importorskip("torch")
+    from curved_lbm.two_d import curved_boundary_warp as cbw
+
+    inst = bt.batch_instances(3, seed=2, sides=(64, 96, 128))
+    inputs = [bt.shell_solver_inputs(i) for i in inst]
+    every, checks = 20, 5
+    hist = []
+    for g, kw in inputs:
+        s = cbw.WarpCurvedBoundaryLBM2D(g, **kw, precision="fp32")
+        prev, h = s.u.snapshot(), []
+        for _ in range(checks):
+            s.run(every)
+            h.append(s.reductions(prev)["rms_du"] / (every * s.dt))
+            s.u.snapshot(prev)
+        hist.append(h)
+    # during the load ramp the residual grows, so a tolerance between the
+    # shells' first residuals stops some shells at once and never the rest
+    tol = float(np.median([h[0] for h in hist]))
+    want = []
+    for h in hist:
+        k = next((i for i, r in enumerate(h) if r <= tol), None)
+        want.append((True, (k + 1) * every, h[k]) if k is not None else (False, checks * every, h[-1]))
+    b = bt.BatchedCurvedBoundaryLBM2D.build(inputs, precision="fp32", graph_steps=graph_steps)
+    res = b.run_to_equilibrium(tol, check_every=every, max_steps=checks * every)
+    got = [(r["converged"], r["steps"], r["residual"]) for r in res]
+    assert got == want
+    assert any(g[0] for g in got) and not all(g[0] for g in got)
+
+
 class _Tensor:
     def __init__(self, x):
         self.x = np.array(x, dtype=float)
@@ -52,14 +87,24 @@ class _Tensor:
         self.x[...] = other.x
 
 
+class _Field:
+    def __init__(self, x):
+        self.tensor = _Tensor(x)
+
+    def snapshot(self, out=None):
+        if out is None:
+            return self.tensor.clone()
+        out.copy_(self.tensor)
+        return out
+
+
 class _StubSolver:
     """Host stand-in with the attributes the batch reads: a displacement
     that moves by rate * dt per step, optionally turning non-finite."""
 
     def __init__(self, rate, dt=1e-3, blow_at=None):
         self._stream = object()
-        self.u = type("Field", (), {})()
-        self.u.tensor = _Tensor([0.0])
+        self.u = _Field([0.0])
         self.steps, self.dt, self.rate, self.blow_at = 0, dt, rate, blow_at
         self.n_active = 10
 
