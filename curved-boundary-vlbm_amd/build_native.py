@@ -17,7 +17,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT_DIR = HERE / "curved_lbm" / "_lib"
 INCLUDE = HERE.parent / "include"
-SOURCES = ["vlbm3d.hip", "vlbm2d.hip", "geom_host.hip"]
+SOURCES = ["vlbm3d.hip", "vlbm2d.hip", "geom_host.hip", "nn.hip"]
 ARCH = "gfx950"
 
 

@@ -28,6 +28,12 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
 
+class NNGrid(C.Structure):
+    """Mirror of ``vlbm_nn_grid`` in include/vlbm_nn.h."""
+
+    _fields_ = [("batch", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32)]
+
+
 class Vlbm3dDesc(C.Structure):
     """Mirror of ``vlbm3d_desc`` in include/vlbm3d.h (field order matters)."""
 
@@ -238,6 +244,12 @@ EXPORTED = (
     "vlbm_geom3d_char_weights",
     "vlbm_geom2d_lsq_weights",
     "vlbm_geom3d_tangent_weights",
+    "vlbm_nn_conv_forward",
+    "vlbm_nn_conv_backward_data",
+    "vlbm_nn_conv_backward_weight",
+    "vlbm_nn_loss",
+    "vlbm_nn_pack_weights",
+    "vlbm_nn_adam",
     "vlbm_last_error",
     "vlbm_abi_version",
 )
@@ -306,6 +318,12 @@ def load_library(path: Path | None = None) -> C.CDLL:
         + [C.c_double, C.c_int32, C.c_int32] + [vp] * 7,
         "vlbm_geom3d_tangent_weights": [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int64] + [vp] * 9
         + [C.c_double] * 3 + [C.c_int32, C.c_int32] + [vp] * 4,
+        "vlbm_nn_conv_forward": [vp, vp, vp, vp, C.c_int32, vp, vp, vp, vp],
+        "vlbm_nn_conv_backward_data": [vp, vp, vp, vp, vp, vp],
+        "vlbm_nn_conv_backward_weight": [vp, vp, vp, vp, C.c_int32, vp, vp, vp],
+        "vlbm_nn_loss": [vp, vp, vp, vp, C.c_int32, C.c_float, vp, vp, C.c_int32, vp, vp],
+        "vlbm_nn_pack_weights": [vp, vp, vp, vp],
+        "vlbm_nn_adam": [C.c_int32, vp, vp, vp, vp] + [C.c_float] * 6 + [vp],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)

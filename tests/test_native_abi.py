@@ -1,6 +1,7 @@
 """The C ABI library loads and exports every entry point of
-include/vlbm3d.h, include/vlbm2d.h and include/vlbm_geom.h; descriptor
-layouts agree with the library.  No GPU work."""
+include/vlbm3d.h, include/vlbm2d.h, include/vlbm_geom.h and include/vlbm_nn.h;
+descriptor layouts agree with the library; the network entry points reject
+bad arguments before any launch.  No GPU work."""
 
 import ctypes as C
 import re
@@ -14,11 +15,12 @@ INCLUDE = Path(__file__).resolve().parents[1] / "include"
 HEADER = INCLUDE / "vlbm3d.h"
 HEADER2 = INCLUDE / "vlbm2d.h"
 HEADER_GEOM = INCLUDE / "vlbm_geom.h"
+HEADER_NN = INCLUDE / "vlbm_nn.h"
 
 
 def declared_functions():
     names = set()
-    for h in (HEADER, HEADER2, HEADER_GEOM):
+    for h in (HEADER, HEADER2, HEADER_GEOM, HEADER_NN):
         names |= set(re.findall(r"^\s*(?:int|const char \*)\s*(vlbm\w*)\s*\(", h.read_text(), flags=re.M))
     return sorted(names)
 
@@ -120,3 +122,21 @@ def test_library_binds_to_the_hip_runtime_torch_loads():
             "libs = {l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l}\n"
             "assert len(libs) == 1, libs\n" % str(_native.Path(_native.__file__).resolve().parents[1]))
     subprocess.run([sys.executable, "-c", code], check=True, timeout=300)
+
+
+def test_nn_entry_points_reject_bad_arguments():
+    lib = _native.load_library()
+    g = _native.NNGrid(2, 4, 4, 4)
+    bad = _native.NNGrid(0, 4, 4, 4)
+    assert lib.vlbm_nn_conv_forward(C.byref(bad), 1, 1, 1, 0, 1, 1, None, None) == -1
+    assert b"positive" in lib.vlbm_last_error()
+    assert lib.vlbm_nn_conv_forward(C.byref(g), None, 1, 1, 0, 1, 1, None, None) == -1
+    assert lib.vlbm_nn_conv_forward(C.byref(g), 1, 1, 1, 7, 1, 1, None, None) == -1
+    assert b"mode" in lib.vlbm_last_error()
+    assert lib.vlbm_nn_conv_forward(C.byref(g), 1, 1, 1, 1, None, None, None, None) == -1
+    assert lib.vlbm_nn_conv_backward_weight(C.byref(g), 1, 1, 1, 0, 1, 1, None) == -1
+    assert b"n_partials" in lib.vlbm_last_error()
+    assert lib.vlbm_nn_loss(C.byref(g), 1, 1, 1, 16, 1.0, 1, 1, 1, 1, None) == -1
+    assert b"n_out" in lib.vlbm_last_error()
+    assert lib.vlbm_nn_adam(-1, None, None, None, None, 0, 0, 0, 0, 0, 0, None) == -1
+    assert lib.vlbm_nn_adam(0, None, None, None, None, 0, 0, 0, 0, 0, 0, None) == 0
