@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -979,6 +980,7 @@ struct vlbm3d_handle {
     LinkTables T;
     Box3 box{};  // faces of the rectangular box solver (d.box != 0)
     hipStream_t stream;
+    int device = 0;  // device current at create
     Timing timing;
     // Library-owned tables built at create: the compacted anchor list of
     // the displacement sweep and the second-neighbour table.
@@ -1265,6 +1267,41 @@ static int run_steps(vlbm3d_handle *h, T *f0, T *f1, double time, int64_t steps,
     return 0;
 }
 
+// Device scratch of the allocation-free reduction (vlbm3d_reduce_noalloc):
+// the per-block partial rows of k_reduce_partial, at most RED_BLOCKS rows of
+// 8 doubles (64 KiB).  Each GPU the library runs on holds its own copy until
+// the process ends.  Host code never passes `scratch` itself to a kernel (on
+// the host it is only a stand-in symbol); it uses the device address looked
+// up once per device below.
+constexpr int SCRATCH_ROWS = RED_BLOCKS;
+static __device__ __align__(256) double scratch[SCRATCH_ROWS * 8];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of `scratch` on `dev`, looked up on the first call for that
+// device with the device current (the caller's current device is restored).
+int scratch_address(int dev, double **out) {
+    if (dev < 0 || dev >= MAX_DEVICES) return fail_msg("vlbm3d_reduce_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        int cur = 0;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return fail("vlbm3d_reduce_noalloc: get device", e);
+        if (cur != dev && (e = hipSetDevice(dev)) != hipSuccess) return fail("vlbm3d_reduce_noalloc: set device", e);
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return fail("vlbm3d_reduce_noalloc: scratch address", e);
+        g_scratch_ptr[dev] = (double *)ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+}  // namespace
+
 extern "C" {
 
 const char *vlbm_last_error(void) { return vlbm_common::last_error().c_str(); }
@@ -1332,6 +1369,7 @@ int vlbm3d_create(const vlbm3d_desc *desc, vlbm3d_handle **out) {
         h->box.len[a] = desc->box_length[a];
     }
     h->stream = (hipStream_t)desc->stream;
+    if (hipGetDevice(&h->device) != hipSuccess) h->device = 0;
     const int rc = build_anchor_tables(h);
     if (rc) {
         h->release();
@@ -1446,6 +1484,26 @@ int vlbm3d_reduce(vlbm3d_handle *h, const double *u_prev, double *out) {
     if (e != hipSuccess) return fail("vlbm3d_reduce: copy", e);
     e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return fail("vlbm3d_reduce: sync", e);
+    return 0;
+}
+
+int vlbm3d_reduce_noalloc(vlbm3d_handle *h, const double *u_prev, double *out_dev) {
+    if (!h || !out_dev) return fail_msg("vlbm3d_reduce_noalloc: null argument");
+    const vlbm3d_desc &d = h->d;
+    int nb = (d.n_active + BLOCK - 1) / BLOCK;
+    if (nb > RED_BLOCKS) nb = RED_BLOCKS;  // k_reduce_partial strides over the rest
+    if (nb > SCRATCH_ROWS)
+        return fail_msg("vlbm3d_reduce_noalloc: " + std::to_string(nb) + " partial rows exceed the scratch (" +
+                        std::to_string(SCRATCH_ROWS) + ")");
+    double *part = nullptr;
+    const int rc = scratch_address(h->device, &part);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_reduce_partial, dim3(nb), dim3(BLOCK), 0, h->stream, (const double *)d.U,
+                       (const double *)d.u, u_prev, part, d.n_active);
+    CHECK_LAUNCH("k_reduce_partial");
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(BLOCK), 0, h->stream, part, nb, (const int32_t *)d.invalid,
+                       out_dev);
+    CHECK_LAUNCH("k_reduce_final");
     return 0;
 }
 

@@ -219,6 +219,7 @@ EXPORTED = (
     "vlbm3d_init_populations",
     "vlbm3d_set_ustar",
     "vlbm3d_reduce",
+    "vlbm3d_reduce_noalloc",
     "vlbm3d_populations_to_f64",
     "vlbm3d_populations_from_f64",
     "vlbm3d_timing_enable",
@@ -290,6 +291,7 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vlbm3d_init_populations": [vp, vp],
         "vlbm3d_set_ustar": [vp, vp],
         "vlbm3d_reduce": [vp, vp, _dp],
+        "vlbm3d_reduce_noalloc": [vp, vp, vp],
         "vlbm3d_populations_to_f64": [vp, vp, vp],
         "vlbm3d_populations_from_f64": [vp, vp, vp],
         "vlbm3d_timing_enable": [vp, C.c_int32],

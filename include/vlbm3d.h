@@ -150,6 +150,16 @@ int vlbm3d_set_ustar(vlbm3d_handle *h, const double *disp);
  * u_prev (nullable, (3,n) device) is an earlier copy of u for the
  * displacement-rate equilibrium residual. */
 int vlbm3d_reduce(vlbm3d_handle *h, const double *u_prev, double *out);
+/* The reductions of vlbm3d_reduce with no allocation, copy or wait: no
+ * counterpart in the reference; for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops).  The 8 results (VLBM_RED_*) are
+ * written to the device array out_dev on the solver stream and are valid once
+ * that stream has completed.  The per-block partials live in a static device
+ * array of 1024 rows x 8 doubles (64 KiB per GPU, held until the process
+ * ends); the partial grid is capped at 1024 blocks, so every lattice fits.
+ * All calls on one GPU share that array: calls that overlap in time on one
+ * GPU (different streams) race. */
+int vlbm3d_reduce_noalloc(vlbm3d_handle *h, const double *u_prev, double *out_dev);
 /* Converts populations between the stored precision and double planes. */
 int vlbm3d_populations_to_f64(vlbm3d_handle *h, const void *f, double *dst);
 int vlbm3d_populations_from_f64(vlbm3d_handle *h, const double *src, void *f);
