@@ -5,10 +5,10 @@ leg, never by the product package.
 
 ``OracleSolver2D`` takes the constructor arguments of
 ``curved_lbm.two_d.curved_boundary_warp.WarpCurvedBoundaryLBM2D`` and a
-``CurvedBoundaryGeometry`` (whose cut links and normal-probe / LSQ weights
-come from the product's host geometry code, as in the reference where the
-same Python builder feeds the kernels), then advances the dense reference
-state with the C restatement of the reference step.  State arrays are numpy
+``CurvedBoundaryGeometry`` (its cut links), rebuilds the normal-probe / LSQ
+weights with its own NumPy restatement of the reference's builder
+(oracle/stencils2d.py) rather than taking the product's, then advances the
+dense reference state with the C restatement of the reference step.  State arrays are numpy
 arrays in the reference layout (components, nx, ny).
 """
 
@@ -20,6 +20,7 @@ from pathlib import Path
 import numpy as np
 
 from . import oracle3d as _o3
+from . import stencils2d
 
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "build" / "libvlbm2d_oracle.so"
@@ -155,12 +156,22 @@ class OracleSolver2D:
         L = g.n_links
         i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)  # noqa: E731
         f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+        if L:
+            xs = (np.arange(self.nx) + 0.5) * self.dx
+            ys = (np.arange(self.ny) + 0.5) * self.dy
+            lsq = stencils2d.boundary_lsq_weights(
+                active=i32(g.active), x_coords=xs, y_coords=ys, link_i=g.link_i, link_j=g.link_j,
+                link_xb=g.link_xb, link_yb=g.link_yb, link_nx=g.link_nx, link_ny=g.link_ny, dx=self.dx,
+                max_stencil=np.asarray(g.lsq_i).shape[1])
+        else:
+            lsq = (g.lsq_i, g.lsq_j, g.lsq_wb, g.lsq_wt, g.lsq_wx, g.lsq_wy, g.lsq_valid)
+        self.lsq = dict(zip(("lsq_i", "lsq_j", "lsq_wb", "lsq_wt", "lsq_wx", "lsq_wy", "lsq_valid"), lsq))
         self._tabs = dict(
             active=i32(g.active), link_i=i32(g.link_i), link_j=i32(g.link_j), link_q=i32(g.link_q),
             link_bid=i32(g.link_boundary_id), link_eta=f64(g.link_eta), link_xb=f64(g.link_xb),
             link_yb=f64(g.link_yb), link_nx=f64(g.link_nx), link_ny=f64(g.link_ny),
-            lsq_i=i32(g.lsq_i), lsq_j=i32(g.lsq_j), lsq_wb=f64(g.lsq_wb), lsq_wt=f64(g.lsq_wt),
-            lsq_wx=f64(g.lsq_wx), lsq_wy=f64(g.lsq_wy), lsq_valid=i32(g.lsq_valid),
+            lsq_i=i32(lsq[0]), lsq_j=i32(lsq[1]), lsq_wb=f64(lsq[2]), lsq_wt=f64(lsq[3]),
+            lsq_wx=f64(lsq[4]), lsq_wy=f64(lsq[5]), lsq_valid=i32(lsq[6]),
             kind_by_id=i32(kind), mode_by_id=i32(mode), params_by_id=f64(params),
         )
         self.dyn_tx = np.zeros(max(L, 1))

@@ -20,6 +20,8 @@ from pathlib import Path
 
 import numpy as np
 
+from . import stencils3d
+
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "build" / "libvlbm3d_oracle.so"
 
@@ -164,7 +166,7 @@ class OracleSolver3D:
         neu_rows = ck[:, lids] == cb.BC_NEUMANN
         self.characteristic_stencil = characteristic_stencil
         if characteristic_stencil is None and self.recon == cb.BOUNDARY_RECON_LOCAL_F and L and np.any(neu_rows):
-            self.characteristic_stencil = cb.build_characteristic_weights(
+            self.characteristic_stencil = stencils3d.characteristic_weights(
                 geometry, tables, link_mask=np.any(neu_rows, axis=0))
         i32, f64 = np.int32, np.float64
         a = self._arrays = {}
@@ -202,7 +204,7 @@ class OracleSolver3D:
         self.tangent_lsq = None
         if self.recon == cb.BOUNDARY_RECON_COMPAT_BFL:
             if tangent_lsq is None:
-                tangent_lsq = cb.build_tangent_lsq_weights(geometry, tables, builder="numpy")
+                tangent_lsq = stencils3d.tangent_weights(geometry, tables)
             self.tangent_lsq = tangent_lsq
             for k in ("lsq_p", "lsq_wt1", "lsq_wt2"):
                 if np.asarray(tangent_lsq[k]).shape != (L, 32):

@@ -1,6 +1,7 @@
-"""The threaded C++ geometry builders of libvlbm (include/vlbm_geom.h)
-against the NumPy restatements of the reference's builders
-(characteristic_stencil3d.py:19-171; curved_boundary_warp.py:332-606).
+"""The threaded C++ geometry builders of libvlbm (include/vlbm_geom.h),
+the only builders of the product, against the oracle's NumPy restatements of
+the reference's builders (oracle/stencils3d.py: characteristic_stencil3d.py:19-171
+and cbw3d:3725-3849; oracle/stencils2d.py: curved_boundary_warp.py:332-606).
 
 Bar: sample indices, slots, probe modes and validity flags identical; value
 and predictor weights bitwise equal (the same unfused expressions); the
@@ -13,6 +14,7 @@ import pytest
 
 import gpu_cases
 from curved_lbm.three_d import characteristic_stencil3d as cs3
+from oracle import stencils2d, stencils3d
 from curved_lbm.three_d import curved_boundary_warp3d as cb3
 from curved_lbm.two_d import curved_boundary_warp as cb
 from curved_lbm.two_d import shapes
@@ -29,7 +31,7 @@ def test_native_characteristic_weights_match_numpy(shell):
     ri, ro, n = shell
     g = gpu_cases.shell_geometry(n, ri, ro)
     tables = cb3._sparse_tables_from_geometry(g)
-    ref = cs3.build_characteristic_weights(g, tables, builder="numpy")
+    ref = stencils3d.characteristic_weights(g, tables)
     got = cs3.build_characteristic_weights(g, tables, builder="native")
     np.testing.assert_array_equal(got["indices"], ref["indices"])
     np.testing.assert_array_equal(got["probe_modes"], ref["probe_modes"])
@@ -48,7 +50,7 @@ def test_native_characteristic_weights_with_link_mask():
     g = gpu_cases.shell_geometry(16, 0.25, 0.45)
     tables = cb3._sparse_tables_from_geometry(g)
     mask = np.asarray(g.link_q) % 2 == 0
-    ref = cs3.build_characteristic_weights(g, tables, link_mask=mask, builder="numpy")
+    ref = stencils3d.characteristic_weights(g, tables, link_mask=mask)
     got = cs3.build_characteristic_weights(g, tables, link_mask=mask, builder="native")
     np.testing.assert_array_equal(got["indices"], ref["indices"])
     assert np.all(got["indices"][:, ~mask] == -1)
@@ -60,9 +62,10 @@ def test_native_characteristic_weights_reject_like_numpy(shell):
     g = gpu_cases.shell_geometry(shell[2], shell[0], shell[1])
     tables = cb3._sparse_tables_from_geometry(g)
     msgs = []
-    for builder in ("numpy", "native"):
+    for build in (lambda: stencils3d.characteristic_weights(g, tables),
+                  lambda: cs3.build_characteristic_weights(g, tables)):
         with pytest.raises(ValueError) as err:
-            cs3.build_characteristic_weights(g, tables, builder=builder)
+            build()
         msgs.append(str(err.value))
     assert msgs[0] == msgs[1]
 
@@ -92,9 +95,11 @@ def test_native_gradient_weights_are_exact_for_quadratics():
 def _build2d(g, builder):
     x = (np.arange(g.nx) + 0.5) * g.dx
     y = (np.arange(g.ny) + 0.5) * g.dy
-    return cb._build_boundary_lsq_weights(active=g.active.astype(np.int32), x_coords=x, y_coords=y,
-                                          link_i=g.link_i, link_j=g.link_j, link_xb=g.link_xb, link_yb=g.link_yb,
-                                          link_nx=g.link_nx, link_ny=g.link_ny, dx=g.dx, builder=builder)
+    kw = dict(active=g.active.astype(np.int32), x_coords=x, y_coords=y, link_i=g.link_i, link_j=g.link_j,
+              link_xb=g.link_xb, link_yb=g.link_yb, link_nx=g.link_nx, link_ny=g.link_ny, dx=g.dx)
+    if builder == "numpy":
+        return stencils2d.boundary_lsq_weights(**kw)
+    return cb._build_boundary_lsq_weights(**kw, builder=builder)
 
 
 CASES2D = {
@@ -145,7 +150,7 @@ def test_unknown_builder_is_rejected():
 def test_native_tangent_weights_match_numpy(shell):
     g = gpu_cases.shell_geometry(shell[2], shell[0], shell[1])
     tables = cb3._sparse_tables_from_geometry(g)
-    ref = cb3.build_tangent_lsq_weights(g, tables, builder="numpy")
+    ref = stencils3d.tangent_weights(g, tables)
     got = cb3.build_tangent_lsq_weights(g, tables, builder="native")
     np.testing.assert_array_equal(got["lsq_p"], ref["lsq_p"])
     np.testing.assert_array_equal(got["lsq_valid"], ref["lsq_valid"])
