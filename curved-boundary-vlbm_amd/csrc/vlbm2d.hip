@@ -22,6 +22,7 @@
 
 #include <cmath>
 #include <string>
+#include <vector>
 
 #include "../../include/vlbm2d.h"
 #include "vlbm2d_device.h"
@@ -150,12 +151,11 @@ __global__ void k2_advance_clock(double *clock, double dt) { clock[0] = clock[0]
 // u* = u + dt/2 v; J flag.  cbw2d:1692-1790 and :3128-3195.
 // With gout (mrt_ghost) also the ghost moment m3 of every component.
 template <typename T, bool COLLIDE>
-__global__ __launch_bounds__(BLOCK) void k2_moments(const T *__restrict__ f, double *__restrict__ U,
+__device__ __forceinline__ void k2_moments_body(const T *__restrict__ f, double *__restrict__ U,
                                                     double *__restrict__ u, double *__restrict__ us,
                                                     int32_t *__restrict__ invalid,
                                                     const int32_t *__restrict__ list, double *__restrict__ gout,
-                                                    Params2 P) {
-    const int t = xcd_block() * BLOCK + threadIdx.x;
+                                                    Params2 P, int t) {
     if (t >= P.na) return;
     const int p = list[t];
     const size_t N = P.N;
@@ -192,6 +192,14 @@ __global__ __launch_bounds__(BLOCK) void k2_moments(const T *__restrict__ f, dou
     }
     if (Um[2] * Um[5] - Um[3] * Um[4] <= P.m.jf) atomicOr(invalid, 1);
 }
+template <typename T, bool COLLIDE>
+__global__ __launch_bounds__(BLOCK) void k2_moments(const T *__restrict__ f, double *__restrict__ U,
+                                                    double *__restrict__ u, double *__restrict__ us,
+                                                    int32_t *__restrict__ invalid,
+                                                    const int32_t *__restrict__ list, double *__restrict__ gout,
+                                                    Params2 P) {
+    k2_moments_body<T, COLLIDE>(f, U, u, us, invalid, list, gout, P, xcd_block() * BLOCK + threadIdx.x);
+}
 
 // Per active node, which lattice neighbours along the axes are on the
 // lattice and active: first (+-x, +-y) and second (+-2x, +-2y) neighbours,
@@ -221,12 +229,11 @@ __global__ __launch_bounds__(BLOCK) void k2_nbmask(const int32_t *__restrict__ l
 // One relaxed nearest-neighbour compatibility sweep of u (cbw2d:3421-3490)
 // with the Dirichlet cut-point anchors of the node's own links
 // (cbw2d:3288-3417) summed in link order, so the sums are deterministic.
-__global__ __launch_bounds__(BLOCK) void k2_sweep(const double *__restrict__ U, const double *__restrict__ u,
+__device__ __forceinline__ void k2_sweep_body(const double *__restrict__ U, const double *__restrict__ u,
                                                   double *__restrict__ un, const int32_t *__restrict__ list,
                                                   const uint8_t *__restrict__ nbm, const int32_t *__restrict__ aoff,
                                                   const int32_t *__restrict__ aids, Links2 T, double time,
-                                                  Params2 P) {
-    const int t = xcd_block() * BLOCK + threadIdx.x;
+                                                  Params2 P, int t) {
     if (t >= P.na) return;
     const int p = list[t];
     const int ny = P.ny;
@@ -304,11 +311,17 @@ __global__ __launch_bounds__(BLOCK) void k2_sweep(const double *__restrict__ U, 
         un[N + p] = a1;
     }
 }
+__global__ __launch_bounds__(BLOCK) void k2_sweep(const double *__restrict__ U, const double *__restrict__ u,
+                                                  double *__restrict__ un, const int32_t *__restrict__ list,
+                                                  const uint8_t *__restrict__ nbm, const int32_t *__restrict__ aoff,
+                                                  const int32_t *__restrict__ aids, Links2 T, double time,
+                                                  Params2 P) {
+    k2_sweep_body(U, u, un, list, nbm, aoff, aids, T, time, P, xcd_block() * BLOCK + threadIdx.x);
+}
 
-__global__ __launch_bounds__(BLOCK) void k2_commit(const double *__restrict__ un, double *__restrict__ u,
+__device__ __forceinline__ void k2_commit_body(const double *__restrict__ un, double *__restrict__ u,
                                                    double *__restrict__ us, const int32_t *__restrict__ list,
-                                                   int na, size_t N) {
-    const int t = xcd_block() * BLOCK + threadIdx.x;
+                                                   int na, size_t N, int t) {
     if (t >= na) return;
     const int p = list[t];
 #pragma unroll
@@ -318,6 +331,11 @@ __global__ __launch_bounds__(BLOCK) void k2_commit(const double *__restrict__ un
         u[k] = un[k];
         us[k] = us[k] + delta;
     }
+}
+__global__ __launch_bounds__(BLOCK) void k2_commit(const double *__restrict__ un, double *__restrict__ u,
+                                                   double *__restrict__ us, const int32_t *__restrict__ list,
+                                                   int na, size_t N) {
+    k2_commit_body(un, u, us, list, na, N, xcd_block() * BLOCK + threadIdx.x);
 }
 
 // Centred / second-order one-sided / first-order one-sided derivative of one
@@ -346,12 +364,11 @@ __device__ __forceinline__ double du2(const double *uc, int nb, int p, int axis,
 // commit.  ug is the displacement the repair differentiates (the sweep's
 // u_next when a sweep ran, else u).
 template <typename R>
-__global__ __launch_bounds__(BLOCK) void k2_record(double *__restrict__ U, const double *__restrict__ ug,
+__device__ __forceinline__ void k2_record_body(double *__restrict__ U, const double *__restrict__ ug,
                                                    double *__restrict__ u, double *__restrict__ us,
                                                    R *__restrict__ rec, const int32_t *__restrict__ list,
                                                    const uint8_t *__restrict__ nbm, int repair, int commit,
-                                                   int32_t *__restrict__ reset_invalid, Params2 P) {
-    const int t = xcd_block() * BLOCK + threadIdx.x;
+                                                   int32_t *__restrict__ reset_invalid, Params2 P, int t) {
     if (t >= P.na) return;
     // Clears the invalid flag ahead of the stream kernel's fused collide
     // (the link rows never set it).
@@ -414,6 +431,14 @@ __global__ __launch_bounds__(BLOCK) void k2_record(double *__restrict__ U, const
     rst_b(rec, (R_BF + 0) * N + p, -w2c * v0);
     rst_b(rec, (R_BF + 1) * N + p, -w2c * v1);
 }
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k2_record(double *__restrict__ U, const double *__restrict__ ug,
+                                                   double *__restrict__ u, double *__restrict__ us,
+                                                   R *__restrict__ rec, const int32_t *__restrict__ list,
+                                                   const uint8_t *__restrict__ nbm, int repair, int commit,
+                                                   int32_t *__restrict__ reset_invalid, Params2 P) {
+    k2_record_body<R>(U, ug, u, us, rec, list, nbm, repair, commit, reset_invalid, P, xcd_block() * BLOCK + threadIdx.x);
+}
 
 // Quadratic one-sided reconstruction of U[comp] along the inward link
 // coordinate (cbw2d:1443-1480).
@@ -445,16 +470,16 @@ __device__ inline double interp_link(const double *U, const int32_t *act, size_t
 // `tangent` rows 16-17 the tangential derivative of u and row 18 the sum of
 // |w_t| (compat_bfl, cbw2d:2221-2230).
 constexpr int GL = 8;  // links per block of 256 threads
-__global__ __launch_bounds__(256) void k2_gather(const double *__restrict__ U, const double *__restrict__ u,
+__device__ __forceinline__ void k2_gather_body(const double *__restrict__ U, const double *__restrict__ u,
                                                  const int32_t *__restrict__ lsq_p, const double *__restrict__ wb,
                                                  const double *__restrict__ wx, const double *__restrict__ wy,
                                                  const double *__restrict__ wt, int tangent, double *__restrict__ cs,
-                                                 int L, size_t N) {
+                                                 int L, size_t N, int blk) {
     __shared__ double sU[8][GL * 32];
     __shared__ double sW[4][GL * 32];
     const int t = threadIdx.x;
     const int ll = t >> 5, m = t & 31;
-    const int ell = blockIdx.x * GL + ll;
+    const int ell = blk * GL + ll;
     double u8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, w4[4] = {0.0, 0.0, 0.0, 0.0};
     if (ell < L && m < MAX_LSQ) {
         const int pp = lsq_p[(size_t)m * L + ell];
@@ -479,7 +504,7 @@ __global__ __launch_bounds__(256) void k2_gather(const double *__restrict__ U, c
     const int nout = tangent ? 19 : 16;
     if (t >= GL * nout) return;
     const int l2 = t / nout, v = t - l2 * nout;
-    const int e2 = blockIdx.x * GL + l2;
+    const int e2 = blk * GL + l2;
     if (e2 >= L) return;
     double acc = 0.0;
     if (v < 16) {
@@ -493,6 +518,13 @@ __global__ __launch_bounds__(256) void k2_gather(const double *__restrict__ U, c
     }
     cs[(size_t)v * L + e2] = acc;
 }
+__global__ __launch_bounds__(256) void k2_gather(const double *__restrict__ U, const double *__restrict__ u,
+                                                 const int32_t *__restrict__ lsq_p, const double *__restrict__ wb,
+                                                 const double *__restrict__ wx, const double *__restrict__ wy,
+                                                 const double *__restrict__ wt, int tangent, double *__restrict__ cs,
+                                                 int L, size_t N) {
+    k2_gather_body(U, u, lsq_p, wb, wx, wy, wt, tangent, cs, L, N, blockIdx.x);
+}
 
 // Incoming populations of every cut link (cbw2d:2064-3098).  use_rec
 // selects post-collision sources (time step: pre-collision f plus this
@@ -501,11 +533,10 @@ __global__ __launch_bounds__(256) void k2_gather(const double *__restrict__ U, c
 // regularized, NEE, quadratic and polynomial variants (recon_ext), so the
 // kernel of the other reconstructions carries none of their code.
 template <typename T, bool EXT>
-__global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T *__restrict__ fdst,
+__device__ __forceinline__ void k2_links_body(const T *__restrict__ fsrc, T *__restrict__ fdst,
                                                    const T *__restrict__ rec, const double *__restrict__ U,
                                                    const double *__restrict__ cs, Links2 Tb, int use_rec,
-                                                   const double *__restrict__ gh, double time, Params2 P) {
-    const int ell = blockIdx.x * LBLOCK + threadIdx.x;
+                                                   const double *__restrict__ gh, double time, Params2 P, int ell) {
     if (ell >= P.L) return;
     const int r = P.recon;
     const size_t N = P.N;
@@ -779,6 +810,13 @@ __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T
         st(fdst, (size_t)(qi * 6 + a) * N + p, a, val);
     }
 }
+template <typename T, bool EXT>
+__global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T *__restrict__ fdst,
+                                                   const T *__restrict__ rec, const double *__restrict__ U,
+                                                   const double *__restrict__ cs, Links2 Tb, int use_rec,
+                                                   const double *__restrict__ gh, double time, Params2 P) {
+    k2_links_body<T, EXT>(fsrc, fdst, rec, U, cs, Tb, use_rec, gh, time, P, blockIdx.x * LBLOCK + threadIdx.x);
+}
 
 // Index along a periodic axis of n nodes (one step off either end wraps).
 __device__ __forceinline__ int wrap_i(int i, int n, int periodic) {
@@ -910,14 +948,13 @@ __global__ __launch_bounds__(BLOCK) void k2_upmask(const int32_t *__restrict__ l
 // four populations of the node, cut-link ones included, before they are
 // stored and summed.
 template <typename T, bool NEXT, int MODE>
-__global__ __launch_bounds__(BLOCK) void k2_stream(const T *__restrict__ f0, T *__restrict__ f1,
+__device__ __forceinline__ void k2_stream_body(const T *__restrict__ f0, T *__restrict__ f1,
                                                    const T *__restrict__ rec, const int32_t *__restrict__ list,
                                                    const uint8_t *__restrict__ upm, double *__restrict__ U,
                                                    double *__restrict__ u, double *__restrict__ us,
                                                    int32_t *__restrict__ invalid, const double *__restrict__ gin,
-                                                   double *__restrict__ gout, Params2 P) {
+                                                   double *__restrict__ gout, Params2 P, int t) {
     constexpr bool MRT = (MODE & 1) != 0, FILT = (MODE & 2) != 0;
-    const int t = xcd_block() * BLOCK + threadIdx.x;
     if (t >= P.na) return;
     const int p = list[t];
     const int up = upm[t];
@@ -1014,6 +1051,15 @@ __global__ __launch_bounds__(BLOCK) void k2_stream(const T *__restrict__ f0, T *
         }
         if (Um[2] * Um[5] - Um[3] * Um[4] <= P.m.jf) atomicOr(invalid, 1);
     }
+}
+template <typename T, bool NEXT, int MODE>
+__global__ __launch_bounds__(BLOCK) void k2_stream(const T *__restrict__ f0, T *__restrict__ f1,
+                                                   const T *__restrict__ rec, const int32_t *__restrict__ list,
+                                                   const uint8_t *__restrict__ upm, double *__restrict__ U,
+                                                   double *__restrict__ u, double *__restrict__ us,
+                                                   int32_t *__restrict__ invalid, const double *__restrict__ gin,
+                                                   double *__restrict__ gout, Params2 P) {
+    k2_stream_body<T, NEXT, MODE>(f0, f1, rec, list, upm, U, u, us, invalid, gin, gout, P, xcd_block() * BLOCK + threadIdx.x);
 }
 
 // f = feq(U) - dt/(2Q) B(U) at active nodes (cbw2d:3873-3876).
@@ -1555,6 +1601,370 @@ static int run_steps2(vlbm2d_handle *h, T *f0, T *f1, double time, int64_t steps
     return 0;
 }
 
+namespace {
+// ---- batches: many lattices, one launch per kernel per step ---------------
+//
+// Every shell of a batch keeps its own handle, buffers and parameters; a
+// batch step issues each kernel once over the work items of all active
+// shells.  Block b of a batched launch (after the XCD remap) belongs to the
+// shell j with start[j] <= b < start[j + 1] of that kernel's block table and
+// works on the shell's local block b - start[j], running the same per-node /
+// per-link code as a single-lattice launch, so every shell's results are
+// bitwise those of stepping it alone.  The boundary time of each shell lives
+// in its own device clock (k2b_advance_clocks adds the shell's dt after every
+// step, the operation the host performs).
+struct Job2 {
+    Params2 P;  // P.clock -> the shell's clock
+    Links2 T;
+    double *U, *u, *u_star, *u_next;
+    void *record;
+    int32_t *invalid;
+    const int32_t *list, *aoff, *aids;
+    const uint8_t *nbm, *upm;
+    const int32_t *lsq_p;
+    const double *wb, *wx, *wy, *wt;
+    double *link_state;
+    void *f[2];  // populations: step parity k reads f[k & 1] and writes f[(k + 1) & 1]
+    int disp, compat, tangent;
+};
+
+enum { BK_NODE = 0, BK_SWEEP, BK_GATHER, BK_LINK, BK_COUNT };
+
+__device__ __forceinline__ int find_job(const int *start, int njobs, int b) {
+    int lo = 0, hi = njobs;  // start[lo] <= b < start[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (start[mid] <= b)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+template <typename T, int NB>
+__global__ __launch_bounds__(NB) void k2b_moments(const Job2 *__restrict__ jobs, const int *__restrict__ start,
+                                                  int njobs, int parity) {
+    const int b = xcd_block();
+    const int j = find_job(start, njobs, b);
+    const Job2 &J = jobs[j];
+    k2_moments_body<T, true>((const T *)J.f[parity], J.U, J.u, J.u_star, J.invalid, J.list, nullptr, J.P,
+                             (b - start[j]) * NB + threadIdx.x);
+}
+
+template <int NB>
+__global__ __launch_bounds__(NB) void k2b_sweep(const Job2 *__restrict__ jobs, const int *__restrict__ start,
+                                                int njobs) {
+    const int b = xcd_block();
+    const int j = find_job(start, njobs, b);
+    const Job2 &J = jobs[j];
+    if (!J.disp) return;  // padding blocks past the table land on the last shell
+    k2_sweep_body(J.U, J.u, J.u_next, J.list, J.nbm, J.aoff, J.aids, J.T, 0.0, J.P, (b - start[j]) * NB + threadIdx.x);
+}
+
+template <typename T, int NB>
+__global__ __launch_bounds__(NB) void k2b_record(const Job2 *__restrict__ jobs, const int *__restrict__ start,
+                                                 int njobs, int next) {
+    const int b = xcd_block();
+    const int j = find_job(start, njobs, b);
+    const Job2 &J = jobs[j];
+    k2_record_body<T>(J.U, J.disp ? J.u_next : J.u, J.u, J.u_star, (T *)J.record, J.list, J.nbm, J.compat, J.disp,
+                      next ? J.invalid : (int32_t *)nullptr, J.P, (b - start[j]) * NB + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k2b_gather(const Job2 *__restrict__ jobs, const int *__restrict__ start,
+                                                  int njobs) {
+    const int b = blockIdx.x;
+    const int j = find_job(start, njobs, b);
+    const Job2 &J = jobs[j];
+    k2_gather_body(J.U, J.u, J.lsq_p, J.wb, J.wx, J.wy, J.wt, J.tangent, J.link_state, J.P.L, J.P.N, b - start[j]);
+}
+
+template <typename T, bool EXT, int NL>
+__global__ __launch_bounds__(NL) void k2b_links(const Job2 *__restrict__ jobs, const int *__restrict__ start,
+                                                int njobs, int parity) {
+    const int b = blockIdx.x;
+    const int j = find_job(start, njobs, b);
+    const Job2 &J = jobs[j];
+    k2_links_body<T, EXT>((const T *)J.f[parity], (T *)J.f[parity ^ 1], (const T *)J.record, J.U, J.link_state, J.T,
+                          1, nullptr, 0.0, J.P, (b - start[j]) * NL + threadIdx.x);
+}
+
+template <typename T, bool NEXT, int NB>
+__global__ __launch_bounds__(NB) void k2b_stream(const Job2 *__restrict__ jobs, const int *__restrict__ start,
+                                                 int njobs, int parity) {
+    const int b = xcd_block();
+    const int j = find_job(start, njobs, b);
+    const Job2 &J = jobs[j];
+    k2_stream_body<T, NEXT, 0>((const T *)J.f[parity], (T *)J.f[parity ^ 1], (const T *)J.record, J.list, J.upm,
+                               J.U, J.u, J.u_star, J.invalid, nullptr, nullptr, J.P, (b - start[j]) * NB + threadIdx.x);
+}
+
+__global__ void k2b_reset_invalid(const Job2 *__restrict__ jobs, int njobs) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < njobs) jobs[j].invalid[0] = 0;
+}
+
+__global__ void k2b_advance_clocks(double *__restrict__ clocks, const double *__restrict__ dts, int njobs) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < njobs) clocks[j] = clocks[j] + dts[j];
+}
+
+}  // namespace
+
+struct vlbm2d_batch {
+    std::vector<vlbm2d_handle *> all;  // every shell, in creation order
+    std::vector<void *> f0, f1;        // their population buffers at parity 0
+    int precision = 0;
+    bool ext = false;
+    hipStream_t stream = nullptr;
+    // device tables of the active shells
+    Job2 *jobs = nullptr;
+    int *start = nullptr;  // BK_COUNT tables of njobs + 1 block offsets
+    double *clocks = nullptr, *dts = nullptr;
+    int njobs = 0, cap = 0;
+    int nblk[BK_COUNT] = {0, 0, 0, 0};
+    int parity = 0;  // populations are in f[parity] of every shell
+    int nb = 256, nl = 64;    // threads per block of the node kernels and of the link kernel
+    std::vector<int> active;  // indices into `all`
+    bool gather = false;
+    // captured step loops of the active set (graph_steps fused steps)
+    int graph_steps = 0;
+    hipStream_t capst = nullptr;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};  // one per starting parity
+    void drop_graphs() {
+        for (auto &g : graph) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    }
+    void release() {
+        drop_graphs();
+        if (capst) (void)hipStreamDestroy(capst);
+        for (void *ptr : {(void *)jobs, (void *)start, (void *)clocks, (void *)dts})
+            if (ptr) (void)hipFree(ptr);
+        jobs = nullptr;
+        start = nullptr;
+        clocks = dts = nullptr;
+        capst = nullptr;
+    }
+};
+
+namespace {
+
+// Builds the job and block tables of the shells listed in b->active.
+int batch_tables(vlbm2d_batch *b) {
+    const int n = (int)b->active.size();
+    if (n > b->cap) {
+        for (void *ptr : {(void *)b->jobs, (void *)b->start, (void *)b->clocks, (void *)b->dts})
+            if (ptr) (void)hipFree(ptr);
+        b->jobs = nullptr;
+        b->start = nullptr;
+        b->clocks = b->dts = nullptr;
+        hipError_t e = hipMalloc((void **)&b->jobs, sizeof(Job2) * (size_t)n);
+        if (e == hipSuccess) e = hipMalloc((void **)&b->start, sizeof(int) * (size_t)(n + 1) * BK_COUNT);
+        if (e == hipSuccess) e = hipMalloc((void **)&b->clocks, sizeof(double) * (size_t)n);
+        if (e == hipSuccess) e = hipMalloc((void **)&b->dts, sizeof(double) * (size_t)n);
+        if (e != hipSuccess) return fail("vlbm2d_batch: tables", e);
+        b->cap = n;
+    }
+    std::vector<Job2> jobs((size_t)n);
+    std::vector<int> start((size_t)(n + 1) * BK_COUNT, 0);
+    std::vector<double> dts((size_t)n);
+    int tot[BK_COUNT] = {0, 0, 0, 0};
+    b->gather = false;
+    for (int k = 0; k < n; ++k) {
+        const int s = b->active[k];
+        vlbm2d_handle *h = b->all[s];
+        const vlbm2d_desc &d = h->d;
+        Job2 &J = jobs[k];
+        J.P = h->P;
+        J.P.clock = b->clocks + k;
+        J.T = h->T;
+        J.U = d.U;
+        J.u = d.u;
+        J.u_star = d.u_star;
+        J.u_next = d.u_next;
+        J.record = d.record;
+        J.invalid = d.invalid;
+        J.list = d.active_list;
+        J.aoff = d.anchor_off;
+        J.aids = d.anchor_ids;
+        J.nbm = h->nbmask;
+        J.upm = h->upmask;
+        J.lsq_p = d.lsq_p;
+        J.wb = d.lsq_wb;
+        J.wx = d.lsq_wx;
+        J.wy = d.lsq_wy;
+        J.wt = d.lsq_wt;
+        J.link_state = d.link_state;
+        J.f[0] = b->f0[s];
+        J.f[1] = b->f1[s];
+        J.disp = d.disp_interval == 1;
+        J.compat = d.compat_interval == 1;
+        J.tangent = recon_compat(d.recon) ? 1 : 0;
+        const bool g = d.n_links > 0 && recon_gather(d.recon);
+        b->gather = b->gather || g;
+        const int NBk = b->nb, NLk = b->nl;
+        const int nb[BK_COUNT] = {(d.n_active + NBk - 1) / NBk, J.disp ? (d.n_active + NBk - 1) / NBk : 0,
+                                  g ? (d.n_links + GL - 1) / GL : 0, (d.n_links + NLk - 1) / NLk};
+        for (int t = 0; t < BK_COUNT; ++t) {
+            start[(size_t)t * (n + 1) + k] = tot[t];
+            tot[t] += nb[t];
+        }
+        dts[k] = d.dt;
+    }
+    for (int t = 0; t < BK_COUNT; ++t) {
+        start[(size_t)t * (n + 1) + n] = tot[t];
+        b->nblk[t] = tot[t];
+    }
+    b->njobs = n;
+    hipError_t e = hipMemcpyAsync(b->jobs, jobs.data(), sizeof(Job2) * (size_t)n, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->dts, dts.data(), sizeof(double) * (size_t)n, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return fail("vlbm2d_batch: upload", e);
+    b->drop_graphs();
+    return 0;
+}
+
+// Node-kernel grids rounded up to whole XCD groups; the extra blocks find
+// the last shell and their threads fall past its node count.
+inline int batch_grid(int blocks) { return ((blocks + vlbm_common::NXCD - 1) / vlbm_common::NXCD) * vlbm_common::NXCD; }
+
+template <typename T, int NB, int NL>
+int batch_step_launches_t(vlbm2d_batch *b, hipStream_t st, int parity, bool next) {
+    const int n = b->njobs;
+    const int *S = b->start;
+    // node grids are rounded up to whole XCD groups; the padding blocks map
+    // to the last shell past its node count (or, in the sweep, skip it)
+    if (b->nblk[BK_SWEEP] > 0) {
+        hipLaunchKernelGGL(k2b_sweep<NB>, dim3(batch_grid(b->nblk[BK_SWEEP])), dim3(NB), 0, st, b->jobs,
+                           S + 1 * (n + 1), n);
+        CHECK_LAUNCH("k2b_sweep");
+    }
+    hipLaunchKernelGGL((k2b_record<T, NB>), dim3(batch_grid(b->nblk[BK_NODE])), dim3(NB), 0, st, b->jobs, S, n,
+                       (int)next);
+    CHECK_LAUNCH("k2b_record");
+    if (b->gather && b->nblk[BK_GATHER] > 0) {
+        hipLaunchKernelGGL(k2b_gather, dim3(b->nblk[BK_GATHER]), dim3(256), 0, st, b->jobs, S + 2 * (n + 1), n);
+        CHECK_LAUNCH("k2b_gather");
+    }
+    if (b->nblk[BK_LINK] > 0) {
+        if (b->ext)
+            hipLaunchKernelGGL((k2b_links<T, true, NL>), dim3(b->nblk[BK_LINK]), dim3(NL), 0, st, b->jobs,
+                               S + 3 * (n + 1), n, parity);
+        else
+            hipLaunchKernelGGL((k2b_links<T, false, NL>), dim3(b->nblk[BK_LINK]), dim3(NL), 0, st, b->jobs,
+                               S + 3 * (n + 1), n, parity);
+        CHECK_LAUNCH("k2b_links");
+    }
+    if (next)
+        hipLaunchKernelGGL((k2b_stream<T, true, NB>), dim3(batch_grid(b->nblk[BK_NODE])), dim3(NB), 0, st, b->jobs, S,
+                           n, parity);
+    else
+        hipLaunchKernelGGL((k2b_stream<T, false, NB>), dim3(batch_grid(b->nblk[BK_NODE])), dim3(NB), 0, st, b->jobs,
+                           S, n, parity);
+    CHECK_LAUNCH("k2b_stream");
+    hipLaunchKernelGGL(k2b_advance_clocks, dim3((n + 255) / 256), dim3(256), 0, st, b->clocks, b->dts, n);
+    CHECK_LAUNCH("k2b_advance_clocks");
+    return 0;
+}
+
+// Block sizes the autotuner chooses from.
+constexpr int TUNE_NB[3] = {128, 256, 512};
+constexpr int TUNE_NL[3] = {32, 64, 128};
+
+template <typename T, int NB>
+int batch_step_nl(vlbm2d_batch *b, hipStream_t st, int parity, bool next) {
+    switch (b->nl) {
+    case 32: return batch_step_launches_t<T, NB, 32>(b, st, parity, next);
+    case 128: return batch_step_launches_t<T, NB, 128>(b, st, parity, next);
+    default: return batch_step_launches_t<T, NB, 64>(b, st, parity, next);
+    }
+}
+
+template <typename T>
+int batch_step_launches(vlbm2d_batch *b, hipStream_t st, int parity, bool next) {
+    switch (b->nb) {
+    case 128: return batch_step_nl<T, 128>(b, st, parity, next);
+    case 512: return batch_step_nl<T, 512>(b, st, parity, next);
+    default: return batch_step_nl<T, 256>(b, st, parity, next);
+    }
+}
+
+template <typename T>
+int batch_moments(vlbm2d_batch *b, hipStream_t st) {
+    const int n = b->njobs;
+    const dim3 g(batch_grid(b->nblk[BK_NODE]));
+    switch (b->nb) {
+    case 128: hipLaunchKernelGGL((k2b_moments<T, 128>), g, dim3(128), 0, st, b->jobs, b->start, n, b->parity); break;
+    case 512: hipLaunchKernelGGL((k2b_moments<T, 512>), g, dim3(512), 0, st, b->jobs, b->start, n, b->parity); break;
+    default: hipLaunchKernelGGL((k2b_moments<T, 256>), g, dim3(256), 0, st, b->jobs, b->start, n, b->parity); break;
+    }
+    CHECK_LAUNCH("k2b_moments");
+    return 0;
+}
+
+template <typename T>
+int batch_capture(vlbm2d_batch *b, int parity, hipGraphExec_t *out) {
+    hipError_t e;
+    if (!b->capst) {
+        e = hipStreamCreateWithFlags(&b->capst, hipStreamNonBlocking);
+        if (e != hipSuccess) return fail("vlbm2d_batch: capture stream", e);
+    }
+    e = hipStreamBeginCapture(b->capst, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return fail("vlbm2d_batch: begin capture", e);
+    int rc = 0;
+    for (int k = 0; k < b->graph_steps && rc == 0; ++k) rc = batch_step_launches<T>(b, b->capst, (parity + k) & 1, true);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(b->capst, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail("vlbm2d_batch: end capture", e);
+    e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail("vlbm2d_batch: instantiate", e);
+    return 0;
+}
+
+// nsteps steps of every active shell from its clock; as run_steps2 for one
+// lattice: the first collide, fused collides inside, the last step without.
+template <typename T>
+int batch_run(vlbm2d_batch *b, int nsteps) {
+    hipStream_t st = b->stream;
+    const int n = b->njobs;
+    if (n == 0 || nsteps <= 0) return 0;
+    hipLaunchKernelGGL(k2b_reset_invalid, dim3((n + 255) / 256), dim3(256), 0, st, b->jobs, n);
+    CHECK_LAUNCH("k2b_reset_invalid");
+    if (int rc = batch_moments<T>(b, st)) return rc;
+    int k = 0;
+    const int K = b->graph_steps;
+    if (K > 0 && nsteps - 1 >= K) {
+        hipGraphExec_t &g = b->graph[b->parity];
+        if (!g) {
+            int rc = batch_capture<T>(b, b->parity, &g);
+            if (rc) return rc;
+        }
+        for (; k + K <= nsteps - 1; k += K) {
+            hipError_t e = hipGraphLaunch(g, st);
+            if (e != hipSuccess) return fail("vlbm2d_batch: graph launch", e);
+        }
+    }
+    for (; k < nsteps; ++k) {
+        int rc = batch_step_launches<T>(b, st, (b->parity + k) & 1, k + 1 < nsteps);
+        if (rc) return rc;
+    }
+    if (nsteps & 1) b->parity ^= 1;
+    return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 int vlbm2d_create(const vlbm2d_desc *desc, vlbm2d_handle **out) {
@@ -1823,6 +2233,185 @@ int vlbm2d_populations_from_f64(vlbm2d_handle *h, const double *src, void *f) {
         hipLaunchKernelGGL(k2_from_f64, dim3(g), dim3(BLOCK), 0, h->stream, src, (float *)f, h->P.N);
     CHECK_LAUNCH("vlbm2d_populations_from_f64");
     return 0;
+}
+
+
+int vlbm2d_batch_create(vlbm2d_handle *const *handles, void *const *f0, void *const *f1, int32_t n, void *stream,
+                        vlbm2d_batch **out) {
+    if (!handles || !f0 || !f1 || !out) return fail_msg("vlbm2d_batch_create: null argument");
+    if (n < 1 || n > 65536) return fail_msg("vlbm2d_batch_create: n must be in [1, 65536]");
+    const int prec = handles[0] ? handles[0]->d.precision : 0;
+    bool ext = false;
+    for (int k = 0; k < n; ++k) {
+        const vlbm2d_handle *h = handles[k];
+        if (!h || !f0[k] || !f1[k]) return fail_msg("vlbm2d_batch_create: null handle or populations");
+        const vlbm2d_desc &d = h->d;
+        if (d.precision != prec) return fail_msg("vlbm2d_batch_create: every shell needs the same precision");
+        if (d.box) return fail_msg("vlbm2d_batch_create: box solvers are not batched");
+        if (h->P.gw != 0.0 || h->P.filt > 0.0)
+            return fail_msg("vlbm2d_batch_create: batched shells use BGK without the kinetic filter");
+        if (d.disp_interval > 1 || d.compat_interval > 1 || d.disp_sweeps != 1)
+            return fail_msg("vlbm2d_batch_create: batched shells need a uniform schedule (intervals 0 or 1, one sweep)");
+        ext = ext || recon_ext(d.recon);
+        for (int j = 0; j < k; ++j)
+            if (handles[j] == h) return fail_msg("vlbm2d_batch_create: a shell appears twice");
+    }
+    vlbm2d_batch *b = new vlbm2d_batch;
+    b->precision = prec;
+    b->ext = ext;
+    b->stream = (hipStream_t)stream;
+    for (int k = 0; k < n; ++k) {
+        vlbm2d_handle *h = handles[k];
+        b->all.push_back(h);
+        b->f0.push_back(f0[k]);
+        b->f1.push_back(f1[k]);
+        b->active.push_back(k);
+        if (!h->upmask_ok) {
+            hipLaunchKernelGGL(k2_upmask, dim3((h->d.n_active + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, b->stream,
+                               h->d.active_list, h->d.active, h->upmask, h->P);
+            hipLaunchKernelGGL(k2_nbmask, dim3((h->d.n_active + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, b->stream,
+                               h->d.active_list, h->d.active, h->nbmask, h->P);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) {
+                delete b;
+                return fail("vlbm2d_batch_create: masks", e);
+            }
+            h->upmask_ok = true;
+        }
+    }
+    for (int k = 0; k < n; ++k) {
+        hipError_t e = hipStreamSynchronize(b->all[k]->stream);
+        if (e != hipSuccess) {
+            delete b;
+            return fail("vlbm2d_batch_create: sync", e);
+        }
+    }
+    const int rc = batch_tables(b);
+    if (rc) {
+        b->release();
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return 0;
+}
+
+int vlbm2d_batch_destroy(vlbm2d_batch *b) {
+    if (!b) return 0;
+    (void)hipStreamSynchronize(b->stream);
+    b->release();
+    delete b;
+    return 0;
+}
+
+int vlbm2d_batch_select(vlbm2d_batch *b, const int32_t *ids, void *const *f0, void *const *f1, int32_t count) {
+    if (!b || (count > 0 && (!ids || !f0 || !f1))) return fail_msg("vlbm2d_batch_select: null argument");
+    if (count < 0 || count > (int)b->all.size()) return fail_msg("vlbm2d_batch_select: count out of range");
+    std::vector<int> act;
+    for (int k = 0; k < count; ++k) {
+        if (ids[k] < 0 || ids[k] >= (int)b->all.size()) return fail_msg("vlbm2d_batch_select: shell id out of range");
+        if (k > 0 && ids[k] <= ids[k - 1]) return fail_msg("vlbm2d_batch_select: ids must be strictly increasing");
+        if (!f0[k] || !f1[k]) return fail_msg("vlbm2d_batch_select: null populations");
+        act.push_back(ids[k]);
+    }
+    hipError_t e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return fail("vlbm2d_batch_select: sync", e);
+    for (int k = 0; k < count; ++k) {
+        b->f0[ids[k]] = f0[k];
+        b->f1[ids[k]] = f1[k];
+    }
+    b->active = act;
+    b->parity = 0;
+    return batch_tables(b);
+}
+
+int vlbm2d_batch_set_graph_steps(vlbm2d_batch *b, int32_t steps) {
+    if (!b) return fail_msg("vlbm2d_batch_set_graph_steps: null batch");
+    if (steps < 0 || steps > 4096) return fail_msg("vlbm2d_batch_set_graph_steps: steps must be in [0, 4096]");
+    hipError_t e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return fail("vlbm2d_batch_set_graph_steps: sync", e);
+    b->drop_graphs();
+    b->graph_steps = steps;
+    return 0;
+}
+
+int vlbm2d_batch_parity(const vlbm2d_batch *b) { return b ? b->parity : -1; }
+
+int vlbm2d_batch_step(vlbm2d_batch *b, const double *times, int32_t nsteps) {
+    if (!b || (b->njobs > 0 && !times)) return fail_msg("vlbm2d_batch_step: null argument");
+    if (nsteps < 0) return fail_msg("vlbm2d_batch_step: nsteps must be non-negative");
+    if (b->njobs == 0 || nsteps == 0) return 0;
+    hipError_t e = hipMemcpyAsync(b->clocks, times, sizeof(double) * (size_t)b->njobs, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return fail("vlbm2d_batch_step: clocks", e);
+    if (b->precision == VLBM_PREC_F64) return batch_run<double>(b, nsteps);
+    return batch_run<float>(b, nsteps);
+}
+
+int vlbm2d_batch_autotune(vlbm2d_batch *b, int32_t steps, int32_t *nb_out, int32_t *nl_out, float *ms_out) {
+    if (!b || !nb_out || !nl_out) return fail_msg("vlbm2d_batch_autotune: null argument");
+    if (steps < 2 || steps > 1000) return fail_msg("vlbm2d_batch_autotune: steps must be in [2, 1000]");
+    if (b->njobs == 0) return fail_msg("vlbm2d_batch_autotune: no shell selected");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e != hipSuccess) return fail("vlbm2d_batch_autotune: events", e);
+    std::vector<double> zeros((size_t)b->njobs, 0.0);
+    float best = -1.0f;
+    int bnb = b->nb, bnl = b->nl, rc = 0;
+    const int keep_graph = b->graph_steps;
+    b->graph_steps = 0;
+    for (int i = 0; i < 3 && rc == 0; ++i)
+        for (int k = 0; k < 3 && rc == 0; ++k) {
+            b->nb = TUNE_NB[i];
+            b->nl = TUNE_NL[k];
+            rc = batch_tables(b);
+            if (rc) break;
+            e = hipMemcpy(b->clocks, zeros.data(), sizeof(double) * zeros.size(), hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                rc = fail("vlbm2d_batch_autotune: clocks", e);
+                break;
+            }
+            rc = b->precision == VLBM_PREC_F64 ? batch_run<double>(b, 1) : batch_run<float>(b, 1);
+            if (rc) break;
+            (void)hipEventRecord(e0, b->stream);
+            rc = b->precision == VLBM_PREC_F64 ? batch_run<double>(b, steps) : batch_run<float>(b, steps);
+            (void)hipEventRecord(e1, b->stream);
+            if (rc) break;
+            e = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) {
+                rc = fail("vlbm2d_batch_autotune: timing", e);
+                break;
+            }
+            if (ms_out) ms_out[3 * i + k] = ms / (float)steps;
+            if (best < 0.0f || ms < best) {
+                best = ms;
+                bnb = b->nb;
+                bnl = b->nl;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->graph_steps = keep_graph;
+    if (rc) return rc;
+    b->nb = bnb;
+    b->nl = bnl;
+    *nb_out = bnb;
+    *nl_out = bnl;
+    return batch_tables(b);
+}
+
+int vlbm2d_batch_set_blocks(vlbm2d_batch *b, int32_t nb, int32_t nl) {
+    if (!b) return fail_msg("vlbm2d_batch_set_blocks: null batch");
+    if ((nb != 128 && nb != 256 && nb != 512) || (nl != 32 && nl != 64 && nl != 128))
+        return fail_msg("vlbm2d_batch_set_blocks: node blocks must be 128, 256 or 512 and link blocks 32, 64 or 128");
+    hipError_t e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return fail("vlbm2d_batch_set_blocks: sync", e);
+    b->nb = nb;
+    b->nl = nl;
+    return batch_tables(b);
 }
 
 }  // extern "C"

@@ -1,16 +1,20 @@
-"""Many independent D2Q4x6 shells stepped concurrently on one GPU.
+"""Many independent D2Q4x6 shells stepped together on one GPU.
 
-The batched uneven workload: superellipse shells of different lattice
-sizes, each a ``WarpCurvedBoundaryLBM2D`` with its own HIP stream (and, for
-uniform schedules, its own captured step graph), advanced together so that
-the small per-shell kernels overlap on the device instead of running one
-shell after another.  Results are bitwise those of stepping every shell on
-its own.  There is no reference counterpart (the reference builds one
-solver object per geometry and steps it alone).
+The batched uneven workload: superellipse shells of different lattice sizes,
+each a ``WarpCurvedBoundaryLBM2D``.  In the default ``mode="batch"`` every
+step issues each kernel once over the work of all shells still stepping
+(libvlbm's vlbm2d_batch: a block table maps every block of a launch to a
+shell and its local block), optionally replayed from captured HIP graphs;
+``mode="streams"`` instead gives every shell its own HIP stream and step
+graph so that the per-shell kernels overlap.  Either way the results are
+bitwise those of stepping every shell on its own.  There is no reference
+counterpart (the reference builds one solver object per geometry and steps
+it alone).
 """
 
 from __future__ import annotations
 
+import ctypes as C
 from dataclasses import dataclass
 import math
 
@@ -108,41 +112,132 @@ def shell_solver_inputs(inst: ShellInstance):
 
 
 class BatchedCurvedBoundaryLBM2D:
-    """Steps a list of 2D solvers concurrently: each was built on its own
-    HIP stream, ``run(n)`` enqueues n steps of every solver before waiting
-    for any, and ``run_to_equilibrium`` keeps stepping the shells that have
-    not yet met the residual tolerance."""
+    """Steps a list of 2D solvers together.  ``mode="batch"``: one launch per
+    kernel per step over all shells still stepping (every solver on the
+    batch's stream); ``mode="streams"``: every solver on its own HIP stream,
+    ``run(n)`` enqueues n steps of every solver before waiting for any.
+    ``run_to_equilibrium`` keeps stepping the shells that have not yet met
+    the residual tolerance."""
 
-    def __init__(self, solvers: list):
-        if len({id(s._stream) for s in solvers}) != len(solvers):
-            raise ValueError("every solver of a batch needs its own stream")
+    def __init__(self, solvers: list, *, mode: str = "streams", graph_steps: int = 0):
+        if mode not in ("batch", "streams"):
+            raise ValueError("mode must be 'batch' or 'streams'")
         self.solvers = list(solvers)
+        self.mode = mode
+        self._batch = None
+        if mode == "streams":
+            if len({id(s._stream) for s in solvers}) != len(solvers):
+                raise ValueError("every solver of a batch needs its own stream")
+            return
+        if len({id(s._stream) for s in solvers}) != 1:
+            raise ValueError("batched solvers must share one stream")
+        from .. import _native
+
+        self._lib = _native.load_library()
+        n = len(self.solvers)
+        hs = (C.c_void_p * n)(*[s._handle.value for s in self.solvers])
+        f0 = (C.c_void_p * n)(*[s._pops[0].ptr for s in self.solvers])
+        f1 = (C.c_void_p * n)(*[s._pops[1].ptr for s in self.solvers])
+        b = C.c_void_p()
+        _native.check(self._lib.vlbm2d_batch_create(hs, f0, f1, n, self.solvers[0]._stream.cuda_stream or None,
+                                                    C.byref(b)), "vlbm2d_batch_create")
+        self._batch = b
+        self._live = list(range(n))
+        _native.check(self._lib.vlbm2d_batch_set_graph_steps(b, int(graph_steps)), "vlbm2d_batch_set_graph_steps")
+
+    def __del__(self):
+        b = getattr(self, "_batch", None)
+        if b is not None and b.value:
+            self._lib.vlbm2d_batch_destroy(b)
+            self._batch = None
 
     @classmethod
-    def build(cls, inputs, *, precision: str = "fp32", graph_steps: int = 0, device: str = "cuda:0"):
-        """One solver per (geometry, kwargs), each constructed on a new
-        default-priority stream."""
+    def build(cls, inputs, *, precision: str = "fp32", graph_steps: int = 0, device: str = "cuda:0",
+              mode: str = "batch"):
+        """One solver per (geometry, kwargs): for ``mode="batch"`` all on one
+        new default-priority stream, for ``"streams"`` each on its own."""
         import torch
 
         solvers = []
+        shared = torch.cuda.Stream(device=device) if mode == "batch" else None
         for g, kw in inputs:
-            with torch.cuda.stream(torch.cuda.Stream(device=device)):
-                solvers.append(cbw.WarpCurvedBoundaryLBM2D(g, **kw, precision=precision, graph_steps=graph_steps,
-                                                           device=device))
-        return cls(solvers)
+            with torch.cuda.stream(shared if shared is not None else torch.cuda.Stream(device=device)):
+                solvers.append(cbw.WarpCurvedBoundaryLBM2D(
+                    g, **kw, precision=precision, graph_steps=0 if mode == "batch" else graph_steps, device=device))
+        return cls(solvers, mode=mode, graph_steps=graph_steps if mode == "batch" else 0)
 
     @property
     def n_active(self) -> int:
         return int(sum(s.n_active for s in self.solvers))
+
+    def autotune(self, steps: int = 4) -> dict:
+        """Batch mode: time every pair of block shapes (node kernels 128 /
+        256 / 512 threads, cut-link kernel 32 / 64 / 128) over `steps`
+        batched steps and keep the fastest; then restart every shell from
+        rest (the timing steps advance the state)."""
+        if self.mode != "batch":
+            raise ValueError("autotune applies to mode='batch'")
+        from .. import _native
+
+        nb, nl = C.c_int32(), C.c_int32()
+        ms = (C.c_float * 9)()
+        _native.check(self._lib.vlbm2d_batch_autotune(self._batch, int(steps), C.byref(nb), C.byref(nl), ms),
+                      "vlbm2d_batch_autotune")
+        for s in self.solvers:
+            s.initialize_identity()
+        self._select(list(range(len(self.solvers))))
+        grid = {f"{a}x{b}": float(ms[3 * i + k]) for i, a in enumerate((128, 256, 512))
+                for k, b in enumerate((32, 64, 128))}
+        return {"node_block": nb.value, "link_block": nl.value, "ms_per_step": grid}
+
+    def set_blocks(self, node_block: int, link_block: int) -> None:
+        from .. import _native
+
+        _native.check(self._lib.vlbm2d_batch_set_blocks(self._batch, int(node_block), int(link_block)),
+                      "vlbm2d_batch_set_blocks")
 
     def synchronize(self) -> None:
         import torch
 
         torch.cuda.synchronize(self.solvers[0].device)
 
+    def _select(self, live) -> None:
+        from .. import _native
+
+        live = sorted(live)
+        n = len(live)
+        ids = (C.c_int32 * max(n, 1))(*live)
+        f0 = (C.c_void_p * max(n, 1))(*[self.solvers[k]._pops[0].ptr for k in live])
+        f1 = (C.c_void_p * max(n, 1))(*[self.solvers[k]._pops[1].ptr for k in live])
+        _native.check(self._lib.vlbm2d_batch_select(self._batch, ids, f0, f1, n), "vlbm2d_batch_select")
+        self._live = live
+
+    def _advance(self, live, nsteps: int) -> None:
+        """nsteps steps of the shells in `live` (all of one batch launch
+        sequence in batch mode)."""
+        if not live or nsteps <= 0:
+            return
+        if self.mode == "streams":
+            for k in live:
+                self.solvers[k].run(nsteps)
+            return
+        from .. import _native
+
+        if list(live) != self._live:
+            self._select(live)
+        times = (C.c_double * len(live))(*[self.solvers[k].time for k in live])
+        _native.check(self._lib.vlbm2d_batch_step(self._batch, times, int(nsteps)), "vlbm2d_batch_step")
+        for k in live:
+            s = self.solvers[k]
+            for _ in range(nsteps):
+                s.time += s.dt
+            s.steps += nsteps
+            if nsteps % 2 == 1:
+                s._pops = [s._pops[1], s._pops[0]]
+            s._fpost_stale = True
+
     def run(self, nsteps: int) -> None:
-        for s in self.solvers:
-            s.run(nsteps)
+        self._advance(list(range(len(self.solvers))), int(nsteps))
         self.synchronize()
 
     def run_to_equilibrium(self, tol: float, *, check_every: int = 200, max_steps: int = 10**6,
@@ -159,8 +254,7 @@ class BatchedCurvedBoundaryLBM2D:
         done = [None] * len(self.solvers)
         while any(d is None for d in done):
             live = [k for k, d in enumerate(done) if d is None]
-            for k in live:
-                self.solvers[k].run(check_every)
+            self._advance(live, check_every)
             for k in live:
                 s = self.solvers[k]
                 red = s.reductions(prev[k])

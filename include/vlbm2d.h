@@ -165,6 +165,34 @@ int vlbm2d_populations_from_f64(vlbm2d_handle *h, const double *src, void *f);
 int vlbm2d_timing_enable(vlbm2d_handle *h, int32_t enable);
 int vlbm2d_timing_read(vlbm2d_handle *h, double *ms, int32_t *counts, int32_t nslots);
 
+/* Batches: many 2D lattices stepped together, one launch per kernel per
+ * step over the work of all selected shells (bitwise the results of stepping
+ * each alone).  No reference counterpart (the reference steps one solver per
+ * geometry, cbw2d:4163-4258).  Requirements: the same population precision,
+ * BGK without the kinetic filter, a uniform schedule (intervals 0 or 1, one
+ * sweep), no box solvers.  f0[k] / f1[k] are shell k's population buffers;
+ * the populations are in f0 while vlbm2d_batch_parity() is 0 and in f1 while
+ * it is 1 (an odd step count flips it).  vlbm2d_batch_select sets the
+ * shells that step (ids strictly increasing) with their current population
+ * buffers (f0: the populations, f1: scratch) and resets the parity to 0.
+ * times: the start time of each selected shell (host array, selection
+ * order). */
+typedef struct vlbm2d_batch vlbm2d_batch;
+int vlbm2d_batch_create(vlbm2d_handle *const *handles, void *const *f0, void *const *f1, int32_t n, void *stream,
+                        vlbm2d_batch **out);
+int vlbm2d_batch_select(vlbm2d_batch *b, const int32_t *ids, void *const *f0, void *const *f1, int32_t count);
+int vlbm2d_batch_set_graph_steps(vlbm2d_batch *b, int32_t steps);
+int vlbm2d_batch_parity(const vlbm2d_batch *b);
+int vlbm2d_batch_step(vlbm2d_batch *b, const double *times, int32_t nsteps);
+int vlbm2d_batch_destroy(vlbm2d_batch *b);
+/* Block shapes of the batched launches: threads per block of the node
+ * kernels (128, 256 or 512) and of the cut-link kernel (32, 64 or 128).
+ * vlbm2d_batch_autotune times `steps` batched steps of the selected shells
+ * with every pair (HIP events on the batch stream), keeps the fastest and
+ * returns it (ms_out, optional: 9 per-step times, node-block major).  It
+ * advances the shells' state: tune before initialising them. */
+int vlbm2d_batch_autotune(vlbm2d_batch *b, int32_t steps, int32_t *nb_out, int32_t *nl_out, float *ms_out);
+int vlbm2d_batch_set_blocks(vlbm2d_batch *b, int32_t nb, int32_t nl);
 #ifdef __cplusplus
 }
 #endif

@@ -1,6 +1,9 @@
-"""The batched uneven 2D workload: the shell draws (host) and, on the GPU,
-that stepping shells concurrently on their own streams gives bitwise the
-result of stepping each alone."""
+"""The batched uneven 2D workload: the shell draws (host); on the GPU, that
+stepping shells together (one launch per kernel per step over all shells,
+or concurrently on their own streams) gives bitwise the result of stepping
+each alone, with and without captured step graphs and after the block-shape
+autotuner; and the seed-0 draw against the CPU oracle: one shell of every
+lattice side and two of the shells that diverge."""
 
 import numpy as np
 import pytest
@@ -21,14 +24,14 @@ def test_batch_draws_are_seeded_and_in_range():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("graph_steps", [0, 10])
-def test_gpu_concurrent_batch_equals_shells_stepped_alone(graph_steps):
+@pytest.mark.parametrize("mode,graph_steps", [("batch", 0), ("batch", 10), ("streams", 0), ("streams", 10)])
+def test_gpu_concurrent_batch_equals_shells_stepped_alone(mode, graph_steps):
     pytest.importorskip("torch")
     from curved_lbm.two_d import curved_boundary_warp as cbw
 
     inst = bt.batch_instances(3, seed=2, sides=(64, 96, 128))
     inputs = [bt.shell_solver_inputs(i) for i in inst]
-    b = bt.BatchedCurvedBoundaryLBM2D.build(inputs, precision="fp32", graph_steps=graph_steps)
+    b = bt.BatchedCurvedBoundaryLBM2D.build(inputs, precision="fp32", graph_steps=graph_steps, mode=mode)
     alone = [cbw.WarpCurvedBoundaryLBM2D(g, **kw, precision="fp32") for g, kw in inputs]
     b.run(40)
     for s in alone:
@@ -42,8 +45,8 @@ def test_gpu_concurrent_batch_equals_shells_stepped_alone(graph_steps):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("graph_steps", [0, 10])
-def test_gpu_batch_residual_checks_equal_shells_alone(graph_steps):
+@pytest.mark.parametrize("mode,graph_steps", [("batch", 0), ("batch", 10), ("streams", 10)])
+def test_gpu_batch_residual_checks_equal_shells_alone(mode, graph_steps):
     """With a finite tolerance the batch's per-check residuals and stopping
     steps are those of each shell stepped alone with the same checks (the
     snapshots of u run on each shell's own stream)."""
@@ -69,11 +72,118 @@ def test_gpu_batch_residual_checks_equal_shells_alone(graph_steps):
     for h in hist:
         k = next((i for i, r in enumerate(h) if r <= tol), None)
         want.append((True, (k + 1) * every, h[k]) if k is not None else (False, checks * every, h[-1]))
-    b = bt.BatchedCurvedBoundaryLBM2D.build(inputs, precision="fp32", graph_steps=graph_steps)
+    b = bt.BatchedCurvedBoundaryLBM2D.build(inputs, precision="fp32", graph_steps=graph_steps, mode=mode)
     res = b.run_to_equilibrium(tol, check_every=every, max_steps=checks * every)
     got = [(r["converged"], r["steps"], r["residual"]) for r in res]
     assert got == want
     assert any(g[0] for g in got) and not all(g[0] for g in got)
+
+
+@pytest.mark.gpu
+def test_gpu_batch_autotune_keeps_results():
+    """After the autotuner's timing steps and the restart from rest, the
+    batch at the chosen block shapes equals the shells stepped alone."""
+    pytest.importorskip("torch")
+    from curved_lbm.two_d import curved_boundary_warp as cbw
+
+    inst = bt.batch_instances(3, seed=4, sides=(64, 96))
+    inputs = [bt.shell_solver_inputs(i) for i in inst]
+    b = bt.BatchedCurvedBoundaryLBM2D.build(inputs, precision="fp32", mode="batch")
+    choice = b.autotune(steps=3)
+    assert choice["node_block"] in (128, 256, 512) and choice["link_block"] in (32, 64, 128)
+    assert len(choice["ms_per_step"]) == 9 and all(v > 0 for v in choice["ms_per_step"].values())
+    alone = [cbw.WarpCurvedBoundaryLBM2D(g, **kw, precision="fp32") for g, kw in inputs]
+    b.run(23)
+    for s in alone:
+        s.run(23)
+    for s, r in zip(b.solvers, alone):
+        assert s.steps == r.steps == 23 and s.time == r.time
+        np.testing.assert_array_equal(s.numpy_field("f0"), r.numpy_field("f0"))
+    for nb, nl in ((128, 32), (512, 128)):
+        b.set_blocks(nb, nl)
+        b.run(4)
+        for s in alone:
+            s.run(4)
+        for s, r in zip(b.solvers, alone):
+            np.testing.assert_array_equal(s.numpy_field("u"), r.numpy_field("u"))
+
+
+SEED0_SIDES = (0, 1, 2, 8, 10)  # the first shell of every side: 384, 512, 128, 192, 256
+SEED0_DIVERGING = (202, 32)      # sides 128 and 512; both blow up within the first ~1000 steps
+
+
+def _oracle_close(s, o, act, what):
+    from oracle.oracle2d import OracleSolver2D  # noqa: F401
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    assert rel(s.system_state()[:, act], o.U[:, act]) < 1e-10, what
+    assert rel(s.displacement()[:, act], o.u[:, act]) < 1e-10, what
+    assert np.max(np.abs(s.numpy_field("f0")[:, act] - o.f0[:, act])) < 1e-12, what
+
+
+@pytest.mark.gpu
+def test_gpu_seed0_batch_matches_oracle():
+    """The seed-0 draw of the 256-shell workload, one shell of every lattice
+    side plus two that diverge later, stepped as one FP64 batch for 12 steps
+    (calls of 5 and 7), against the CPU oracle shell by shell."""
+    pytest.importorskip("torch")
+    from oracle.oracle2d import OracleSolver2D
+
+    inst = bt.batch_instances(256, seed=0)
+    pick = SEED0_SIDES + SEED0_DIVERGING
+    assert {inst[k].n for k in pick} == set(bt.SIDES)
+    inputs = [bt.shell_solver_inputs(inst[k]) for k in pick]
+    b = bt.BatchedCurvedBoundaryLBM2D.build(inputs, precision="fp64", mode="batch")
+    oras = [OracleSolver2D(g, **kw) for g, kw in inputs]
+    for n in (5, 7):
+        b.run(n)
+        for o in oras:
+            o.run(n)
+    for k, s, o in zip(pick, b.solvers, oras):
+        s.refresh_current()
+        o.refresh_current()
+        _oracle_close(s, o, s.active_mask(), f"shell {k}")
+        assert np.max(np.abs(o.U[:2])) > 0.0
+
+
+@pytest.mark.gpu
+def test_gpu_seed0_diverging_shells_diverge_with_the_oracle():
+    """Shells 202 and 32 of the seed-0 draw: batched FP64 steps and the
+    oracle agree (1e-8 relative) while the state is moderate, and both turn
+    invalid (non-finite or J <= 0) in the same 50-step window."""
+    pytest.importorskip("torch")
+    from oracle.oracle2d import OracleSolver2D
+
+    inst = bt.batch_instances(256, seed=0)
+    inputs = [bt.shell_solver_inputs(inst[k]) for k in SEED0_DIVERGING]
+    b = bt.BatchedCurvedBoundaryLBM2D.build(inputs, precision="fp64", mode="batch")
+    oras = [OracleSolver2D(g, **kw) for g, kw in inputs]
+    broke = [None, None]
+    live = [0, 1]
+    for window in range(40):
+        b._advance(live, 50)
+        for k in list(live):
+            oras[k].run(50)
+            s, o = b.solvers[k], oras[k]
+            s.refresh_current()
+            o.refresh_current()
+            act = s.active_mask()
+            Jo = o.deformation_jacobian()[act]
+            o_bad = (not np.isfinite(o.U[:, act]).all()) or Jo.min() <= 0.0
+            red = s.reductions()
+            s_bad = bool(red["nonfinite"]) or red["min_J"] <= 0.0
+            assert s_bad == o_bad, (SEED0_DIVERGING[k], window, red, float(Jo.min()))
+            if o_bad:
+                broke[k] = window
+                live.remove(k)
+            elif np.max(np.abs(o.U[:2, act])) < 0.5:
+                Us, Uo = s.system_state()[:, act], o.U[:, act]
+                assert np.linalg.norm(Us - Uo) <= 1e-8 * np.linalg.norm(Uo), (SEED0_DIVERGING[k], window)
+        if not live:
+            break
+    assert broke[0] is not None and broke[1] is not None, broke
 
 
 class _Tensor:

@@ -298,6 +298,33 @@ def test_gpu_annulus96_3000_steps_matches_oracle():
     assert np.max(np.abs(Jg - Jo)) <= 1e-10 and Jg.min() > 0.0
 
 
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_gpu_annulus96_3000_steps_fp32_error():
+    """The same workload with FP32 populations against the FP64 oracle after
+    3000 steps: relative l2 error of u, v and F - I below 1e-6 and J within
+    1e-8."""
+    n = 96
+    dt = (1.0 / n) / 10.0
+    g, kw, _ = gc.annulus(n, params=(0.05, 1000 * dt))
+    gpu = cb.WarpCurvedBoundaryLBM2D(g, **kw, precision="fp32")
+    ora = OracleSolver2D(g, **kw)
+    gpu.run(3000)
+    ora.run(3000)
+    gpu.refresh_current()
+    ora.refresh_current()
+    act = gpu.active_mask()
+    eye = np.array([1.0, 0.0, 0.0, 1.0])[:, None]
+    Ug, Uo = gpu.system_state()[:, act], ora.U[:, act]
+    errs = {"u": rel(gpu.displacement()[:, act], ora.u[:, act]), "v": rel(Ug[:2], Uo[:2]),
+            "F_minus_I": rel(Ug[2:] - eye, Uo[2:] - eye),
+            "J": float(np.max(np.abs(gpu.deformation_jacobian()[act] - ora.deformation_jacobian()[act])))}
+    print(errs)
+    assert errs["u"] < 1e-6 and errs["v"] < 1e-6 and errs["F_minus_I"] < 1e-6 and errs["J"] < 1e-8
+
+
 @pytest.mark.parametrize("over", [{}, {"collision_model": "mrt_ghost", "ghost_omega": 1.3,
                                       "kinetic_filter_strength": 0.1}])
 def test_gpu_graph_step_loop_matches_direct_launches(over):
