@@ -67,7 +67,14 @@ struct Params {
     double disp_relax, w2;
     int disp_bid;
     double blend;
+    const double *clock;  // device time of the step start (captured step loops), or nullptr
 };
+
+// Boundary time of a step: the host value, or clock + dt/2 when the launch
+// belongs to a captured step loop that keeps the time on the device.
+__device__ __forceinline__ double step_time(double tb, const Params &P) {
+    return P.clock ? P.clock[0] + 0.5 * P.dt : tb;
+}
 
 // Population access.  double planes hold f; float planes hold f - f_rest
 // where f_rest[q][a] = 1/6 on the diagonal of F (a = 3, 7, 11), 0 elsewhere.
@@ -134,6 +141,9 @@ __device__ __forceinline__ double fpost(const T *f, const T *rec, size_t n, int 
 }
 
 __global__ void k_reset_invalid(int32_t *invalid) { invalid[0] = 0; }
+__global__ void k_set_clock(double *clock, double t) { clock[0] = t; }
+// time += dt, the same FP64 operation the host performs after each step.
+__global__ void k_advance_clock(double *clock, double dt) { clock[0] = clock[0] + dt; }
 
 // Moments with the half-step source; u = u* + dt/2 v and (collide only)
 // u* = u + dt/2 v.  cbw3d:2110-2189 and :2193-2280.
@@ -189,6 +199,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(
     double d[3] = {0.0, 0.0, 0.0}, r[3] = {0.0, 0.0, 0.0};
     const int l0 = aoff[p], l1 = aoff[p + 1];
     if (l1 > l0) {
+        time = step_time(time, P);
         double F[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) F[k] = U[(3 + k) * n + p];
@@ -575,6 +586,7 @@ __global__ __launch_bounds__(128) void k_prepare(const double *__restrict__ U, c
                                                  LinkTables T, double t, Params P) {
     const int ell = blockIdx.x * 128 + threadIdx.x;
     if (ell >= P.L) return;
+    t = step_time(t, P);
     const size_t n = P.n, L = P.L;
     const int bid = T.lbid[ell];
     const int mode = T.mode_by_id[bid];
@@ -912,6 +924,7 @@ __global__ __launch_bounds__(128) void k_box_sides3(const T *fsrc, T *fdst, cons
     if (nx * ny > m) m = nx * ny;
     const long long t = (long long)blockIdx.x * 128 + threadIdx.x;
     if (t >= 6LL * m) return;
+    time = step_time(time, P);
     const int side = (int)(t / m), s = (int)(t - (long long)side * m);
     const int kind = B.kind[side];
     if (kind == 0) return;
@@ -982,15 +995,35 @@ struct vlbm3d_handle {
     hipStream_t stream;
     int device = 0;  // device current at create
     Timing timing;
+    // Captured step loops: graph_steps fused steps per graph, replayed on
+    // the solver stream; the time lives in `clock` on the device.
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        void *f0 = nullptr, *f1 = nullptr;
+    };
+    int graph_steps = 0;
+    hipStream_t cap = nullptr;
+    double *clock = nullptr;
+    StepGraph graphs[2];
+    void drop_graphs() {
+        for (auto &g : graphs) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            g = StepGraph{};
+        }
+    }
     // Library-owned tables built at create: the compacted anchor list of
     // the displacement sweep and the second-neighbour table.
     int32_t *anc_off = nullptr, *anc_key = nullptr, *nbr2 = nullptr;
     double *anc_dat = nullptr;
     void release() {
-        for (void *ptr : {(void *)anc_off, (void *)anc_key, (void *)nbr2, (void *)anc_dat})
+        drop_graphs();
+        if (cap) (void)hipStreamDestroy(cap);
+        cap = nullptr;
+        for (void *ptr : {(void *)anc_off, (void *)anc_key, (void *)nbr2, (void *)anc_dat, (void *)clock})
             if (ptr) (void)hipFree(ptr);
         anc_off = anc_key = nbr2 = nullptr;
         anc_dat = nullptr;
+        clock = nullptr;
     }
 };
 
@@ -1132,7 +1165,7 @@ static LinkTables make_tables(const vlbm3d_desc &d) {
 // characteristic closure's J flags.
 template <typename T>
 static int run_boundary(vlbm3d_handle *h, hipStream_t st, const T *src, T *dst, double t, int use_rec,
-                        int32_t *invalid) {
+                        int32_t *invalid, const Params &P) {
     const vlbm3d_desc &d = h->d;
     if (d.box) {
         const Box3 &B = h->box;
@@ -1141,7 +1174,7 @@ static int run_boundary(vlbm3d_handle *h, hipStream_t st, const T *src, T *dst, 
         if ((long long)B.n[0] * B.n[1] > m) m = (long long)B.n[0] * B.n[1];
         TimedSlot ts(h->timing, st, VLBM_SLOT_LINKS);
         hipLaunchKernelGGL((k_box_sides3<T>), dim3((unsigned)((6 * m + 127) / 128)), dim3(128), 0, st, src, dst,
-                           (const T *)d.record, (const double *)d.U, use_rec, B, t, h->P);
+                           (const T *)d.record, (const double *)d.U, use_rec, B, t, P);
         CHECK_LAUNCH("k_box_sides3");
         return 0;
     }
@@ -1164,27 +1197,124 @@ static int run_boundary(vlbm3d_handle *h, hipStream_t st, const T *src, T *dst, 
     {
         TimedSlot ts(h->timing, st, VLBM_SLOT_PREPARE);
         hipLaunchKernelGGL(k_prepare, dim3(gl), dim3(128), 0, st, (const double *)d.U, (const double *)d.u, h->T, t,
-                           h->P);
+                           P);
         CHECK_LAUNCH("k_prepare");
     }
     TimedSlot ts(h->timing, st, VLBM_SLOT_LINKS);
     hipLaunchKernelGGL((k_links<T>), dim3(gl), dim3(128), 0, st, src, dst, (const T *)d.record,
-                       (const double *)d.U, h->T, invalid, use_rec, d.has_char, h->P);
+                       (const double *)d.U, h->T, invalid, use_rec, d.has_char, P);
     CHECK_LAUNCH("k_links");
     return 0;
 }
 
 template <typename T>
-static int launch_stream(vlbm3d_handle *h, bool next, const T *f0, T *f1) {
+static int launch_stream(vlbm3d_handle *h, bool next, const T *f0, T *f1, const Params &P) {
     const vlbm3d_desc &d = h->d;
     const int gn = xcd_grid(d.n_active);
     if (next)
         hipLaunchKernelGGL((k_stream<T, true>), dim3(gn), dim3(BLOCK), 0, h->stream, f0, f1, (const T *)d.record,
-                           d.neighbor, d.U, d.u, d.u_star, d.invalid, h->P);
+                           d.neighbor, d.U, d.u, d.u_star, d.invalid, P);
     else
         hipLaunchKernelGGL((k_stream<T, false>), dim3(gn), dim3(BLOCK), 0, h->stream, f0, f1, (const T *)d.record,
-                           d.neighbor, d.U, d.u, d.u_star, d.invalid, h->P);
+                           d.neighbor, d.U, d.u, d.u_star, d.invalid, P);
     CHECK_LAUNCH("k_stream");
+    return 0;
+}
+
+// Launches of one step from (time, steps) on the solver stream: sweeps,
+// record, boundary and the stream kernel (with the next step's collide when
+// `next`).  `last` marks the call's last step; P carries the clock pointer
+// inside captured step loops.
+template <typename T>
+static int launch_step3(vlbm3d_handle *h, const Params &P, T *f0, T *f1, double time, int64_t steps, bool next,
+                        bool last) {
+    const vlbm3d_desc &d = h->d;
+    const int gn = xcd_grid(d.n_active);
+    TimedSlot whole(h->timing, h->stream, VLBM_SLOT_STEP);
+    const double tb = time + 0.5 * d.dt;
+    const bool disp = d.disp_interval > 0 && steps % d.disp_interval == 0;
+    const bool compat = d.compat_interval > 0 && steps % d.compat_interval == 0;
+    if (disp) {
+        for (int s = 0; s < d.disp_sweeps; ++s) {
+            TimedSlot ts(h->timing, h->stream, VLBM_SLOT_SWEEP);
+            hipLaunchKernelGGL(k_sweep, dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)d.U,
+                               (const double *)d.u, d.u_next, d.neighbor, (const int32_t *)h->anc_off,
+                               (const int32_t *)h->anc_key, (const double *)h->anc_dat, d.params_by_id, tb, P);
+            CHECK_LAUNCH("k_sweep");
+            if (s + 1 < d.disp_sweeps) {
+                TimedSlot tc(h->timing, h->stream, VLBM_SLOT_COMMIT);
+                hipLaunchKernelGGL(k_commit, dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)d.u_next, d.u,
+                                   d.u_star, d.n_active);
+                CHECK_LAUNCH("k_commit");
+            }
+        }
+    }
+    {
+        TimedSlot ts(h->timing, h->stream, VLBM_SLOT_RECORD);
+        // Within a call only the gather pack consumes the post-repair F of
+        // the local_f closure; the U planes get it on the call's last step.
+        const bool pack = d.has_char && d.n_char_links > 0;
+        const int write_U = (last || d.recon != R_LOCALF || !pack) ? 1 : 0;
+        // The flag is cleared here when the stream kernel performs the next
+        // collide; the links then report only on the call's last step, where
+        // the reference's flag holds the last collide's and the last
+        // closure's checks.
+        hipLaunchKernelGGL(k_record<T>, dim3(gn), dim3(BLOCK), 0, h->stream, d.U,
+                           disp ? (const double *)d.u_next : (const double *)d.u, d.u, d.u_star, (T *)d.record,
+                           d.neighbor, (const int32_t *)h->nbr2, pack ? d.char_pack_slot : (const int32_t *)nullptr,
+                           d.char_pack, (int)compat, (int)disp, write_U, last ? (int32_t *)nullptr : d.invalid, P);
+        CHECK_LAUNCH("k_record");
+    }
+    int32_t *link_flag = next ? (int32_t *)nullptr : d.invalid;
+    int rc = run_boundary<T>(h, h->stream, f0, f1, tb, 1, link_flag, P);
+    if (rc) return rc;
+    TimedSlot ts(h->timing, h->stream, VLBM_SLOT_STREAM);
+    return launch_stream<T>(h, next, f0, f1, P);
+}
+
+// Captures graph_steps fused steps (never a call's last step) with the time
+// kept on the device; the schedule must be the same every step.
+template <typename T>
+static int capture_steps3(vlbm3d_handle *h, T *f0, T *f1, hipGraphExec_t *out) {
+    hipError_t e;
+    if (!h->cap) {
+        e = hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking);
+        if (e != hipSuccess) return fail("vlbm3d: capture stream", e);
+    }
+    if (!h->clock) {
+        e = hipMalloc((void **)&h->clock, sizeof(double));
+        if (e != hipSuccess) return fail("vlbm3d: clock", e);
+    }
+    Params P = h->P;
+    P.clock = h->clock;
+    hipStream_t keep = h->stream;
+    const bool timing = h->timing.on;
+    h->timing.on = false;
+    h->stream = h->cap;
+    e = hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        h->stream = keep;
+        h->timing.on = timing;
+        return fail("vlbm3d: begin capture", e);
+    }
+    int rc = 0;
+    for (int k = 0; k < h->graph_steps && rc == 0; ++k) {
+        rc = (k % 2 == 0) ? launch_step3<T>(h, P, f0, f1, 0.0, 0, true, false)
+                          : launch_step3<T>(h, P, f1, f0, 0.0, 0, true, false);
+        if (rc == 0) hipLaunchKernelGGL(k_advance_clock, dim3(1), dim3(1), 0, h->cap, h->clock, h->d.dt);
+    }
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(h->cap, &graph);
+    h->stream = keep;
+    h->timing.on = timing;
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail("vlbm3d: end capture", e);
+    e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail("vlbm3d: instantiate", e);
     return 0;
 }
 
@@ -1195,69 +1325,50 @@ static int launch_stream(vlbm3d_handle *h, bool next, const T *f0, T *f1) {
 // kernel therefore runs before the stream kernel, which reads the link rows
 // back to complete each node's moments.  The last step of the call streams
 // without the fused collide, so U, u and u* end as the reference leaves them
-// (post-repair state, committed displacement).
+// (post-repair state, committed displacement).  With graph_steps = K > 0 and
+// a uniform schedule, blocks of K fused steps are replayed from a captured
+// graph (an even K keeps f0 / f1 where they started).
 template <typename T>
 static int run_steps(vlbm3d_handle *h, T *f0, T *f1, double time, int64_t steps, int nsteps) {
     const vlbm3d_desc &d = h->d;
     const int gn = xcd_grid(d.n_active);
-    const Params &P = h->P;
     if (nsteps <= 0) return 0;
     hipLaunchKernelGGL(k_reset_invalid, dim3(1), dim3(1), 0, h->stream, d.invalid);
     {
         TimedSlot ts(h->timing, h->stream, VLBM_SLOT_COLLIDE);
         hipLaunchKernelGGL((k_collide<T, true>), dim3(gn), dim3(BLOCK), 0, h->stream, (const T *)f0, d.U, d.u,
-                           d.u_star, d.invalid, P);
+                           d.u_star, d.invalid, h->P);
         CHECK_LAUNCH("k_collide");
     }
-    for (int k = 0; k < nsteps; ++k) {
-        TimedSlot whole(h->timing, h->stream, VLBM_SLOT_STEP);
-        const double tb = time + 0.5 * d.dt;
-        const bool disp = d.disp_interval > 0 && steps % d.disp_interval == 0;
-        const bool compat = d.compat_interval > 0 && steps % d.compat_interval == 0;
-        if (disp) {
-            for (int s = 0; s < d.disp_sweeps; ++s) {
-                TimedSlot ts(h->timing, h->stream, VLBM_SLOT_SWEEP);
-                hipLaunchKernelGGL(k_sweep, dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)d.U,
-                                   (const double *)d.u, d.u_next, d.neighbor, (const int32_t *)h->anc_off,
-                                   (const int32_t *)h->anc_key, (const double *)h->anc_dat, d.params_by_id, tb,
-                                   P);
-                CHECK_LAUNCH("k_sweep");
-                if (s + 1 < d.disp_sweeps) {
-                    TimedSlot tc(h->timing, h->stream, VLBM_SLOT_COMMIT);
-                    hipLaunchKernelGGL(k_commit, dim3(gn), dim3(BLOCK), 0, h->stream, (const double *)d.u_next,
-                                       d.u, d.u_star, d.n_active);
-                    CHECK_LAUNCH("k_commit");
-                }
-            }
-        }
-        {
-            TimedSlot ts(h->timing, h->stream, VLBM_SLOT_RECORD);
-            // Within a call only the gather pack consumes the post-repair F of
-            // the local_f closure; the U planes get it on the call's last step.
-            const bool last = k + 1 == nsteps;
-            const bool pack = d.has_char && d.n_char_links > 0;
-            const int write_U = (last || d.recon != R_LOCALF || !pack) ? 1 : 0;
-            // The flag is cleared here when the stream kernel performs the
-            // next collide; the links then report only on the call's last
-            // step, where the reference's flag holds the last collide's and
-            // the last closure's checks.
-            hipLaunchKernelGGL(k_record<T>, dim3(gn), dim3(BLOCK), 0, h->stream, d.U,
-                               disp ? (const double *)d.u_next : (const double *)d.u, d.u, d.u_star,
-                               (T *)d.record, d.neighbor, (const int32_t *)h->nbr2,
-                               pack ? d.char_pack_slot : (const int32_t *)nullptr,
-                               d.char_pack, (int)compat, (int)disp, write_U,
-                               last ? (int32_t *)nullptr : d.invalid, P);
-            CHECK_LAUNCH("k_record");
-        }
-        const bool next = k + 1 < nsteps;
-        int32_t *link_flag = next ? (int32_t *)nullptr : d.invalid;
-        int rc = run_boundary<T>(h, h->stream, f0, f1, tb, 1, link_flag);
-        if (rc) return rc;
-        {
-            TimedSlot ts(h->timing, h->stream, VLBM_SLOT_STREAM);
-            rc = launch_stream<T>(h, next, f0, f1);
+    int k = 0;
+    const bool uniform = d.disp_interval <= 1 && d.compat_interval <= 1;
+    const int K = h->graph_steps;
+    if (K > 0 && uniform && !h->timing.on && nsteps - 1 >= K) {
+        vlbm3d_handle::StepGraph *g = nullptr;
+        for (auto &c : h->graphs)
+            if (c.exec && c.f0 == (void *)f0 && c.f1 == (void *)f1) g = &c;
+        if (!g) {
+            g = !h->graphs[0].exec ? &h->graphs[0] : &h->graphs[1];
+            if (g->exec) (void)hipGraphExecDestroy(g->exec);
+            *g = vlbm3d_handle::StepGraph{};
+            int rc = capture_steps3<T>(h, f0, f1, &g->exec);
             if (rc) return rc;
+            g->f0 = f0;
+            g->f1 = f1;
         }
+        hipLaunchKernelGGL(k_set_clock, dim3(1), dim3(1), 0, h->stream, h->clock, time);
+        CHECK_LAUNCH("k_set_clock");
+        for (; k + K <= nsteps - 1; k += K) {
+            hipError_t e = hipGraphLaunch(g->exec, h->stream);
+            if (e != hipSuccess) return fail("vlbm3d: graph launch", e);
+            for (int j = 0; j < K; ++j) time += d.dt;
+            steps += K;
+        }
+    }
+    for (; k < nsteps; ++k) {
+        const bool next = k + 1 < nsteps;
+        int rc = launch_step3<T>(h, h->P, f0, f1, time, steps, next, !next);
+        if (rc) return rc;
         T *tmp = f0;
         f0 = f1;
         f1 = tmp;
@@ -1399,6 +1510,17 @@ int vlbm3d_timing_read(vlbm3d_handle *h, double *ms, int32_t *counts, int32_t ns
     return h->timing.read(h->stream, ms, counts, nslots);
 }
 
+int vlbm3d_set_graph_steps(vlbm3d_handle *h, int32_t steps) {
+    if (!h) return fail_msg("vlbm3d_set_graph_steps: null handle");
+    if (steps < 0 || steps > 4096) return fail_msg("vlbm3d_set_graph_steps: steps must be in [0, 4096]");
+    if (steps % 2) return fail_msg("vlbm3d_set_graph_steps: steps must be even");
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail("vlbm3d_set_graph_steps: sync", e);
+    h->drop_graphs();
+    h->graph_steps = steps;
+    return 0;
+}
+
 int vlbm3d_step(vlbm3d_handle *h, void *f0, void *f1, double time, int64_t steps, int32_t nsteps) {
     if (!h || !f0 || !f1) return fail_msg("vlbm3d_step: null argument");
     if (nsteps < 0) return fail_msg("vlbm3d_step: nsteps must be non-negative");
@@ -1433,17 +1555,17 @@ int vlbm3d_apply_boundary(vlbm3d_handle *h, void *f_src, void *f_dst, double t, 
         if (in_place) {
             hipLaunchKernelGGL(k_copy_pop<double>, dim3(gc), dim3(BLOCK), 0, h->stream, s, t2, count);
             CHECK_LAUNCH("k_copy_pop");
-            return run_boundary<double>(h, h->stream, t2, s, t, 0, d.invalid);
+            return run_boundary<double>(h, h->stream, t2, s, t, 0, d.invalid, h->P);
         }
-        return run_boundary<double>(h, h->stream, s, t2, t, 0, d.invalid);
+        return run_boundary<double>(h, h->stream, s, t2, t, 0, d.invalid, h->P);
     }
     float *s = (float *)f_src, *t2 = (float *)f_dst;
     if (in_place) {
         hipLaunchKernelGGL(k_copy_pop<float>, dim3(gc), dim3(BLOCK), 0, h->stream, s, t2, count);
         CHECK_LAUNCH("k_copy_pop");
-        return run_boundary<float>(h, h->stream, t2, s, t, 0, d.invalid);
+        return run_boundary<float>(h, h->stream, t2, s, t, 0, d.invalid, h->P);
     }
-    return run_boundary<float>(h, h->stream, s, t2, t, 0, d.invalid);
+    return run_boundary<float>(h, h->stream, s, t2, t, 0, d.invalid, h->P);
 }
 
 int vlbm3d_init_populations(vlbm3d_handle *h, void *f) {

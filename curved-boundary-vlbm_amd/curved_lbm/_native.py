@@ -214,6 +214,7 @@ EXPORTED = (
     "vlbm3d_create",
     "vlbm3d_destroy",
     "vlbm3d_step",
+    "vlbm3d_set_graph_steps",
     "vlbm3d_refresh",
     "vlbm3d_apply_boundary",
     "vlbm3d_init_populations",
@@ -238,6 +239,14 @@ EXPORTED = (
     "vlbm2d_set_dynamic_traction",
     "vlbm2d_set_graph_steps",
     "vlbm2d_reduce",
+    "vlbm2d_batch_create",
+    "vlbm2d_batch_select",
+    "vlbm2d_batch_set_graph_steps",
+    "vlbm2d_batch_parity",
+    "vlbm2d_batch_step",
+    "vlbm2d_batch_destroy",
+    "vlbm2d_batch_autotune",
+    "vlbm2d_batch_set_blocks",
     "vlbm2d_populations_to_f64",
     "vlbm2d_populations_from_f64",
     "vlbm2d_timing_enable",
@@ -286,6 +295,7 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vlbm3d_create": [C.POINTER(Vlbm3dDesc), C.POINTER(vp)],
         "vlbm3d_destroy": [vp],
         "vlbm3d_step": [vp, vp, vp, C.c_double, C.c_int64, C.c_int32],
+        "vlbm3d_set_graph_steps": [vp, C.c_int32],
         "vlbm3d_refresh": [vp, vp],
         "vlbm3d_apply_boundary": [vp, vp, vp, C.c_double, C.c_int32],
         "vlbm3d_init_populations": [vp, vp],
@@ -310,6 +320,14 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vlbm2d_set_dynamic_traction": [vp, C.c_int32],
         "vlbm2d_set_graph_steps": [vp, C.c_int32],
         "vlbm2d_reduce": [vp, vp, _dp],
+        "vlbm2d_batch_create": [vp, vp, vp, C.c_int32, vp, C.POINTER(vp)],
+        "vlbm2d_batch_select": [vp, vp, vp, vp, C.c_int32],
+        "vlbm2d_batch_set_graph_steps": [vp, C.c_int32],
+        "vlbm2d_batch_parity": [vp],
+        "vlbm2d_batch_step": [vp, _dp, C.c_int32],
+        "vlbm2d_batch_destroy": [vp],
+        "vlbm2d_batch_autotune": [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_float)],
+        "vlbm2d_batch_set_blocks": [vp, C.c_int32, C.c_int32],
         "vlbm2d_populations_to_f64": [vp, vp, vp],
         "vlbm2d_populations_from_f64": [vp, vp, vp],
         "vlbm2d_timing_enable": [vp, C.c_int32],

@@ -599,86 +599,19 @@ def build_tangent_lsq_weights(geometry, tables, *, max_stencil: int = MAX_LSQ_ST
     nearest active nodes within (r + 1/4) cells of the cut point, r = 3, 4, 5
     in turn, a Gaussian-weighted quadratic least-squares fit (rank 10,
     cond <= 1e9), and the derivative weights along t1, t2 of the normal's
-    basis.  ``builder="native"`` runs the threaded C++ builder of libvlbm,
-    ``"numpy"`` the restatement below (same samples and validity; weights
-    equal to rounding).  Returns ``lsq_p`` (active-node index), ``lsq_wt1``,
-    ``lsq_wt2`` as (L, max_stencil) and ``lsq_valid`` (L)."""
+    basis, by the threaded C++ builder of libvlbm (include/vlbm_geom.h; the
+    reference's samples and validity, weights to rounding).  Returns
+    ``lsq_p`` (active-node index), ``lsq_wt1``, ``lsq_wt2`` as
+    (L, max_stencil) and ``lsq_valid`` (L)."""
+    if builder != "native":
+        raise ValueError("tangent builder must be 'native'")
+    from .. import _native
+
     g = geometry
-    if builder == "native":
-        from .. import _native
-
-        p, w1, w2, valid = _native.geom3d_tangent_weights(
-            tables["active_id"], g.link_i, g.link_j, g.link_k, g.link_xb, g.link_yb, g.link_zb, g.link_nx,
-            g.link_ny, g.link_nz, g.dx, g.dy, g.dz, max_stencil)
-        return {"lsq_p": p, "lsq_wt1": w1, "lsq_wt2": w2, "lsq_valid": valid}
-    if builder != "numpy":
-        raise ValueError("tangent builder must be 'native' or 'numpy'")
-    L = int(g.n_links)
-    out_p = np.full((L, max_stencil), -1, dtype=np.int32)
-    wt1 = np.zeros((L, max_stencil))
-    wt2 = np.zeros((L, max_stencil))
-    valid = np.zeros(L, dtype=np.int32)
-    active_id = np.asarray(tables["active_id"], dtype=np.int32)
-    nx, ny, nz = g.nx, g.ny, g.nz
-    dx, dy, dz = float(g.dx), float(g.dy), float(g.dz)
-    xs = (np.arange(nx, dtype=np.float64) + 0.5) * dx
-    ys = (np.arange(ny, dtype=np.float64) + 0.5) * dy
-    zs = (np.arange(nz, dtype=np.float64) + 0.5) * dz
-
-    def fit(ell, radius):
-        xb, yb, zb = float(g.link_xb[ell]), float(g.link_yb[ell]), float(g.link_zb[ell])
-        t1, t2 = _basis_np(np.array([g.link_nx[ell], g.link_ny[ell], g.link_nz[ell]], dtype=np.float64))
-        i0, j0, k0 = int(g.link_i[ell]), int(g.link_j[ell]), int(g.link_k[ell])
-        r2max = (float(radius) + 0.25) ** 2
-        cand = []
-        for ii in range(max(0, i0 - radius), min(nx, i0 + radius + 1)):
-            for jj in range(max(0, j0 - radius), min(ny, j0 + radius + 1)):
-                for kk in range(max(0, k0 - radius), min(nz, k0 + radius + 1)):
-                    pid = int(active_id[ii, jj, kk])
-                    if pid < 0:
-                        continue
-                    rx, ry, rz = (float(xs[ii]) - xb) / dx, (float(ys[jj]) - yb) / dy, (float(zs[kk]) - zb) / dz
-                    r2 = rx * rx + ry * ry + rz * rz
-                    if r2 <= r2max:
-                        cand.append((r2, pid, rx, ry, rz))
-        cand.sort(key=lambda c: c[0])
-        if len(cand) < 10:
-            return None
-        cand = cand[:max_stencil]
-        A = np.array([(1.0, rx, ry, rz, 0.5 * rx * rx, rx * ry, rx * rz, 0.5 * ry * ry, ry * rz, 0.5 * rz * rz)
-                      for _r2, _p, rx, ry, rz in cand])
-        wd = np.array([math.exp(-0.5 * c[0] / (1.75 * 1.75)) + 1.0e-4 for c in cand])
-        if np.linalg.matrix_rank(A, tol=1.0e-10) < 10:
-            return None
-        M = A.T @ (A * wd[:, None])
-        cond = np.linalg.cond(M)
-        if not np.isfinite(cond) or cond > 1.0e9:
-            return None
-        coeff = np.linalg.solve(M + 1.0e-12 * np.eye(10), A.T * wd[None, :])
-        w1 = np.array([0.0, t1[0] / dx, t1[1] / dy, t1[2] / dz] + [0.0] * 6) @ coeff
-        w2 = np.array([0.0, t2[0] / dx, t2[1] / dy, t2[2] / dz] + [0.0] * 6) @ coeff
-        h = max(dx, dy, dz)
-        if not (np.all(np.isfinite(w1)) and np.all(np.isfinite(w2))):
-            return None
-        if abs(float(np.sum(w1))) * h > 1.0e-6 or abs(float(np.sum(w2))) * h > 1.0e-6:
-            return None
-        if float(np.sum(np.abs(w1))) * h > 12.0 or float(np.sum(np.abs(w2))) * h > 12.0:
-            return None
-        return cand, w1, w2
-
-    for ell in range(L):
-        for radius in (3, 4, 5):
-            res = fit(ell, radius)
-            if res is not None:
-                break
-        if res is None:
-            continue
-        cand, w1, w2 = res
-        for m, c in enumerate(cand):
-            out_p[ell, m] = c[1]
-            wt1[ell, m], wt2[ell, m] = float(w1[m]), float(w2[m])
-        valid[ell] = 1
-    return {"lsq_p": out_p, "lsq_wt1": wt1, "lsq_wt2": wt2, "lsq_valid": valid}
+    p, w1, w2, valid = _native.geom3d_tangent_weights(
+        tables["active_id"], g.link_i, g.link_j, g.link_k, g.link_xb, g.link_yb, g.link_zb, g.link_nx,
+        g.link_ny, g.link_nz, g.dx, g.dy, g.dz, max_stencil)
+    return {"lsq_p": p, "lsq_wt1": w1, "lsq_wt2": w2, "lsq_valid": valid}
 
 
 # ---- host boundary-value helpers (cbw3d:802-960) -----------------------------
@@ -844,6 +777,8 @@ class WarpSparseCurvedBoundaryLBM3D:
     Storage is O(n_active): populations (72, n_active), fields (c, n_active).
     ``precision="fp32"`` stores populations as float deviations from the rest
     state while moments, kinematics, traction solves and reductions stay FP64.
+    ``graph_steps`` (even, 0 = off) replays blocks of that many steps of a
+    ``run`` from captured HIP graphs (bitwise the same results).
     """
 
     def __init__(
@@ -879,6 +814,7 @@ class WarpSparseCurvedBoundaryLBM3D:
         normal_jacobian: Literal["finite_difference", "acoustic"] | None = None,
         neumann_solver: Literal["newton", "energy"] | None = None,
         precision: Precision = "fp64",
+        graph_steps: int = 0,
         _box=None,
     ):
         # _box: dict(side_kind (6), value_mode (6), params (6, 16), lengths (3)) of
@@ -886,6 +822,9 @@ class WarpSparseCurvedBoundaryLBM3D:
         # (vector_nonlinear_elastic_warp3d.WarpNonlinearElasticLBM3D); None for
         # a curved geometry.
         self._box_sides = _box
+        if int(graph_steps) < 0 or int(graph_steps) % 2:
+            raise ValueError("graph_steps must be a non-negative even count")
+        self.graph_steps = int(graph_steps)
         self.geometry = geometry
         self.nx, self.ny, self.nz = geometry.nx, geometry.ny, geometry.nz
         self.length_x, self.length_y, self.length_z = geometry.length_x, geometry.length_y, geometry.length_z
@@ -1122,6 +1061,10 @@ class WarpSparseCurvedBoundaryLBM3D:
         _native.check(self._lib.vlbm3d_create(C.byref(d), C.byref(h)), "vlbm3d_create")
         self._handle = h
         self._native = _native
+        self._apply_graph_steps()
+
+    def _apply_graph_steps(self) -> None:
+        self._check(self._lib.vlbm3d_set_graph_steps(self._handle, self.graph_steps), "vlbm3d_set_graph_steps")
 
     def _fill_box_desc(self, d, box) -> None:
         d.box = 1
@@ -1148,6 +1091,7 @@ class WarpSparseCurvedBoundaryLBM3D:
         h = C.c_void_p()
         self._check(self._lib.vlbm3d_create(C.byref(self._desc), C.byref(h)), "vlbm3d_create")
         self._handle = h
+        self._apply_graph_steps()
 
     def __del__(self):
         h = getattr(self, "_handle", None)
@@ -1168,6 +1112,7 @@ class WarpSparseCurvedBoundaryLBM3D:
         h = C.c_void_p()
         self._check(self._lib.vlbm3d_create(C.byref(self._desc), C.byref(h)), "vlbm3d_create")
         self._handle = h
+        self._apply_graph_steps()
 
     def _to_device(self, host: np.ndarray):
         """Host array -> device tensor, copied on the solver's stream so the
@@ -1269,6 +1214,20 @@ class WarpSparseCurvedBoundaryLBM3D:
 
     def clear_dynamic_traction_override(self) -> None:
         self.dynamic_traction_override_enabled.zero_()
+
+    def set_boundary_params(self, boundary_id: int, params) -> None:
+        """Replace the value-mode parameters of one boundary id (its row of
+        the per-id table, up to 16 values) for the following steps; the
+        boundary kinds and value modes stay.  Used to step a family of loads
+        on one geometry without rebuilding the solver."""
+        bid = int(boundary_id)
+        if not 0 <= bid < MAX_BOUNDARY_IDS:
+            raise ValueError(f"boundary id must be in [0, {MAX_BOUNDARY_IDS})")
+        vals = [float(v) for v in tuple(params)[:PARAMS_PER_ID]]
+        row = np.zeros(PARAMS_PER_ID)
+        row[: len(vals)] = vals
+        self.params_by_id[bid] = row
+        self.params_by_id_wp.assign(self.params_by_id)
 
     # ---- initialisation (cbw3d:4176-4279) ----------------------------------
     def _active_from_dense(self, arr, ncomp, name):

@@ -136,6 +136,12 @@ int vlbm3d_destroy(vlbm3d_handle *h);
  * newest populations are in f0 when nsteps is even and in f1 when odd.
  * f0/f1 are double planes (precision F64) or float planes (precision F32). */
 int vlbm3d_step(vlbm3d_handle *h, void *f0, void *f1, double time, int64_t steps, int32_t nsteps);
+/* Replays blocks of `steps` fused steps (even, 0 = off) from captured HIP
+ * graphs whenever a vlbm3d_step call has a uniform schedule (sweep and F
+ * update every step or never); the boundary time is then kept on the device.
+ * Results are bitwise those of direct launches.  No reference counterpart
+ * (the reference launches every kernel from the host, cbw3d:4549-4589). */
+int vlbm3d_set_graph_steps(vlbm3d_handle *h, int32_t steps);
 /* Moments of f into U, u = u_star + dt/2 v, invalid flag. */
 int vlbm3d_refresh(vlbm3d_handle *h, const void *f);
 /* Boundary reconstruction at time t.  in_place = 0: reads f_src, writes the

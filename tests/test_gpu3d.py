@@ -250,3 +250,28 @@ def test_gpu_x_tube_is_the_relabelled_z_tube(n, steps):
         b = _relabel_z_to_x(getattr(sz, name).numpy())
         assert np.linalg.norm(a - b) <= 1e-10 * np.linalg.norm(b), name
     assert np.max(np.abs(sx.u.numpy())) > 1e-3
+
+
+def test_gpu3d_graph_step_loop_matches_direct_launches():
+    """Captured step graphs (8 fused steps each) replayed across calls of
+    odd and even lengths, with eager work between them (refresh, reductions,
+    host reads) and a load change in the device parameter table, give
+    bitwise the populations and fields of direct launches; the step graph is
+    also rejected for odd counts."""
+    g, kw = gpu_cases.traction_shell(16, traction=0.03, ramp_steps=30)
+    a = cb.WarpCurvedBoundaryLBM3D(g, **kw, graph_steps=0)
+    b = cb.WarpCurvedBoundaryLBM3D(g, **kw, graph_steps=8)
+    for calls, n in enumerate((37, 20, 9, 64, 33)):
+        if calls == 3:
+            for s in (a, b):
+                s.set_boundary_params(cb.BOUNDARY_ID_OUTER, (0.02, 30 * s.dt))
+        a.run(n)
+        b.run(n)
+        for name in ("f0", "U", "u", "u_star"):
+            np.testing.assert_array_equal(getattr(a, name).numpy(), getattr(b, name).numpy(),
+                                          err_msg=f"{name} call {calls}")
+        assert a.reductions() == b.reductions()
+    assert a.time == b.time and a.steps == b.steps
+    assert np.max(np.abs(a.u.numpy())) > 1e-6
+    with pytest.raises(ValueError):
+        cb.WarpCurvedBoundaryLBM3D(g, **kw, graph_steps=3)
