@@ -126,10 +126,10 @@ class BatchedCurvedBoundaryLBM2D:
         self.mode = mode
         self._batch = None
         if mode == "streams":
-            if len({id(s._stream) for s in solvers}) != len(solvers):
+            if len({s._stream.cuda_stream for s in solvers}) != len(solvers):
                 raise ValueError("every solver of a batch needs its own stream")
             return
-        if len({id(s._stream) for s in solvers}) != 1:
+        if len({s._stream.cuda_stream for s in solvers}) != 1:
             raise ValueError("batched solvers must share one stream")
         from .. import _native
 

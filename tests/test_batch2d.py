@@ -115,11 +115,14 @@ SEED0_DIVERGING = (202, 32)      # sides 128 and 512; both blow up within the fi
 def _oracle_close(s, o, act, what):
     from oracle.oracle2d import OracleSolver2D  # noqa: F401
 
-    def rel(a, b):
-        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    # the single-solver bar: 1e-10 relative plus an absolute floor of 1e-14
+    # per entry, since after a few steps u is still ~1e-8 and its entries near
+    # the symmetry axes are rounding-level
+    def close(a, b):
+        return np.linalg.norm(a - b) <= 1e-10 * np.linalg.norm(b) + 1e-14 * np.sqrt(b.size)
 
-    assert rel(s.system_state()[:, act], o.U[:, act]) < 1e-10, what
-    assert rel(s.displacement()[:, act], o.u[:, act]) < 1e-10, what
+    assert close(s.system_state()[:, act], o.U[:, act]), what
+    assert close(s.displacement()[:, act], o.u[:, act]), what
     assert np.max(np.abs(s.numpy_field("f0")[:, act] - o.f0[:, act])) < 1e-12, what
 
 
@@ -150,9 +153,17 @@ def test_gpu_seed0_batch_matches_oracle():
 
 @pytest.mark.gpu
 def test_gpu_seed0_diverging_shells_diverge_with_the_oracle():
-    """Shells 202 and 32 of the seed-0 draw: batched FP64 steps and the
-    oracle agree (1e-8 relative) while the state is moderate, and both turn
-    invalid (non-finite or J <= 0) in the same 50-step window."""
+    """Shells 202 and 32 of the seed-0 draw: batched FP64 steps against the
+    oracle while the state is moderate, and both turn invalid (non-finite or
+    J <= 0) in the same 50-step window.
+
+    On the way to blowing up these shells amplify any rounding difference
+    exponentially, so a fixed relative bar does not fit late windows.  A
+    second oracle, started from populations perturbed by 1e-13 relative,
+    measures that amplification: the batch may differ from the oracle by
+    1e-10 relative or by ten times the perturbed oracle's difference,
+    whichever is larger.  A wrong term gives an O(1) difference from the
+    first window, far above both."""
     pytest.importorskip("torch")
     from oracle.oracle2d import OracleSolver2D
 
@@ -160,15 +171,21 @@ def test_gpu_seed0_diverging_shells_diverge_with_the_oracle():
     inputs = [bt.shell_solver_inputs(inst[k]) for k in SEED0_DIVERGING]
     b = bt.BatchedCurvedBoundaryLBM2D.build(inputs, precision="fp64", mode="batch")
     oras = [OracleSolver2D(g, **kw) for g, kw in inputs]
+    twins = [OracleSolver2D(g, **kw) for g, kw in inputs]
+    rng = np.random.default_rng(5)
+    for t in twins:
+        t.f0 *= 1.0 + 1e-13 * rng.standard_normal(t.f0.shape)
     broke = [None, None]
     live = [0, 1]
     for window in range(40):
         b._advance(live, 50)
         for k in list(live):
             oras[k].run(50)
-            s, o = b.solvers[k], oras[k]
+            twins[k].run(50)
+            s, o, w = b.solvers[k], oras[k], twins[k]
             s.refresh_current()
             o.refresh_current()
+            w.refresh_current()
             act = s.active_mask()
             Jo = o.deformation_jacobian()[act]
             o_bad = (not np.isfinite(o.U[:, act]).all()) or Jo.min() <= 0.0
@@ -179,8 +196,10 @@ def test_gpu_seed0_diverging_shells_diverge_with_the_oracle():
                 broke[k] = window
                 live.remove(k)
             elif np.max(np.abs(o.U[:2, act])) < 0.5:
-                Us, Uo = s.system_state()[:, act], o.U[:, act]
-                assert np.linalg.norm(Us - Uo) <= 1e-8 * np.linalg.norm(Uo), (SEED0_DIVERGING[k], window)
+                Us, Uo, Uw = s.system_state()[:, act], o.U[:, act], w.U[:, act]
+                d_gpu, d_twin = np.linalg.norm(Us - Uo), np.linalg.norm(Uw - Uo)
+                assert d_gpu <= max(1e-10 * np.linalg.norm(Uo), 10.0 * d_twin), \
+                    (SEED0_DIVERGING[k], window, d_gpu, d_twin)
         if not live:
             break
     assert broke[0] is not None and broke[1] is not None, broke
@@ -213,7 +232,8 @@ class _StubSolver:
     that moves by rate * dt per step, optionally turning non-finite."""
 
     def __init__(self, rate, dt=1e-3, blow_at=None):
-        self._stream = object()
+        self._stream = type("Stream", (), {})()
+        self._stream.cuda_stream = id(self._stream)  # a distinct raw handle per stub
         self.u = _Field([0.0])
         self.steps, self.dt, self.rate, self.blow_at = 0, dt, rate, blow_at
         self.n_active = 10
