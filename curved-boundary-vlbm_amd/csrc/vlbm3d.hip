@@ -68,6 +68,11 @@ struct Params {
     int disp_bid;
     double blend;
     const double *clock;  // device time of the step start (captured step loops), or nullptr
+    // Lattice speed along each axis (all c except for an anisotropic box,
+    // vnew3d:1240-1367) and cr[a] = c / cs[a], the factor that turns the
+    // record's F-flux planes (kept with 1/c) into the axis's 1/cs[a]; cr is
+    // exactly 1 for equal speeds, so isotropic results do not change.
+    double cs[3], cr[3];
 };
 
 // Boundary time of a step: the host value, or clock + dt/2 when the launch
@@ -118,17 +123,18 @@ __device__ __forceinline__ void rst_a(float *r, size_t i, int a, double w, doubl
 __device__ __forceinline__ void rst_b(double *r, size_t i, double v) { r[i] = v; }
 __device__ __forceinline__ void rst_b(float *r, size_t i, double v) { r[i] = (float)v; }
 
-// fpost[q][a] at `node` from its pre-collision value fv and the node's record.
+// fpost[q][a] at `node` from its pre-collision value fv and the node's
+// record; fs = Params::cr[q % 3] scales the F-flux term to the speed of q's axis.
 template <typename R>
 __device__ __forceinline__ double fpost_v(double fv, const R *rec, size_t n, int q, int a, int node, double om1,
-                                          double w) {
+                                          double w, double fs) {
     const int axis = q % 3;
     const double s = q < 3 ? 1.0 : -1.0;
     double b = 0.0;
     if (a < 3) {
         b = rld_b(rec, (size_t)(R_BV + 3 * a + axis) * n + node);
     } else if ((a - 3) % 3 == axis) {
-        b = rld_b(rec, (size_t)(R_BF + (a - 3) / 3) * n + node);
+        b = rld_b(rec, (size_t)(R_BF + (a - 3) / 3) * n + node) * fs;
     }
     return om1 * fv + rld(rec, (size_t)a * n + node, a, w) + s * b;
 }
@@ -136,8 +142,8 @@ __device__ __forceinline__ double fpost_v(double fv, const R *rec, size_t n, int
 // fpost[q][a] at node m from pre-collision f and the node's record.
 template <typename T>
 __device__ __forceinline__ double fpost(const T *f, const T *rec, size_t n, int q, int a, int node, double om1,
-                                        double w) {
-    return fpost_v(ld(f, (size_t)(q * 12 + a) * n + node, a), rec, n, q, a, node, om1, w);
+                                        double w, double fs) {
+    return fpost_v(ld(f, (size_t)(q * 12 + a) * n + node, a), rec, n, q, a, node, om1, w, fs);
 }
 
 __global__ void k_reset_invalid(int32_t *invalid) { invalid[0] = 0; }
@@ -381,7 +387,7 @@ __global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const 
     }
     const double w = P.omega;
     const double st = P.dt * (2.0 - w) * (1.0 / 12.0);
-    const double w6 = w / 6.0, w2c = w / (2.0 * P.c), h2c = 0.5 / P.c;
+    const double w6 = w / 6.0, w2c = w / (2.0 * P.c);
 #pragma unroll
     for (int a = 0; a < 3; ++a)
         rst_a(rec, (R_AV + a) * n + p, R_AV + a, w,
@@ -391,7 +397,10 @@ __global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const 
         rst_a(rec, (R_AF + k) * n + p, R_AF + k, w,
               w6 * Fo[k] + st * source_comp(3 + k, P.src_mode, P.src, v[0], v[1], v[2]) + dF[k] / 6.0);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) rst_b(rec, (R_BV + k) * n + p, -w2c * Po[k] - h2c * dP[k]);
+    for (int k = 0; k < 9; ++k) {
+        const double ca = P.cs[k % 3];  // plane 3 r + axis: the speed of its axis
+        rst_b(rec, (R_BV + k) * n + p, -(w / (2.0 * ca)) * Po[k] - (0.5 / ca) * dP[k]);
+    }
 #pragma unroll
     for (int i = 0; i < 3; ++i) rst_b(rec, (R_BF + i) * n + p, -w2c * v[i]);
 }
@@ -424,7 +433,8 @@ __global__ __launch_bounds__(BLOCK) void k_stream(const T *__restrict__ f0, T *_
 #pragma unroll
             for (int a = 0; a < NC; ++a) {
                 const double x =
-                    st_nt(f1, (size_t)(q * 12 + a) * n + p, a, fpost_v(fv[a], rec, n, q, a, src, om1, P.omega));
+                    st_nt(f1, (size_t)(q * 12 + a) * n + p, a,
+                          fpost_v(fv[a], rec, n, q, a, src, om1, P.omega, P.cr[q % 3]));
                 if (NEXT) Um[a] += x;
             }
         } else if (NEXT) {
@@ -726,8 +736,8 @@ __global__ __launch_bounds__(128) void k_links(const T *__restrict__ fsrc, T *__
         }
         double fo, fi;
         if (use_rec) {
-            fo = fpost(fsrc, rec, n, qo, a, p, om1, P.omega);
-            fi = fpost(fsrc, rec, n, qi, a, p, om1, P.omega);
+            fo = fpost(fsrc, rec, n, qo, a, p, om1, P.omega, P.cr[qo % 3]);
+            fi = fpost(fsrc, rec, n, qi, a, p, om1, P.omega, P.cr[qi % 3]);
         } else {
             fo = ld(fsrc, (size_t)(qo * 12 + a) * n + p, a);
             fi = ld(fsrc, (size_t)(qi * 12 + a) * n + p, a);
@@ -739,7 +749,8 @@ __global__ __launch_bounds__(128) void k_links(const T *__restrict__ fsrc, T *__
         } else {
             double upv = fo;
             if (up >= 0)
-                upv = use_rec ? fpost(fsrc, rec, n, qo, a, up, om1, P.omega) : ld(fsrc, (size_t)(qo * 12 + a) * n + up, a);
+                upv = use_rec ? fpost(fsrc, rec, n, qo, a, up, om1, P.omega, P.cr[qo % 3])
+                              : ld(fsrc, (size_t)(qo * 12 + a) * n + up, a);
             val = D * (2.0 * eta * fo + (1.0 - 2.0 * eta) * upv) + S;
         }
         outv[a] = val;
@@ -790,7 +801,7 @@ __global__ __launch_bounds__(BLOCK) void k_init_pop(const double *__restrict__ U
             else if ((a - 3) % 3 == axis)
                 flux = -Uv[(a - 3) / 3];
             const double B = source_comp(a, P.src_mode, P.src, Uv[0], Uv[1], Uv[2]);
-            double fv = (1.0 / 6.0) * (Uv[a] + 3.0 * (s * flux / P.c));
+            double fv = (1.0 / 6.0) * (Uv[a] + 3.0 * (s * flux / P.cs[axis]));
             fv -= 0.5 * P.dt / 6.0 * B;
             st(f, (size_t)(q * 12 + a) * n + p, a, fv);
         }
@@ -961,7 +972,9 @@ __global__ __launch_bounds__(128) void k_box_sides3(const T *fsrc, T *fdst, cons
     double fo[12];
 #pragma unroll
     for (int a = 0; a < 12; ++a)
-        fo[a] = use_rec ? fpost(fsrc, rec, n, qo, a, p, om1, P.omega) : ld(fsrc, (size_t)(qo * 12 + a) * n + p, a);
+        fo[a] = use_rec ? fpost(fsrc, rec, n, qo, a, p, om1, P.omega, P.cr[axis])
+                        : ld(fsrc, (size_t)(qo * 12 + a) * n + p, a);
+    const double cn = P.cs[axis];  // speed along the face normal (vnew3d:2123-2127)
     double val[12];
     if (kind == BC_DIR) {
 #pragma unroll
@@ -970,14 +983,14 @@ __global__ __launch_bounds__(128) void k_box_sides3(const T *fsrc, T *fdst, cons
         for (int r = 0; r < 3; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                val[3 + 3 * r + c] = fo[3 + 3 * r + c] + (c == axis ? (double)outward : 0.0) * v[r] / P.c;
+                val[3 + 3 * r + c] = fo[3 + 3 * r + c] + (c == axis ? (double)outward : 0.0) * v[r] / cn;
     } else {
         double F[9];
 #pragma unroll
         for (int a = 0; a < 9; ++a) F[a] = U[(size_t)(3 + a) * n + p];
         solve_normal_column3(F, axis, sign, v, P.m);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) val[c] = fo[c] + v[c] / P.c;
+        for (int c = 0; c < 3; ++c) val[c] = fo[c] + v[c] / cn;
 #pragma unroll
         for (int a = 0; a < 9; ++a) val[3 + a] = -fo[3 + a] + F[a] / 3.0;
     }
@@ -1122,6 +1135,10 @@ static Params make_params(const vlbm3d_desc &d) {
     P.w2 = d.disp_weight * d.disp_weight;
     P.disp_bid = d.disp_boundary_id;
     P.blend = d.compat_blend;
+    for (int a = 0; a < 3; ++a) {
+        P.cs[a] = (d.box && d.box_speed[a] > 0.0) ? d.box_speed[a] : d.c;
+        P.cr[a] = d.c / P.cs[a];
+    }
     return P;
 }
 

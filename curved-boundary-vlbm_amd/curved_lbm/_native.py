@@ -15,7 +15,7 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libvlbm.so"
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 ABI2_VERSION = 6
 PREC_F64 = 0
 PREC_F32 = 1
@@ -125,6 +125,7 @@ class Vlbm3dDesc(C.Structure):
         ("box_value_mode", C.c_int32 * 6),
         ("box_params", C.c_double * 96),
         ("box_length", C.c_double * 3),
+        ("box_speed", C.c_double * 3),
         ("stream", C.c_void_p),
     ]
 

@@ -184,18 +184,23 @@ class WarpNonlinearElasticLBM3D:
         if nx < 4 or ny < 4 or nz < 4:
             raise ValueError("nx, ny, and nz must be at least 4")
         speeds = _speed_tuple(lattice_speed)
-        if max(speeds) - min(speeds) > 0.0:
-            raise ValueError("anisotropic lattice speeds are not provided by the HIP solver")
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.length_x, self.length_y, self.length_z = float(length_x), float(length_y), float(length_z)
         self.dx, self.dy, self.dz = self.length_x / self.nx, self.length_y / self.ny, self.length_z / self.nz
-        if max(abs(self.dx - self.dy), abs(self.dx - self.dz)) > 1.0e-12 * max(1.0, self.dx):
-            raise ValueError("directional spacings and lattice speeds must define one common dt")
         self.material = material
         self.boundaries = boundaries or BoxBoundarySet.periodic()
         self.boundaries.validate_periodic_pairs()
-        self.lattice_speed_x = self.lattice_speed_y = self.lattice_speed_z = speeds[0]
+        # per-axis lattice speeds with one common dt (reference :2342-2361);
+        # the scalar lattice_speed is the limiting (smallest) one
+        self.lattice_speed_x, self.lattice_speed_y, self.lattice_speed_z = speeds
         self.lattice_speeds = speeds
+        if not 0.0 < float(collision_omega) <= 2.0:
+            raise ValueError("collision_omega must be in (0, 2]")
+        if check_linearized_cfl and material.linearized_stability_ratio(min(speeds)) >= 1.0:
+            raise ValueError("linearized D3Q6 stability ratio must be below one")
+        directional_dt = (self.dx / speeds[0], self.dy / speeds[1], self.dz / speeds[2])
+        if max(abs(v - directional_dt[0]) for v in directional_dt[1:]) > 1.0e-12 * max(1.0, abs(directional_dt[0])):
+            raise ValueError("directional spacings and lattice speeds must define one common dt")
         ei, ef = np.zeros(0, dtype=np.int32), np.zeros(0)
         geometry = cbw.CurvedBoundaryGeometry3D(
             nx=self.nx, ny=self.ny, nz=self.nz, length_x=self.length_x, length_y=self.length_y,
@@ -205,11 +210,11 @@ class WarpNonlinearElasticLBM3D:
         self._s = cbw.WarpCurvedBoundaryLBM3D(
             geometry, material=material, boundary=cbw.CurvedBoundarySpec3D(kind="neumann"),
             lattice_speed=speeds[0], collision_omega=collision_omega, device=device,
-            check_linearized_cfl=check_linearized_cfl, jacobian_floor=jacobian_floor, source_mode=source_mode,
+            check_linearized_cfl=False, jacobian_floor=jacobian_floor, source_mode=source_mode,
             source_vector=source_vector, damping_gamma=damping_gamma, precision=precision,
             _box=self._box_dict(self.boundaries))
         s = self._s
-        self.lattice_speed, self.collision_omega, self.dt = s.lattice_speed, s.collision_omega, s.dt
+        self.lattice_speed, self.collision_omega, self.dt = min(speeds), s.collision_omega, s.dt
         self.device, self.jacobian_floor = s.device, s.jacobian_floor
         self.source_mode, self.source_params = s.source_mode, s.source_params
         self.shape = (self.nx, self.ny, self.nz)
@@ -219,7 +224,8 @@ class WarpNonlinearElasticLBM3D:
 
     def _box_dict(self, b: BoxBoundarySet) -> dict:
         return {"side_kind": np.array(b.side_kind), "value_mode": np.array(b.value_mode),
-                "params": np.array(b.params), "lengths": (self.length_x, self.length_y, self.length_z)}
+                "params": np.array(b.params), "lengths": (self.length_x, self.length_y, self.length_z),
+                "speeds": tuple(self.lattice_speeds)}
 
     time = property(lambda self: self._s.time, lambda self, v: setattr(self._s, "time", v))
     steps = property(lambda self: self._s.steps, lambda self, v: setattr(self._s, "steps", v))

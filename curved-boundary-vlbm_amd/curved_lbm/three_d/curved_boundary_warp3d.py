@@ -1071,6 +1071,7 @@ class WarpSparseCurvedBoundaryLBM3D:
         for a, n in enumerate(self.active_np.shape):
             d.box_n[a] = int(n)
             d.box_length[a] = float(box["lengths"][a])
+            d.box_speed[a] = float(box.get("speeds", (self.lattice_speed,) * 3)[a])
         for s in range(6):
             d.box_side_kind[s] = int(box["side_kind"][s])
             d.box_value_mode[s] = int(box["value_mode"][s])

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define VLBM3D_ABI_VERSION 6
+#define VLBM3D_ABI_VERSION 7
 #define VLBM3D_MAX_IDS 16
 #define VLBM3D_NPRM 16 /* parameters per boundary id: params_by_id is (MAX_IDS, NPRM) */
 #define VLBM3D_MAX_CHAR 64
@@ -115,6 +115,9 @@ typedef struct {
     int32_t box_value_mode[6];       /* box value modes 0..5 */
     double box_params[96];           /* 16 params per face */
     double box_length[3];
+    double box_speed[3];             /* lattice speeds along x, y, z (one dt: dx/cx = dy/cy = dz/cz), as
+                                        vnew3d:2342-2361; a zero entry means c.  The curved solver
+                                        ignores them. */
     void *stream;                    /* hipStream_t; NULL = null stream */
 } vlbm3d_desc;
 

@@ -54,7 +54,7 @@ class _State(C.Structure):
         ("dyn_enabled", _dp), ("dyn_traction", _dp),
         ("lkind", _dp), ("lcomp_kind", _dp), ("lbv", _dp), ("lFb", _dp), ("lPb", _dp),
         ("box", C.c_int32), ("box_n", C.c_int32 * 3), ("side_kind", C.c_int32 * 6), ("side_mode", C.c_int32 * 6),
-        ("side_params", C.c_double * 96), ("box_length", C.c_double * 3),
+        ("side_params", C.c_double * 96), ("box_length", C.c_double * 3), ("box_speed", C.c_double * 3),
     ]
 
 
@@ -316,7 +316,9 @@ class OracleBox3D(OracleSolver3D):
     """CPU FP64 oracle of the rectangular-box solver
     (vector_nonlinear_elastic_warp3d.py:2305-2744): every node active, no
     cut links, periodic axes wrapped in the neighbour table, half-way faces
-    (_apply_boundary_side_kernel, :1999-2200).  One lattice speed."""
+    (_apply_boundary_side_kernel, :1999-2200).  lattice_speed is one speed or
+    the three per-axis speeds of one common dt (:2342-2361, the anisotropic
+    equilibrium of :1240-1367 and the face-normal speed of :2123-2127)."""
 
     def __init__(self, nx, ny, nz, *, material, boundaries=None, length_x=1.0, length_y=1.0, length_z=1.0,
                  lattice_speed=5.0, collision_omega=1.8, jacobian_floor=1e-12, source_mode="none",
@@ -336,10 +338,18 @@ class OracleBox3D(OracleSolver3D):
         tables = cb._sparse_tables_from_geometry(g)
         self._box_shape = (int(nx), int(ny), int(nz))
         tables["neighbor"] = self._neighbor(tables, self.boundaries)
+        from curved_lbm.three_d import vector_nonlinear_elastic_warp3d as ve3
+
+        self.lattice_speeds = ve3._speed_tuple(lattice_speed)
+        h = (float(length_x) / int(nx), float(length_y) / int(ny), float(length_z) / int(nz))
+        dts = [h[a] / self.lattice_speeds[a] for a in range(3)]
+        if max(abs(d - dts[0]) for d in dts[1:]) > 1e-12 * max(1.0, abs(dts[0])):
+            raise ValueError("directional spacings and lattice speeds must define one common dt")
         super().__init__(g, material=material, boundary=cb.CurvedBoundarySpec3D(kind="neumann"),
-                         lattice_speed=lattice_speed, collision_omega=collision_omega, boundary_reconstruction="bfl",
-                         jacobian_floor=jacobian_floor, source_mode=source_mode, source_vector=source_vector,
-                         damping_gamma=damping_gamma, threads=threads, tables=tables)
+                         lattice_speed=self.lattice_speeds[0], collision_omega=collision_omega,
+                         boundary_reconstruction="bfl", jacobian_floor=jacobian_floor, source_mode=source_mode,
+                         source_vector=source_vector, damping_gamma=damping_gamma, threads=threads, tables=tables,
+                         check_linearized_cfl=False)
         self._set_faces(self.boundaries)
         self.initialize_identity()
 
@@ -352,6 +362,7 @@ class OracleBox3D(OracleSolver3D):
         for a in range(3):
             s.box_n[a] = self._box_shape[a]
             s.box_length[a] = self.length[a]
+            s.box_speed[a] = self.lattice_speeds[a]
         for k in range(6):
             s.side_kind[k] = int(b.side_kind[k])
             s.side_mode[k] = int(b.value_mode[k])
@@ -383,7 +394,7 @@ class OracleBox3D(OracleSolver3D):
         if init_order == 2:
             b = self.boundaries
             f = ve3._second_order_populations_np(
-                U, self.material, self.lattice_speed, dx=self.dx, dy=self.dy, dz=self.dz, dt=self.dt,
+                U, self.material, self.lattice_speeds, dx=self.dx, dy=self.dy, dz=self.dz, dt=self.dt,
                 periodic_x=b.periodic_x, periodic_y=b.periodic_y, periodic_z=b.periodic_z,
                 source_mode=self.source_mode, source_params=self.source_params)
             self.f0[...] = f.reshape(72, n)

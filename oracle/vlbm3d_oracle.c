@@ -187,6 +187,22 @@ static inline double equilibrium_comp(int q, int a, const double *U, const doubl
     return (1.0 / 6.0) * (U[a] + (3.0 / c) * (sx * phix + sy * phiy + sz * phiz));
 }
 
+/* Lattice speed along an axis: the box's per-axis speeds (vnew3d:2342-2361),
+ * else the single c. */
+static inline double axis_speed(const or3d_state *s, int axis) {
+    return (s->box && s->box_speed[axis] > 0.0) ? s->box_speed[axis] : s->c;
+}
+
+/* vnew3d.py:1240-1367 (_equilibrium_anisotropic_comp): equal speeds take
+ * the isotropic form, otherwise each axis's flux is divided by its speed. */
+static inline double equilibrium_aniso(const or3d_state *s, int q, int a, const double *U, const double *P) {
+    const double cx = axis_speed(s, 0), cy = axis_speed(s, 1), cz = axis_speed(s, 2);
+    if (cx == cy && cx == cz) return equilibrium_comp(q, a, U, P, cx);
+    double sx = dir_x(q), sy = dir_y(q), sz = dir_z(q);
+    double phix = flux_comp(0, a, U, P), phiy = flux_comp(1, a, U, P), phiz = flux_comp(2, a, U, P);
+    return (1.0 / 6.0) * (U[a] + 3.0 * (sx * phix / cx + sy * phiy / cy + sz * phiz / cz));
+}
+
 /* vnew3d.py:1372-1390 */
 static inline double source_comp(int a, int mode, const double *params, double vx, double vy,
                                  double vz) {
@@ -266,7 +282,7 @@ int or3d_init_populations(or3d_state *s, const double *Uin) {
             for (int a = 0; a < NC; ++a) {
                 /* directional flux: sum over axes of s[axis] * flux[axis] / c */
                 double dfl = 0.0;
-                for (int ax = 0; ax < 3; ++ax) dfl += sd[ax] * flux_comp(ax, a, U, P) / s->c;
+                for (int ax = 0; ax < 3; ++ax) dfl += sd[ax] * flux_comp(ax, a, U, P) / axis_speed(s, ax);
                 double f = (1.0 / 6.0) * (U[a] + 3.0 * dfl);
                 f -= 0.5 * s->dt / 6.0 * B[a];
                 s->f0[IDX(q * 12 + a, p, n)] = f;
@@ -296,7 +312,7 @@ static void collide_inplace(or3d_state *s) {
         PIOLA(U + 3, P);
         for (int q = 0; q < NQ; ++q)
             for (int a = 0; a < NC; ++a) {
-                double feq = equilibrium_comp(q, a, U, P, s->c);
+                double feq = s->box ? equilibrium_aniso(s, q, a, U, P) : equilibrium_comp(q, a, U, P, s->c);
                 double old = s->f0[IDX(q * 12 + a, p, n)];
                 double st = source_comp(a, s->source_mode, s->source_params, U[0], U[1], U[2]);
                 s->f0[IDX(q * 12 + a, p, n)] =
@@ -1135,11 +1151,11 @@ static void solve_normal_column(const or3d_state *s, const double *Fin, int axis
  * right, bottom, top, front, back. */
 static void apply_sides(or3d_state *s, const double *fpost, double *f1, double tb) {
     const int nx = s->box_n[0], ny = s->box_n[1], nz = s->box_n[2], n = s->n_active;
-    const double c = s->c;
     for (int side = 0; side < 6; ++side) {
         const int kind = s->side_kind[side];
         if (kind == 0) continue;
         const int axis = side / 2, hi = side & 1;
+        const double c = axis_speed(s, axis); /* speed along the face normal, vnew3d:2123-2127 */
         const int nn = axis == 0 ? ny * nz : (axis == 1 ? nx * nz : nx * ny);
         const int q_out = hi ? axis : axis + 3, q_in = hi ? axis + 3 : axis;
         const double outward = hi ? 1.0 : -1.0;

@@ -70,6 +70,7 @@ typedef struct {
     int32_t side_kind[6], side_mode[6];
     double side_params[96];
     double box_length[3];
+    double box_speed[3];  /* lattice speeds along x, y, z; a zero entry means c */
 } or3d_state;
 
 int or3d_refresh(or3d_state *s);
