@@ -96,26 +96,56 @@ def train(family, n, loads, U, F, *, epochs=40, batch=8, lr=2e-3, seed=0, val_fr
     hist = []
     t0 = time.perf_counter()
     it = 0
+    # A single oversized Adam step can switch off most ReLUs at once, after
+    # which the network only outputs its last bias.  Guard: a batch loss that
+    # is not finite or exceeds ten times the running mean (after a short
+    # warm-up) rolls the parameters back to the best epoch so far, clears
+    # the Adam moments and halves the learning rate; the best epoch's
+    # parameters are the result.
+    best_loss, best = math.inf, op.params.detach().clone()
+    ema, lr_scale, rollbacks = None, 1.0, 0
     for ep in range(epochs):
         perm = rng.permutation(trn)
-        acc = torch.zeros((), dtype=torch.float64, device=device)
+        acc, used = 0.0, 0
         for b in range(steps_per_epoch):
             idx = torch.from_numpy(perm[b * batch:(b + 1) * batch]).to(device)
             torch.index_select(X, 0, idx, out=op.acts[0])
             torch.index_select(T, 0, idx, out=tbuf)
             op.forward()
-            acc += op.loss_and_backward(tbuf, chw, inv)[0]
-            rate = lr * 0.5 * (1.0 + math.cos(math.pi * it / total))
-            op.adam_step(max(rate, 1e-5))
+            loss = op.loss_and_backward(tbuf, chw, inv)
+            op.stream.synchronize()
+            lv = float(loss[0].item())
             it += 1
-        loss = float(acc.item()) / steps_per_epoch
+            if not math.isfinite(lv) or (ema is not None and it > 50 and lv > 10.0 * ema):
+                op.params.copy_(best)
+                op.adam_m.zero_()
+                op.adam_v.zero_()
+                op.adam_t = 0
+                op._pack()
+                lr_scale *= 0.5
+                rollbacks += 1
+                log(f"[train] {family.name} epoch {ep + 1}: batch loss {lv:.3e} against a running "
+                    f"{ema:.3e}; back to the best epoch, learning rate x{lr_scale:g}")
+                continue
+            ema = lv if ema is None else 0.95 * ema + 0.05 * lv
+            acc += lv
+            used += 1
+            rate = lr_scale * lr * 0.5 * (1.0 + math.cos(math.pi * (it - 1) / total))
+            op.adam_step(max(rate, 1e-5 * lr_scale))
+        loss = acc / max(used, 1)
         hist.append(loss)
+        if used and loss < best_loss:
+            best_loss = loss
+            best.copy_(op.params)
         if ep % max(1, epochs // 10) == 0 or ep == epochs - 1:
             log(f"[train] {family.name} epoch {ep + 1}/{epochs} loss {loss:.4e} ({time.perf_counter() - t0:.1f} s)")
+    op.params.copy_(best)
+    op._pack()
     train_s = time.perf_counter() - t0
     errs = evaluate(op, family, n, [loads[k] for k in val], U[val], F[val], device) if n_val else {}
     return op, {"loss_history": hist, "train_s": train_s, "val_index": val.tolist(), "val_errors": errs,
-                "iterations": total, "batch": batch, "channel_weights": chw[:12].tolist()}
+                "iterations": total, "batch": batch, "channel_weights": chw[:12].tolist(), "best_loss": best_loss,
+                "rollbacks": rollbacks}
 
 
 def evaluate(op, family, n, loads, U, F, device):
