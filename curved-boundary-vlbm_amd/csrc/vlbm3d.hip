@@ -116,12 +116,20 @@ __device__ __forceinline__ double rld(const float *r, size_t i, int a, double w)
 }
 __device__ __forceinline__ double rld_b(const double *r, size_t i) { return r[i]; }
 __device__ __forceinline__ double rld_b(const float *r, size_t i) { return (double)r[i]; }
-__device__ __forceinline__ void rst_a(double *r, size_t i, int, double, double v) { r[i] = v; }
-__device__ __forceinline__ void rst_a(float *r, size_t i, int a, double w, double v) {
-    r[i] = (float)(v - w * rest(a));
+// Records are written once per step and read by other kernels, so the
+// stores are non-temporal: they do not displace the neighbour data that the
+// writing kernel itself reuses from L2.
+__device__ __forceinline__ void rst_a(double *r, size_t i, int, double, double v) {
+    __builtin_nontemporal_store(v, r + i);
 }
-__device__ __forceinline__ void rst_b(double *r, size_t i, double v) { r[i] = v; }
-__device__ __forceinline__ void rst_b(float *r, size_t i, double v) { r[i] = (float)v; }
+__device__ __forceinline__ void rst_a(float *r, size_t i, int a, double w, double v) {
+    __builtin_nontemporal_store((float)(v - w * rest(a)), r + i);
+}
+__device__ __forceinline__ void rst_b(double *r, size_t i, double v) { __builtin_nontemporal_store(v, r + i); }
+__device__ __forceinline__ void rst_b(float *r, size_t i, double v) { __builtin_nontemporal_store((float)v, r + i); }
+// Per-node state touched once by a kernel (no neighbour reuse).
+__device__ __forceinline__ double nt_ld(const double *a, size_t i) { return __builtin_nontemporal_load(a + i); }
+__device__ __forceinline__ void nt_st(double *a, size_t i, double v) { __builtin_nontemporal_store(v, a + i); }
 
 // fpost[q][a] at `node` from its pre-collision value fv and the node's
 // record; fs = Params::cr[q % 3] scales the F-flux term to the speed of q's axis.
@@ -251,7 +259,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const double uc = u[c * n + p];
-        un[c * n + p] = d[c] > 0.0 ? uc + P.disp_relax * (r[c] / d[c] - uc) : uc;
+        nt_st(un, (size_t)c * n + p, d[c] > 0.0 ? uc + P.disp_relax * (r[c] / d[c] - uc) : uc);
     }
 }
 
@@ -287,6 +295,23 @@ __device__ __forceinline__ double du_axis(const double *uc, int p, int pp, int p
         return (u0 - um) / h;
     }
     return 0.0;
+}
+
+// Neighbour table of the stream kernel: the neighbour table with each cut
+// link's missing entry (node link_p, direction link_q) replaced by
+// -(ell + 2), so the fused collide finds the link's incoming population in
+// the link-major copy that k_links writes (one 96-byte record instead of
+// twelve population planes); -1 stays a missing neighbour without a link.
+__global__ __launch_bounds__(BLOCK) void k_nbr_pull(const int32_t *__restrict__ nbr, const int32_t *__restrict__ lp,
+                                                    const int32_t *__restrict__ lq, int32_t *__restrict__ out,
+                                                    int32_t *__restrict__ bad, int n, int L) {
+    const int ell = blockIdx.x * BLOCK + threadIdx.x;
+    if (ell >= L) return;
+    const size_t i = (size_t)lq[ell] * n + lp[ell];
+    if (nbr[i] >= 0)
+        atomicOr(bad, 1);
+    else
+        out[i] = -(ell + 2);
 }
 
 // Second neighbours: nbr2[q][p] = nbr[q][nbr[q][p]] (or -1).
@@ -363,7 +388,7 @@ __global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const 
             }
             if (write_U)
 #pragma unroll
-                for (int k = 0; k < 9; ++k) U[(3 + k) * n + p] = Fn[k];
+                for (int k = 0; k < 9; ++k) nt_st(U, (size_t)(3 + k) * n + p, Fn[k]);
         }
     }
     if (pack_slot) {
@@ -380,9 +405,9 @@ __global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const 
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const size_t i = (size_t)c * n + p;
-            const double delta = ugrad[i] - u[i];
-            u[i] = ugrad[i];
-            us[i] = us[i] + delta;
+            const double un = ugrad[i], delta = un - nt_ld(u, i);
+            nt_st(u, i, un);
+            nt_st(us, i, nt_ld(us, i) + delta);
         }
     }
     const double w = P.omega;
@@ -412,7 +437,7 @@ __global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const 
 // source, u = u* + dt/2 v, u* = u + dt/2 v, and the J check.
 template <typename T, bool NEXT>
 __global__ __launch_bounds__(BLOCK) void k_stream(const T *__restrict__ f0, T *__restrict__ f1,
-                                                  const T *__restrict__ rec,
+                                                  const T *__restrict__ rec, const T *__restrict__ flink,
                                                   const int32_t *__restrict__ nbr, double *__restrict__ U,
                                                   double *__restrict__ u, double *__restrict__ us,
                                                   int32_t *__restrict__ invalid, Params P) {
@@ -438,8 +463,16 @@ __global__ __launch_bounds__(BLOCK) void k_stream(const T *__restrict__ f0, T *_
                 if (NEXT) Um[a] += x;
             }
         } else if (NEXT) {
+            // the cut link's population, written by k_links: from its
+            // link-major copy, else (box faces) from the population planes
+            if (src <= -2) {
+                const T *fl = flink + (size_t)(-2 - src) * NC;
 #pragma unroll
-            for (int a = 0; a < NC; ++a) Um[a] += ld(f1, (size_t)(q * 12 + a) * n + p, a);
+                for (int a = 0; a < NC; ++a) Um[a] += ld(fl, (size_t)a, a);
+            } else {
+#pragma unroll
+                for (int a = 0; a < NC; ++a) Um[a] += ld(f1, (size_t)(q * 12 + a) * n + p, a);
+            }
         }
     }
     if (NEXT) {
@@ -454,12 +487,12 @@ __global__ __launch_bounds__(BLOCK) void k_stream(const T *__restrict__ f0, T *_
             Um[2] = Um[2] / denom;
         }
 #pragma unroll
-        for (int a = 0; a < NC; ++a) U[a * n + p] = Um[a];
+        for (int a = 0; a < NC; ++a) nt_st(U, (size_t)a * n + p, Um[a]);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const double uk = us[k * n + p] + hdt * Um[k];
-            u[k * n + p] = uk;
-            us[k * n + p] = uk + hdt * Um[k];
+            const double uk = nt_ld(us, (size_t)k * n + p) + hdt * Um[k];
+            nt_st(u, (size_t)k * n + p, uk);
+            nt_st(us, (size_t)k * n + p, uk + hdt * Um[k]);
         }
         if (det3(Um + 3) <= P.m.jf) atomicOr(invalid, 1);
     }
@@ -694,6 +727,7 @@ __global__ __launch_bounds__(128) void k_prepare(const double *__restrict__ U, c
 // (time step) or the raw populations (standalone reconstruction).
 template <typename T>
 __global__ __launch_bounds__(128) void k_links(const T *__restrict__ fsrc, T *__restrict__ fdst,
+                                               T *__restrict__ flink,
                                                const T *__restrict__ rec, const double *__restrict__ U,
                                                LinkTables Tb, int32_t *__restrict__ invalid, int use_rec,
                                                int has_char, Params P) {
@@ -777,6 +811,8 @@ __global__ __launch_bounds__(128) void k_links(const T *__restrict__ fsrc, T *__
     }
 #pragma unroll
     for (int a = 0; a < NC; ++a) st(fdst, (size_t)(qi * 12 + a) * n + p, a, outv[a]);
+#pragma unroll
+    for (int a = 0; a < NC; ++a) st(flink, (size_t)ell * NC + a, a, outv[a]);
 }
 
 // f = feq(U) - dt/(2Q) B(U) (vnew3d:535-560 as used by cbw3d:4251-4253).
@@ -1028,14 +1064,20 @@ struct vlbm3d_handle {
     // the displacement sweep and the second-neighbour table.
     int32_t *anc_off = nullptr, *anc_key = nullptr, *nbr2 = nullptr;
     double *anc_dat = nullptr;
+    // Stream kernel's neighbour table and the link-major copy of the cut
+    // links' populations (L x 12 in the population precision).
+    int32_t *nbr_pull = nullptr;
+    void *flink = nullptr;
     void release() {
         drop_graphs();
         if (cap) (void)hipStreamDestroy(cap);
         cap = nullptr;
-        for (void *ptr : {(void *)anc_off, (void *)anc_key, (void *)nbr2, (void *)anc_dat, (void *)clock})
+        for (void *ptr : {(void *)anc_off, (void *)anc_key, (void *)nbr2, (void *)anc_dat, (void *)clock,
+                          (void *)nbr_pull, flink})
             if (ptr) (void)hipFree(ptr);
-        anc_off = anc_key = nbr2 = nullptr;
+        anc_off = anc_key = nbr2 = nbr_pull = nullptr;
         anc_dat = nullptr;
+        flink = nullptr;
         clock = nullptr;
     }
 };
@@ -1113,6 +1155,27 @@ static int build_anchor_tables(vlbm3d_handle *h) {
                        (const int32_t *)d.neighbor, h->nbr2, (int)n);
     e = hipGetLastError();
     if (e != hipSuccess) return fail("vlbm3d_create: k_nbr2", e);
+    // stream kernel's neighbour table and the link-major population copy
+    const size_t esz = d.precision == VLBM_PREC_F32 ? sizeof(float) : sizeof(double);
+    e = hipMalloc((void **)&h->nbr_pull, (size_t)NQ * n * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc(&h->flink, std::max<size_t>(L, 1) * NC * esz);
+    int32_t *bad = nullptr;
+    if (e == hipSuccess) e = hipMalloc((void **)&bad, sizeof(int32_t));
+    if (e == hipSuccess) e = hipMemsetAsync(bad, 0, sizeof(int32_t), h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h->nbr_pull, d.neighbor, (size_t)NQ * n * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                           h->stream);
+    if (e == hipSuccess && L > 0) {
+        hipLaunchKernelGGL(k_nbr_pull, dim3((unsigned)((L + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, h->stream,
+                           (const int32_t *)d.neighbor, d.link_p, d.link_q, h->nbr_pull, bad, (int)n, (int)L);
+        e = hipGetLastError();
+    }
+    int32_t hbad = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&hbad, bad, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (bad) (void)hipFree(bad);
+    if (e != hipSuccess) return fail("vlbm3d_create: stream neighbour table", e);
+    if (hbad) return fail_msg("vlbm3d_create: a cut link's neighbour is present in the neighbour table");
     return 0;
 }
 
@@ -1218,7 +1281,7 @@ static int run_boundary(vlbm3d_handle *h, hipStream_t st, const T *src, T *dst, 
         CHECK_LAUNCH("k_prepare");
     }
     TimedSlot ts(h->timing, st, VLBM_SLOT_LINKS);
-    hipLaunchKernelGGL((k_links<T>), dim3(gl), dim3(128), 0, st, src, dst, (const T *)d.record,
+    hipLaunchKernelGGL((k_links<T>), dim3(gl), dim3(128), 0, st, src, dst, (T *)h->flink, (const T *)d.record,
                        (const double *)d.U, h->T, invalid, use_rec, d.has_char, P);
     CHECK_LAUNCH("k_links");
     return 0;
@@ -1230,10 +1293,10 @@ static int launch_stream(vlbm3d_handle *h, bool next, const T *f0, T *f1, const 
     const int gn = xcd_grid(d.n_active);
     if (next)
         hipLaunchKernelGGL((k_stream<T, true>), dim3(gn), dim3(BLOCK), 0, h->stream, f0, f1, (const T *)d.record,
-                           d.neighbor, d.U, d.u, d.u_star, d.invalid, P);
+                           (const T *)h->flink, (const int32_t *)h->nbr_pull, d.U, d.u, d.u_star, d.invalid, P);
     else
         hipLaunchKernelGGL((k_stream<T, false>), dim3(gn), dim3(BLOCK), 0, h->stream, f0, f1, (const T *)d.record,
-                           d.neighbor, d.U, d.u, d.u_star, d.invalid, P);
+                           (const T *)h->flink, d.neighbor, d.U, d.u, d.u_star, d.invalid, P);
     CHECK_LAUNCH("k_stream");
     return 0;
 }

@@ -141,7 +141,9 @@ def test_gpu_shell128_fp32_against_fp64_oracle():
     for k in (5, 7):
         gpu.run(k)
         ora.run(k)
+    # both sides refreshed: the moments of the newest populations
     gpu.refresh_current()
+    ora.refresh_current()
     _assert_fp32(_fp32_errors(gpu.U.numpy(), gpu.u.numpy(), ora.U, ora.u, 3))
     assert not gpu.invalid_state()
 
@@ -164,6 +166,7 @@ def test_gpu_superellipse1024_fp32_against_fp64_oracle():
         gpu.run(k)
         ora.run(k)
     gpu.refresh_current()
+    ora.refresh_current()
     act = gpu.active_mask()
     _assert_fp32(_fp32_errors(gpu.system_state()[:, act], gpu.displacement()[:, act], ora.U[:, act],
                               ora.u[:, act], 2))
