@@ -64,13 +64,20 @@ def main() -> int:
     ap.add_argument("--twist-deg", type=float, default=15.0, help="tube workload: end twist in degrees")
     ap.add_argument("--batch-count", type=int, default=256, help="shells of the batch workload")
     ap.add_argument("--graph-steps", type=int, default=50, help="captured steps per graph (batch workload)")
+    ap.add_argument("--batch-mode", choices=("batch", "streams"), default="batch",
+                    help="batch workload: one launch per kernel per step over all shells, or a stream per shell")
     ap.add_argument("--n", type=int, default=0, help="lattice side (default 128 / 1024 / 96 by workload)")
     ap.add_argument("--precision", choices=("fp64", "fp32"), default=None, help="populations (fp64; batch fp32)")
     ap.add_argument("--cpu-steps", type=int, default=40, help="oracle steps timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--chunk", type=int, default=500, help="steps per library call")
     ap.add_argument("--equilibrium", action="store_true",
-                    help="run FP64 and FP32 from rest to the rms_v tolerance and report wall-s, steps and FP32 error")
+                    help="only the FP64 / FP32 runs from rest to the residual tolerance (wall-s, steps, FP32 error)")
+    ap.add_argument("--window", choices=("loaded", "rest"), default="loaded",
+                    help="timed steps start from the loaded equilibrium (3D: the FP64 run to the residual; 2D: past "
+                         "the load ramp) or from rest")
+    ap.add_argument("--no-warm", action="store_true", help="skip the equilibrium clause (plain / FP32 / warm start)")
+    ap.add_argument("--cpu-repeats", type=int, default=5, help="repeats of the CPU baseline sample (median)")
     ap.add_argument("--tol", type=float, default=1e-9, help="equilibrium residual rms_v tolerance")
     ap.add_argument("--check-every", type=int, default=200)
     ap.add_argument("--max-steps", type=int, default=60000)
@@ -111,6 +118,24 @@ def main() -> int:
             s.run(c)
             k -= c
 
+    ramp_time = case["ramp_steps"] * g.dx / kw["lattice_speed"]
+    plain = None
+    if args.window == "loaded":
+        if case["dim"] == 3:
+            # the loaded regime: this solver's own run from rest to the
+            # equilibrium residual (also the FP64 plain leg of the
+            # equilibrium clause when the populations are FP64)
+            torch.cuda.synchronize()
+            t_eq = time.perf_counter()
+            rec = s.run_to_equilibrium(args.tol, check_every=args.check_every, min_time=ramp_time,
+                                       max_steps=args.max_steps)
+            torch.cuda.synchronize()
+            plain = {"wall_s": time.perf_counter() - t_eq, "steps": rec["steps"], "converged": rec["converged"]}
+            print(f"[bench] {args.precision} plain run to the residual: {plain}", file=sys.stderr, flush=True)
+            s.refresh_current()
+            plain["state"] = (s.U.numpy().copy(), s.u.numpy().copy())
+        else:
+            advance(case["ramp_steps"] + 200)
     advance(args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -179,23 +204,15 @@ def main() -> int:
         torch.cuda.synchronize()
         del src, dst
 
+    eq = None
+    if (args.workload in ("shell", "tube") and args.precision == "fp64" and plain is not None
+            and not args.no_warm):
+        eq = equilibrium_clause(args, case, plain)
+        print(f"[bench] equilibrium clause: {json.dumps(eq)}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
-        threads = min(16, cpu_count())
-        os.environ["OMP_NUM_THREADS"] = str(threads)
-        ora = case["oracle"](s, threads)
-        ora.run(1)
-        tc = time.perf_counter()
-        ora.run(args.cpu_steps)
-        cpu_s = time.perf_counter() - tc
-        cpu = {
-            "value": round(n * args.cpu_steps / cpu_s / 1e6, 3),
-            "unit": "MLUPS",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"{args.cpu_steps} FP64 steps of the same {case['short']} after 1 warm-up step, "
-                      f"C oracle with OpenMP ({cpu_s:.2f} s)",
-        }
+        cpu = cpu_baseline(args, case, s, n)
 
     out = {
         "metric": METRIC if args.workload == "shell" else case["metric"],
@@ -216,14 +233,124 @@ def main() -> int:
             "n_active": n,
             "n_links": int(g.n_links),
             "precision": args.precision,
+            "window": ("loaded: timed steps continue from this solver's own run from rest to the equilibrium "
+                       "residual" if case["dim"] == 3 else "loaded: timed steps start after the load ramp")
+            if args.window == "loaded" else "rest: timed steps start after the warmup from rest",
             "box_mlups": round(box_mlups, 2),
             "setup_s": round(setup_s, 2),
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if eq is not None:
+        out["equilibrium"] = eq
     print(json.dumps(out))
     return 0
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(args, case, s, n) -> dict:
+    """The C oracle (FP64, OpenMP) on this host's cores: the median of
+    --cpu-repeats repeats of --cpu-steps steps of the same workload from the
+    GPU solver's current state, after one warm-up step.  Threads: the CPUs
+    this process may use, capped by OMP_NUM_THREADS when the host sets it
+    (the CPU share of one GPU on a shared box)."""
+    import numpy as np
+
+    aff = cpu_count()
+    threads = min(aff, int(os.environ.get("OMP_NUM_THREADS", aff) or aff))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    ora = case["oracle"](s, threads)
+    ora.run(1)
+    times = []
+    for _ in range(max(1, args.cpu_repeats)):
+        tc = time.perf_counter()
+        ora.run(args.cpu_steps)
+        times.append(time.perf_counter() - tc)
+    med = float(np.median(times))
+    return {
+        "value": round(n * args.cpu_steps / med / 1e6, 3),
+        "unit": "MLUPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"median of {len(times)} repeats of {args.cpu_steps} FP64 steps of the same {case['short']} "
+                  f"(each {min(times):.2f}-{max(times):.2f} s) after 1 warm-up step, C oracle with OpenMP on "
+                  f"{threads} threads of {aff} affinity CPUs ({cpu_model()})",
+    }
+
+
+def equilibrium_clause(args, case, plain) -> dict:
+    """Wall-seconds to the equilibrium residual: FP64 populations from rest
+    (the run that brought the timed solver to the loaded state), FP32 populations from
+    rest, and FP32 populations from the learned warm start (prediction
+    included; falls back to rest if the prediction fails its checks); the
+    FP32 and warm results against the FP64 one (relative l2 of u and F, max
+    |J| difference)."""
+    import numpy as np
+    import torch
+
+    g, kw = case["geometry"], case["kwargs"]
+    ramp_time = case["ramp_steps"] * g.dx / kw["lattice_speed"]
+    U64, u64 = plain["state"]  # the FP64 run's state when it met the residual
+
+    def fields(solver):
+        solver.refresh_current()
+        return solver.U.numpy().copy(), solver.u.numpy().copy()
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    def err(U, u):
+        J = np.linalg.det(np.moveaxis(U[3:].reshape(3, 3, -1), -1, 0))
+        J64 = np.linalg.det(np.moveaxis(U64[3:].reshape(3, 3, -1), -1, 0))
+        return {"u_rel": rel(u, u64), "F_rel": rel(U[3:], U64[3:]),
+                "F_minus_I_rel": rel(U[3:] - np.eye(3).reshape(9, 1), U64[3:] - np.eye(3).reshape(9, 1)),
+                "J_max_abs": float(np.max(np.abs(J - J64)))}
+
+    out = {"tol": args.tol, "criterion": "rms displacement rate |u(t) - u(t - T)| / T, T = %d steps" % args.check_every,
+           "fp64_plain_s": round(plain["wall_s"], 3), "fp64_plain_steps": plain["steps"]}
+    s32 = case["solver"]("fp32")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r32 = s32.run_to_equilibrium(args.tol, check_every=args.check_every, min_time=ramp_time, max_steps=args.max_steps)
+    torch.cuda.synchronize()
+    out["fp32_s"] = round(time.perf_counter() - t0, 3)
+    out["fp32_steps"] = r32["steps"]
+    out["fp32_error"] = err(*fields(s32))
+    del s32
+    model_path = ROOT / "curved-boundary-vlbm_amd" / "curved_lbm" / "surrogate" / "weights" / case["model"]
+    if model_path.exists():
+        from curved_lbm.surrogate.train import WarmStartModel
+
+        model = WarmStartModel(model_path)
+        sw = case["solver"]("fp32")
+        rw = model.start(sw, case["load"], tol=args.tol, check_every=args.check_every, max_steps=args.max_steps)
+        out["fp32_warm_s"] = round(rw["wall_s"], 3)
+        out["fp32_warm_steps"] = rw["steps"]
+        out["warm_start_used"] = bool(rw["warm"])
+        out["fallback_reason"] = rw["fallback_reason"]
+        out["predict_s"] = round(rw["predict_s"], 4)
+        out["fp32_warm_error"] = err(*fields(sw))
+        out["speedup"] = round(plain["wall_s"] / rw["wall_s"], 3)
+        out["model"] = {"file": str(model_path.relative_to(ROOT)), "family": model.meta["family"],
+                        "lattice": list(model.family.lattice(model.n)), "samples": model.meta["converged"],
+                        "held_out_errors": model.meta["val_errors"]}
+        del sw
+    else:
+        out["fp32_warm_s"] = None
+        out["speedup"] = None
+        out["model"] = f"missing: {model_path.name}"
+    out["speedup_fp32_plain"] = round(plain["wall_s"] / out["fp32_s"], 3)
+    return out
 
 
 def make_case(args) -> dict:
@@ -245,6 +372,7 @@ def make_case(args) -> dict:
             "geometry": g, "kwargs": kw, "fq": 72, "dim": 3, "stream_kernel": "k_stream",
             "solver": lambda prec: cb.WarpCurvedBoundaryLBM3D(g, **kw, precision=prec), "oracle": oracle,
             "ramp_steps": 4000, "short": f"{args.n}^3 shell", "lattice": [args.n] * 3, "metric": METRIC,
+            "model": "shell64.npz", "load": {"traction": 0.03},
             "description": f"D3Q6x12 spherical shell {args.n}^3, Ri 0.2L clamped, Ro 0.45L nominal traction 0.03 "
                            "ramped linearly over 4000 steps, manuscript settings",
         }
@@ -268,6 +396,7 @@ def make_case(args) -> dict:
             "geometry": g, "kwargs": kw, "fq": 72, "dim": 3, "stream_kernel": "k_stream",
             "solver": lambda prec: cb.WarpCurvedBoundaryLBM3D(g, **kw, precision=prec), "oracle": oracle_t,
             "ramp_steps": ramp, "short": "x 2:1:1 tube", "lattice": shape,
+            "model": "tube24.npz", "load": {"stretch": 1.0 + args.stretch, "twist": math.radians(args.twist_deg)},
             "metric": f"MLUPS at D3Q6x12 {shape[0]}x{shape[1]}x{shape[2]} x-axis tube",
             "description": f"D3Q6x12 tube along x on {shape[0]}x{shape[1]}x{shape[2]} (length 2, outer / inner radius "
                            f"0.4 / 0.2), traction-free walls, end x = 0 clamped, end x = 2 moved by the exact stretch "
@@ -306,9 +435,9 @@ def make_case(args) -> dict:
 
 
 def batch(args) -> int:
-    """Aggregate throughput of a batch of uneven 2D shells stepped
-    concurrently (one stream and step graph per shell), beside the same
-    shells stepped one after another.  ``value`` counts the node updates of
+    """Aggregate throughput of a batch of uneven 2D shells stepped together
+    (by default one launch per kernel per step over all shells, block shapes
+    from the autotuner), beside the same shells stepped one after another.  ``value`` counts the node updates of
     the shells that stayed finite only (a shell that diverges still costs
     its kernel time).  With --equilibrium: every shell run to the
     displacement-rate residual --tol after its load ramp, per-shell steps."""
@@ -320,12 +449,19 @@ def batch(args) -> int:
     t0 = time.time()
     inst = bt.batch_instances(args.batch_count, seed=0)
     b = bt.BatchedCurvedBoundaryLBM2D.build([bt.shell_solver_inputs(i) for i in inst], precision=prec,
-                                            graph_steps=args.graph_steps)
+                                            graph_steps=args.graph_steps, mode=args.batch_mode)
+    tune = b.autotune(steps=3) if args.batch_mode == "batch" else None
     setup_s = time.time() - t0
     n_act = b.n_active
-    print(f"[bench] batch setup {setup_s:.1f}s shells={len(inst)} n_active={n_act}", file=sys.stderr, flush=True)
-    workload = (f"{len(inst)} superellipse shells, seed 0, sides {sorted({i.n for i in inst})}, {prec} populations, "
-                f"one stream and {args.graph_steps}-step graph per shell")
+    print(f"[bench] batch setup {setup_s:.1f}s shells={len(inst)} n_active={n_act} autotune={tune}",
+          file=sys.stderr, flush=True)
+    if args.batch_mode == "batch":
+        workload = (f"{len(inst)} superellipse shells, seed 0, sides {sorted({i.n for i in inst})}, {prec} "
+                    f"populations, one launch per kernel per step over all shells, {args.graph_steps}-step "
+                    f"captured graphs, block shapes {tune['node_block']}/{tune['link_block']} from the autotuner")
+    else:
+        workload = (f"{len(inst)} superellipse shells, seed 0, sides {sorted({i.n for i in inst})}, {prec} "
+                    f"populations, one stream and {args.graph_steps}-step graph per shell")
     if args.equilibrium:
         return batch_equilibrium(args, b, inst, prec, workload, setup_s)
     steps, warm = args.steps, args.warmup
@@ -352,7 +488,7 @@ def batch(args) -> int:
         "config": {"workload": workload, "shells": len(inst), "n_active": n_act, "setup_s": round(setup_s, 1),
                    "all_shells_mlups": round(n_act * steps / conc / 1e6, 2),
                    "serial_mlups": round(n_ok * steps / serial / 1e6, 2),
-                   "concurrency_speedup": round(serial / conc, 3), "nonfinite_shells": bad},
+                   "concurrency_speedup": round(serial / conc, 3), "nonfinite_shells": bad, "autotune": tune},
     }
     print(json.dumps(out), flush=True)
     return 0
