@@ -196,7 +196,7 @@ class WarmStartModel:
         self.op.set_input(inputs(self.family, self.n, p, self.device))
         return decode(self.family, self.n, p, self.op.forward())
 
-    def prepare(self, p, *, symmetric=True, integrate=True, integrate_tol=1e-6):
+    def prepare(self, p, *, symmetric=True, integrate=None, integrate_tol=1e-6):
         """(u, F, info) on the coarse lattice.  The network's error breaks the
         load's symmetry and so excites the lowest antisymmetric modes
         (rocking and torsion of the shell, bending of the tube), which the
@@ -204,9 +204,11 @@ class WarmStartModel:
         symmetric error; the group average removes those components.  u is
         then re-integrated from F (least squares pinned at the Dirichlet
         walls) starting from the network's u, which leaves u the displacement
-        of the predicted F."""
+        of the predicted F (by default where the family asks for it)."""
         u, F = self.predict(p)
         info = {}
+        if integrate is None:
+            integrate = self.family.integrate_u
         if symmetric:
             u, F = ws.symmetrize(u, F, self._mask, self.family.symmetry_group())
         if integrate:
@@ -215,7 +217,7 @@ class WarmStartModel:
                                                 u0=u, tol=integrate_tol)
         return u, F, info
 
-    def start(self, solver, p, *, tol=1e-9, check_every=200, max_steps=10**6, symmetric=True, integrate=True,
+    def start(self, solver, p, *, tol=1e-9, check_every=200, max_steps=10**6, symmetric=True, integrate=None,
               **kw):
         """Warm-started run of a solver of the same family (any lattice
         side that refines the coarse one); returns the run record with the
