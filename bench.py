@@ -20,9 +20,11 @@ stretch --stretch plus twist --twist-deg, ramped over 2000 steps), local_f.
 ``--workload batch``: --batch-count superellipse shells drawn from seed 0
 (exponent U[2,6], a U[0.30,0.45] L, b/a U[0.5,1], rotation U[0,180) deg,
 inner circle 0.1 L clamped, traction U[0.01,0.05] ramped over 5000 steps,
-side from {128,192,256,384,512}), FP32 populations by default, every shell
-on its own HIP stream and step graph; reports the aggregate MLUPS of the
-concurrent batch and of the same shells stepped one after another.
+side from {128,192,256,384,512}), FP32 populations by default, stepped as
+one batch (one launch per kernel per step over all shells, captured step
+graphs, block shapes chosen by the autotuner; ``--batch-mode streams``: every
+shell on its own stream and graph); reports the aggregate MLUPS of the batch
+and of the same shells stepped one after another.
 
 Prints one JSON line: MLUPS (active-node updates per second) of the whole
 step with the populations resident in HBM, the roofline of the dominant
@@ -292,7 +294,9 @@ def equilibrium_clause(args, case, plain) -> dict:
     """Wall-seconds to the equilibrium residual: FP64 populations from rest
     (the run that brought the timed solver to the loaded state), FP32 populations from
     rest, and FP32 populations from the learned warm start (prediction
-    included; falls back to rest if the prediction fails its checks); the
+    included, after one untimed pass of the prediction path whose first-use
+    cost is reported as predict_first_use_s; falls back to rest if the
+    prediction fails its checks); the
     FP32 and warm results against the FP64 one (relative l2 of u and F, max
     |J| difference)."""
     import numpy as np
@@ -333,6 +337,9 @@ def equilibrium_clause(args, case, plain) -> dict:
 
         model = WarmStartModel(model_path)
         sw = case["solver"]("fp32")
+        # one untimed pass of the prediction path: the first use of its
+        # torch operators in a process loads their kernels (reported apart)
+        out["predict_first_use_s"] = round(model.warm_up((sw.nx, sw.ny, sw.nz)), 3)
         rw = model.start(sw, case["load"], tol=args.tol, check_every=args.check_every, max_steps=args.max_steps)
         out["fp32_warm_s"] = round(rw["wall_s"], 3)
         out["fp32_warm_steps"] = rw["steps"]

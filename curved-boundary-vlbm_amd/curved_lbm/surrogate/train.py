@@ -217,6 +217,18 @@ class WarmStartModel:
                                                 u0=u, tol=integrate_tol)
         return u, F, info
 
+    def warm_up(self, fine_shape) -> float:
+        """Runs the prediction path once (a training load, carried to
+        fine_shape) so that the first real prediction does not pay the
+        one-time loading of the torch kernels it uses; returns the seconds
+        taken."""
+        torch.cuda.synchronize(self.device)
+        t0 = time.perf_counter()
+        u, F, _ = self.prepare(self.meta["loads"][0])
+        ws.carry_to_lattice(u, F, self._mask, tuple(fine_shape))
+        torch.cuda.synchronize(self.device)
+        return time.perf_counter() - t0
+
     def start(self, solver, p, *, tol=1e-9, check_every=200, max_steps=10**6, symmetric=True, integrate=None,
               **kw):
         """Warm-started run of a solver of the same family (any lattice
