@@ -59,7 +59,8 @@ def run_case(model, p, n, *, tol=1e-9, max_steps=None, carry=True, warm_precisio
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         u, F, _ = model.prepare(p)
-        rw = ws.run_warm(sw, u, F, t_start=ramp_steps * sw.dt, tol=tol, check_every=every, max_steps=max_steps)
+        rw = ws.run_warm(sw, u, F, t_start=ramp_steps * sw.dt, tol=tol, check_every=every, max_steps=max_steps,
+                         max_initial_residual=fam.max_initial_residual)
         rw["wall_s"] = time.perf_counter() - t1
     Uw, uw = _fields(sw)
     del sw

@@ -241,6 +241,7 @@ class WarmStartModel:
         torch.cuda.synchronize(self.device)
         t_pred = time.perf_counter() - t0
         ramp_time = self.family.ramp_steps(solver.nx if self.family.name == "shell" else solver.ny) * solver.dt
+        kw.setdefault("max_initial_residual", self.family.max_initial_residual)
         out = ws.run_warm(solver, uf, Ff, t_start=ramp_time, tol=tol, check_every=check_every, max_steps=max_steps,
                           **kw)
         out["predict_s"] = t_pred
