@@ -363,13 +363,16 @@ __device__ __forceinline__ double du2(const double *uc, int nb, int p, int axis,
 // cbw2d:1820-1843 into the per-node record, plus the last displacement
 // commit.  ug is the displacement the repair differentiates (the sweep's
 // u_next when a sweep ran, else u).
-template <typename R>
+template <typename R, int LAW = -1>
 __device__ __forceinline__ void k2_record_body(double *__restrict__ U, const double *__restrict__ ug,
                                                    double *__restrict__ u, double *__restrict__ us,
                                                    R *__restrict__ rec, const int32_t *__restrict__ list,
                                                    const uint8_t *__restrict__ nbm, int repair, int commit,
                                                    int32_t *__restrict__ reset_invalid, Params2 P, int t) {
     if (t >= P.na) return;
+    // LAW >= 0 fixes the constitutive law at compile time (the launch checks
+    // it is the solver's), so the inlined Piola maps keep only its code.
+    if (LAW >= 0) P.m.model = LAW;
     // Clears the invalid flag ahead of the stream kernel's fused collide
     // (the link rows never set it).
     if (reset_invalid && t == 0) reset_invalid[0] = 0;
@@ -431,13 +434,13 @@ __device__ __forceinline__ void k2_record_body(double *__restrict__ U, const dou
     rst_b(rec, (R_BF + 0) * N + p, -w2c * v0);
     rst_b(rec, (R_BF + 1) * N + p, -w2c * v1);
 }
-template <typename R>
+template <typename R, int LAW>
 __global__ __launch_bounds__(BLOCK) void k2_record(double *__restrict__ U, const double *__restrict__ ug,
                                                    double *__restrict__ u, double *__restrict__ us,
                                                    R *__restrict__ rec, const int32_t *__restrict__ list,
                                                    const uint8_t *__restrict__ nbm, int repair, int commit,
                                                    int32_t *__restrict__ reset_invalid, Params2 P) {
-    k2_record_body<R>(U, ug, u, us, rec, list, nbm, repair, commit, reset_invalid, P, xcd_block() * BLOCK + threadIdx.x);
+    k2_record_body<R, LAW>(U, ug, u, us, rec, list, nbm, repair, commit, reset_invalid, P, xcd_block() * BLOCK + threadIdx.x);
 }
 
 // Quadratic one-sided reconstruction of U[comp] along the inward link
@@ -532,12 +535,13 @@ __global__ __launch_bounds__(256) void k2_gather(const double *__restrict__ U, c
 // read as they are (standalone reconstruction).  EXT instantiates the
 // regularized, NEE, quadratic and polynomial variants (recon_ext), so the
 // kernel of the other reconstructions carries none of their code.
-template <typename T, bool EXT>
+template <typename T, bool EXT, int LAW = -1>
 __device__ __forceinline__ void k2_links_body(const T *__restrict__ fsrc, T *__restrict__ fdst,
                                                    const T *__restrict__ rec, const double *__restrict__ U,
                                                    const double *__restrict__ cs, Links2 Tb, int use_rec,
                                                    const double *__restrict__ gh, double time, Params2 P, int ell) {
     if (ell >= P.L) return;
+    if (LAW >= 0) P.m.model = LAW;
     const int r = P.recon;
     const size_t N = P.N;
     const int L = P.L;
@@ -810,12 +814,12 @@ __device__ __forceinline__ void k2_links_body(const T *__restrict__ fsrc, T *__r
         st(fdst, (size_t)(qi * 6 + a) * N + p, a, val);
     }
 }
-template <typename T, bool EXT>
+template <typename T, bool EXT, int LAW = -1>
 __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T *__restrict__ fdst,
                                                    const T *__restrict__ rec, const double *__restrict__ U,
                                                    const double *__restrict__ cs, Links2 Tb, int use_rec,
                                                    const double *__restrict__ gh, double time, Params2 P) {
-    k2_links_body<T, EXT>(fsrc, fdst, rec, U, cs, Tb, use_rec, gh, time, P, blockIdx.x * LBLOCK + threadIdx.x);
+    k2_links_body<T, EXT, LAW>(fsrc, fdst, rec, U, cs, Tb, use_rec, gh, time, P, blockIdx.x * LBLOCK + threadIdx.x);
 }
 
 // Index along a periodic axis of n nodes (one step off either end wraps).
@@ -1412,12 +1416,11 @@ static int launch_links2(vlbm2d_handle *h, const Params2 &P, hipStream_t st, con
     }
     TimedSlot tl(h->timing, st, VLBM_SLOT_LINKS);
     const int gl = (d.n_links + LBLOCK - 1) / LBLOCK;
-    if (recon_ext(d.recon))
-        hipLaunchKernelGGL((k2_links<T, true>), dim3(gl), dim3(LBLOCK), 0, st, f0, f1, (const T *)d.record,
-                           (const double *)d.U, (const double *)d.link_state, h->T, 1, gin, tb, P);
-    else
-        hipLaunchKernelGGL((k2_links<T, false>), dim3(gl), dim3(LBLOCK), 0, st, f0, f1, (const T *)d.record,
-                           (const double *)d.U, (const double *)d.link_state, h->T, 1, gin, tb, P);
+    const bool nh = d.model == M_NH;
+    auto kern = recon_ext(d.recon) ? (nh ? k2_links<T, true, M_NH> : k2_links<T, true, -1>)
+                                   : (nh ? k2_links<T, false, M_NH> : k2_links<T, false, -1>);
+    hipLaunchKernelGGL(kern, dim3(gl), dim3(LBLOCK), 0, st, f0, f1, (const T *)d.record, (const double *)d.U,
+                       (const double *)d.link_state, h->T, 1, gin, tb, P);
     CHECK_LAUNCH("k2_links");
     return 0;
 }
@@ -1485,7 +1488,8 @@ static int launch_step2(vlbm2d_handle *h, const Params2 &P, T *f0, T *f1, double
     }
     {
         TimedSlot ts(h->timing, st, VLBM_SLOT_RECORD);
-        hipLaunchKernelGGL(k2_record<T>, dim3(gn), dim3(BLOCK), 0, st, d.U,
+        auto kern = d.model == M_NH ? k2_record<T, M_NH> : k2_record<T, -1>;
+        hipLaunchKernelGGL(kern, dim3(gn), dim3(BLOCK), 0, st, d.U,
                            disp ? (const double *)d.u_next : (const double *)d.u, d.u, d.u_star, (T *)d.record,
                            d.active_list, (const uint8_t *)h->nbmask, (int)compat, (int)disp,
                            next ? d.invalid : (int32_t *)nullptr,
@@ -1662,13 +1666,13 @@ __global__ __launch_bounds__(NB) void k2b_sweep(const Job2 *__restrict__ jobs, c
     k2_sweep_body(J.U, J.u, J.u_next, J.list, J.nbm, J.aoff, J.aids, J.T, 0.0, J.P, (b - start[j]) * NB + threadIdx.x);
 }
 
-template <typename T, int NB>
+template <typename T, int NB, int LAW>
 __global__ __launch_bounds__(NB) void k2b_record(const Job2 *__restrict__ jobs, const int *__restrict__ start,
                                                  int njobs, int next) {
     const int b = xcd_block();
     const int j = find_job(start, njobs, b);
     const Job2 &J = jobs[j];
-    k2_record_body<T>(J.U, J.disp ? J.u_next : J.u, J.u, J.u_star, (T *)J.record, J.list, J.nbm, J.compat, J.disp,
+    k2_record_body<T, LAW>(J.U, J.disp ? J.u_next : J.u, J.u, J.u_star, (T *)J.record, J.list, J.nbm, J.compat, J.disp,
                       next ? J.invalid : (int32_t *)nullptr, J.P, (b - start[j]) * NB + threadIdx.x);
 }
 
@@ -1680,13 +1684,13 @@ __global__ __launch_bounds__(256) void k2b_gather(const Job2 *__restrict__ jobs,
     k2_gather_body(J.U, J.u, J.lsq_p, J.wb, J.wx, J.wy, J.wt, J.tangent, J.link_state, J.P.L, J.P.N, b - start[j]);
 }
 
-template <typename T, bool EXT, int NL>
+template <typename T, bool EXT, int NL, int LAW>
 __global__ __launch_bounds__(NL) void k2b_links(const Job2 *__restrict__ jobs, const int *__restrict__ start,
                                                 int njobs, int parity) {
     const int b = blockIdx.x;
     const int j = find_job(start, njobs, b);
     const Job2 &J = jobs[j];
-    k2_links_body<T, EXT>((const T *)J.f[parity], (T *)J.f[parity ^ 1], (const T *)J.record, J.U, J.link_state, J.T,
+    k2_links_body<T, EXT, LAW>((const T *)J.f[parity], (T *)J.f[parity ^ 1], (const T *)J.record, J.U, J.link_state, J.T,
                           1, nullptr, 0.0, J.P, (b - start[j]) * NL + threadIdx.x);
 }
 
@@ -1717,6 +1721,7 @@ struct vlbm2d_batch {
     std::vector<void *> f0, f1;        // their population buffers at parity 0
     int precision = 0;
     bool ext = false;
+    bool nh = false;  // every shell neo-Hooke: record and link kernels specialised on that law
     hipStream_t stream = nullptr;
     // device tables of the active shells
     Job2 *jobs = nullptr;
@@ -1845,20 +1850,17 @@ int batch_step_launches_t(vlbm2d_batch *b, hipStream_t st, int parity, bool next
                            S + 1 * (n + 1), n);
         CHECK_LAUNCH("k2b_sweep");
     }
-    hipLaunchKernelGGL((k2b_record<T, NB>), dim3(batch_grid(b->nblk[BK_NODE])), dim3(NB), 0, st, b->jobs, S, n,
-                       (int)next);
+    auto rec = b->nh ? k2b_record<T, NB, M_NH> : k2b_record<T, NB, -1>;
+    hipLaunchKernelGGL(rec, dim3(batch_grid(b->nblk[BK_NODE])), dim3(NB), 0, st, b->jobs, S, n, (int)next);
     CHECK_LAUNCH("k2b_record");
     if (b->gather && b->nblk[BK_GATHER] > 0) {
         hipLaunchKernelGGL(k2b_gather, dim3(b->nblk[BK_GATHER]), dim3(256), 0, st, b->jobs, S + 2 * (n + 1), n);
         CHECK_LAUNCH("k2b_gather");
     }
     if (b->nblk[BK_LINK] > 0) {
-        if (b->ext)
-            hipLaunchKernelGGL((k2b_links<T, true, NL>), dim3(b->nblk[BK_LINK]), dim3(NL), 0, st, b->jobs,
-                               S + 3 * (n + 1), n, parity);
-        else
-            hipLaunchKernelGGL((k2b_links<T, false, NL>), dim3(b->nblk[BK_LINK]), dim3(NL), 0, st, b->jobs,
-                               S + 3 * (n + 1), n, parity);
+        auto links = b->ext ? (b->nh ? k2b_links<T, true, NL, M_NH> : k2b_links<T, true, NL, -1>)
+                            : (b->nh ? k2b_links<T, false, NL, M_NH> : k2b_links<T, false, NL, -1>);
+        hipLaunchKernelGGL(links, dim3(b->nblk[BK_LINK]), dim3(NL), 0, st, b->jobs, S + 3 * (n + 1), n, parity);
         CHECK_LAUNCH("k2b_links");
     }
     if (next)
@@ -2241,7 +2243,7 @@ int vlbm2d_batch_create(vlbm2d_handle *const *handles, void *const *f0, void *co
     if (!handles || !f0 || !f1 || !out) return fail_msg("vlbm2d_batch_create: null argument");
     if (n < 1 || n > 65536) return fail_msg("vlbm2d_batch_create: n must be in [1, 65536]");
     const int prec = handles[0] ? handles[0]->d.precision : 0;
-    bool ext = false;
+    bool ext = false, nh = true;
     for (int k = 0; k < n; ++k) {
         const vlbm2d_handle *h = handles[k];
         if (!h || !f0[k] || !f1[k]) return fail_msg("vlbm2d_batch_create: null handle or populations");
@@ -2253,12 +2255,14 @@ int vlbm2d_batch_create(vlbm2d_handle *const *handles, void *const *f0, void *co
         if (d.disp_interval > 1 || d.compat_interval > 1 || d.disp_sweeps != 1)
             return fail_msg("vlbm2d_batch_create: batched shells need a uniform schedule (intervals 0 or 1, one sweep)");
         ext = ext || recon_ext(d.recon);
+        nh = nh && d.model == M_NH;
         for (int j = 0; j < k; ++j)
             if (handles[j] == h) return fail_msg("vlbm2d_batch_create: a shell appears twice");
     }
     vlbm2d_batch *b = new vlbm2d_batch;
     b->precision = prec;
     b->ext = ext;
+    b->nh = nh;
     b->stream = (hipStream_t)stream;
     for (int k = 0; k < n; ++k) {
         vlbm2d_handle *h = handles[k];

@@ -63,7 +63,11 @@ __device__ __forceinline__ int cy(int q) { return q == 1 ? 1 : (q == 3 ? -1 : 0)
 
 __device__ __forceinline__ double clamp_j(double j, double f) { return j < f ? f : j; }
 
+// First Piola stress of the 2D laws, evaluated without FMA contraction (each
+// product and sum rounded, as the FP64 oracle is built): the characteristic
+// closure differentiates it by central differences with step ~1e-6.
 __device__ __forceinline__ void piola2(const double *F, const Material2 &m, double *P) {
+#pragma clang fp contract(off)
     const double F11 = F[0], F12 = F[1], F21 = F[2], F22 = F[3];
     const double lam = m.lam, mu = m.mu;
     if (m.model == M_SVK) {
