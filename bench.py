@@ -331,27 +331,49 @@ def equilibrium_clause(args, case, plain) -> dict:
     out["fp32_steps"] = r32["steps"]
     out["fp32_error"] = err(*fields(s32))
     del s32
-    model_path = ROOT / "curved-boundary-vlbm_amd" / "curved_lbm" / "surrogate" / "weights" / case["model"]
+    wdir = ROOT / "curved-boundary-vlbm_amd" / "curved_lbm" / "surrogate" / "weights"
+    model_path = wdir / case["model"]
     if model_path.exists():
+        from curved_lbm.surrogate.refine import RefinedWarmStartModel
         from curved_lbm.surrogate.train import WarmStartModel
 
-        model = WarmStartModel(model_path)
-        sw = case["solver"]("fp32")
-        # one untimed pass of the prediction path: the first use of its
-        # torch operators in a process loads their kernels (reported apart)
-        out["predict_first_use_s"] = round(model.warm_up((sw.nx, sw.ny, sw.nz)), 3)
-        rw = model.start(sw, case["load"], tol=args.tol, check_every=args.check_every, max_steps=args.max_steps)
+        coarse = WarmStartModel(model_path)
+        # the second stage (a network on the solver's own lattice correcting
+        # the carried coarse prediction) where one is trained for this lattice
+        rpath = wdir / case.get("refine", "")
+        model = coarse
+        if case.get("refine") and rpath.exists():
+            refined = RefinedWarmStartModel(rpath, base=coarse)
+            if tuple(refined.family.lattice(refined.n)) == tuple(case["lattice"]):
+                model = refined
+
+        def warm_run(m):
+            sw = case["solver"]("fp32")
+            # one untimed pass of the prediction path: the first use of its
+            # torch operators in a process loads their kernels (reported apart)
+            first = m.warm_up((sw.nx, sw.ny, sw.nz))
+            rw = m.start(sw, case["load"], tol=args.tol, check_every=args.check_every, max_steps=args.max_steps)
+            return rw, first, err(*fields(sw))
+
+        rw, first, e = warm_run(model)
+        out["predict_first_use_s"] = round(first, 3)
         out["fp32_warm_s"] = round(rw["wall_s"], 3)
         out["fp32_warm_steps"] = rw["steps"]
         out["warm_start_used"] = bool(rw["warm"])
         out["fallback_reason"] = rw["fallback_reason"]
         out["predict_s"] = round(rw["predict_s"], 4)
-        out["fp32_warm_error"] = err(*fields(sw))
+        out["fp32_warm_error"] = e
         out["speedup"] = round(plain["wall_s"] / rw["wall_s"], 3)
-        out["model"] = {"file": str(model_path.relative_to(ROOT)), "family": model.meta["family"],
-                        "lattice": list(model.family.lattice(model.n)), "samples": model.meta["converged"],
-                        "held_out_errors": model.meta["val_errors"]}
-        del sw
+        out["model"] = {"file": str(model_path.relative_to(ROOT)), "family": coarse.meta["family"],
+                        "lattice": list(coarse.family.lattice(coarse.n)), "samples": coarse.meta["converged"],
+                        "held_out_errors": coarse.meta["val_errors"]}
+        if model is not coarse:
+            out["model"]["refine"] = {"file": str(rpath.relative_to(ROOT)), "lattice": list(case["lattice"]),
+                                      "samples": model.meta["used"], "held_out_errors": model.meta["val_errors"]}
+            rc, _, ec = warm_run(coarse)
+            out["fp32_warm_coarse_only"] = {"s": round(rc["wall_s"], 3), "steps": rc["steps"],
+                                            "warm_start_used": bool(rc["warm"]), "error": ec,
+                                            "speedup": round(plain["wall_s"] / rc["wall_s"], 3)}
     else:
         out["fp32_warm_s"] = None
         out["speedup"] = None
@@ -379,7 +401,7 @@ def make_case(args) -> dict:
             "geometry": g, "kwargs": kw, "fq": 72, "dim": 3, "stream_kernel": "k_stream",
             "solver": lambda prec: cb.WarpCurvedBoundaryLBM3D(g, **kw, precision=prec), "oracle": oracle,
             "ramp_steps": 4000, "short": f"{args.n}^3 shell", "lattice": [args.n] * 3, "metric": METRIC,
-            "model": "shell64.npz", "load": {"traction": 0.03},
+            "model": "shell64.npz", "refine": "shell128_refine.npz", "load": {"traction": 0.03},
             "description": f"D3Q6x12 spherical shell {args.n}^3, Ri 0.2L clamped, Ro 0.45L nominal traction 0.03 "
                            "ramped linearly over 4000 steps, manuscript settings",
         }

@@ -1,12 +1,13 @@
 """Held-out evaluation of a trained warm-start network.
 
     python -m curved_lbm.surrogate.evaluate --model tube24.npz --pairs 64 --full 4
+    python -m curved_lbm.surrogate.evaluate --model shell128_refine.npz --pairs 8
 
 draws ``pairs`` loads from numpy's default_rng(seed) (a seed the training
 draw did not use) and, for each, runs the solver to the displacement-rate
 residual from rest (FP64 populations) and from the network's prepared
 prediction (FP32 populations, or FP64 with --warm-precision), on the
-network's own lattice; ``full`` of them are also
+network's own lattice (a refinement stage's lattice is the full one); ``full`` of them are also
 run on the family's full lattice (the prediction carried to it).  Prints one
 JSON record: per case the steps and wall-seconds of both starts, whether the
 warm start was used or fell back, and the relative l2 difference of u and F
@@ -25,7 +26,7 @@ import torch
 
 from ..three_d import curved_boundary_warp3d as cb
 from .. import warm_start as ws
-from .train import WarmStartModel
+from .refine import load_stage
 
 
 def _fields(s):
@@ -81,7 +82,7 @@ def main(argv=None) -> int:
     ap.add_argument("--warm-precision", choices=("fp32", "fp64"), default="fp32")
     ap.add_argument("--out", default="")
     args = ap.parse_args(argv)
-    model = WarmStartModel(args.model)
+    model = load_stage(args.model)
     fam = model.family
     rng = np.random.default_rng(args.seed)
     loads = [fam.draw(rng) for _ in range(args.pairs)]

@@ -75,19 +75,36 @@ def train(family, n, loads, U, F, *, epochs=40, batch=8, lr=2e-3, seed=0, val_fr
     X = torch.cat([inputs(family, n, loads[k], device) for k in range(N)])
     T = torch.cat([targets(family, n, loads[k], U[k].astype(np.float64), F[k].astype(np.float64), device)
                    for k in range(N)])
+    op, rec = fit(X, T, trn, family.lattice(n), n_in=family.n_in, active=float(family.geometry(n).active.sum()),
+                  name=family.name, epochs=epochs, batch=batch, lr=lr, seed=seed, device=device, log=log)
+    errs = evaluate(op, family, n, [loads[k] for k in val], U[val], F[val], device) if n_val else {}
+    rec.update(val_index=val.tolist(), val_errors=errs)
+    return op, rec
+
+
+def fit(X, T, trn, shape, *, n_in, active, name, epochs=40, batch=8, lr=2e-3, seed=0, device="cuda:0", log=print):
+    """Trains a ConvOperator on inputs X (N, padded, 32) BF16 and targets T
+    (N, padded, 16) FP32 (channels 0-11 the prediction, 15 the voxel weight)
+    over the samples ``trn``; ``active`` is the number of weighted voxels per
+    sample.  Returns the operator with the best epoch's parameters and the
+    training record."""
     batch = min(batch, len(trn))
-    op = ConvOperator(family.lattice(n), batch=batch, n_in=family.n_in, n_out=12, device=device, seed=seed)
+    op = ConvOperator(shape, batch=batch, n_in=n_in, n_out=12, device=device, seed=seed)
     # every output channel weighted by the inverse of its mean square over
     # the training targets, so displacement and gradient rows count alike;
     # a channel the prior already fits (mean square near zero) is floored at
     # 1% of the channel mean, since the inverse of a near-zero mean square
     # makes the loss follow that channel's noise and the training unstable
-    act = T[trn, ..., 15] > 0
-    ms = torch.stack([(T[trn, ..., c][act] ** 2).mean() for c in range(12)])
+    sq = torch.zeros(12, dtype=torch.float64, device=device)
+    cnt = 0.0
+    for k in trn:  # sample by sample: a fine-lattice target set is several GB
+        act = T[int(k), ..., 15] > 0
+        sq += (T[int(k)][act][:, :12].double() ** 2).sum(0)
+        cnt += float(act.sum())
+    ms = (sq / max(cnt, 1.0)).float()
     ms = ms.clamp(min=max(1e-2 * float(ms.mean()), 1e-12))
     chw = torch.zeros(CH, dtype=torch.float32, device=device)
     chw[:12] = (1.0 / ms).float()
-    active = float(family.geometry(n).active.sum())
     inv = 1.0 / (batch * active * 12)
     tbuf = torch.zeros((batch,) + T.shape[1:], dtype=T.dtype, device=device)
     rng = np.random.default_rng(seed + 1)
@@ -124,7 +141,7 @@ def train(family, n, loads, U, F, *, epochs=40, batch=8, lr=2e-3, seed=0, val_fr
                 op._pack()
                 lr_scale *= 0.5
                 rollbacks += 1
-                log(f"[train] {family.name} epoch {ep + 1}: batch loss {lv:.3e} against a running "
+                log(f"[train] {name} epoch {ep + 1}: batch loss {lv:.3e} against a running "
                     f"{ema:.3e}; back to the best epoch, learning rate x{lr_scale:g}")
                 continue
             ema = lv if ema is None else 0.95 * ema + 0.05 * lv
@@ -138,14 +155,12 @@ def train(family, n, loads, U, F, *, epochs=40, batch=8, lr=2e-3, seed=0, val_fr
             best_loss = loss
             best.copy_(op.params)
         if ep % max(1, epochs // 10) == 0 or ep == epochs - 1:
-            log(f"[train] {family.name} epoch {ep + 1}/{epochs} loss {loss:.4e} ({time.perf_counter() - t0:.1f} s)")
+            log(f"[train] {name} epoch {ep + 1}/{epochs} loss {loss:.4e} ({time.perf_counter() - t0:.1f} s)")
     op.params.copy_(best)
     op._pack()
     train_s = time.perf_counter() - t0
-    errs = evaluate(op, family, n, [loads[k] for k in val], U[val], F[val], device) if n_val else {}
-    return op, {"loss_history": hist, "train_s": train_s, "val_index": val.tolist(), "val_errors": errs,
-                "iterations": total, "batch": batch, "channel_weights": chw[:12].tolist(), "best_loss": best_loss,
-                "rollbacks": rollbacks}
+    return op, {"loss_history": hist, "train_s": train_s, "iterations": total, "batch": batch,
+                "channel_weights": chw[:12].tolist(), "best_loss": best_loss, "rollbacks": rollbacks}
 
 
 def evaluate(op, family, n, loads, U, F, device):

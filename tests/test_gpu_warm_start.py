@@ -85,6 +85,21 @@ def test_gpu_trained_shell_warm_start_beats_the_plain_start():
     assert warm["steps"] < 0.6 * plain["steps"]
     Uw, uw = _fields(sw)
     assert _rel(uw, u64) < 1e-6 and _rel(Uw[3:], U64[3:]) < 1e-6
+    del sw
+
+    # the second stage on the 128^3 lattice starts closer: fewer steps than
+    # the coarse stage alone, the same equilibrium
+    from curved_lbm.surrogate.refine import RefinedWarmStartModel
+
+    refined = RefinedWarmStartModel(WEIGHTS / "shell128_refine.npz", base=model)
+    assert all(abs(q["traction"] - 0.03) > 1e-9 for q in refined.meta["loads"])  # held out
+    s2 = cb.WarpCurvedBoundaryLBM3D(g, **fam.solver_kwargs(n, p), precision="fp32")
+    warm2 = refined.start(s2, p, tol=1e-9, check_every=200, max_steps=60000)
+    print({"refined_steps": warm2["steps"], "warm": warm2["warm"], "predict_s": warm2["predict_s"]})
+    assert warm2["converged"] and warm2["warm"]
+    assert warm2["steps"] < warm["steps"]
+    U2, u2 = _fields(s2)
+    assert _rel(u2, u64) < 1e-6 and _rel(U2[3:], U64[3:]) < 1e-6
 
 
 def test_gpu_trained_tube_warm_start_on_its_lattice():

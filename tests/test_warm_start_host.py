@@ -226,3 +226,63 @@ def test_symmetrize_keeps_invariant_fields_and_averages_the_rest(name):
     bad[0, 0, 0] = ~bad[0, 0, 0]
     with pytest.raises(ValueError):
         ws.symmetrize(ur, Fr, bad, fam.symmetry_group())
+
+
+def test_refine_targets_invert_to_the_equilibrium_and_inputs_carry_the_base_fields():
+    """The second stage's target, (u - u_c, F - F_c) / (REL s), gives back u
+    and F when added to the base fields as the stage's prediction is
+    (u = u_c + REL s y); its input holds the family's load-independent
+    features, the load channel and the base fields scaled by s."""
+    from curved_lbm.surrogate import refine as rf
+
+    fam = fm.FAMILIES["shell"]
+    n, p = 16, {"traction": 0.027}
+    shape = fam.lattice(n)
+    rng = np.random.default_rng(3)
+    uc = torch.from_numpy(rng.normal(scale=1e-3, size=(3,) + shape))
+    Fc = torch.from_numpy(np.eye(3).reshape(9, 1, 1, 1) + rng.normal(scale=1e-2, size=(9,) + shape))
+    u = uc + torch.from_numpy(rng.normal(scale=1e-5, size=(3,) + shape))
+    F = Fc + torch.from_numpy(rng.normal(scale=1e-4, size=(9,) + shape))
+    t = rf.targets(fam, n, p, uc, Fc, u, F)
+    assert tuple(t.shape) == (1,) + tuple(m + 2 for m in shape) + (16,)
+    core = t[0, 1:-1, 1:-1, 1:-1].permute(3, 0, 1, 2).double()
+    s = rf.REL * fam.scale(p)
+    act = torch.from_numpy(fam.geometry(n).active.astype(bool))
+    # the target is stored in FP32: one rounding of the correction
+    tu = 1.2e-7 * float((u - uc).abs().max())
+    tF = 1.2e-7 * float((F - Fc).abs().max())
+    np.testing.assert_allclose((uc + s * core[0:3])[:, act].numpy(), u[:, act].numpy(), rtol=0, atol=tu)
+    np.testing.assert_allclose((Fc + s * core[3:12])[:, act].numpy(), F[:, act].numpy(), rtol=0, atol=tF)
+    assert torch.equal(core[15].bool(), act) and float(t[0, 0].abs().max()) == 0.0
+
+    inp = rf._Inputs(fam, n, "cpu")
+    x = inp.build(p, uc, Fc)[0]
+    feat = fam.features(n, p)
+    k = fam.n_in
+    xc = x[1:-1, 1:-1, 1:-1].float()
+    np.testing.assert_array_equal(xc[..., :k].numpy(), torch.from_numpy(feat).to(torch.bfloat16).float().numpy())
+    want_u = (uc / fam.scale(p)).permute(1, 2, 3, 0).to(torch.bfloat16).float()
+    want_F = ((Fc - torch.eye(3, dtype=Fc.dtype).reshape(9, 1, 1, 1)) / fam.scale(p)).permute(1, 2, 3, 0)
+    assert torch.equal(xc[..., k:k + 3], want_u)
+    assert torch.equal(xc[..., k + 3:k + 12], want_F.to(torch.bfloat16).float())
+    assert float(xc[..., k + 12:].abs().max()) == 0.0 and float(x[0].abs().max()) == 0.0
+    assert inp.n_in == k + 12
+
+
+def test_refine_stage_weights_record():
+    """The shipped second stage: a refinement of the shipped coarse shell
+    network on the 128^3 lattice, trained on loads that leave the
+    benchmark's traction 0.03 out, with its held-out errors below its
+    base's."""
+    import json
+    from pathlib import Path
+
+    w = Path(fm.__file__).parent / "weights"
+    z = np.load(w / "shell128_refine.npz", allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    assert meta["stage"] == "refine" and meta["family"] == "shell" and meta["n"] == 128
+    assert (w / meta["base"]).exists() and meta["n_in"] == fm.FAMILIES["shell"].n_in + 12
+    assert all(abs(q["traction"] - 0.03) > 1e-9 for q in meta["loads"])
+    ve = meta["val_errors"]
+    assert ve["u"]["max"] < 0.5 * ve["base_u"]["mean"]
+    assert ve["F_minus_I"]["max"] < 0.5 * ve["base_F_minus_I"]["mean"]
