@@ -1268,17 +1268,29 @@ class WarpSparseCurvedBoundaryLBM3D:
             raise ValueError(
                 "sparse 3-D curved solver currently supports init_order=1; use dense class for higher-order initialization"
             )
+        # U_active / disp_active: host arrays, or device tensors (the warm
+        # start installs its prediction without a host round trip)
+        on_device = isinstance(U_active, self._torch.Tensor)
         self.U.assign(U_active)
         self._check(self._lib.vlbm3d_init_populations(self._handle, self.f0.ptr), "init_populations")
         self.f1.zero_()
-        self.u_star.assign(np.ascontiguousarray(disp_active - 0.5 * self.dt * U_active[:3]))
+        if on_device:
+            with self._torch.cuda.stream(self._stream):
+                us = disp_active - 0.5 * self.dt * U_active[:3]
+            self.u_star.assign(us)
+        else:
+            self.u_star.assign(np.ascontiguousarray(disp_active - 0.5 * self.dt * U_active[:3]))
         self.time = 0.0
         self.steps = 0
         self.refresh_current()
         if apply_initial_boundaries:
             self.apply_boundary_reconstruction(initial_boundary_time, in_place=True)
             self.refresh_current()
-            disp = self._to_device(np.asarray(disp_active, dtype=np.float64))
+            if on_device:
+                with self._torch.cuda.stream(self._stream):
+                    disp = disp_active.to(device=self.device, dtype=self._torch.float64).contiguous()
+            else:
+                disp = self._to_device(np.asarray(disp_active, dtype=np.float64))
             self._check(self._lib.vlbm3d_set_ustar(self._handle, disp.data_ptr()), "set_ustar")
             self._torch.cuda.synchronize(self.device)
             self.refresh_current()

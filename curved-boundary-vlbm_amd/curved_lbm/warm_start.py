@@ -124,12 +124,49 @@ def transform(u: torch.Tensor, F: torch.Tensor, perm, sign) -> tuple[torch.Tenso
     return u2, F2.reshape(9, *uu.shape[1:])
 
 
+def _is_cube_group(group) -> bool:
+    """Whether group is the full 48-element symmetry group of the cube."""
+    import itertools
+
+    full = {(tuple(p), tuple(s)) for p in itertools.permutations(range(3)) for s in itertools.product((1, -1), repeat=3)}
+    return len(group) == 48 and {(tuple(p), tuple(s)) for p, s in group} == full
+
+
+def _symmetrize_cube(u: torch.Tensor, F: torch.Tensor, mask: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+    """The average over the cube group in nine moves instead of 48: every
+    element is uniquely a sign flip after an axis permutation, so the group
+    average is the average over the eight flips of the average over the six
+    permutations, and the flip average is three successive averages over
+    {identity, flip of one axis} (the flips commute)."""
+    m = mask[None].to(u.dtype)
+    for perm, sign in (((1, 0, 2), (1, 1, 1)), ((1, 2, 0), (1, 1, 1)), ((0, 1, 2), (-1, 1, 1)),
+                       ((0, 1, 2), (1, -1, 1)), ((0, 1, 2), (1, 1, -1))):
+        mm, _ = transform(m.expand(3, *mask.shape), m.expand(9, *mask.shape), perm, sign)
+        if not torch.equal(mm[0].abs(), m[0]):
+            raise ValueError(f"the active cells are not invariant under {perm}, {sign}")
+    import itertools
+
+    su, sF = torch.zeros_like(u), torch.zeros_like(F)
+    for perm in itertools.permutations(range(3)):
+        a, b = transform(u, F, perm, (1, 1, 1))
+        su += a
+        sF += b
+    u, F = su / 6.0, sF / 6.0
+    for axis in range(3):
+        sign = tuple(-1 if k == axis else 1 for k in range(3))
+        a, b = transform(u, F, (0, 1, 2), sign)
+        u, F = 0.5 * (u + a), 0.5 * (F + b)
+    return u, F
+
+
 def symmetrize(u: torch.Tensor, F: torch.Tensor, mask: torch.Tensor, group) -> tuple[torch.Tensor, torch.Tensor]:
     """Average of the fields over a symmetry group of the problem (a list
     of (perm, sign) closed under composition, the identity included).  The
     discrete equilibrium of a problem with that symmetry is invariant, so
     the average keeps it and removes the error components that excite the
     antisymmetric modes.  Raises if the active mask is not invariant."""
+    if _is_cube_group(group):
+        return _symmetrize_cube(u, F, mask)
     su, sF = torch.zeros_like(u), torch.zeros_like(F)
     m = mask[None].to(u.dtype)
     for perm, sign in group:
@@ -237,9 +274,11 @@ def install(solver, u_dense: torch.Tensor, F_dense: torch.Tensor, t_start: float
     ak = torch.from_numpy(solver.active_k.astype(np.int64)).to(u_dense.device)
     U = torch.zeros((12, solver.n_active), dtype=torch.float64, device=u_dense.device)
     U[3:12] = F_dense[:, ai, aj, ak]
-    disp = u_dense[:, ai, aj, ak]
-    solver.initialize_from_numpy(U=U.cpu().numpy(), displacement=disp.cpu().numpy(),
-                                 initial_boundary_time=float(t_start))
+    disp = u_dense[:, ai, aj, ak].to(torch.float64)
+    # initialize_from_numpy's path (cbw3d:4216-4236) with the active-node
+    # values handed over on the device
+    torch.cuda.synchronize(u_dense.device)
+    solver._initialize_from_active(U, disp, initial_boundary_time=float(t_start))
     solver.time = float(t_start)
 
 

@@ -286,3 +286,29 @@ def test_refine_stage_weights_record():
     ve = meta["val_errors"]
     assert ve["u"]["max"] < 0.5 * ve["base_u"]["mean"]
     assert ve["F_minus_I"]["max"] < 0.5 * ve["base_F_minus_I"]["mean"]
+
+
+def test_cube_group_average_in_nine_moves_equals_the_48_term_average():
+    """The fast cube-group average (permutations, then one flip per axis)
+    equals the plain average over all 48 signed permutations, and it leaves
+    a field that is already symmetric unchanged."""
+    import itertools
+
+    n = 10
+    X = np.stack(_cells((n, n, n))) - 0.5
+    mask = torch.from_numpy(np.sqrt((X ** 2).sum(0)) < 0.4)
+    rng = np.random.default_rng(5)
+    u = torch.from_numpy(rng.normal(size=(3, n, n, n)))
+    F = torch.from_numpy(rng.normal(size=(9, n, n, n)))
+    group = [(p, s) for p in itertools.permutations(range(3)) for s in itertools.product((1, -1), repeat=3)]
+    fu, fF = ws.symmetrize(u, F, mask, group)
+    su, sF = torch.zeros_like(u), torch.zeros_like(F)
+    for p, s in group:
+        a, b = ws.transform(u, F, p, s)
+        su += a
+        sF += b
+    np.testing.assert_allclose(fu.numpy(), (su / 48).numpy(), rtol=0, atol=1e-14)
+    np.testing.assert_allclose(fF.numpy(), (sF / 48).numpy(), rtol=0, atol=1e-14)
+    gu, gF = ws.symmetrize(fu, fF, mask, group)
+    np.testing.assert_allclose(gu.numpy(), fu.numpy(), rtol=0, atol=1e-14)
+    np.testing.assert_allclose(gF.numpy(), fF.numpy(), rtol=0, atol=1e-14)
