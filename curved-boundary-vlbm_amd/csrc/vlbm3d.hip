@@ -55,7 +55,7 @@ namespace {
 constexpr int RED_BLOCKS = 1024;
 
 // Record layout (24 planes): A_v[3], A_F[9], B_v[r*3+axis] (9), B_F[3].
-enum { R_AV = 0, R_AF = 3, R_BV = 12, R_BF = 21 };
+enum { R_AV = 0, R_AF = 3, R_BV = 12, R_BF = 21, RECORD_PLANES = 24 };
 
 struct Params {
     int n, L;
@@ -81,6 +81,21 @@ __device__ __forceinline__ double step_time(double tb, const Params &P) {
     return P.clock ? P.clock[0] + 0.5 * P.dt : tb;
 }
 
+// Element offsets of population and record planes.  double: plane-major,
+// plane * n + node.  float: adjacent planes paired, pair (plane / 2) * n +
+// node holds planes 2k and 2k + 1 side by side, so the twelve values of one
+// direction at a node (and the record's A planes) are six aligned 8-byte
+// pairs: the pull stream then moves 8 bytes per lane and load, where single
+// floats stream at about two thirds of the rate.  The reference's (72, n)
+// view of FP32 populations exists only through the conversion kernels.
+__device__ __forceinline__ size_t pofs(const double *, int plane, size_t node, size_t n) {
+    return (size_t)plane * n + node;
+}
+__device__ __forceinline__ size_t pofs(const float *, int plane, size_t node, size_t n) {
+    return (((size_t)(plane >> 1) * n + node) << 1) + (size_t)(plane & 1);
+}
+typedef float float_pair __attribute__((ext_vector_type(2)));
+
 // Population access.  double planes hold f; float planes hold f - f_rest
 // where f_rest[q][a] = 1/6 on the diagonal of F (a = 3, 7, 11), 0 elsewhere.
 __device__ __forceinline__ double rest(int a) { return (a == 3 || a == 7 || a == 11) ? (1.0 / 6.0) : 0.0; }
@@ -105,6 +120,31 @@ __device__ __forceinline__ double st_nt(float *f, size_t i, int a, double v) {
     __builtin_nontemporal_store(x, f + i);
     return (double)x + rest(a);
 }
+// Planes (a, a + 1), a even, at one node: one 8-byte access for float pairs.
+__device__ __forceinline__ void ld2_nt(const double *f, int plane, size_t node, size_t n, int a, double &x0,
+                                       double &x1) {
+    x0 = ld_nt(f, pofs(f, plane, node, n), a);
+    x1 = ld_nt(f, pofs(f, plane + 1, node, n), a + 1);
+}
+__device__ __forceinline__ void ld2_nt(const float *f, int plane, size_t node, size_t n, int a, double &x0,
+                                       double &x1) {
+    const float_pair v = __builtin_nontemporal_load((const float_pair *)(f + pofs(f, plane, node, n)));
+    x0 = (double)v.x + rest(a);
+    x1 = (double)v.y + rest(a + 1);
+}
+// Store planes (a, a + 1), a even; returns the values as stored.
+__device__ __forceinline__ void st2_nt(double *f, int plane, size_t node, size_t n, int a, double &v0, double &v1) {
+    v0 = st_nt(f, pofs(f, plane, node, n), a, v0);
+    v1 = st_nt(f, pofs(f, plane + 1, node, n), a + 1, v1);
+}
+__device__ __forceinline__ void st2_nt(float *f, int plane, size_t node, size_t n, int a, double &v0, double &v1) {
+    float_pair x;
+    x.x = (float)(v0 - rest(a));
+    x.y = (float)(v1 - rest(a + 1));
+    __builtin_nontemporal_store(x, (float_pair *)(f + pofs(f, plane, node, n)));
+    v0 = (double)x.x + rest(a);
+    v1 = (double)x.y + rest(a + 1);
+}
 
 // Records are stored in the precision of the populations.  double records
 // hold the values themselves; float records hold A - omega f_rest on the A
@@ -127,6 +167,30 @@ __device__ __forceinline__ void rst_a(float *r, size_t i, int a, double w, doubl
 }
 __device__ __forceinline__ void rst_b(double *r, size_t i, double v) { __builtin_nontemporal_store(v, r + i); }
 __device__ __forceinline__ void rst_b(float *r, size_t i, double v) { __builtin_nontemporal_store((float)v, r + i); }
+// Record planes (k, k + 1), k even, at one node; `fa` for each the A-plane
+// index whose rest part the float record subtracts, or -1 for a B plane.
+__device__ __forceinline__ void rst2(double *r, int k, size_t node, size_t n, int fa0, int fa1, double w, double v0,
+                                     double v1) {
+    __builtin_nontemporal_store(v0, r + pofs(r, k, node, n));
+    __builtin_nontemporal_store(v1, r + pofs(r, k + 1, node, n));
+}
+__device__ __forceinline__ void rst2(float *r, int k, size_t node, size_t n, int fa0, int fa1, double w, double v0,
+                                     double v1) {
+    float_pair x;
+    x.x = (float)(fa0 >= 0 ? v0 - w * rest(fa0) : v0);
+    x.y = (float)(fa1 >= 0 ? v1 - w * rest(fa1) : v1);
+    __builtin_nontemporal_store(x, (float_pair *)(r + pofs(r, k, node, n)));
+}
+// A planes (a, a + 1), a even, of the record at one node.
+__device__ __forceinline__ void rld2(const double *r, int a, size_t node, size_t n, double w, double &x0, double &x1) {
+    x0 = r[pofs(r, a, node, n)];
+    x1 = r[pofs(r, a + 1, node, n)];
+}
+__device__ __forceinline__ void rld2(const float *r, int a, size_t node, size_t n, double w, double &x0, double &x1) {
+    const float_pair v = *(const float_pair *)(r + pofs(r, a, node, n));
+    x0 = (double)v.x + w * rest(a);
+    x1 = (double)v.y + w * rest(a + 1);
+}
 // Per-node state touched once by a kernel (no neighbour reuse).
 __device__ __forceinline__ double nt_ld(const double *a, size_t i) { return __builtin_nontemporal_load(a + i); }
 __device__ __forceinline__ void nt_st(double *a, size_t i, double v) { __builtin_nontemporal_store(v, a + i); }
@@ -140,18 +204,33 @@ __device__ __forceinline__ double fpost_v(double fv, const R *rec, size_t n, int
     const double s = q < 3 ? 1.0 : -1.0;
     double b = 0.0;
     if (a < 3) {
-        b = rld_b(rec, (size_t)(R_BV + 3 * a + axis) * n + node);
+        b = rld_b(rec, pofs(rec, R_BV + 3 * a + axis, node, n));
     } else if ((a - 3) % 3 == axis) {
-        b = rld_b(rec, (size_t)(R_BF + (a - 3) / 3) * n + node) * fs;
+        b = rld_b(rec, pofs(rec, R_BF + (a - 3) / 3, node, n)) * fs;
     }
-    return om1 * fv + rld(rec, (size_t)a * n + node, a, w) + s * b;
+    return om1 * fv + rld(rec, pofs(rec, a, node, n), a, w) + s * b;
+}
+
+// fpost_v with the record's A value already loaded (Av).
+template <typename R>
+__device__ __forceinline__ double fpost_a(double fv, double Av, const R *rec, size_t n, int q, int a, int node,
+                                          double om1, double fs) {
+    const int axis = q % 3;
+    const double s = q < 3 ? 1.0 : -1.0;
+    double b = 0.0;
+    if (a < 3) {
+        b = rld_b(rec, pofs(rec, R_BV + 3 * a + axis, node, n));
+    } else if ((a - 3) % 3 == axis) {
+        b = rld_b(rec, pofs(rec, R_BF + (a - 3) / 3, node, n)) * fs;
+    }
+    return om1 * fv + Av + s * b;
 }
 
 // fpost[q][a] at node m from pre-collision f and the node's record.
 template <typename T>
 __device__ __forceinline__ double fpost(const T *f, const T *rec, size_t n, int q, int a, int node, double om1,
                                         double w, double fs) {
-    return fpost_v(ld(f, (size_t)(q * 12 + a) * n + node, a), rec, n, q, a, node, om1, w, fs);
+    return fpost_v(ld(f, pofs(f, q * 12 + a, node, n), a), rec, n, q, a, node, om1, w, fs);
 }
 
 __global__ void k_reset_invalid(int32_t *invalid) { invalid[0] = 0; }
@@ -174,7 +253,7 @@ __global__ __launch_bounds__(BLOCK) void k_collide(const T *__restrict__ f, doub
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
-        for (int a = 0; a < NC; ++a) Um[a] += ld(f, (size_t)(q * 12 + a) * n + p, a);
+        for (int a = 0; a < NC; ++a) Um[a] += ld(f, pofs(f, q * 12 + a, p, n), a);
     const double hdt = 0.5 * P.dt;
     if (P.src_mode == SRC_CONST) {
 #pragma unroll
@@ -413,21 +492,22 @@ __global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const 
     const double w = P.omega;
     const double st = P.dt * (2.0 - w) * (1.0 / 12.0);
     const double w6 = w / 6.0, w2c = w / (2.0 * P.c);
+    double rv[RECORD_PLANES];
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
-        rst_a(rec, (R_AV + a) * n + p, R_AV + a, w,
-              w6 * v[a] + st * source_comp(a, P.src_mode, P.src, v[0], v[1], v[2]));
+    for (int a = 0; a < 3; ++a) rv[R_AV + a] = w6 * v[a] + st * source_comp(a, P.src_mode, P.src, v[0], v[1], v[2]);
 #pragma unroll
     for (int k = 0; k < 9; ++k)
-        rst_a(rec, (R_AF + k) * n + p, R_AF + k, w,
-              w6 * Fo[k] + st * source_comp(3 + k, P.src_mode, P.src, v[0], v[1], v[2]) + dF[k] / 6.0);
+        rv[R_AF + k] = w6 * Fo[k] + st * source_comp(3 + k, P.src_mode, P.src, v[0], v[1], v[2]) + dF[k] / 6.0;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const double ca = P.cs[k % 3];  // plane 3 r + axis: the speed of its axis
-        rst_b(rec, (R_BV + k) * n + p, -(w / (2.0 * ca)) * Po[k] - (0.5 / ca) * dP[k]);
+        rv[R_BV + k] = -(w / (2.0 * ca)) * Po[k] - (0.5 / ca) * dP[k];
     }
 #pragma unroll
-    for (int i = 0; i < 3; ++i) rst_b(rec, (R_BF + i) * n + p, -w2c * v[i]);
+    for (int i = 0; i < 3; ++i) rv[R_BF + i] = -w2c * v[i];
+#pragma unroll
+    for (int k = 0; k < RECORD_PLANES; k += 2)
+        rst2(rec, k, p, n, k < R_BV ? k : -1, k + 1 < R_BV ? k + 1 : -1, w, rv[k], rv[k + 1]);
 }
 
 // Pull stream of the post-collision populations; the entries whose upstream
@@ -452,15 +532,21 @@ __global__ __launch_bounds__(BLOCK) void k_stream(const T *__restrict__ f0, T *_
     for (int q = 0; q < NQ; ++q) {
         const int src = nbr[opp(q) * n + p];
         if (src >= 0) {
-            double fv[NC];
+            // components in pairs (a, a + 1): one 8-byte access each for float planes
+            double fv[NC], A[NC];
 #pragma unroll
-            for (int a = 0; a < NC; ++a) fv[a] = ld_nt(f0, (size_t)(q * 12 + a) * n + src, a);
+            for (int h = 0; h < NC / 2; ++h) ld2_nt(f0, q * 12 + 2 * h, src, n, 2 * h, fv[2 * h], fv[2 * h + 1]);
 #pragma unroll
-            for (int a = 0; a < NC; ++a) {
-                const double x =
-                    st_nt(f1, (size_t)(q * 12 + a) * n + p, a,
-                          fpost_v(fv[a], rec, n, q, a, src, om1, P.omega, P.cr[q % 3]));
-                if (NEXT) Um[a] += x;
+            for (int h = 0; h < NC / 2; ++h) rld2(rec, 2 * h, src, n, P.omega, A[2 * h], A[2 * h + 1]);
+#pragma unroll
+            for (int h = 0; h < NC / 2; ++h) {
+                double x0 = fpost_a(fv[2 * h], A[2 * h], rec, n, q, 2 * h, src, om1, P.cr[q % 3]);
+                double x1 = fpost_a(fv[2 * h + 1], A[2 * h + 1], rec, n, q, 2 * h + 1, src, om1, P.cr[q % 3]);
+                st2_nt(f1, q * 12 + 2 * h, p, n, 2 * h, x0, x1);
+                if (NEXT) {
+                    Um[2 * h] += x0;
+                    Um[2 * h + 1] += x1;
+                }
             }
         } else if (NEXT) {
             // the cut link's population, written by k_links: from its
@@ -471,7 +557,7 @@ __global__ __launch_bounds__(BLOCK) void k_stream(const T *__restrict__ f0, T *_
                 for (int a = 0; a < NC; ++a) Um[a] += ld(fl, (size_t)a, a);
             } else {
 #pragma unroll
-                for (int a = 0; a < NC; ++a) Um[a] += ld(f1, (size_t)(q * 12 + a) * n + p, a);
+                for (int a = 0; a < NC; ++a) Um[a] += ld(f1, pofs(f1, q * 12 + a, p, n), a);
             }
         }
     }
@@ -773,8 +859,8 @@ __global__ __launch_bounds__(128) void k_links(const T *__restrict__ fsrc, T *__
             fo = fpost(fsrc, rec, n, qo, a, p, om1, P.omega, P.cr[qo % 3]);
             fi = fpost(fsrc, rec, n, qi, a, p, om1, P.omega, P.cr[qi % 3]);
         } else {
-            fo = ld(fsrc, (size_t)(qo * 12 + a) * n + p, a);
-            fi = ld(fsrc, (size_t)(qi * 12 + a) * n + p, a);
+            fo = ld(fsrc, pofs(fsrc, qo * 12 + a, p, n), a);
+            fi = ld(fsrc, pofs(fsrc, qi * 12 + a, p, n), a);
         }
         double val;
         if (eta >= 0.5) {
@@ -784,7 +870,7 @@ __global__ __launch_bounds__(128) void k_links(const T *__restrict__ fsrc, T *__
             double upv = fo;
             if (up >= 0)
                 upv = use_rec ? fpost(fsrc, rec, n, qo, a, up, om1, P.omega, P.cr[qo % 3])
-                              : ld(fsrc, (size_t)(qo * 12 + a) * n + up, a);
+                              : ld(fsrc, pofs(fsrc, qo * 12 + a, up, n), a);
             val = D * (2.0 * eta * fo + (1.0 - 2.0 * eta) * upv) + S;
         }
         outv[a] = val;
@@ -810,7 +896,7 @@ __global__ __launch_bounds__(128) void k_links(const T *__restrict__ fsrc, T *__
         }
     }
 #pragma unroll
-    for (int a = 0; a < NC; ++a) st(fdst, (size_t)(qi * 12 + a) * n + p, a, outv[a]);
+    for (int a = 0; a < NC; ++a) st(fdst, pofs(fdst, qi * 12 + a, p, n), a, outv[a]);
 #pragma unroll
     for (int a = 0; a < NC; ++a) st(flink, (size_t)ell * NC + a, a, outv[a]);
 }
@@ -839,7 +925,7 @@ __global__ __launch_bounds__(BLOCK) void k_init_pop(const double *__restrict__ U
             const double B = source_comp(a, P.src_mode, P.src, Uv[0], Uv[1], Uv[2]);
             double fv = (1.0 / 6.0) * (Uv[a] + 3.0 * (s * flux / P.cs[axis]));
             fv -= 0.5 * P.dt / 6.0 * B;
-            st(f, (size_t)(q * 12 + a) * n + p, a, fv);
+            st(f, pofs(f, q * 12 + a, p, n), a, fv);
         }
     }
 }
@@ -858,17 +944,18 @@ __global__ void k_copy_pop(const T *__restrict__ s, T *__restrict__ d, size_t co
     if (i < count) d[i] = s[i];
 }
 
+// Conversions between the (72, n) double planes and the stored float pairs.
 __global__ void k_pop_to_f64(const float *__restrict__ f, double *__restrict__ d, int n) {
     const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= (size_t)FQ * n) return;
-    const int a = (int)((i / n) % 12);
-    d[i] = ld(f, i, a);
+    const int plane = (int)(i / n);
+    d[i] = ld(f, pofs(f, plane, i - (size_t)plane * n, n), plane % 12);
 }
 __global__ void k_pop_from_f64(const double *__restrict__ s, float *__restrict__ f, int n) {
     const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= (size_t)FQ * n) return;
-    const int a = (int)((i / n) % 12);
-    st(f, i, a, s[i]);
+    const int plane = (int)(i / n);
+    st(f, pofs(f, plane, i - (size_t)plane * n, n), plane % 12, s[i]);
 }
 
 // Deterministic two-pass reductions over U and u.
@@ -1009,7 +1096,7 @@ __global__ __launch_bounds__(128) void k_box_sides3(const T *fsrc, T *fdst, cons
 #pragma unroll
     for (int a = 0; a < 12; ++a)
         fo[a] = use_rec ? fpost(fsrc, rec, n, qo, a, p, om1, P.omega, P.cr[axis])
-                        : ld(fsrc, (size_t)(qo * 12 + a) * n + p, a);
+                        : ld(fsrc, pofs(fsrc, qo * 12 + a, p, n), a);
     const double cn = P.cs[axis];  // speed along the face normal (vnew3d:2123-2127)
     double val[12];
     if (kind == BC_DIR) {
@@ -1031,7 +1118,7 @@ __global__ __launch_bounds__(128) void k_box_sides3(const T *fsrc, T *fdst, cons
         for (int a = 0; a < 9; ++a) val[3 + a] = -fo[3 + a] + F[a] / 3.0;
     }
 #pragma unroll
-    for (int a = 0; a < 12; ++a) st(fdst, (size_t)(qi * 12 + a) * n + p, a, val[a]);
+    for (int a = 0; a < 12; ++a) st(fdst, pofs(fdst, qi * 12 + a, p, n), a, val[a]);
 }
 
 }  // namespace
