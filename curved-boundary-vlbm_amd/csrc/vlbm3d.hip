@@ -81,20 +81,20 @@ __device__ __forceinline__ double step_time(double tb, const Params &P) {
     return P.clock ? P.clock[0] + 0.5 * P.dt : tb;
 }
 
-// Element offsets of population and record planes.  double: plane-major,
-// plane * n + node.  float: adjacent planes paired, pair (plane / 2) * n +
-// node holds planes 2k and 2k + 1 side by side, so the twelve values of one
-// direction at a node (and the record's A planes) are six aligned 8-byte
-// pairs: the pull stream then moves 8 bytes per lane and load, where single
-// floats stream at about two thirds of the rate.  The reference's (72, n)
-// view of FP32 populations exists only through the conversion kernels.
-__device__ __forceinline__ size_t pofs(const double *, int plane, size_t node, size_t n) {
-    return (size_t)plane * n + node;
-}
-__device__ __forceinline__ size_t pofs(const float *, int plane, size_t node, size_t n) {
+// Element offsets of population and record planes: adjacent planes paired,
+// pair (plane / 2) * n + node holds planes 2k and 2k + 1 side by side, so the
+// twelve values of one direction at a node (and the record's A planes) are
+// six aligned pairs (8 bytes in FP32, 16 in FP64): the pull stream then moves
+// twice the bytes per lane and access (single floats stream at about two
+// thirds of the rate of pairs; single doubles at 0.93 of it).  The
+// reference's (72, n) view of the populations exists only through the
+// conversion kernels.
+template <typename T>
+__device__ __forceinline__ size_t pofs(const T *, int plane, size_t node, size_t n) {
     return (((size_t)(plane >> 1) * n + node) << 1) + (size_t)(plane & 1);
 }
 typedef float float_pair __attribute__((ext_vector_type(2)));
+typedef double double_pair __attribute__((ext_vector_type(2)));
 
 // Population access.  double planes hold f; float planes hold f - f_rest
 // where f_rest[q][a] = 1/6 on the diagonal of F (a = 3, 7, 11), 0 elsewhere.
@@ -121,10 +121,11 @@ __device__ __forceinline__ double st_nt(float *f, size_t i, int a, double v) {
     return (double)x + rest(a);
 }
 // Planes (a, a + 1), a even, at one node: one 8-byte access for float pairs.
-__device__ __forceinline__ void ld2_nt(const double *f, int plane, size_t node, size_t n, int a, double &x0,
+__device__ __forceinline__ void ld2_nt(const double *f, int plane, size_t node, size_t n, int, double &x0,
                                        double &x1) {
-    x0 = ld_nt(f, pofs(f, plane, node, n), a);
-    x1 = ld_nt(f, pofs(f, plane + 1, node, n), a + 1);
+    const double_pair v = __builtin_nontemporal_load((const double_pair *)(f + pofs(f, plane, node, n)));
+    x0 = v.x;
+    x1 = v.y;
 }
 __device__ __forceinline__ void ld2_nt(const float *f, int plane, size_t node, size_t n, int a, double &x0,
                                        double &x1) {
@@ -133,9 +134,11 @@ __device__ __forceinline__ void ld2_nt(const float *f, int plane, size_t node, s
     x1 = (double)v.y + rest(a + 1);
 }
 // Store planes (a, a + 1), a even; returns the values as stored.
-__device__ __forceinline__ void st2_nt(double *f, int plane, size_t node, size_t n, int a, double &v0, double &v1) {
-    v0 = st_nt(f, pofs(f, plane, node, n), a, v0);
-    v1 = st_nt(f, pofs(f, plane + 1, node, n), a + 1, v1);
+__device__ __forceinline__ void st2_nt(double *f, int plane, size_t node, size_t n, int, double &v0, double &v1) {
+    double_pair x;
+    x.x = v0;
+    x.y = v1;
+    __builtin_nontemporal_store(x, (double_pair *)(f + pofs(f, plane, node, n)));
 }
 __device__ __forceinline__ void st2_nt(float *f, int plane, size_t node, size_t n, int a, double &v0, double &v1) {
     float_pair x;
@@ -169,10 +172,12 @@ __device__ __forceinline__ void rst_b(double *r, size_t i, double v) { __builtin
 __device__ __forceinline__ void rst_b(float *r, size_t i, double v) { __builtin_nontemporal_store((float)v, r + i); }
 // Record planes (k, k + 1), k even, at one node; `fa` for each the A-plane
 // index whose rest part the float record subtracts, or -1 for a B plane.
-__device__ __forceinline__ void rst2(double *r, int k, size_t node, size_t n, int fa0, int fa1, double w, double v0,
+__device__ __forceinline__ void rst2(double *r, int k, size_t node, size_t n, int, int, double, double v0,
                                      double v1) {
-    __builtin_nontemporal_store(v0, r + pofs(r, k, node, n));
-    __builtin_nontemporal_store(v1, r + pofs(r, k + 1, node, n));
+    double_pair x;
+    x.x = v0;
+    x.y = v1;
+    __builtin_nontemporal_store(x, (double_pair *)(r + pofs(r, k, node, n)));
 }
 __device__ __forceinline__ void rst2(float *r, int k, size_t node, size_t n, int fa0, int fa1, double w, double v0,
                                      double v1) {
@@ -182,9 +187,10 @@ __device__ __forceinline__ void rst2(float *r, int k, size_t node, size_t n, int
     __builtin_nontemporal_store(x, (float_pair *)(r + pofs(r, k, node, n)));
 }
 // A planes (a, a + 1), a even, of the record at one node.
-__device__ __forceinline__ void rld2(const double *r, int a, size_t node, size_t n, double w, double &x0, double &x1) {
-    x0 = r[pofs(r, a, node, n)];
-    x1 = r[pofs(r, a + 1, node, n)];
+__device__ __forceinline__ void rld2(const double *r, int a, size_t node, size_t n, double, double &x0, double &x1) {
+    const double_pair v = *(const double_pair *)(r + pofs(r, a, node, n));
+    x0 = v.x;
+    x1 = v.y;
 }
 __device__ __forceinline__ void rld2(const float *r, int a, size_t node, size_t n, double w, double &x0, double &x1) {
     const float_pair v = *(const float_pair *)(r + pofs(r, a, node, n));
@@ -944,7 +950,18 @@ __global__ void k_copy_pop(const T *__restrict__ s, T *__restrict__ d, size_t co
     if (i < count) d[i] = s[i];
 }
 
-// Conversions between the (72, n) double planes and the stored float pairs.
+// Conversions between the (72, n) double planes and the stored pairs.
+template <bool TO_PLANES>
+__global__ void k_pop_layout(const double *__restrict__ s, double *__restrict__ d, int n) {
+    const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= (size_t)FQ * n) return;
+    const int plane = (int)(i / n);
+    const size_t k = pofs(s, plane, i - (size_t)plane * n, n);
+    if (TO_PLANES)
+        d[i] = s[k];
+    else
+        d[k] = s[i];
+}
 __global__ void k_pop_to_f64(const float *__restrict__ f, double *__restrict__ d, int n) {
     const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= (size_t)FQ * n) return;
@@ -1801,7 +1818,8 @@ int vlbm3d_populations_to_f64(vlbm3d_handle *h, const void *f, double *dst) {
     const size_t count = (size_t)FQ * h->d.n_active;
     const int g = (int)((count + BLOCK - 1) / BLOCK);
     if (h->d.precision == VLBM_PREC_F64)
-        hipLaunchKernelGGL(k_copy_pop<double>, dim3(g), dim3(BLOCK), 0, h->stream, (const double *)f, dst, count);
+        hipLaunchKernelGGL(k_pop_layout<true>, dim3(g), dim3(BLOCK), 0, h->stream, (const double *)f, dst,
+                           h->d.n_active);
     else
         hipLaunchKernelGGL(k_pop_to_f64, dim3(g), dim3(BLOCK), 0, h->stream, (const float *)f, dst, h->d.n_active);
     CHECK_LAUNCH("populations_to_f64");
@@ -1813,7 +1831,7 @@ int vlbm3d_populations_from_f64(vlbm3d_handle *h, const double *src, void *f) {
     const size_t count = (size_t)FQ * h->d.n_active;
     const int g = (int)((count + BLOCK - 1) / BLOCK);
     if (h->d.precision == VLBM_PREC_F64)
-        hipLaunchKernelGGL(k_copy_pop<double>, dim3(g), dim3(BLOCK), 0, h->stream, src, (double *)f, count);
+        hipLaunchKernelGGL(k_pop_layout<false>, dim3(g), dim3(BLOCK), 0, h->stream, src, (double *)f, h->d.n_active);
     else
         hipLaunchKernelGGL(k_pop_from_f64, dim3(g), dim3(BLOCK), 0, h->stream, src, (float *)f, h->d.n_active);
     CHECK_LAUNCH("populations_from_f64");

@@ -114,24 +114,32 @@ class _NullContext:
 
 
 class PopulationArray(DeviceArray):
-    """Population planes (72, n).  In FP32 mode the device holds float
-    deviations from the rest state; ``numpy()``/``assign`` convert to and from
-    the full double-precision populations through the HIP library."""
+    """Population planes as the reference sees them.  In FP32 mode the
+    device holds float deviations from the rest state, and an owner whose
+    ``populations_opaque`` is set (the 3D solver: adjacent planes paired per
+    node) keeps its own layout in FP64 too; ``numpy()``/``assign`` then
+    convert to and from the full double-precision planes through the HIP
+    library."""
 
     def __init__(self, tensor: torch.Tensor, owner, stream=None):
         super().__init__(tensor, stream)
         self._owner = owner
 
+    def _converted(self) -> bool:
+        return self.tensor.dtype != torch.float64 or getattr(self._owner, "populations_opaque", False)
+
     def numpy(self) -> np.ndarray:
-        if self.tensor.dtype == torch.float64:
+        if not self._converted():
             return super().numpy()
         return self._owner._populations_to_host(self)
 
     def assign(self, arr) -> None:
-        if self.tensor.dtype == torch.float64:
+        if not self._converted():
             return super().assign(arr)
         if isinstance(arr, DeviceArray):
             arr = arr.numpy()
+        if isinstance(arr, torch.Tensor):
+            arr = arr.detach().to("cpu").numpy()
         self._owner._populations_from_host(self, np.asarray(arr, dtype=np.float64))
 
 

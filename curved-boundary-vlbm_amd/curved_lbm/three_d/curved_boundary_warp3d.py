@@ -995,6 +995,9 @@ class WarpSparseCurvedBoundaryLBM3D:
         self.dynamic_traction_override_enabled = DeviceArray.zeros((L,), i32, dev)
         self.dynamic_traction_override = DeviceArray.zeros((3, L), f64, dev)
         pop_dtype = np.float64 if self.precision == "fp64" else np.float32
+        # the library pairs adjacent population planes per node: f0 / f1
+        # read and write through its conversions (PopulationArray)
+        self.populations_opaque = True
         self.f0 = PopulationArray(DeviceArray.zeros((FQ, n), pop_dtype, dev).tensor, self)
         self.f1 = PopulationArray(DeviceArray.zeros((FQ, n), pop_dtype, dev).tensor, self)
         self.U = DeviceArray.zeros((NCOMP, n), f64, dev)

@@ -4,8 +4,13 @@
  * C ABI only: plain pointers and sizes.  Every device pointer in a
  * vlbm3d_desc is owned by the caller (the Python layer allocates them as
  * device tensors) and must stay valid until vlbm3d_destroy.  Fields are
- * stored as (components, n_active) planes; populations as (72, n_active)
- * planes with index q*12 + a and directions (+x, +y, +z, -x, -y, -z).
+ * stored as (components, n_active) planes.  Populations (72 planes, index
+ * q*12 + a, directions (+x, +y, +z, -x, -y, -z)) and the record (24 planes)
+ * are held with adjacent planes paired: element (plane, node) sits at
+ * ((plane / 2) * n_active + node) * 2 + plane % 2, so a node's planes 2k and
+ * 2k + 1 are one 16-byte (FP64) or 8-byte (FP32) pair.  A population buffer
+ * is therefore opaque; vlbm3d_populations_to_f64 / _from_f64 convert to and
+ * from (72, n_active) double planes.
  *
  * Every entry point returns 0 on success and a negative code on failure;
  * vlbm_last_error() then describes the failure.  Work is enqueued on the
@@ -32,7 +37,7 @@
 extern "C" {
 #endif
 
-#define VLBM3D_ABI_VERSION 7
+#define VLBM3D_ABI_VERSION 8
 #define VLBM3D_MAX_IDS 16
 #define VLBM3D_NPRM 16 /* parameters per boundary id: params_by_id is (MAX_IDS, NPRM) */
 #define VLBM3D_MAX_CHAR 64
@@ -71,8 +76,9 @@ typedef struct {
     int32_t n_char_nodes;   /* number of distinct stencil sample nodes */
     /* state (device) */
     double *U, *u, *u_star, *u_next; /* (12,n), (3,n), (3,n), (3,n) */
-    void *record;                    /* (24,n) deferred-collision record, in the population precision
-                                        (float records hold A - omega f_rest on the A planes) */
+    void *record;                    /* (24,n) deferred-collision record, in the population precision and
+                                        the paired-plane layout (float records hold A - omega f_rest on
+                                        the A planes) */
     int32_t *invalid;                /* (1) */
     double *reduce_scratch;          /* (>= 8 * 1024) */
     /* tables (device) */
@@ -169,7 +175,8 @@ int vlbm3d_reduce(vlbm3d_handle *h, const double *u_prev, double *out);
  * All calls on one GPU share that array: calls that overlap in time on one
  * GPU (different streams) race. */
 int vlbm3d_reduce_noalloc(vlbm3d_handle *h, const double *u_prev, double *out_dev);
-/* Converts populations between the stored precision and double planes. */
+/* Converts populations between the stored layout and precision and
+ * (72, n_active) double planes. */
 int vlbm3d_populations_to_f64(vlbm3d_handle *h, const void *f, double *dst);
 int vlbm3d_populations_from_f64(vlbm3d_handle *h, const double *src, void *f);
 /* Per-kernel timing with HIP events on the solver stream (for measurement).
