@@ -81,6 +81,18 @@ struct Links2 {
     const int32_t *act;
 };
 
+// Element offsets of population and record planes: adjacent planes paired,
+// pair (plane / 2) * N + p holds planes 2k and 2k + 1 side by side, so the six
+// values of one direction at a node are three aligned 16-byte (FP64) or
+// 8-byte (FP32) pairs, which the pull stream moves as one access each.  The
+// reference's (24, nx, ny) view exists only through the conversion kernels.
+template <typename T>
+__device__ __forceinline__ size_t pofs2(const T *, int plane, size_t p, size_t N) {
+    return (((size_t)(plane >> 1) * N + p) << 1) + (size_t)(plane & 1);
+}
+typedef float float_pair2 __attribute__((ext_vector_type(2)));
+typedef double double_pair2 __attribute__((ext_vector_type(2)));
+
 // Population access.  double planes hold f; float planes hold f - f_rest
 // where f_rest[q][a] = 1/4 on the diagonal of F (a = 2, 5), 0 elsewhere.
 __device__ __forceinline__ double rest(int a) { return (a == 2 || a == 5) ? 0.25 : 0.0; }
@@ -101,6 +113,32 @@ __device__ __forceinline__ double st_nt(float *f, size_t i, int a, double v) {
     const float x = (float)(v - rest(a));
     __builtin_nontemporal_store(x, f + i);
     return (double)x + rest(a);
+}
+// Planes (plane, plane + 1) at node p, plane even (components a, a + 1).
+__device__ __forceinline__ void ld2_nt(const double *f, int plane, size_t p, size_t N, int, double &x0, double &x1) {
+    const double_pair2 v = __builtin_nontemporal_load((const double_pair2 *)(f + pofs2(f, plane, p, N)));
+    x0 = v.x;
+    x1 = v.y;
+}
+__device__ __forceinline__ void ld2_nt(const float *f, int plane, size_t p, size_t N, int a, double &x0, double &x1) {
+    const float_pair2 v = __builtin_nontemporal_load((const float_pair2 *)(f + pofs2(f, plane, p, N)));
+    x0 = (double)v.x + rest(a);
+    x1 = (double)v.y + rest(a + 1);
+}
+// Store planes (plane, plane + 1); v0 / v1 return the values as stored.
+__device__ __forceinline__ void st2_nt(double *f, int plane, size_t p, size_t N, int, double &v0, double &v1) {
+    double_pair2 x;
+    x.x = v0;
+    x.y = v1;
+    __builtin_nontemporal_store(x, (double_pair2 *)(f + pofs2(f, plane, p, N)));
+}
+__device__ __forceinline__ void st2_nt(float *f, int plane, size_t p, size_t N, int a, double &v0, double &v1) {
+    float_pair2 x;
+    x.x = (float)(v0 - rest(a));
+    x.y = (float)(v1 - rest(a + 1));
+    __builtin_nontemporal_store(x, (float_pair2 *)(f + pofs2(f, plane, p, N)));
+    v0 = (double)x.x + rest(a);
+    v1 = (double)x.y + rest(a + 1);
 }
 
 // Records are stored in the precision of the populations; float records
@@ -126,11 +164,11 @@ __device__ __forceinline__ double fpost_v(double fv, const R *rec, size_t N, int
     const double s = q < 2 ? 1.0 : -1.0;
     double b = 0.0;
     if (a < 2) {
-        b = rld_b(rec, (size_t)(R_BV + 2 * a + axis) * N + p);
+        b = rld_b(rec, pofs2(rec, R_BV + 2 * a + axis, p, N));
     } else if (((a - 2) & 1) == axis) {
-        b = rld_b(rec, (size_t)(R_BF + (a - 2) / 2) * N + p);
+        b = rld_b(rec, pofs2(rec, R_BF + (a - 2) / 2, p, N));
     }
-    return om1 * fv + rld(rec, (size_t)(R_A + a) * N + p, a, w) + s * b;
+    return om1 * fv + rld(rec, pofs2(rec, R_A + a, p, N), a, w) + s * b;
 }
 
 // Ghost part of the mrt_ghost collision (cbw2d:1847-2041).  Relaxing the
@@ -166,7 +204,7 @@ __device__ __forceinline__ void k2_moments_body(const T *__restrict__ f, double 
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
         for (int a = 0; a < NC; ++a) {
-            const double x = ld(f, (size_t)(q * 6 + a) * N + p, a);
+            const double x = ld(f, pofs2(f, q * 6 + a, p, N), a);
             Um[a] += x;
             Gm[a] += (q & 1) ? -x : x;
         }
@@ -423,16 +461,16 @@ __device__ __forceinline__ void k2_record_body(double *__restrict__ U, const dou
     const double w = P.omega;
     const double st = P.dt * (2.0 - w) * 0.125;
     const double w4 = 0.25 * w, w2c = w / (2.0 * P.c), h2c = 0.5 / P.c;
-    rst_a(rec, (R_A + 0) * N + p, 0, w, w4 * v0 + st * source2(0, P.src_mode, P.src, v0, v1));
-    rst_a(rec, (R_A + 1) * N + p, 1, w, w4 * v1 + st * source2(1, P.src_mode, P.src, v0, v1));
+    rst_a(rec, pofs2(rec, R_A + 0, p, N), 0, w, w4 * v0 + st * source2(0, P.src_mode, P.src, v0, v1));
+    rst_a(rec, pofs2(rec, R_A + 1, p, N), 1, w, w4 * v1 + st * source2(1, P.src_mode, P.src, v0, v1));
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-        rst_a(rec, (R_A + 2 + k) * N + p, 2 + k, w,
+        rst_a(rec, pofs2(rec, R_A + 2 + k, p, N), 2 + k, w,
               w4 * Fo[k] + st * source2(2 + k, P.src_mode, P.src, v0, v1) + 0.25 * dF[k]);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) rst_b(rec, (R_BV + k) * N + p, -w2c * Po[k] - h2c * dP[k]);
-    rst_b(rec, (R_BF + 0) * N + p, -w2c * v0);
-    rst_b(rec, (R_BF + 1) * N + p, -w2c * v1);
+    for (int k = 0; k < 4; ++k) rst_b(rec, pofs2(rec, R_BV + k, p, N), -w2c * Po[k] - h2c * dP[k]);
+    rst_b(rec, pofs2(rec, R_BF + 0, p, N), -w2c * v0);
+    rst_b(rec, pofs2(rec, R_BF + 1, p, N), -w2c * v1);
 }
 template <typename R, int LAW>
 __global__ __launch_bounds__(BLOCK) void k2_record(double *__restrict__ U, const double *__restrict__ ug,
@@ -568,7 +606,7 @@ __device__ __forceinline__ void k2_links_body(const T *__restrict__ fsrc, T *__r
     const double om1 = 1.0 - P.omega;
     // post-collision population q, component a of node nd
     auto fps = [&](int q, int a, int nd) -> double {
-        const size_t k = (size_t)(q * 6 + a) * N + nd;
+        const size_t k = pofs2(fsrc, q * 6 + a, nd, N);
         if (!use_rec) return ld(fsrc, k, a);
         double v = fpost_v(ld(fsrc, k, a), rec, N, q, a, nd, om1, P.omega);
         if (gh) v += ghost_term(gh, N, q, a, nd, P.gw);
@@ -661,7 +699,7 @@ __device__ __forceinline__ void k2_links_body(const T *__restrict__ fsrc, T *__r
             double Pg[4];
             piola2(g + 2, P.m, Pg);
             for (int a = 0; a < 6; ++a)
-                st(fdst, (size_t)(qi * 6 + a) * N + p, a, feq2(qi, a, g, Pg, P.c) + hreg(qi, a, p, node, Pn));
+                st(fdst, pofs2(fdst, qi * 6 + a, p, N), a, feq2(qi, a, g, Pg, P.c) + hreg(qi, a, p, node, Pn));
             return;
         }
         if (r == R_PAIR_NEE || r == R_SAME_NEE || r == R_COMPAT_NEE) {
@@ -678,7 +716,7 @@ __device__ __forceinline__ void k2_links_body(const T *__restrict__ fsrc, T *__r
             for (int a = 0; a < 6; ++a) {
                 const double Dp = (kind == BC_DIR) ? (a < 2 ? -1.0 : 1.0) : (a >= 2 ? -1.0 : 1.0);
                 const double h = r == R_SAME_NEE ? hreg(qi, a, p, node, Pn) : Dp * hreg(qo, a, p, node, Pn);
-                st(fdst, (size_t)(qi * 6 + a) * N + p, a, feq2(qi, a, b, Pbb, P.c) + h);
+                st(fdst, pofs2(fdst, qi * 6 + a, p, N), a, feq2(qi, a, b, Pbb, P.c) + h);
             }
             return;
         }
@@ -811,7 +849,7 @@ __device__ __forceinline__ void k2_links_body(const T *__restrict__ fsrc, T *__r
                 val = D * (2.0 * eta * fo + (1.0 - 2.0 * eta) * upv) + S;
             }
         }
-        st(fdst, (size_t)(qi * 6 + a) * N + p, a, val);
+        st(fdst, pofs2(fdst, qi * 6 + a, p, N), a, val);
     }
 }
 template <typename T, bool EXT, int LAW = -1>
@@ -896,7 +934,7 @@ __global__ __launch_bounds__(LBLOCK) void k2_box_sides(const T *fsrc, T *fdst, c
     double fo[6];
 #pragma unroll
     for (int a = 0; a < 6; ++a) {
-        const size_t k = (size_t)(qo * 6 + a) * N + p;
+        const size_t k = pofs2(fsrc, qo * 6 + a, p, N);
         fo[a] = use_rec ? fpost_v(ld(fsrc, k, a), rec, N, qo, a, p, om1, P.omega) : ld(fsrc, k, a);
     }
     double val[6];
@@ -916,7 +954,7 @@ __global__ __launch_bounds__(LBLOCK) void k2_box_sides(const T *fsrc, T *fdst, c
         for (int k = 0; k < 4; ++k) val[2 + k] = -fo[2 + k] + 0.5 * F[k];
     }
 #pragma unroll
-    for (int a = 0; a < 6; ++a) st(fdst, (size_t)(qi * 6 + a) * N + p, a, val[a]);
+    for (int a = 0; a < 6; ++a) st(fdst, pofs2(fdst, qi * 6 + a, p, N), a, val[a]);
 }
 
 // Bit q of upm[t]: the upstream node of population q at the active node
@@ -975,18 +1013,23 @@ __device__ __forceinline__ void k2_stream_body(const T *__restrict__ f0, T *__re
             const int iu = wrap_i(i - cx(q), P.nx, P.per_x), ju = wrap_i(j - cy(q), ny, P.per_y);
             if ((up >> q) & 1) {
                 const int src = iu * ny + ju;
+                // components in pairs (a, a + 1): one access each
                 double fv[NC];
 #pragma unroll
-                for (int a = 0; a < NC; ++a) fv[a] = ld_nt(f0, (size_t)(q * 6 + a) * N + src, a);
+                for (int h = 0; h < NC / 2; ++h) ld2_nt(f0, q * 6 + 2 * h, src, N, 2 * h, fv[2 * h], fv[2 * h + 1]);
 #pragma unroll
-                for (int a = 0; a < NC; ++a) {
-                    const double x =
-                        st_nt(f1, (size_t)(q * 6 + a) * N + p, a, fpost_v(fv[a], rec, N, q, a, src, om1, P.omega));
-                    if (NEXT) Um[a] += x;
+                for (int h = 0; h < NC / 2; ++h) {
+                    double x0 = fpost_v(fv[2 * h], rec, N, q, 2 * h, src, om1, P.omega);
+                    double x1 = fpost_v(fv[2 * h + 1], rec, N, q, 2 * h + 1, src, om1, P.omega);
+                    st2_nt(f1, q * 6 + 2 * h, p, N, 2 * h, x0, x1);
+                    if (NEXT) {
+                        Um[2 * h] += x0;
+                        Um[2 * h + 1] += x1;
+                    }
                 }
             } else if (NEXT) {
 #pragma unroll
-                for (int a = 0; a < NC; ++a) Um[a] += ld(f1, (size_t)(q * 6 + a) * N + p, a);
+                for (int a = 0; a < NC; ++a) Um[a] += ld(f1, pofs2(f1, q * 6 + a, p, N), a);
             }
         }
     } else {
@@ -1000,13 +1043,13 @@ __device__ __forceinline__ void k2_stream_body(const T *__restrict__ f0, T *__re
                 const int src = iu * ny + ju;
 #pragma unroll
                 for (int a = 0; a < NC; ++a) {
-                    double x = fpost_v(ld_nt(f0, (size_t)(q * 6 + a) * N + src, a), rec, N, q, a, src, om1, P.omega);
+                    double x = fpost_v(ld_nt(f0, pofs2(f0, q * 6 + a, src, N), a), rec, N, q, a, src, om1, P.omega);
                     if (MRT) x += ghost_term(gin, N, q, a, src, P.gw);
                     fq[q][a] = x;
                 }
             } else {
 #pragma unroll
-                for (int a = 0; a < NC; ++a) fq[q][a] = ld(f1, (size_t)(q * 6 + a) * N + p, a);
+                for (int a = 0; a < NC; ++a) fq[q][a] = ld(f1, pofs2(f1, q * 6 + a, p, N), a);
             }
         }
         if (FILT) {
@@ -1027,7 +1070,7 @@ __device__ __forceinline__ void k2_stream_body(const T *__restrict__ f0, T *__re
         for (int q = 0; q < NQ; ++q)
 #pragma unroll
             for (int a = 0; a < NC; ++a) {
-                const double x = (FILT || okq[q]) ? st_nt(f1, (size_t)(q * 6 + a) * N + p, a, fq[q][a]) : fq[q][a];
+                const double x = (FILT || okq[q]) ? st_nt(f1, pofs2(f1, q * 6 + a, p, N), a, fq[q][a]) : fq[q][a];
                 Um[a] += x;
                 Gm[a] += (q & 1) ? -x : x;
             }
@@ -1083,7 +1126,7 @@ __global__ __launch_bounds__(BLOCK) void k2_init_pop(const double *__restrict__ 
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
         for (int a = 0; a < NC; ++a)
-            st(f, (size_t)(q * 6 + a) * N + p, a, feq2(q, a, Uv, Pf, P.c) - cs * source2(a, P.src_mode, P.src, Uv[0], Uv[1]));
+            st(f, pofs2(f, q * 6 + a, p, N), a, feq2(q, a, Uv, Pf, P.c) - cs * source2(a, P.src_mode, P.src, Uv[0], Uv[1]));
 }
 
 __global__ __launch_bounds__(BLOCK) void k2_set_ustar(const double *__restrict__ disp, const double *__restrict__ U,
@@ -1128,11 +1171,11 @@ __global__ __launch_bounds__(BLOCK) void k2_post(const T *__restrict__ f, T *__r
     for (int a = 0; a < NC; ++a) {
         double fv[NQ];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) fv[q] = ld(f, (size_t)(q * 6 + a) * P.N + p, a);
+        for (int q = 0; q < NQ; ++q) fv[q] = ld(f, pofs2(f, q * 6 + a, p, P.N), a);
         const double g = P.gw * (fv[0] - fv[1] + fv[2] - fv[3]);  // mrt_ghost term, 0 for bgk
 #pragma unroll
         for (int q = 0; q < NQ; ++q)
-            st(fp, (size_t)(q * 6 + a) * P.N + p, a,
+            st(fp, pofs2(fp, q * 6 + a, p, P.N), a,
                fpost_v(fv[q], rec, P.N, q, a, p, om1, P.omega) + ((q & 1) ? -g : g));
     }
 }
@@ -1147,11 +1190,11 @@ __global__ __launch_bounds__(BLOCK) void k2_filter(T *__restrict__ f, const int3
     for (int a = 0; a < NC; ++a) {
         double fv[NQ];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) fv[q] = ld(f, (size_t)(q * 6 + a) * P.N + p, a);
+        for (int q = 0; q < NQ; ++q) fv[q] = ld(f, pofs2(f, q * 6 + a, p, P.N), a);
         const double quarter = 0.25 * (-P.filt * (fv[0] - fv[1] + fv[2] - fv[3]));
 #pragma unroll
         for (int q = 0; q < NQ; ++q)
-            st(f, (size_t)(q * 6 + a) * P.N + p, a, (q & 1) ? fv[q] - quarter : fv[q] + quarter);
+            st(f, pofs2(f, q * 6 + a, p, P.N), a, (q & 1) ? fv[q] - quarter : fv[q] + quarter);
     }
 }
 
@@ -1160,17 +1203,20 @@ __global__ void k2_copy(const T *__restrict__ s, T *__restrict__ d, size_t count
     const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i < count) d[i] = s[i];
 }
-__global__ void k2_to_f64(const float *__restrict__ f, double *__restrict__ d, size_t N) {
+// Conversions between the (24, nx, ny) double planes and the stored pairs.
+template <typename T>
+__global__ void k2_to_f64(const T *__restrict__ f, double *__restrict__ d, size_t N) {
     const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= (size_t)FQ * N) return;
-    const int a = (int)((i / N) % 6);
-    d[i] = ld(f, i, a);
+    const int plane = (int)(i / N);
+    d[i] = ld(f, pofs2(f, plane, i - (size_t)plane * N, N), plane % 6);
 }
-__global__ void k2_from_f64(const double *__restrict__ s, float *__restrict__ f, size_t N) {
+template <typename T>
+__global__ void k2_from_f64(const double *__restrict__ s, T *__restrict__ f, size_t N) {
     const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= (size_t)FQ * N) return;
-    const int a = (int)((i / N) % 6);
-    st(f, i, a, s[i]);
+    const int plane = (int)(i / N);
+    st(f, pofs2(f, plane, i - (size_t)plane * N, N), plane % 6, s[i]);
 }
 
 // Deterministic two-pass reductions over the active nodes.
@@ -2218,9 +2264,9 @@ int vlbm2d_populations_to_f64(vlbm2d_handle *h, const void *f, double *dst) {
     const size_t count = (size_t)FQ * h->P.N;
     const int g = (int)((count + BLOCK - 1) / BLOCK);
     if (h->d.precision == VLBM_PREC_F64)
-        hipLaunchKernelGGL(k2_copy<double>, dim3(g), dim3(BLOCK), 0, h->stream, (const double *)f, dst, count);
+        hipLaunchKernelGGL(k2_to_f64<double>, dim3(g), dim3(BLOCK), 0, h->stream, (const double *)f, dst, h->P.N);
     else
-        hipLaunchKernelGGL(k2_to_f64, dim3(g), dim3(BLOCK), 0, h->stream, (const float *)f, dst, h->P.N);
+        hipLaunchKernelGGL(k2_to_f64<float>, dim3(g), dim3(BLOCK), 0, h->stream, (const float *)f, dst, h->P.N);
     CHECK_LAUNCH("vlbm2d_populations_to_f64");
     return 0;
 }
@@ -2230,9 +2276,9 @@ int vlbm2d_populations_from_f64(vlbm2d_handle *h, const double *src, void *f) {
     const size_t count = (size_t)FQ * h->P.N;
     const int g = (int)((count + BLOCK - 1) / BLOCK);
     if (h->d.precision == VLBM_PREC_F64)
-        hipLaunchKernelGGL(k2_copy<double>, dim3(g), dim3(BLOCK), 0, h->stream, src, (double *)f, count);
+        hipLaunchKernelGGL(k2_from_f64<double>, dim3(g), dim3(BLOCK), 0, h->stream, src, (double *)f, h->P.N);
     else
-        hipLaunchKernelGGL(k2_from_f64, dim3(g), dim3(BLOCK), 0, h->stream, src, (float *)f, h->P.N);
+        hipLaunchKernelGGL(k2_from_f64<float>, dim3(g), dim3(BLOCK), 0, h->stream, src, (float *)f, h->P.N);
     CHECK_LAUNCH("vlbm2d_populations_from_f64");
     return 0;
 }

@@ -810,6 +810,9 @@ class WarpCurvedBoundaryLBM2D:
         self.link_traction_x = DeviceArray.zeros((max(L, 1),), f64, dev)
         self.link_traction_y = DeviceArray.zeros((max(L, 1),), f64, dev)
         pop_dtype = np.float64 if self.precision == "fp64" else np.float32
+        # the library pairs adjacent population planes per node: the
+        # population arrays read and write through its conversions
+        self.populations_opaque = True
         shp = (FQ, nx, ny, 1)
         self._pops = [PopulationArray(DeviceArray.zeros(shp, pop_dtype, dev).tensor, self) for _ in range(2)]
         self._fpost = PopulationArray(DeviceArray.zeros(shp, pop_dtype, dev).tensor, self)

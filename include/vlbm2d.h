@@ -4,10 +4,15 @@
  * C ABI only: plain pointers and sizes.  Every device pointer in a
  * vlbm2d_desc is owned by the caller and must stay valid until
  * vlbm2d_destroy.  Fields are dense (components, nx, ny) planes with node
- * index i*ny + j, the reference's (components, nx, ny, 1) arrays;
- * populations are (24, nx, ny) planes with index q*6 + a, directions
- * E, N, W, S.  Only active nodes are read or written: entries of inactive
- * nodes keep the value the caller gave them (zero in the Python layer).
+ * index i*ny + j, the reference's (components, nx, ny, 1) arrays.
+ * Populations (24 planes, index q*6 + a, directions E, N, W, S) and the
+ * record (12 planes) are held with adjacent planes paired: element
+ * (plane, node) sits at ((plane / 2) * nx*ny + node) * 2 + plane % 2, so a
+ * node's planes 2k and 2k + 1 are one 16-byte (FP64) or 8-byte (FP32) pair;
+ * a population buffer is opaque and vlbm2d_populations_to_f64 / _from_f64
+ * convert to and from (24, nx, ny) double planes.  Only active nodes are
+ * read or written: entries of inactive nodes keep the value the caller gave
+ * them (zero in the Python layer).
  *
  * Every entry point returns 0 on success and a negative code on failure;
  * vlbm_last_error() (declared in vlbm3d.h) then describes the failure.
@@ -49,7 +54,7 @@
 extern "C" {
 #endif
 
-#define VLBM2D_ABI_VERSION 6
+#define VLBM2D_ABI_VERSION 7
 #define VLBM2D_MAX_IDS 8
 #define VLBM2D_MAX_LSQ 24
 #define VLBM2D_RECORD 12
@@ -87,7 +92,8 @@ typedef struct {
     /* state (device) */
     double *U, *u, *u_star, *u_next; /* (6|2, nx, ny) */
     void *record;                    /* (12, nx, ny) deferred-collision record, in the population
-                                        precision (float: A - omega f_rest on the A planes) */
+                                        precision and the paired-plane layout (float: A - omega
+                                        f_rest on the A planes) */
     int32_t *invalid;                /* (1) */
     double *ghost;                   /* (2, 6, nx, ny) ghost moments of the pre-collision populations,
                                         two step parities (mrt_ghost only; else may be NULL) */
@@ -158,7 +164,8 @@ int vlbm2d_set_dynamic_traction(vlbm2d_handle *h, int32_t enabled);
 int vlbm2d_set_graph_steps(vlbm2d_handle *h, int32_t steps);
 /* Synchronous reductions over the active nodes (VLBM_RED_* slots). */
 int vlbm2d_reduce(vlbm2d_handle *h, const double *u_prev, double *out);
-/* Population precision conversion over whole (24, nx, ny) planes. */
+/* Conversion between the stored layout and precision and whole
+ * (24, nx, ny) double planes. */
 int vlbm2d_populations_to_f64(vlbm2d_handle *h, const void *f, double *dst);
 int vlbm2d_populations_from_f64(vlbm2d_handle *h, const double *src, void *f);
 /* Per-kernel timing, slots VLBM_SLOT_* of vlbm3d.h (no reference counterpart). */
