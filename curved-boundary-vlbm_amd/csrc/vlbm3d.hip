@@ -133,6 +133,30 @@ __device__ __forceinline__ void ld2_nt(const float *f, int plane, size_t node, s
     x0 = (double)v.x + rest(a);
     x1 = (double)v.y + rest(a + 1);
 }
+// Cached pair accesses for the scattered link kernel: planes (a, a + 1),
+// a even, at one node, and a pair of entries at element i (i even).
+__device__ __forceinline__ void ld2(const double *f, int plane, size_t node, size_t n, int, double &x0, double &x1) {
+    const double_pair v = *(const double_pair *)(f + pofs(f, plane, node, n));
+    x0 = v.x;
+    x1 = v.y;
+}
+__device__ __forceinline__ void ld2(const float *f, int plane, size_t node, size_t n, int a, double &x0, double &x1) {
+    const float_pair v = *(const float_pair *)(f + pofs(f, plane, node, n));
+    x0 = (double)v.x + rest(a);
+    x1 = (double)v.y + rest(a + 1);
+}
+__device__ __forceinline__ void st2(double *f, size_t i, int, double v0, double v1) {
+    double_pair x;
+    x.x = v0;
+    x.y = v1;
+    *(double_pair *)(f + i) = x;
+}
+__device__ __forceinline__ void st2(float *f, size_t i, int a, double v0, double v1) {
+    float_pair x;
+    x.x = (float)(v0 - rest(a));
+    x.y = (float)(v1 - rest(a + 1));
+    *(float_pair *)(f + i) = x;
+}
 // Store planes (a, a + 1), a even; returns the values as stored.
 __device__ __forceinline__ void st2_nt(double *f, int plane, size_t node, size_t n, int, double &v0, double &v1) {
     double_pair x;
@@ -837,49 +861,76 @@ __global__ __launch_bounds__(128) void k_links(const T *__restrict__ fsrc, T *__
     const bool char_rows = has_char && (kk[0] == BC_NEU || kk[1] == BC_NEU || kk[2] == BC_NEU);
     double outv[NC];
 #pragma unroll
-    for (int a = 0; a < NC; ++a) {
-        const int row = a < 3 ? a : (a - 3) / 3;
-        outv[a] = 0.0;
-        // Traction rows are overwritten by the characteristic closure below.
-        if (char_rows && kk[row] == BC_NEU) continue;
-        double D = 1.0, S = 0.0;
-        if (kk[row] == BC_DIR) {
-            if (a < 3) {
-                D = -1.0;
-                S = v[a] / 3.0;
-            } else {
-                const int col = (a - 3) - row * 3;
-                S = -din[col] * v[row] / P.c;
-            }
-        } else {
-            if (a < 3) {
-                S = -(din[0] * Tb.lPb[(3 * a) * L + ell] + din[1] * Tb.lPb[(3 * a + 1) * L + ell] +
-                      din[2] * Tb.lPb[(3 * a + 2) * L + ell]) / P.c;
-            } else {
-                D = -1.0;
-                S = Tb.lFb[(a - 3) * L + ell] / 3.0;
-            }
+    for (int h = 0; h < NC / 2; ++h) {
+        const int a0 = 2 * h;
+        bool need[2];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int a = a0 + e, row = a < 3 ? a : (a - 3) / 3;
+            // Traction rows are overwritten by the characteristic closure below.
+            need[e] = !(char_rows && kk[row] == BC_NEU);
         }
-        double fo, fi;
+        outv[a0] = outv[a0 + 1] = 0.0;
+        if (!need[0] && !need[1]) continue;
+        // The pair's populations in both directions and its record entries
+        // at p (and upstream of p), one access per pair; qo and qi share an
+        // axis, so the record's flux entry is the same for both.
+        const bool upstream = eta < 0.5 && up >= 0;
+        double fo[2], fi[2], fu[2] = {0.0, 0.0};
         if (use_rec) {
-            fo = fpost(fsrc, rec, n, qo, a, p, om1, P.omega, P.cr[qo % 3]);
-            fi = fpost(fsrc, rec, n, qi, a, p, om1, P.omega, P.cr[qi % 3]);
+            double vo[2], vi[2], A[2];
+            ld2(fsrc, qo * 12 + a0, p, n, a0, vo[0], vo[1]);
+            ld2(fsrc, qi * 12 + a0, p, n, a0, vi[0], vi[1]);
+            rld2(rec, a0, p, n, P.omega, A[0], A[1]);
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                fo[e] = fpost_a(vo[e], A[e], rec, n, qo, a0 + e, p, om1, P.cr[qo % 3]);
+                fi[e] = fpost_a(vi[e], A[e], rec, n, qi, a0 + e, p, om1, P.cr[qi % 3]);
+            }
+            if (upstream) {
+                double vu[2], Au[2];
+                ld2(fsrc, qo * 12 + a0, up, n, a0, vu[0], vu[1]);
+                rld2(rec, a0, up, n, P.omega, Au[0], Au[1]);
+#pragma unroll
+                for (int e = 0; e < 2; ++e) fu[e] = fpost_a(vu[e], Au[e], rec, n, qo, a0 + e, up, om1, P.cr[qo % 3]);
+            }
         } else {
-            fo = ld(fsrc, pofs(fsrc, qo * 12 + a, p, n), a);
-            fi = ld(fsrc, pofs(fsrc, qi * 12 + a, p, n), a);
+            ld2(fsrc, qo * 12 + a0, p, n, a0, fo[0], fo[1]);
+            ld2(fsrc, qi * 12 + a0, p, n, a0, fi[0], fi[1]);
+            if (upstream) ld2(fsrc, qo * 12 + a0, up, n, a0, fu[0], fu[1]);
         }
-        double val;
-        if (eta >= 0.5) {
-            const double alpha = 1.0 / (2.0 * eta);
-            val = alpha * (D * fo + S) + (1.0 - alpha) * fi;
-        } else {
-            double upv = fo;
-            if (up >= 0)
-                upv = use_rec ? fpost(fsrc, rec, n, qo, a, up, om1, P.omega, P.cr[qo % 3])
-                              : ld(fsrc, pofs(fsrc, qo * 12 + a, up, n), a);
-            val = D * (2.0 * eta * fo + (1.0 - 2.0 * eta) * upv) + S;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            if (!need[e]) continue;
+            const int a = a0 + e, row = a < 3 ? a : (a - 3) / 3;
+            double D = 1.0, S = 0.0;
+            if (kk[row] == BC_DIR) {
+                if (a < 3) {
+                    D = -1.0;
+                    S = v[a] / 3.0;
+                } else {
+                    const int col = (a - 3) - row * 3;
+                    S = -din[col] * v[row] / P.c;
+                }
+            } else {
+                if (a < 3) {
+                    S = -(din[0] * Tb.lPb[(3 * a) * L + ell] + din[1] * Tb.lPb[(3 * a + 1) * L + ell] +
+                          din[2] * Tb.lPb[(3 * a + 2) * L + ell]) / P.c;
+                } else {
+                    D = -1.0;
+                    S = Tb.lFb[(a - 3) * L + ell] / 3.0;
+                }
+            }
+            double val;
+            if (eta >= 0.5) {
+                const double alpha = 1.0 / (2.0 * eta);
+                val = alpha * (D * fo[e] + S) + (1.0 - alpha) * fi[e];
+            } else {
+                const double upv = upstream ? fu[e] : fo[e];
+                val = D * (2.0 * eta * fo[e] + (1.0 - 2.0 * eta) * upv) + S;
+            }
+            outv[a] = val;
         }
-        outv[a] = val;
     }
     if (char_rows) {
         double Ub[NC], Gx[NC], Gy[NC], Gz[NC], inc[NC];
@@ -902,9 +953,10 @@ __global__ __launch_bounds__(128) void k_links(const T *__restrict__ fsrc, T *__
         }
     }
 #pragma unroll
-    for (int a = 0; a < NC; ++a) st(fdst, pofs(fdst, qi * 12 + a, p, n), a, outv[a]);
-#pragma unroll
-    for (int a = 0; a < NC; ++a) st(flink, (size_t)ell * NC + a, a, outv[a]);
+    for (int h = 0; h < NC / 2; ++h) {
+        st2(fdst, pofs(fdst, qi * 12 + 2 * h, p, n), 2 * h, outv[2 * h], outv[2 * h + 1]);
+        st2(flink, (size_t)ell * NC + 2 * h, 2 * h, outv[2 * h], outv[2 * h + 1]);
+    }
 }
 
 // f = feq(U) - dt/(2Q) B(U) (vnew3d:535-560 as used by cbw3d:4251-4253).
