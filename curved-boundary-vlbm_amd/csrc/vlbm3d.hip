@@ -632,12 +632,13 @@ __global__ __launch_bounds__(BLOCK) void k_pack(const double *__restrict__ U, co
 constexpr int GLANES = 2;
 constexpr int GC = NC / GLANES;  // components per lane
 static_assert(NC % GLANES == 0, "components must split evenly over the lanes of a link");
-__global__ __launch_bounds__(BLOCK) void k_gather(const double *__restrict__ pack, const int32_t *__restrict__ clink,
+constexpr int GATHER_BLOCK = 256;
+__global__ __launch_bounds__(GATHER_BLOCK) void k_gather(const double *__restrict__ pack, const int32_t *__restrict__ clink,
                                                   const int32_t *__restrict__ gidx, const double *__restrict__ gw,
                                                   const int32_t *__restrict__ pidx, const double *__restrict__ pw,
                                                   double *__restrict__ cs, int L, int Lc) {
     constexpr int NG = MAX_CHAR - 16;
-    const int t = xcd_block() * BLOCK + threadIdx.x;
+    const int t = xcd_block() * GATHER_BLOCK + threadIdx.x;
     const int c = t / GLANES;
     const int a0 = GC * (t % GLANES);
     if (c >= Lc) return;
@@ -741,9 +742,11 @@ struct LinkTables {
 
 // Boundary values, F predictor and the normal-traction solve per link
 // (cbw3d:2888-3151).
-__global__ __launch_bounds__(128) void k_prepare(const double *__restrict__ U, const double *__restrict__ uu,
-                                                 LinkTables T, double t, Params P) {
-    const int ell = blockIdx.x * 128 + threadIdx.x;
+// One wave per block, as for the link kernel below.
+constexpr int PREP_BLOCK = 64;
+__global__ __launch_bounds__(PREP_BLOCK) void k_prepare(const double *__restrict__ U, const double *__restrict__ uu,
+                                                        LinkTables T, double t, Params P) {
+    const int ell = blockIdx.x * PREP_BLOCK + threadIdx.x;
     if (ell >= P.L) return;
     t = step_time(t, P);
     const size_t n = P.n, L = P.L;
@@ -841,13 +844,17 @@ __global__ __launch_bounds__(128) void k_prepare(const double *__restrict__ U, c
 // identities (cbw3d:3155-3241), then the characteristic closure on the
 // traction rows (cbw3d:2813-2884).  `use_rec` selects post-collision sources
 // (time step) or the raw populations (standalone reconstruction).
+// One wave per block: the one or two waves per SIMD that the link count gives
+// spread more evenly over the CUs than 128-thread blocks do (75 000 links:
+// 4.6 blocks per CU instead of 2.3).
+constexpr int LINK_BLOCK = 64;
 template <typename T>
-__global__ __launch_bounds__(128) void k_links(const T *__restrict__ fsrc, T *__restrict__ fdst,
-                                               T *__restrict__ flink,
-                                               const T *__restrict__ rec, const double *__restrict__ U,
-                                               LinkTables Tb, int32_t *__restrict__ invalid, int use_rec,
-                                               int has_char, Params P) {
-    const int ell = blockIdx.x * 128 + threadIdx.x;
+__global__ __launch_bounds__(LINK_BLOCK) void k_links(const T *__restrict__ fsrc, T *__restrict__ fdst,
+                                                      T *__restrict__ flink,
+                                                      const T *__restrict__ rec, const double *__restrict__ U,
+                                                      LinkTables Tb, int32_t *__restrict__ invalid, int use_rec,
+                                                      int has_char, Params P) {
+    const int ell = blockIdx.x * LINK_BLOCK + threadIdx.x;
     if (ell >= P.L) return;
     const size_t n = P.n, L = P.L;
     const int p = Tb.lp[ell], qo = Tb.lq[ell], qi = opp(qo);
@@ -1423,21 +1430,21 @@ static int run_boundary(vlbm3d_handle *h, hipStream_t st, const T *src, T *dst, 
                                d.char_pack, d.n_active, d.n_char_nodes);
             CHECK_LAUNCH("k_pack");
         }
-        const int g = xcd_grid((int)((size_t)d.n_char_links * GLANES));
-        hipLaunchKernelGGL(k_gather, dim3(g), dim3(BLOCK), 0, st, (const double *)d.char_pack, d.char_link,
+        const int g = xcd_grid((int)((size_t)d.n_char_links * GLANES), GATHER_BLOCK);
+        hipLaunchKernelGGL(k_gather, dim3(g), dim3(GATHER_BLOCK), 0, st, (const double *)d.char_pack, d.char_link,
                            d.char_gidx, d.char_gw, d.char_pidx, d.char_pw, d.char_state, d.n_links,
                            d.n_char_links);
         CHECK_LAUNCH("k_gather");
     }
-    const int gl = (d.n_links + 127) / 128;
     {
         TimedSlot ts(h->timing, st, VLBM_SLOT_PREPARE);
-        hipLaunchKernelGGL(k_prepare, dim3(gl), dim3(128), 0, st, (const double *)d.U, (const double *)d.u, h->T, t,
+        hipLaunchKernelGGL(k_prepare, dim3((d.n_links + PREP_BLOCK - 1) / PREP_BLOCK), dim3(PREP_BLOCK), 0, st, (const double *)d.U, (const double *)d.u, h->T, t,
                            P);
         CHECK_LAUNCH("k_prepare");
     }
     TimedSlot ts(h->timing, st, VLBM_SLOT_LINKS);
-    hipLaunchKernelGGL((k_links<T>), dim3(gl), dim3(128), 0, st, src, dst, (T *)h->flink, (const T *)d.record,
+    hipLaunchKernelGGL((k_links<T>), dim3((d.n_links + LINK_BLOCK - 1) / LINK_BLOCK), dim3(LINK_BLOCK), 0, st, src,
+                       dst, (T *)h->flink, (const T *)d.record,
                        (const double *)d.U, h->T, invalid, use_rec, d.has_char, P);
     CHECK_LAUNCH("k_links");
     return 0;
