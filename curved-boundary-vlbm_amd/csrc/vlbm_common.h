@@ -22,7 +22,9 @@ inline int fail_msg(const std::string &m) {
     return -1;
 }
 
-constexpr int BLOCK = 256;
+// Node-kernel block: one wave.  On the 128^3 shell the whole step is 1.4%
+// faster than with 256-thread blocks (512: 3% slower), bitwise the same.
+constexpr int BLOCK = 64;
 constexpr int NXCD = 8;
 
 // Grid sizes rounded up to a multiple of the XCD count; block b of the launch
