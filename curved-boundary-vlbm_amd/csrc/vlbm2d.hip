@@ -1922,7 +1922,7 @@ int batch_step_launches_t(vlbm2d_batch *b, hipStream_t st, int parity, bool next
 }
 
 // Block sizes the autotuner chooses from.
-constexpr int TUNE_NB[3] = {128, 256, 512};
+constexpr int TUNE_NB[4] = {64, 128, 256, 512};
 constexpr int TUNE_NL[3] = {32, 64, 128};
 
 template <typename T, int NB>
@@ -1937,6 +1937,7 @@ int batch_step_nl(vlbm2d_batch *b, hipStream_t st, int parity, bool next) {
 template <typename T>
 int batch_step_launches(vlbm2d_batch *b, hipStream_t st, int parity, bool next) {
     switch (b->nb) {
+    case 64: return batch_step_nl<T, 64>(b, st, parity, next);
     case 128: return batch_step_nl<T, 128>(b, st, parity, next);
     case 512: return batch_step_nl<T, 512>(b, st, parity, next);
     default: return batch_step_nl<T, 256>(b, st, parity, next);
@@ -1948,6 +1949,7 @@ int batch_moments(vlbm2d_batch *b, hipStream_t st) {
     const int n = b->njobs;
     const dim3 g(batch_grid(b->nblk[BK_NODE]));
     switch (b->nb) {
+    case 64: hipLaunchKernelGGL((k2b_moments<T, 64>), g, dim3(64), 0, st, b->jobs, b->start, n, b->parity); break;
     case 128: hipLaunchKernelGGL((k2b_moments<T, 128>), g, dim3(128), 0, st, b->jobs, b->start, n, b->parity); break;
     case 512: hipLaunchKernelGGL((k2b_moments<T, 512>), g, dim3(512), 0, st, b->jobs, b->start, n, b->parity); break;
     default: hipLaunchKernelGGL((k2b_moments<T, 256>), g, dim3(256), 0, st, b->jobs, b->start, n, b->parity); break;
@@ -2411,7 +2413,7 @@ int vlbm2d_batch_autotune(vlbm2d_batch *b, int32_t steps, int32_t *nb_out, int32
     int bnb = b->nb, bnl = b->nl, rc = 0;
     const int keep_graph = b->graph_steps;
     b->graph_steps = 0;
-    for (int i = 0; i < 3 && rc == 0; ++i)
+    for (int i = 0; i < 4 && rc == 0; ++i)
         for (int k = 0; k < 3 && rc == 0; ++k) {
             b->nb = TUNE_NB[i];
             b->nl = TUNE_NL[k];
@@ -2455,8 +2457,8 @@ int vlbm2d_batch_autotune(vlbm2d_batch *b, int32_t steps, int32_t *nb_out, int32
 
 int vlbm2d_batch_set_blocks(vlbm2d_batch *b, int32_t nb, int32_t nl) {
     if (!b) return fail_msg("vlbm2d_batch_set_blocks: null batch");
-    if ((nb != 128 && nb != 256 && nb != 512) || (nl != 32 && nl != 64 && nl != 128))
-        return fail_msg("vlbm2d_batch_set_blocks: node blocks must be 128, 256 or 512 and link blocks 32, 64 or 128");
+    if ((nb != 64 && nb != 128 && nb != 256 && nb != 512) || (nl != 32 && nl != 64 && nl != 128))
+        return fail_msg("vlbm2d_batch_set_blocks: node blocks must be 64, 128, 256 or 512 and link blocks 32, 64 or 128");
     hipError_t e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) return fail("vlbm2d_batch_set_blocks: sync", e);
     b->nb = nb;

@@ -16,7 +16,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libvlbm.so"
 
 ABI_VERSION = 8
-ABI2_VERSION = 7
+ABI2_VERSION = 8
 PREC_F64 = 0
 PREC_F32 = 1
 

@@ -171,8 +171,8 @@ class BatchedCurvedBoundaryLBM2D:
         return int(sum(s.n_active for s in self.solvers))
 
     def autotune(self, steps: int = 4) -> dict:
-        """Batch mode: time every pair of block shapes (node kernels 128 /
-        256 / 512 threads, cut-link kernel 32 / 64 / 128) over `steps`
+        """Batch mode: time every pair of block shapes (node kernels 64 /
+        128 / 256 / 512 threads, cut-link kernel 32 / 64 / 128) over `steps`
         batched steps and keep the fastest; then restart every shell from
         rest (the timing steps advance the state)."""
         if self.mode != "batch":
@@ -180,13 +180,13 @@ class BatchedCurvedBoundaryLBM2D:
         from .. import _native
 
         nb, nl = C.c_int32(), C.c_int32()
-        ms = (C.c_float * 9)()
+        ms = (C.c_float * 12)()
         _native.check(self._lib.vlbm2d_batch_autotune(self._batch, int(steps), C.byref(nb), C.byref(nl), ms),
                       "vlbm2d_batch_autotune")
         for s in self.solvers:
             s.initialize_identity()
         self._select(list(range(len(self.solvers))))
-        grid = {f"{a}x{b}": float(ms[3 * i + k]) for i, a in enumerate((128, 256, 512))
+        grid = {f"{a}x{b}": float(ms[3 * i + k]) for i, a in enumerate((64, 128, 256, 512))
                 for k, b in enumerate((32, 64, 128))}
         return {"node_block": nb.value, "link_block": nl.value, "ms_per_step": grid}
 

@@ -54,7 +54,7 @@
 extern "C" {
 #endif
 
-#define VLBM2D_ABI_VERSION 7
+#define VLBM2D_ABI_VERSION 8
 #define VLBM2D_MAX_IDS 8
 #define VLBM2D_MAX_LSQ 24
 #define VLBM2D_RECORD 12
@@ -193,10 +193,10 @@ int vlbm2d_batch_parity(const vlbm2d_batch *b);
 int vlbm2d_batch_step(vlbm2d_batch *b, const double *times, int32_t nsteps);
 int vlbm2d_batch_destroy(vlbm2d_batch *b);
 /* Block shapes of the batched launches: threads per block of the node
- * kernels (128, 256 or 512) and of the cut-link kernel (32, 64 or 128).
+ * kernels (64, 128, 256 or 512) and of the cut-link kernel (32, 64 or 128).
  * vlbm2d_batch_autotune times `steps` batched steps of the selected shells
  * with every pair (HIP events on the batch stream), keeps the fastest and
- * returns it (ms_out, optional: 9 per-step times, node-block major).  It
+ * returns it (ms_out, optional: 12 per-step times, node-block major).  It
  * advances the shells' state: tune before initialising them. */
 int vlbm2d_batch_autotune(vlbm2d_batch *b, int32_t steps, int32_t *nb_out, int32_t *nl_out, float *ms_out);
 int vlbm2d_batch_set_blocks(vlbm2d_batch *b, int32_t nb, int32_t nl);

@@ -90,8 +90,8 @@ def test_gpu_batch_autotune_keeps_results():
     inputs = [bt.shell_solver_inputs(i) for i in inst]
     b = bt.BatchedCurvedBoundaryLBM2D.build(inputs, precision="fp32", mode="batch")
     choice = b.autotune(steps=3)
-    assert choice["node_block"] in (128, 256, 512) and choice["link_block"] in (32, 64, 128)
-    assert len(choice["ms_per_step"]) == 9 and all(v > 0 for v in choice["ms_per_step"].values())
+    assert choice["node_block"] in (64, 128, 256, 512) and choice["link_block"] in (32, 64, 128)
+    assert len(choice["ms_per_step"]) == 12 and all(v > 0 for v in choice["ms_per_step"].values())
     alone = [cbw.WarpCurvedBoundaryLBM2D(g, **kw, precision="fp32") for g, kw in inputs]
     b.run(23)
     for s in alone:
@@ -99,7 +99,7 @@ def test_gpu_batch_autotune_keeps_results():
     for s, r in zip(b.solvers, alone):
         assert s.steps == r.steps == 23 and s.time == r.time
         np.testing.assert_array_equal(s.numpy_field("f0"), r.numpy_field("f0"))
-    for nb, nl in ((128, 32), (512, 128)):
+    for nb, nl in ((64, 64), (128, 32), (512, 128)):
         b.set_blocks(nb, nl)
         b.run(4)
         for s in alone:
