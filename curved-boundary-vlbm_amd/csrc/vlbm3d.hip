@@ -387,23 +387,28 @@ __global__ __launch_bounds__(BLOCK) void k_commit(const double *__restrict__ un,
 
 // Centred / second-order one-sided / first-order derivative of u along one
 // axis (cbw3d:2544-2571).  pp/pm are the neighbours along +axis/-axis and
-// pp2/pm2 the next ones (second-neighbour table); every value the branch
-// might use is loaded unconditionally from an index that falls back to p, so
-// the loads of all branches issue together.
+// pp2/pm2 the next ones (second-neighbour table).  Every value any case
+// might use is loaded from an index that falls back to p, and the case only
+// selects the numerator and the spacing of the one division: with no branch
+// around a load, the loads of all nine derivatives issue together.
 __device__ __forceinline__ double du_axis(const double *uc, int p, int pp, int pm, int pp2, int pm2, double h) {
     const bool fwd = pp >= 0 && pm < 0 && pp2 >= 0, bwd = pm >= 0 && pp < 0 && pm2 >= 0;
     const double u0 = uc[p], up = uc[pp >= 0 ? pp : p], um = uc[pm >= 0 ? pm : p];
     const double up2 = uc[fwd ? pp2 : p], um2 = uc[bwd ? pm2 : p];
-    if (pp >= 0 && pm >= 0) return (up - um) / (2.0 * h);
-    if (pp >= 0) {
-        if (pp2 >= 0) return (-3.0 * u0 + 4.0 * up - up2) / (2.0 * h);
-        return (up - u0) / h;
+    const double central = up - um, fwd2 = -3.0 * u0 + 4.0 * up - up2, fwd1 = up - u0;
+    const double bwd2 = 3.0 * u0 - 4.0 * um + um2, bwd1 = u0 - um;
+    double num = 0.0, den = 1.0;
+    if (pp >= 0 && pm >= 0) {
+        num = central;
+        den = 2.0 * h;
+    } else if (pp >= 0) {
+        num = pp2 >= 0 ? fwd2 : fwd1;
+        den = pp2 >= 0 ? 2.0 * h : h;
+    } else if (pm >= 0) {
+        num = pm2 >= 0 ? bwd2 : bwd1;
+        den = pm2 >= 0 ? 2.0 * h : h;
     }
-    if (pm >= 0) {
-        if (pm2 >= 0) return (3.0 * u0 - 4.0 * um + um2) / (2.0 * h);
-        return (u0 - um) / h;
-    }
-    return 0.0;
+    return num / den;
 }
 
 // Neighbour table of the stream kernel: the neighbour table with each cut
