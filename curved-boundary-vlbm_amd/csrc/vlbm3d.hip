@@ -350,19 +350,35 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(
     for (int k = 0; k < 9; ++k) Fp[k] = U[(3 + k) * n + p];
     // Only the column of F along the link axis enters: the reference's
     // off-axis terms are multiplied by a zero spacing and add exact zeros.
+    // The neighbours' values are loaded from an index that falls back to p
+    // and a missing neighbour's terms are dropped by selects, so no load
+    // waits behind a branch.
+    int pn[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) pn[q] = nbr[q * n + p];
+    double Fq[NQ][3], uq[NQ][3];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        const int pn = nbr[q * n + p];
-        if (pn < 0) continue;
+        const size_t src = pn[q] >= 0 ? pn[q] : p;
+        const int j = q % 3;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            Fq[q][c] = U[(3 + 3 * c + j) * n + src];
+            uq[q][c] = u[c * n + src];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const bool live = pn[q] >= 0;
         const int j = q % 3;
         const double h = (q < 3 ? 1.0 : -1.0) * (j == 0 ? P.dx : (j == 1 ? P.dy : P.dz));
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            double s = Fp[3 * c + j] + U[(3 + 3 * c + j) * n + pn];
+            double s = Fp[3 * c + j] + Fq[q][c];
             if (j == c) s -= 2.0;
             const double g = 0.5 * (s * h);
-            r[c] += u[c * n + pn] - g;
-            d[c] += 1.0;
+            r[c] = live ? r[c] + (uq[q][c] - g) : r[c];
+            d[c] = live ? d[c] + 1.0 : d[c];
         }
     }
 #pragma unroll
