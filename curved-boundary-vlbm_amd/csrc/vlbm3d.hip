@@ -256,6 +256,21 @@ __device__ __forceinline__ double fpost_a(double fv, double Av, const R *rec, si
     return om1 * fv + Av + s * b;
 }
 
+// fpost_a with the record's flux entries of q's axis already loaded:
+// B[r] = B_V[3 r + axis(q)], B[3 + r] = B_F[r].
+__device__ __forceinline__ double fpost_b(double fv, double Av, const double *B, int q, int a, double om1,
+                                          double fs) {
+    const int axis = q % 3;
+    const double s = q < 3 ? 1.0 : -1.0;
+    double b = 0.0;
+    if (a < 3) {
+        b = B[a];
+    } else if ((a - 3) % 3 == axis) {
+        b = B[3 + (a - 3) / 3] * fs;
+    }
+    return om1 * fv + Av + s * b;
+}
+
 // fpost[q][a] at node m from pre-collision f and the node's record.
 template <typename T>
 __device__ __forceinline__ double fpost(const T *f, const T *rec, size_t n, int q, int a, int node, double om1,
@@ -561,6 +576,10 @@ __global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const 
         rst2(rec, k, p, n, k < R_BV ? k : -1, k + 1 < R_BV ? k + 1 : -1, w, rv[k], rv[k + 1]);
 }
 
+// Directions per load group of the pull: 3 (214 VGPRs FP64, two waves per
+// SIMD) measured fastest; 1, 2 and 6 are 2-5% slower.
+constexpr int STREAM_GROUP = 3;
+
 // Pull stream of the post-collision populations; the entries whose upstream
 // node is missing belong to cut links and were written by k_links.  With
 // NEXT the kernel also performs the moment part of the next step's collide
@@ -579,20 +598,41 @@ __global__ __launch_bounds__(BLOCK) void k_stream(const T *__restrict__ f0, T *_
     double Um[NC];
 #pragma unroll
     for (int a = 0; a < NC; ++a) Um[a] = 0.0;
+    // All six upstream indices first, then the directions in groups whose
+    // loads are all issued before the group's stores: the vector-memory
+    // counter drains in order over loads and stores alike, so a load issued
+    // after a store cannot be waited for without waiting for the store.
+    int srcs[NQ];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        const int src = nbr[opp(q) * n + p];
+    for (int q = 0; q < NQ; ++q) srcs[q] = nbr[opp(q) * n + p];
+#pragma unroll
+    for (int q0 = 0; q0 < NQ; q0 += STREAM_GROUP) {
+        // components in pairs (a, a + 1): one 8-byte access each for float
+        // planes; a missing upstream node reads node p instead, unused
+        double fv[STREAM_GROUP][NC], A[STREAM_GROUP][NC], B[STREAM_GROUP][6];
+#pragma unroll
+        for (int g = 0; g < STREAM_GROUP; ++g) {
+            const int q = q0 + g, axis = q % 3;
+            const size_t src = srcs[q] >= 0 ? (size_t)srcs[q] : (size_t)p;
+#pragma unroll
+            for (int h = 0; h < NC / 2; ++h)
+                ld2_nt(f0, q * 12 + 2 * h, src, n, 2 * h, fv[g][2 * h], fv[g][2 * h + 1]);
+#pragma unroll
+            for (int h = 0; h < NC / 2; ++h) rld2(rec, 2 * h, src, n, P.omega, A[g][2 * h], A[g][2 * h + 1]);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                B[g][r] = rld_b(rec, pofs(rec, R_BV + 3 * r + axis, src, n));
+                B[g][3 + r] = rld_b(rec, pofs(rec, R_BF + r, src, n));
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < STREAM_GROUP; ++g) {
+        const int q = q0 + g, src = srcs[q];
         if (src >= 0) {
-            // components in pairs (a, a + 1): one 8-byte access each for float planes
-            double fv[NC], A[NC];
-#pragma unroll
-            for (int h = 0; h < NC / 2; ++h) ld2_nt(f0, q * 12 + 2 * h, src, n, 2 * h, fv[2 * h], fv[2 * h + 1]);
-#pragma unroll
-            for (int h = 0; h < NC / 2; ++h) rld2(rec, 2 * h, src, n, P.omega, A[2 * h], A[2 * h + 1]);
 #pragma unroll
             for (int h = 0; h < NC / 2; ++h) {
-                double x0 = fpost_a(fv[2 * h], A[2 * h], rec, n, q, 2 * h, src, om1, P.cr[q % 3]);
-                double x1 = fpost_a(fv[2 * h + 1], A[2 * h + 1], rec, n, q, 2 * h + 1, src, om1, P.cr[q % 3]);
+                double x0 = fpost_b(fv[g][2 * h], A[g][2 * h], B[g], q, 2 * h, om1, P.cr[q % 3]);
+                double x1 = fpost_b(fv[g][2 * h + 1], A[g][2 * h + 1], B[g], q, 2 * h + 1, om1, P.cr[q % 3]);
                 st2_nt(f1, q * 12 + 2 * h, p, n, 2 * h, x0, x1);
                 if (NEXT) {
                     Um[2 * h] += x0;
@@ -610,6 +650,7 @@ __global__ __launch_bounds__(BLOCK) void k_stream(const T *__restrict__ f0, T *_
 #pragma unroll
                 for (int a = 0; a < NC; ++a) Um[a] += ld(f1, pofs(f1, q * 12 + a, p, n), a);
             }
+        }
         }
     }
     if (NEXT) {
