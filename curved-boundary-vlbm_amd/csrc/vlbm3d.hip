@@ -486,15 +486,27 @@ __global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const 
                                                   int32_t *__restrict__ reset_invalid, Params P) {
     const int p = xcd_block() * BLOCK + threadIdx.x;
     if (p >= P.n) return;
-    // Clears the invalid flag ahead of the stream kernel's fused collide.
-    if (reset_invalid && p == 0) reset_invalid[0] = 0;
     const size_t n = P.n;
+    // Every load comes before the first store: the vector-memory counter
+    // drains in order over loads and stores, so a load behind a store would
+    // wait for the store.
     double v[3], Fo[9];
 #pragma unroll
     for (int k = 0; k < 3; ++k) v[k] = U[k * n + p];
 #pragma unroll
     for (int k = 0; k < 9; ++k) Fo[k] = U[(3 + k) * n + p];
+    const int sl = pack_slot ? pack_slot[p] : -1;
+    double un[3] = {0.0, 0.0, 0.0}, uo[3] = {0.0, 0.0, 0.0}, uso[3] = {0.0, 0.0, 0.0};
+    if (commit)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const size_t i = (size_t)c * n + p;
+            un[c] = ugrad[i];
+            uo[c] = nt_ld(u, i);
+            uso[c] = nt_ld(us, i);
+        }
     double Po[9], dP[9], dF[9], Fnew[9];
+    bool repaired = false;
     piola(Fo, P.m, Po);
 #pragma unroll
     for (int k = 0; k < 9; ++k) Fnew[k] = Fo[k];
@@ -531,28 +543,26 @@ __global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const 
                 dF[k] = Fn[k] - Fo[k];
                 Fnew[k] = Fn[k];
             }
-            if (write_U)
-#pragma unroll
-                for (int k = 0; k < 9; ++k) nt_st(U, (size_t)(3 + k) * n + p, Fn[k]);
+            repaired = true;
         }
     }
-    if (pack_slot) {
-        const int sl = pack_slot[p];
-        if (sl >= 0) {
-            double *pr = pack + (size_t)sl * NC;
+    if (repaired && write_U)
 #pragma unroll
-            for (int k = 0; k < 3; ++k) pr[k] = v[k];
+        for (int k = 0; k < 9; ++k) nt_st(U, (size_t)(3 + k) * n + p, Fnew[k]);
+    if (sl >= 0) {
+        double *pr = pack + (size_t)sl * NC;
 #pragma unroll
-            for (int k = 0; k < 9; ++k) pr[3 + k] = Fnew[k];
-        }
+        for (int k = 0; k < 3; ++k) pr[k] = v[k];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) pr[3 + k] = Fnew[k];
     }
     if (commit) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const size_t i = (size_t)c * n + p;
-            const double un = ugrad[i], delta = un - nt_ld(u, i);
-            nt_st(u, i, un);
-            nt_st(us, i, nt_ld(us, i) + delta);
+            const double delta = un[c] - uo[c];
+            nt_st(u, i, un[c]);
+            nt_st(us, i, uso[c] + delta);
         }
     }
     const double w = P.omega;
@@ -574,6 +584,8 @@ __global__ __launch_bounds__(BLOCK) void k_record(double *__restrict__ U, const 
 #pragma unroll
     for (int k = 0; k < RECORD_PLANES; k += 2)
         rst2(rec, k, p, n, k < R_BV ? k : -1, k + 1 < R_BV ? k + 1 : -1, w, rv[k], rv[k + 1]);
+    // Clears the invalid flag ahead of the stream kernel's fused collide.
+    if (reset_invalid && p == 0) reset_invalid[0] = 0;
 }
 
 // Directions per load group of the pull: 3 (214 VGPRs FP64, two waves per
