@@ -617,6 +617,10 @@ __global__ __launch_bounds__(BLOCK) void k_stream(const T *__restrict__ f0, T *_
     int srcs[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) srcs[q] = nbr[opp(q) * n + p];
+    double uso[3] = {0.0, 0.0, 0.0};
+    if (NEXT)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) uso[k] = nt_ld(us, (size_t)k * n + p);
 #pragma unroll
     for (int q0 = 0; q0 < NQ; q0 += STREAM_GROUP) {
         // components in pairs (a, a + 1): one 8-byte access each for float
@@ -680,7 +684,7 @@ __global__ __launch_bounds__(BLOCK) void k_stream(const T *__restrict__ f0, T *_
         for (int a = 0; a < NC; ++a) nt_st(U, (size_t)a * n + p, Um[a]);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const double uk = nt_ld(us, (size_t)k * n + p) + hdt * Um[k];
+            const double uk = uso[k] + hdt * Um[k];
             nt_st(u, (size_t)k * n + p, uk);
             nt_st(us, (size_t)k * n + p, uk + hdt * Um[k]);
         }
@@ -836,12 +840,9 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prepare(const double *__restrict
     if (T.dyn_en[ell] != 0)
 #pragma unroll
         for (int c = 0; c < 3; ++c) tr[c] = T.dyn_t[c * L + ell];
-    T.lkind[ell] = T.kind_by_id[bid];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        T.lck[c * L + ell] = kk[c];
-        T.lbv[c * L + ell] = tr[c];
-    }
+    // The link's tables are written after its last load (a load behind a
+    // store would wait for the store).
+    const int lkind = T.kind_by_id[bid];
     double Fb[9] = {1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0};
     double Pb[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const bool any_neu = kk[0] == BC_NEU || kk[1] == BC_NEU || kk[2] == BC_NEU;
@@ -906,6 +907,12 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prepare(const double *__restrict
         const bool all_neu = kk[0] == BC_NEU && kk[1] == BC_NEU && kk[2] == BC_NEU;
         solve_Fb(F, nv, tr, kk, !all_neu, P.solver, P.m, Fb);
         piola(Fb, P.m, Pb);
+    }
+    T.lkind[ell] = lkind;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        T.lck[c * L + ell] = kk[c];
+        T.lbv[c * L + ell] = tr[c];
     }
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
