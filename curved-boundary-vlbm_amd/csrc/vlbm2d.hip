@@ -155,6 +155,21 @@ __device__ __forceinline__ void rst_a(float *r, size_t i, int a, double w, doubl
 }
 __device__ __forceinline__ void rst_b(double *r, size_t i, double v) { r[i] = v; }
 __device__ __forceinline__ void rst_b(float *r, size_t i, double v) { r[i] = (float)v; }
+// Record planes (k, k + 1), k even, at node p in one access; fa0 / fa1 the
+// A-plane index whose rest part a float record subtracts, or -1 for a B plane.
+__device__ __forceinline__ void rst2(double *r, int k, size_t p, size_t N, int, int, double, double v0, double v1) {
+    double_pair2 x;
+    x.x = v0;
+    x.y = v1;
+    *(double_pair2 *)(r + pofs2(r, k, p, N)) = x;
+}
+__device__ __forceinline__ void rst2(float *r, int k, size_t p, size_t N, int fa0, int fa1, double w, double v0,
+                                     double v1) {
+    float_pair2 x;
+    x.x = (float)(fa0 >= 0 ? v0 - w * rest(fa0) : v0);
+    x.y = (float)(fa1 >= 0 ? v1 - w * rest(fa1) : v1);
+    *(float_pair2 *)(r + pofs2(r, k, p, N)) = x;
+}
 
 // fpost[q][a] at node p from its pre-collision value fv and the node's record.
 template <typename R>
@@ -425,17 +440,25 @@ __device__ __forceinline__ void k2_record_body(double *__restrict__ U, const dou
     // LAW >= 0 fixes the constitutive law at compile time (the launch checks
     // it is the solver's), so the inlined Piola maps keep only its code.
     if (LAW >= 0) P.m.model = LAW;
-    // Clears the invalid flag ahead of the stream kernel's fused collide
-    // (the link rows never set it).
-    if (reset_invalid && t == 0) reset_invalid[0] = 0;
     const int p = list[t];
     const int ny = P.ny, nx = P.nx;
     const int i = p / ny, j = p - i * ny;
     const size_t N = P.N;
+    // Every load before the first store (a load behind a store would wait
+    // for the store: the vector-memory counter drains in order).
     const double v0 = U[p], v1 = U[N + p];
     double Fo[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) Fo[k] = U[(2 + k) * N + p];
+    double ugc[2] = {0.0, 0.0}, uc[2] = {0.0, 0.0}, usc[2] = {0.0, 0.0};
+    if (commit)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const size_t k = (size_t)c * N + p;
+            ugc[c] = ug[k];
+            uc[c] = u[k];
+            usc[c] = us[k];
+        }
     double Po[4], dP[4] = {0.0, 0.0, 0.0, 0.0}, dF[4] = {0.0, 0.0, 0.0, 0.0};
     piola2(Fo, P.m, Po);
     bool do_rep = repair != 0;
@@ -467,24 +490,29 @@ __device__ __forceinline__ void k2_record_body(double *__restrict__ U, const dou
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
             const size_t k = (size_t)c * N + p;
-            const double delta = ug[k] - u[k];
-            u[k] = ug[k];
-            us[k] = us[k] + delta;
+            const double delta = ugc[c] - uc[c];
+            u[k] = ugc[c];
+            us[k] = usc[c] + delta;
         }
     }
     const double w = P.omega;
     const double st = P.dt * (2.0 - w) * 0.125;
     const double w4 = 0.25 * w, w2c = w / (2.0 * P.c), h2c = 0.5 / P.c;
-    rst_a(rec, pofs2(rec, R_A + 0, p, N), 0, w, w4 * v0 + st * source2(0, P.src_mode, P.src, v0, v1));
-    rst_a(rec, pofs2(rec, R_A + 1, p, N), 1, w, w4 * v1 + st * source2(1, P.src_mode, P.src, v0, v1));
+    double rv[12];
+    rv[R_A + 0] = w4 * v0 + st * source2(0, P.src_mode, P.src, v0, v1);
+    rv[R_A + 1] = w4 * v1 + st * source2(1, P.src_mode, P.src, v0, v1);
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-        rst_a(rec, pofs2(rec, R_A + 2 + k, p, N), 2 + k, w,
-              w4 * Fo[k] + st * source2(2 + k, P.src_mode, P.src, v0, v1) + 0.25 * dF[k]);
+        rv[R_A + 2 + k] = w4 * Fo[k] + st * source2(2 + k, P.src_mode, P.src, v0, v1) + 0.25 * dF[k];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) rst_b(rec, pofs2(rec, R_BV + k, p, N), -w2c * Po[k] - h2c * dP[k]);
-    rst_b(rec, pofs2(rec, R_BF + 0, p, N), -w2c * v0);
-    rst_b(rec, pofs2(rec, R_BF + 1, p, N), -w2c * v1);
+    for (int k = 0; k < 4; ++k) rv[R_BV + k] = -w2c * Po[k] - h2c * dP[k];
+    rv[R_BF + 0] = -w2c * v0;
+    rv[R_BF + 1] = -w2c * v1;
+#pragma unroll
+    for (int k = 0; k < 12; k += 2) rst2(rec, k, p, N, k < R_BV ? k : -1, k + 1 < R_BV ? k + 1 : -1, w, rv[k], rv[k + 1]);
+    // Clears the invalid flag ahead of the stream kernel's fused collide
+    // (the link rows never set it).
+    if (reset_invalid && t == 0) reset_invalid[0] = 0;
 }
 template <typename R, int LAW>
 __global__ __launch_bounds__(BLOCK) void k2_record(double *__restrict__ U, const double *__restrict__ ug,
@@ -807,6 +835,9 @@ __device__ __forceinline__ void k2_links_body(const T *__restrict__ fsrc, T *__r
         }
         return (sv + 1.0) * v0 - sv * v1;
     };
+    // All six values first, then the stores: a load behind a store would wait
+    // for the store (the vector-memory counter drains in order).
+    double vals[6];
 #pragma unroll
     for (int a = 0; a < 6; ++a) {
         double val;
@@ -863,8 +894,10 @@ __device__ __forceinline__ void k2_links_body(const T *__restrict__ fsrc, T *__r
                 val = D * (2.0 * eta * fo + (1.0 - 2.0 * eta) * upv) + S;
             }
         }
-        st(fdst, pofs2(fdst, qi * 6 + a, p, N), a, val);
+        vals[a] = val;
     }
+#pragma unroll
+    for (int a = 0; a < 6; ++a) st(fdst, pofs2(fdst, qi * 6 + a, p, N), a, vals[a]);
 }
 template <typename T, bool EXT, int LAW = -1>
 __global__ __launch_bounds__(LBLOCK) void k2_links(const T *__restrict__ fsrc, T *__restrict__ fdst,
