@@ -725,17 +725,26 @@ __global__ __launch_bounds__(GATHER_BLOCK) void k_gather(const double *__restric
     for (int k = 0; k < GC; ++k) val[k] = gx[k] = gy[k] = gz[k] = 0.0;
     // Samples are consumed in stencil order up to the first empty slot; the
     // indices, weights and records of GU slots are loaded before any is
-    // used, so each lane keeps GU sample loads in flight.  Four (120 VGPRs,
-    // four waves per SIMD) rather than eight (190 VGPRs, two waves): the
-    // short stencils stop earlier and more waves hide the index -> record
-    // load chain.
+    // used, so each lane keeps GU sample loads in flight (four rather than
+    // eight: the short stencils stop earlier).  The sixteen value samples'
+    // loads, free of branches, are all issued together.
     constexpr int GU = 4;
     static_assert(NG % GU == 0, "gradient stencil length must be a multiple of the unroll");
+    // No load sits behind a branch: an empty slot reads row 0 of the pack and
+    // its terms are dropped by selects; the next chunk's indices are
+    // requested before this chunk's records, so the index -> record chain
+    // costs one round trip per chunk instead of two.
     bool alive = true;
+    int nidx[GU];
+#pragma unroll
+    for (int k = 0; k < GU; ++k) nidx[k] = gidx[(size_t)k * Lc + c];
     for (int m0 = 0; m0 < NG && alive; m0 += GU) {
         int sidx[GU];
 #pragma unroll
-        for (int k = 0; k < GU; ++k) sidx[k] = gidx[(size_t)(m0 + k) * Lc + c];
+        for (int k = 0; k < GU; ++k) sidx[k] = nidx[k];
+        if (m0 + GU < NG)
+#pragma unroll
+            for (int k = 0; k < GU; ++k) nidx[k] = gidx[(size_t)(m0 + GU + k) * Lc + c];
         bool use[GU];
 #pragma unroll
         for (int k = 0; k < GU; ++k) {
@@ -745,29 +754,22 @@ __global__ __launch_bounds__(GATHER_BLOCK) void k_gather(const double *__restric
         double w1[GU], w2[GU], w3[GU], x[GU][GC];
 #pragma unroll
         for (int k = 0; k < GU; ++k) {
-            w1[k] = w2[k] = w3[k] = 0.0;
+            const int m = m0 + k;
+            w1[k] = gw[((size_t)0 * NG + m) * Lc + c];
+            w2[k] = gw[((size_t)1 * NG + m) * Lc + c];
+            w3[k] = gw[((size_t)2 * NG + m) * Lc + c];
+            const double *rp = pack + (size_t)(use[k] ? sidx[k] : 0) * NC + a0;
 #pragma unroll
-            for (int j = 0; j < GC; ++j) x[k][j] = 0.0;
-            if (use[k]) {
-                const int m = m0 + k;
-                w1[k] = gw[((size_t)0 * NG + m) * Lc + c];
-                w2[k] = gw[((size_t)1 * NG + m) * Lc + c];
-                w3[k] = gw[((size_t)2 * NG + m) * Lc + c];
-                const double *rp = pack + (size_t)sidx[k] * NC + a0;
-#pragma unroll
-                for (int j = 0; j < GC; ++j) x[k][j] = rp[j];
-            }
+            for (int j = 0; j < GC; ++j) x[k][j] = rp[j];
         }
 #pragma unroll
-        for (int k = 0; k < GU; ++k) {
-            if (!use[k]) continue;
+        for (int k = 0; k < GU; ++k)
 #pragma unroll
             for (int j = 0; j < GC; ++j) {
-                gx[j] += w1[k] * x[k][j];
-                gy[j] += w2[k] * x[k][j];
-                gz[j] += w3[k] * x[k][j];
+                gx[j] = use[k] ? gx[j] + w1[k] * x[k][j] : gx[j];
+                gy[j] = use[k] ? gy[j] + w2[k] * x[k][j] : gy[j];
+                gz[j] = use[k] ? gz[j] + w3[k] * x[k][j] : gz[j];
             }
-        }
     }
     for (int m0 = 0; m0 < 16; m0 += GU) {
         int sidx[GU];
@@ -776,22 +778,15 @@ __global__ __launch_bounds__(GATHER_BLOCK) void k_gather(const double *__restric
         double w0[GU], x[GU][GC];
 #pragma unroll
         for (int k = 0; k < GU; ++k) {
-            w0[k] = 0.0;
+            w0[k] = pw[(size_t)(m0 + k) * Lc + c];
+            const double *rp = pack + (size_t)(sidx[k] >= 0 ? sidx[k] : 0) * NC + a0;
 #pragma unroll
-            for (int j = 0; j < GC; ++j) x[k][j] = 0.0;
-            if (sidx[k] >= 0) {
-                w0[k] = pw[(size_t)(m0 + k) * Lc + c];
-                const double *rp = pack + (size_t)sidx[k] * NC + a0;
-#pragma unroll
-                for (int j = 0; j < GC; ++j) x[k][j] = rp[j];
-            }
+            for (int j = 0; j < GC; ++j) x[k][j] = rp[j];
         }
 #pragma unroll
-        for (int k = 0; k < GU; ++k) {
-            if (sidx[k] < 0) continue;
+        for (int k = 0; k < GU; ++k)
 #pragma unroll
-            for (int j = 0; j < GC; ++j) val[j] += w0[k] * x[k][j];
-        }
+            for (int j = 0; j < GC; ++j) val[j] = sidx[k] >= 0 ? val[j] + w0[k] * x[k][j] : val[j];
     }
     const int ell = clink[c];
 #pragma unroll
