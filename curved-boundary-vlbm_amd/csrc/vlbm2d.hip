@@ -171,6 +171,29 @@ __device__ __forceinline__ void rst2(float *r, int k, size_t p, size_t N, int fa
     *(float_pair2 *)(r + pofs2(r, k, p, N)) = x;
 }
 
+// Record planes (k, k + 1), k even, at node p in one access: A planes
+// (rld2, rest part added back for float records) and B planes (rldb2).
+__device__ __forceinline__ void rld2(const double *r, int k, size_t p, size_t N, double, double &x0, double &x1) {
+    const double_pair2 v = *(const double_pair2 *)(r + pofs2(r, k, p, N));
+    x0 = v.x;
+    x1 = v.y;
+}
+__device__ __forceinline__ void rld2(const float *r, int k, size_t p, size_t N, double w, double &x0, double &x1) {
+    const float_pair2 v = *(const float_pair2 *)(r + pofs2(r, k, p, N));
+    x0 = (double)v.x + w * rest(k - R_A);
+    x1 = (double)v.y + w * rest(k - R_A + 1);
+}
+__device__ __forceinline__ void rldb2(const double *r, int k, size_t p, size_t N, double &x0, double &x1) {
+    const double_pair2 v = *(const double_pair2 *)(r + pofs2(r, k, p, N));
+    x0 = v.x;
+    x1 = v.y;
+}
+__device__ __forceinline__ void rldb2(const float *r, int k, size_t p, size_t N, double &x0, double &x1) {
+    const float_pair2 v = *(const float_pair2 *)(r + pofs2(r, k, p, N));
+    x0 = (double)v.x;
+    x1 = (double)v.y;
+}
+
 // fpost[q][a] at node p from its pre-collision value fv and the node's record.
 template <typename R>
 __device__ __forceinline__ double fpost_v(double fv, const R *rec, size_t N, int q, int a, int p, double om1,
@@ -1084,12 +1107,10 @@ __device__ __forceinline__ void k2_stream_body(const T *__restrict__ f0, T *__re
                 for (int h = 0; h < NC / 2; ++h)
                     ld2_nt(f0, q * 6 + 2 * h, src, N, 2 * h, fv[g][2 * h], fv[g][2 * h + 1]);
 #pragma unroll
-                for (int a = 0; a < NC; ++a) A[g][a] = rld(rec, pofs2(rec, R_A + a, src, N), a, P.omega);
+                for (int h = 0; h < NC / 2; ++h) rld2(rec, R_A + 2 * h, src, N, P.omega, A[g][2 * h], A[g][2 * h + 1]);
 #pragma unroll
-                for (int r = 0; r < 2; ++r) {
-                    B[g][r] = rld_b(rec, pofs2(rec, R_BV + 2 * r + axis, src, N));
-                    B[g][2 + r] = rld_b(rec, pofs2(rec, R_BF + r, src, N));
-                }
+                for (int r = 0; r < 2; ++r) B[g][r] = rld_b(rec, pofs2(rec, R_BV + 2 * r + axis, src, N));
+                rldb2(rec, R_BF, src, N, B[g][2], B[g][3]);
             }
 #pragma unroll
             for (int g = 0; g < STREAM_GROUP2; ++g) {
