@@ -643,30 +643,30 @@ __global__ __launch_bounds__(BLOCK) void k_stream(const T *__restrict__ f0, T *_
         }
 #pragma unroll
         for (int g = 0; g < STREAM_GROUP; ++g) {
-        const int q = q0 + g, src = srcs[q];
-        if (src >= 0) {
+            const int q = q0 + g, src = srcs[q];
+            if (src >= 0) {
 #pragma unroll
-            for (int h = 0; h < NC / 2; ++h) {
-                double x0 = fpost_b(fv[g][2 * h], A[g][2 * h], B[g], q, 2 * h, om1, P.cr[q % 3]);
-                double x1 = fpost_b(fv[g][2 * h + 1], A[g][2 * h + 1], B[g], q, 2 * h + 1, om1, P.cr[q % 3]);
-                st2_nt(f1, q * 12 + 2 * h, p, n, 2 * h, x0, x1);
-                if (NEXT) {
-                    Um[2 * h] += x0;
-                    Um[2 * h + 1] += x1;
+                for (int h = 0; h < NC / 2; ++h) {
+                    double x0 = fpost_b(fv[g][2 * h], A[g][2 * h], B[g], q, 2 * h, om1, P.cr[q % 3]);
+                    double x1 = fpost_b(fv[g][2 * h + 1], A[g][2 * h + 1], B[g], q, 2 * h + 1, om1, P.cr[q % 3]);
+                    st2_nt(f1, q * 12 + 2 * h, p, n, 2 * h, x0, x1);
+                    if (NEXT) {
+                        Um[2 * h] += x0;
+                        Um[2 * h + 1] += x1;
+                    }
+                }
+            } else if (NEXT) {
+                // the cut link's population, written by k_links: from its
+                // link-major copy, else (box faces) from the population planes
+                if (src <= -2) {
+                    const T *fl = flink + (size_t)(-2 - src) * NC;
+#pragma unroll
+                    for (int a = 0; a < NC; ++a) Um[a] += ld(fl, (size_t)a, a);
+                } else {
+#pragma unroll
+                    for (int a = 0; a < NC; ++a) Um[a] += ld(f1, pofs(f1, q * 12 + a, p, n), a);
                 }
             }
-        } else if (NEXT) {
-            // the cut link's population, written by k_links: from its
-            // link-major copy, else (box faces) from the population planes
-            if (src <= -2) {
-                const T *fl = flink + (size_t)(-2 - src) * NC;
-#pragma unroll
-                for (int a = 0; a < NC; ++a) Um[a] += ld(fl, (size_t)a, a);
-            } else {
-#pragma unroll
-                for (int a = 0; a < NC; ++a) Um[a] += ld(f1, pofs(f1, q * 12 + a, p, n), a);
-            }
-        }
         }
     }
     if (NEXT) {
