@@ -81,19 +81,26 @@ __device__ __forceinline__ double step_time(double tb, const Params &P) {
     return P.clock ? P.clock[0] + 0.5 * P.dt : tb;
 }
 
-// Element offsets of population and record planes: adjacent planes paired,
-// pair (plane / 2) * n + node holds planes 2k and 2k + 1 side by side, so the
-// twelve values of one direction at a node (and the record's A planes) are
-// six aligned pairs (8 bytes in FP32, 16 in FP64): the pull stream then moves
-// twice the bytes per lane and access (single floats stream at about two
-// thirds of the rate of pairs; single doubles at 0.93 of it).  The
+// Element offsets of population and record planes in 16-byte groups: FP64
+// planes paired, pair (plane / 2) * n + node holding planes 2k and 2k + 1
+// side by side; FP32 planes in fours (below).  The twelve values of one
+// direction at a node (and the record's A planes) are six FP64 pairs or three
+// FP32 quads, so the pull stream moves 16 bytes per lane and access (single
+// floats stream at about two thirds of the rate of pairs, single doubles at
+// 0.93 of it; FP32 quads took the FP32 pull from 146 to 137 us).  The
 // reference's (72, n) view of the populations exists only through the
 // conversion kernels.
 template <typename T>
 __device__ __forceinline__ size_t pofs(const T *, int plane, size_t node, size_t n) {
     return (((size_t)(plane >> 1) * n + node) << 1) + (size_t)(plane & 1);
 }
+// FP32 planes in fours: quad (plane / 4) * n + node holds planes 4k .. 4k + 3
+// (16 bytes, like an FP64 pair); each even/odd pair stays adjacent inside it.
+__device__ __forceinline__ size_t pofs(const float *, int plane, size_t node, size_t n) {
+    return (((size_t)(plane >> 2) * n + node) << 2) + (size_t)(plane & 3);
+}
 typedef float float_pair __attribute__((ext_vector_type(2)));
+typedef float float_quad __attribute__((ext_vector_type(4)));
 typedef double double_pair __attribute__((ext_vector_type(2)));
 
 // Population access.  double planes hold f; float planes hold f - f_rest
@@ -630,11 +637,27 @@ __global__ __launch_bounds__(BLOCK) void k_stream(const T *__restrict__ f0, T *_
         for (int g = 0; g < STREAM_GROUP; ++g) {
             const int q = q0 + g, axis = q % 3;
             const size_t src = srcs[q] >= 0 ? (size_t)srcs[q] : (size_t)p;
+            if constexpr (sizeof(T) == 4) {
+                // FP32: the direction's twelve values and the record's A planes as three quads each
 #pragma unroll
-            for (int h = 0; h < NC / 2; ++h)
-                ld2_nt(f0, q * 12 + 2 * h, src, n, 2 * h, fv[g][2 * h], fv[g][2 * h + 1]);
+                for (int h = 0; h < NC / 4; ++h) {
+                    const float_quad v = __builtin_nontemporal_load(
+                        (const float_quad *)((const float *)f0 + pofs((const float *)f0, q * 12 + 4 * h, src, n)));
+                    const float_quad r = *(const float_quad *)((const float *)rec +
+                                                               pofs((const float *)rec, 4 * h, src, n));
 #pragma unroll
-            for (int h = 0; h < NC / 2; ++h) rld2(rec, 2 * h, src, n, P.omega, A[g][2 * h], A[g][2 * h + 1]);
+                    for (int e = 0; e < 4; ++e) {
+                        fv[g][4 * h + e] = (double)v[e] + rest(4 * h + e);
+                        A[g][4 * h + e] = (double)r[e] + P.omega * rest(4 * h + e);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int h = 0; h < NC / 2; ++h)
+                    ld2_nt(f0, q * 12 + 2 * h, src, n, 2 * h, fv[g][2 * h], fv[g][2 * h + 1]);
+#pragma unroll
+                for (int h = 0; h < NC / 2; ++h) rld2(rec, 2 * h, src, n, P.omega, A[g][2 * h], A[g][2 * h + 1]);
+            }
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
                 B[g][r] = rld_b(rec, pofs(rec, R_BV + 3 * r + axis, src, n));

@@ -15,7 +15,7 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libvlbm.so"
 
-ABI_VERSION = 8
+ABI_VERSION = 9
 ABI2_VERSION = 8
 PREC_F64 = 0
 PREC_F32 = 1

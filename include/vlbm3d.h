@@ -6,11 +6,11 @@
  * device tensors) and must stay valid until vlbm3d_destroy.  Fields are
  * stored as (components, n_active) planes.  Populations (72 planes, index
  * q*12 + a, directions (+x, +y, +z, -x, -y, -z)) and the record (24 planes)
- * are held with adjacent planes paired: element (plane, node) sits at
- * ((plane / 2) * n_active + node) * 2 + plane % 2, so a node's planes 2k and
- * 2k + 1 are one 16-byte (FP64) or 8-byte (FP32) pair.  A population buffer
- * is therefore opaque; vlbm3d_populations_to_f64 / _from_f64 convert to and
- * from (72, n_active) double planes.
+ * are held in 16-byte groups of adjacent planes: FP64 element (plane, node)
+ * sits at ((plane / 2) * n_active + node) * 2 + plane % 2 (pairs), FP32 at
+ * ((plane / 4) * n_active + node) * 4 + plane % 4 (quads).  A population
+ * buffer is therefore opaque; vlbm3d_populations_to_f64 / _from_f64 convert
+ * to and from (72, n_active) double planes.
  *
  * Every entry point returns 0 on success and a negative code on failure;
  * vlbm_last_error() then describes the failure.  Work is enqueued on the
@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define VLBM3D_ABI_VERSION 8
+#define VLBM3D_ABI_VERSION 9
 #define VLBM3D_MAX_IDS 16
 #define VLBM3D_NPRM 16 /* parameters per boundary id: params_by_id is (MAX_IDS, NPRM) */
 #define VLBM3D_MAX_CHAR 64
@@ -77,7 +77,7 @@ typedef struct {
     /* state (device) */
     double *U, *u, *u_star, *u_next; /* (12,n), (3,n), (3,n), (3,n) */
     void *record;                    /* (24,n) deferred-collision record, in the population precision and
-                                        the paired-plane layout (float records hold A - omega f_rest on
+                                        the grouped-plane layout (float records hold A - omega f_rest on
                                         the A planes) */
     int32_t *invalid;                /* (1) */
     double *reduce_scratch;          /* (>= 8 * 1024) */
